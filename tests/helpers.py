@@ -1,0 +1,60 @@
+"""Shared test helpers: brute-force forests and forest comparison."""
+import numpy as np
+
+import brute
+from cudatree_amd import datasets
+
+FIELDS = ("feature", "threshold", "left", "right", "value", "n_samples", "depth")
+
+
+def case_boot_ids(case):
+    if not case["boot"]:
+        return None
+    return datasets.bootstrap_ids(len(case["y"]), case["n_trees"], case["seed"] & 0xFFFF)
+
+
+def brute_forest(case, orc):
+    """Forest grown by the brute-force builder with the oracle's feature
+    permutation as input."""
+    X, y, K = case["X"], case["y"], case["K"]
+    N, F = X.shape
+    ids = case_boot_ids(case)
+    trees = []
+    for t in range(case["n_trees"]):
+        samples = list(range(N)) if ids is None else [int(i) for i in ids[t]]
+
+        def perm(depth, start, t=t):
+            return orc.feature_perm(case["seed"], t, depth, start, F, case["identity"])
+
+        trees.append(brute.build_tree(X, y, K, samples, case["m"], perm, case["mss"], case["max_depth"]))
+    return concat_trees(trees)
+
+
+def concat_trees(trees):
+    out = {"tree_off": np.zeros(len(trees) + 1, np.int64)}
+    out["tree_off"][1:] = np.cumsum([len(t["feature"]) for t in trees])
+    for k in FIELDS:
+        out[k] = np.concatenate([t[k] for t in trees])
+    return out
+
+
+def oracle_forest(case, orc):
+    return orc.fit(case["X"], case["y"], case["K"], case["n_trees"], case["m"],
+                   boot_ids=case_boot_ids(case), seed=case["seed"],
+                   min_samples_split=case["mss"], max_depth=case["max_depth"],
+                   identity_perm=case["identity"])
+
+
+def forest_diff(a, b, fields=FIELDS):
+    """First differing field as a readable string, or None if identical
+    (thresholds compared bit for bit)."""
+    if not np.array_equal(np.asarray(a["tree_off"]), np.asarray(b["tree_off"])):
+        return "tree_off %s vs %s" % (a["tree_off"], b["tree_off"])
+    for k in fields:
+        x, y = np.asarray(a[k]), np.asarray(b[k])
+        if k == "threshold":
+            x, y = x.astype(np.float32).view(np.uint32), y.astype(np.float32).view(np.uint32)
+        if x.shape != y.shape or not np.array_equal(x, y):
+            idx = np.nonzero(x != y)[0] if x.shape == y.shape else []
+            return "%s differs (first at %s)" % (k, idx[:5])
+    return None
