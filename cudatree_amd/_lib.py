@@ -1,0 +1,325 @@
+"""ctypes binding of libcudatree_amd.so (the C-ABI of include/cudatree_amd.h).
+
+The library is the only compute path: if it is missing or cannot reach a
+GPU, every call raises.  There is no CPU fallback.
+"""
+import ctypes
+import os
+import threading
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libcudatree_amd.so")
+
+CTA_MODE_BATCHED = 0
+CTA_MODE_HYBRID = 1
+CTA_MODE_PER_NODE = 2
+MODES = {"batched": CTA_MODE_BATCHED, "hybrid": CTA_MODE_HYBRID, "per_node": CTA_MODE_PER_NODE}
+
+MAX_SAMPLES = 4194303
+MAX_FEATURES = 65535
+MAX_CLASSES = 1024
+
+_u32p = ctypes.POINTER(ctypes.c_uint32)
+_i32p = ctypes.POINTER(ctypes.c_int32)
+_i64p = ctypes.POINTER(ctypes.c_int64)
+_u64p = ctypes.POINTER(ctypes.c_uint64)
+_f32p = ctypes.POINTER(ctypes.c_float)
+_u8p = ctypes.POINTER(ctypes.c_uint8)
+_vp = ctypes.c_void_p
+
+
+class FitOpts(ctypes.Structure):
+    _fields_ = [
+        ("seed", ctypes.c_uint64),
+        ("max_features", ctypes.c_int32),
+        ("min_samples_split", ctypes.c_int32),
+        ("max_depth", ctypes.c_int32),
+        ("identity_perm", ctypes.c_int32),
+        ("mode", ctypes.c_int32),
+        ("bfs_threshold", ctypes.c_int64),
+        ("scan_chunk", ctypes.c_int32),
+        ("resh_chunk", ctypes.c_int32),
+        ("trees_per_batch", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 5),
+    ]
+
+
+class KernelStat(ctypes.Structure):
+    _fields_ = [
+        ("name", ctypes.c_char * 32),
+        ("launches", ctypes.c_uint64),
+        ("total_ms", ctypes.c_double),
+        ("alg_bytes", ctypes.c_uint64),
+    ]
+
+
+# name -> (restype, argtypes); every entry is declared in include/cudatree_amd.h
+SIGNATURES = {
+    "cta_last_error": (ctypes.c_char_p, []),
+    "cta_version": (ctypes.c_int, []),
+    "cta_ctx_create": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(_vp)]),
+    "cta_ctx_destroy": (ctypes.c_int, [_vp]),
+    "cta_ctx_synchronize": (ctypes.c_int, [_vp]),
+    "cta_ctx_mem_info": (ctypes.c_int, [_vp, _u64p, _u64p]),
+    "cta_dataset_create": (ctypes.c_int, [_vp, _f32p, _u32p, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+                                          ctypes.c_int32, ctypes.POINTER(_vp)]),
+    "cta_dataset_create_device": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+                                                 ctypes.c_int32, ctypes.POINTER(_vp)]),
+    "cta_dataset_destroy": (ctypes.c_int, [_vp]),
+    "cta_dataset_info": (ctypes.c_int, [_vp, _i32p, _i32p, _i32p, _i32p]),
+    "cta_dataset_sorted": (ctypes.c_int, [_vp, _u32p]),
+    "cta_fit_opts_default": (None, [ctypes.POINTER(FitOpts)]),
+    "cta_fit": (ctypes.c_int, [_vp, _vp, ctypes.c_int32, _u32p, _i64p, ctypes.POINTER(FitOpts),
+                               ctypes.POINTER(_vp)]),
+    "cta_forest_destroy": (ctypes.c_int, [_vp]),
+    "cta_forest_info": (ctypes.c_int, [_vp, _i32p, _i64p, _i32p, _i32p]),
+    "cta_forest_export": (ctypes.c_int, [_vp, _i64p, _i32p, _f32p, _u32p, _u32p, _u32p, _u32p, _u32p]),
+    "cta_forest_import": (ctypes.c_int, [_vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, _i64p, _i32p, _f32p,
+                                         _u32p, _u32p, _u32p, ctypes.POINTER(_vp)]),
+    "cta_predict": (ctypes.c_int, [_vp, _vp, _f32p, ctypes.c_int64, ctypes.c_int32, _u32p, _u32p]),
+    "cta_predict_device": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int64, ctypes.c_int32, _vp, _vp]),
+    "cta_ctx_set_profiling": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "cta_ctx_reset_stats": (ctypes.c_int, [_vp]),
+    "cta_ctx_kernel_stats": (ctypes.c_int, [_vp, ctypes.POINTER(KernelStat), ctypes.c_int32, _i32p]),
+    "cta_k_split_scan": (ctypes.c_int, [_vp, _vp, _u32p, ctypes.c_uint32, _i32p, ctypes.c_int32, ctypes.c_int32,
+                                        _i32p, _u32p, _u64p, _u64p]),
+    "cta_k_reshuffle": (ctypes.c_int, [_vp, _vp, _u32p, ctypes.c_uint32, _u8p, ctypes.c_int32, _u32p]),
+}
+
+
+class CtaError(RuntimeError):
+    """A failed library call: `.code` is the CTA_E* status."""
+
+    def __init__(self, code, msg):
+        super().__init__("cudatree_amd error %d: %s" % (code, msg))
+        self.code = code
+
+
+_lib = None
+_lock = threading.Lock()
+
+
+def load():
+    """The loaded library; raises if it has not been built."""
+    global _lib
+    with _lock:
+        if _lib is None:
+            if not os.path.exists(LIB_PATH):
+                raise ImportError("libcudatree_amd.so not built at %s (run __graft_entry__.build() or "
+                                  "make -C cudatree_amd/csrc)" % LIB_PATH)
+            lib = ctypes.CDLL(LIB_PATH)
+            for name, (res, args) in SIGNATURES.items():
+                fn = getattr(lib, name)
+                fn.restype = res
+                fn.argtypes = args
+            _lib = lib
+    return _lib
+
+
+def check(rc):
+    if rc != 0:
+        msg = load().cta_last_error()
+        raise CtaError(rc, msg.decode() if msg else "")
+
+
+def ptr(a, t):
+    return None if a is None else a.ctypes.data_as(t)
+
+
+class Context:
+    """One HIP device with its own stream (cta_ctx)."""
+
+    def __init__(self, device=0):
+        self.lib = load()
+        h = _vp()
+        check(self.lib.cta_ctx_create(device, ctypes.byref(h)))
+        self.h = h
+        self.device = device
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.cta_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def synchronize(self):
+        check(self.lib.cta_ctx_synchronize(self.h))
+
+    def mem_info(self):
+        f, t = ctypes.c_uint64(), ctypes.c_uint64()
+        check(self.lib.cta_ctx_mem_info(self.h, ctypes.byref(f), ctypes.byref(t)))
+        return f.value, t.value
+
+    def set_profiling(self, on):
+        check(self.lib.cta_ctx_set_profiling(self.h, int(bool(on))))
+
+    def reset_stats(self):
+        check(self.lib.cta_ctx_reset_stats(self.h))
+
+    def kernel_stats(self):
+        n = ctypes.c_int32()
+        check(self.lib.cta_ctx_kernel_stats(self.h, None, 0, ctypes.byref(n)))
+        arr = (KernelStat * max(n.value, 1))()
+        check(self.lib.cta_ctx_kernel_stats(self.h, arr, n.value, ctypes.byref(n)))
+        return {arr[i].name.decode(): {"launches": arr[i].launches, "ms": arr[i].total_ms,
+                                       "alg_bytes": arr[i].alg_bytes} for i in range(n.value)}
+
+
+_ctx = {}
+
+
+def default_context(device=0):
+    with _lock:
+        c = _ctx.get(device)
+    if c is None:
+        c = Context(device)
+        with _lock:
+            _ctx.setdefault(device, c)
+            c = _ctx[device]
+    return c
+
+
+class Dataset:
+    """Training set resident on the device, presorted (cta_dataset)."""
+
+    def __init__(self, ctx, X, y, K, idx_bits=0):
+        self.ctx = ctx
+        self.lib = ctx.lib
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        y = np.ascontiguousarray(y, dtype=np.uint32)
+        if X.ndim != 2 or y.ndim != 1 or len(y) != X.shape[0]:
+            raise ValueError("X must be (N, F) and y (N,)")
+        N, F = X.shape
+        h = _vp()
+        check(self.lib.cta_dataset_create(ctx.h, ptr(X, _f32p), ptr(y, _u32p), N, F, K, idx_bits, ctypes.byref(h)))
+        self.h = h
+        self.N, self.F, self.K = N, F, K
+        b = ctypes.c_int32()
+        check(self.lib.cta_dataset_info(h, None, None, None, ctypes.byref(b)))
+        self.idx_bits = b.value
+
+    def sorted_rows(self):
+        out = np.zeros((self.F, self.N), np.uint32)
+        check(self.lib.cta_dataset_sorted(self.h, ptr(out, _u32p)))
+        return out
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.cta_dataset_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+FOREST_FIELDS = ("feature", "threshold", "left", "right", "value", "n_samples", "depth")
+
+
+class Forest:
+    """A fitted or imported forest (cta_forest), resident for prediction."""
+
+    def __init__(self, ctx, h):
+        self.ctx = ctx
+        self.lib = ctx.lib
+        self.h = h
+        T, n, F, K = ctypes.c_int32(), ctypes.c_int64(), ctypes.c_int32(), ctypes.c_int32()
+        check(self.lib.cta_forest_info(h, ctypes.byref(T), ctypes.byref(n), ctypes.byref(F), ctypes.byref(K)))
+        self.n_trees, self.n_nodes, self.F, self.K = T.value, n.value, F.value, K.value
+
+    @classmethod
+    def fit(cls, ctx, ds, n_trees, opts, boot_ids=None):
+        if boot_ids is not None:
+            flat = np.ascontiguousarray(np.concatenate(boot_ids).astype(np.uint32))
+            off = np.zeros(n_trees + 1, np.int64)
+            off[1:] = np.cumsum([len(b) for b in boot_ids])
+        else:
+            flat = off = None
+        h = _vp()
+        check(ctx.lib.cta_fit(ctx.h, ds.h, n_trees, ptr(flat, _u32p), ptr(off, _i64p), ctypes.byref(opts),
+                              ctypes.byref(h)))
+        return cls(ctx, h)
+
+    @classmethod
+    def from_arrays(cls, ctx, arrays, F, K):
+        off = np.ascontiguousarray(arrays["tree_off"], dtype=np.int64)
+        T = len(off) - 1
+        a = {k: np.ascontiguousarray(arrays[k], dtype=dt) for k, dt in
+             (("feature", np.int32), ("threshold", np.float32), ("left", np.uint32), ("right", np.uint32),
+              ("value", np.uint32))}
+        h = _vp()
+        check(ctx.lib.cta_forest_import(ctx.h, T, F, K, ptr(off, _i64p), ptr(a["feature"], _i32p),
+                                        ptr(a["threshold"], _f32p), ptr(a["left"], _u32p), ptr(a["right"], _u32p),
+                                        ptr(a["value"], _u32p), ctypes.byref(h)))
+        return cls(ctx, h)
+
+    def export(self):
+        off = np.zeros(self.n_trees + 1, np.int64)
+        n = self.n_nodes
+        out = {"tree_off": off, "feature": np.zeros(n, np.int32), "threshold": np.zeros(n, np.float32)}
+        for k in ("left", "right", "value", "n_samples", "depth"):
+            out[k] = np.zeros(n, np.uint32)
+        check(self.lib.cta_forest_export(self.h, ptr(off, _i64p), ptr(out["feature"], _i32p),
+                                         ptr(out["threshold"], _f32p), ptr(out["left"], _u32p),
+                                         ptr(out["right"], _u32p), ptr(out["value"], _u32p),
+                                         ptr(out["n_samples"], _u32p), ptr(out["depth"], _u32p)))
+        return out
+
+    def predict(self, X, votes=False):
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        if X.ndim != 2 or X.shape[1] != self.F:
+            raise ValueError("X must be (R, %d)" % self.F)
+        R = X.shape[0]
+        labels = np.zeros(R, np.uint32)
+        v = np.zeros((R, self.K), np.uint32) if votes else None
+        check(self.lib.cta_predict(self.ctx.h, self.h, ptr(X, _f32p), R, self.F, ptr(labels, _u32p), ptr(v, _u32p)))
+        return (labels, v) if votes else labels
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.cta_forest_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def default_opts():
+    o = FitOpts()
+    load().cta_fit_opts_default(ctypes.byref(o))
+    return o
+
+
+def split_scan(ctx, ds, rows, features, chunk=0):
+    """Device split scan of one node (test hook)."""
+    rows = np.ascontiguousarray(rows, dtype=np.uint32)
+    feats = np.ascontiguousarray(features, dtype=np.int32)
+    n = rows.shape[1]
+    bf, bp, num, den = ctypes.c_int32(), ctypes.c_uint32(), ctypes.c_uint64(), ctypes.c_uint64()
+    check(ctx.lib.cta_k_split_scan(ctx.h, ds.h, ptr(rows, _u32p), n, ptr(feats, _i32p), len(feats), chunk,
+                                   ctypes.byref(bf), ctypes.byref(bp), ctypes.byref(num), ctypes.byref(den)))
+    if bf.value < 0:
+        return None
+    return bf.value, bp.value, num.value, den.value
+
+
+def reshuffle(ctx, ds, rows, mark, chunk=0):
+    """Device stable partition of one node's rows (test hook)."""
+    rows = np.ascontiguousarray(rows, dtype=np.uint32)
+    mark = np.ascontiguousarray(mark, dtype=np.uint8)
+    out = np.zeros_like(rows)
+    check(ctx.lib.cta_k_reshuffle(ctx.h, ds.h, ptr(rows, _u32p), rows.shape[1], ptr(mark, _u8p), chunk,
+                                  ptr(out, _u32p)))
+    return out

@@ -1,0 +1,1225 @@
+// api.hip -- host side of libcudatree_amd.so: the C-ABI declared in
+// include/cudatree_amd.h, the fit driver (batched / hybrid / per-node
+// schedules over the kernels of fit_kernels.hip), export to canonical
+// pre-order trees, and prediction.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/cudatree_amd.h"
+#include "aux_kernels.hip"
+#include "fit_kernels.hip"
+
+using namespace cta;
+
+// ---- errors ------------------------------------------------------------------
+namespace {
+thread_local std::string g_err;
+
+struct Error : std::runtime_error {
+  int code;
+  Error(int c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+
+[[noreturn]] void fail(int code, const char *fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  throw Error(code, buf);
+}
+
+void hip_check(hipError_t e, const char *what) {
+  if (e != hipSuccess) fail(CTA_EHIP, "%s: %s", what, hipGetErrorString(e));
+}
+#define HIPC(x) hip_check((x), #x)
+
+template <class F>
+int guarded(F &&f) {
+  try {
+    f();
+    g_err.clear();
+    return CTA_OK;
+  } catch (const Error &e) {
+    g_err = e.what();
+    return e.code;
+  } catch (const std::bad_alloc &) {
+    g_err = "host allocation failed";
+    return CTA_ENOMEM;
+  } catch (const std::exception &e) {
+    g_err = e.what();
+    return CTA_EINTERNAL;
+  }
+}
+
+// Device allocation owned by RAII.
+struct DevMem {
+  void *p = nullptr;
+  size_t bytes = 0;
+  DevMem() = default;
+  explicit DevMem(size_t n) { alloc(n); }
+  void alloc(size_t n) {
+    release();
+    if (n == 0) n = 1;
+    hipError_t e = hipMalloc(&p, n);
+    if (e != hipSuccess) {
+      p = nullptr;
+      (void)hipGetLastError();
+      fail(CTA_ENOMEM, "hipMalloc of %zu bytes failed: %s", n, hipGetErrorString(e));
+    }
+    bytes = n;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  ~DevMem() { release(); }
+  DevMem(const DevMem &) = delete;
+  DevMem &operator=(const DevMem &) = delete;
+  template <class T>
+  T *as() const {
+    return static_cast<T *>(p);
+  }
+};
+
+uint32_t grid_for(size_t n, int block = 256, uint32_t cap = 65535u * 16) {
+  size_t g = (n + block - 1) / block;
+  if (g < 1) g = 1;
+  return (uint32_t)std::min<size_t>(g, cap);
+}
+}  // namespace
+
+// ---- context -------------------------------------------------------------------
+struct KStat {
+  uint64_t launches = 0;
+  double ms = 0;
+  uint64_t bytes = 0;
+};
+
+struct cta_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::recursive_mutex mu;
+  int num_cu = 256;
+  bool profiling = false;
+  std::map<std::string, KStat> stats;
+  struct Pending {
+    std::string name;
+    hipEvent_t a, b;
+  };
+  std::vector<Pending> pending;
+  std::vector<hipEvent_t> pool;
+
+  hipEvent_t ev() {
+    if (!pool.empty()) {
+      hipEvent_t e = pool.back();
+      pool.pop_back();
+      return e;
+    }
+    hipEvent_t e;
+    HIPC(hipEventCreate(&e));
+    return e;
+  }
+  // Resolves recorded events (after a synchronisation).
+  void resolve() {
+    for (auto &p : pending) {
+      float ms = 0;
+      HIPC(hipEventSynchronize(p.b));
+      HIPC(hipEventElapsedTime(&ms, p.a, p.b));
+      KStat &s = stats[p.name];
+      s.launches += 1;
+      s.ms += ms;
+      pool.push_back(p.a);
+      pool.push_back(p.b);
+    }
+    pending.clear();
+  }
+  void sync() {
+    HIPC(hipStreamSynchronize(stream));
+    if (profiling) resolve();
+  }
+  void add_bytes(const char *name, uint64_t b) {
+    if (profiling) stats[name].bytes += b;
+  }
+};
+
+// Brackets one launch with events on the context stream when profiling.
+struct Timed {
+  cta_ctx *c;
+  const char *name;
+  hipEvent_t a = nullptr, b = nullptr;
+  Timed(cta_ctx *ctx, const char *n) : c(ctx), name(n) {
+    if (c->profiling) {
+      a = c->ev();
+      b = c->ev();
+      HIPC(hipEventRecord(a, c->stream));
+    }
+  }
+  ~Timed() {
+    if (c->profiling && a) {
+      (void)hipEventRecord(b, c->stream);
+      c->pending.push_back({name, a, b});
+    }
+  }
+};
+
+// ---- dataset -------------------------------------------------------------------
+struct cta_dataset {
+  cta_ctx *ctx = nullptr;
+  int32_t N = 0, F = 0, K = 0, idx_bits = 32, lab_bits = 8;
+  DevMem XT, y, si0;
+};
+
+// ---- forest --------------------------------------------------------------------
+struct cta_forest {
+  cta_ctx *ctx = nullptr;
+  int32_t T = 0, F = 0, K = 0;
+  std::vector<int64_t> tree_off{0};
+  std::vector<int32_t> feature;
+  std::vector<float> threshold;
+  std::vector<uint32_t> left, right, value, nsamp, depth;
+  DevMem nodes, roots;
+  bool device_ready = false;
+};
+
+namespace {
+
+void upload_forest(cta_forest *f) {
+  if (f->device_ready) return;
+  const size_t n = f->feature.size();
+  std::vector<PNode> h(std::max<size_t>(n, 1));
+  for (int32_t t = 0; t < f->T; ++t) {
+    const int64_t base = f->tree_off[t];
+    for (int64_t j = base; j < f->tree_off[t + 1]; ++j) {
+      PNode &p = h[j];
+      const bool leaf = f->left[j] == 0;
+      p.thr = f->threshold[j];
+      p.feat = leaf ? -1 : f->feature[j];
+      p.right = leaf ? 0u : (uint32_t)(base + f->right[j]);
+      p.value = f->value[j];
+    }
+  }
+  std::vector<uint32_t> roots(std::max(f->T, 1));
+  for (int32_t t = 0; t < f->T; ++t) roots[t] = (uint32_t)f->tree_off[t];
+  f->nodes.alloc(sizeof(PNode) * h.size());
+  f->roots.alloc(sizeof(uint32_t) * roots.size());
+  HIPC(hipMemcpyAsync(f->nodes.p, h.data(), sizeof(PNode) * h.size(), hipMemcpyHostToDevice, f->ctx->stream));
+  HIPC(hipMemcpyAsync(f->roots.p, roots.data(), sizeof(uint32_t) * roots.size(), hipMemcpyHostToDevice,
+                      f->ctx->stream));
+  f->ctx->sync();
+  f->device_ready = true;
+}
+
+// ---- fit driver ------------------------------------------------------------------
+struct HostEntry {
+  FNode nd;
+  std::vector<uint32_t> cnt;
+};
+
+template <class IdxT, class LabT>
+struct Fitter {
+  cta_ctx *ctx;
+  cta_dataset *ds;
+  const cta_fit_opts &o;
+  int32_t N, F, K, m;
+  uint32_t Cs, Cf, Cr;
+  hipStream_t st;
+  uint32_t G;  // blocks of the wave-loop kernels
+
+  Fitter(cta_ctx *c, cta_dataset *d, const cta_fit_opts &opts) : ctx(c), ds(d), o(opts) {
+    N = d->N;
+    F = d->F;
+    K = d->K;
+    m = opts.max_features;
+    Cs = opts.scan_chunk > 0 ? (uint32_t)opts.scan_chunk : 8192u;
+    Cr = opts.resh_chunk > 0 ? (uint32_t)opts.resh_chunk : 8192u;
+    Cf = 8192u;
+    st = c->stream;
+    G = (uint32_t)std::max(64, ctx->num_cu * 8);
+  }
+
+  size_t smem() const { return (size_t)kWavesPerBlock * 2 * K * sizeof(uint32_t); }
+
+  // Bytes of device memory one resident tree needs (rows, marks, level state).
+  size_t bytes_per_tree() const {
+    const size_t rows = 2 * (size_t)F * N * sizeof(IdxT) + N;
+    const size_t per_sample = 2 * sizeof(FNode) + 3 * 4 * (size_t)K + sizeof(NAux) + 4 * (size_t)m +
+                              2 * 7 * 4 + (sizeof(Task) + sizeof(ScanRes)) + 2 * sizeof(Task);
+    return rows + per_sample * (size_t)N / 2 + 4096;
+  }
+
+  // Grows trees [t0, t0 + Tb) and appends them to `out` in pre-order.
+  void run_batch(int32_t t0, int32_t Tb, const uint32_t *boot_ids, const int64_t *boot_off, cta_forest *out) {
+    const bool boot = boot_ids != nullptr;
+    std::vector<uint64_t> off(Tb + 1, 0);
+    uint64_t n_tot = 0;
+    for (int32_t t = 0; t < Tb; ++t) {
+      const uint64_t n = boot ? (uint64_t)(boot_off[t0 + t + 1] - boot_off[t0 + t]) : (uint64_t)N;
+      off[t + 1] = off[t] + n;
+      n_tot += n;
+    }
+    const uint64_t cap_q = n_tot / 2 + Tb + 64;
+    const uint64_t cap_nodes = 2 * n_tot + Tb + 64;
+    const uint64_t cap_scan = cap_q + 3ull * m * n_tot / Cs + 64;
+    const uint64_t cap_resh = std::max<uint64_t>(cap_q + 3ull * F * n_tot / Cr + 64, (uint64_t)Tb * F * (N / Cr + 2));
+    const uint64_t cap_fill = cap_q + n_tot / Cf + 64;
+    const uint64_t cap_hist = (uint64_t)m * K * (2 * n_tot / Cs + 64);
+    const uint64_t cap_rcnt = std::max<uint64_t>((uint64_t)F * (2 * n_tot / Cr + 64), (uint64_t)Tb * F * (N / Cr + 2));
+    if (cap_scan > 0xFFFFFFF0ull || cap_resh > 0xFFFFFFF0ull || cap_nodes > 0xFFFFFFF0ull || cap_hist > 0xFFFFFFF0ull)
+      fail(CTA_EUNSUPPORTED, "batch of %d trees too large for 32-bit task indices", Tb);
+
+    DevMem buf0((size_t)Tb * F * N * sizeof(IdxT)), buf1((size_t)Tb * F * N * sizeof(IdxT));
+    DevMem mark((size_t)Tb * N);
+    DevMem front[2] = {DevMem(cap_q * sizeof(FNode)), DevMem(cap_q * sizeof(FNode))};
+    DevMem cnt[2] = {DevMem(cap_q * K * 4), DevMem(cap_q * K * 4)};
+    DevMem cntL(cap_q * K * 4), aux(cap_q * sizeof(NAux)), feats(cap_q * m * 4);
+    DevMem scan_tasks(cap_scan * sizeof(Task)), res(cap_scan * sizeof(ScanRes)), hist(cap_hist * 4);
+    DevMem fill_tasks(cap_fill * sizeof(Task)), resh_tasks(cap_resh * sizeof(Task)), rcnt(cap_rcnt * 4);
+    DevMem a_feat(cap_nodes * 4), a_thr(cap_nodes * 4), a_left(cap_nodes * 4), a_right(cap_nodes * 4),
+        a_value(cap_nodes * 4), a_nsamp(cap_nodes * 4), a_depth(cap_nodes * 4);
+    DevMem ctl(sizeof(Ctl)), root_cnt((size_t)Tb * K * 4), d_off((Tb + 1) * 8);
+    DevMem d_ids;
+
+    FitArgs<IdxT, LabT> a = {};
+    a.X_T = ds->XT.as<float>();
+    a.y = ds->y.as<LabT>();
+    a.si0 = ds->si0.as<IdxT>();
+    a.buf0 = buf0.as<IdxT>();
+    a.buf1 = buf1.as<IdxT>();
+    a.mark = mark.as<uint8_t>();
+    a.N = N;
+    a.F = F;
+    a.K = K;
+    a.m = m;
+    a.mss = o.min_samples_split;
+    a.max_depth = o.max_depth;
+    a.identity = o.identity_perm;
+    a.tree_base = (uint32_t)t0;
+    a.seed = o.seed;
+    a.Cs = Cs;
+    a.Cf = Cf;
+    a.Cr = Cr;
+    a.cntL = cntL.as<uint32_t>();
+    a.aux = aux.as<NAux>();
+    a.feats = feats.as<int32_t>();
+    a.scan_tasks = scan_tasks.as<Task>();
+    a.res = res.as<ScanRes>();
+    a.hist = hist.as<uint32_t>();
+    a.fill_tasks = fill_tasks.as<Task>();
+    a.resh_tasks = resh_tasks.as<Task>();
+    a.rcnt = rcnt.as<uint32_t>();
+    a.arena = Arena{a_feat.as<int32_t>(), a_thr.as<float>(), a_left.as<uint32_t>(), a_right.as<uint32_t>(),
+                    a_value.as<uint32_t>(), a_nsamp.as<uint32_t>(), a_depth.as<uint32_t>()};
+    a.ctl = ctl.as<Ctl>();
+    a.cap_q = (uint32_t)cap_q;
+    a.cap_scan = (uint32_t)cap_scan;
+    a.cap_hist = (uint32_t)cap_hist;
+    a.cap_fill = (uint32_t)cap_fill;
+    a.cap_resh = (uint32_t)cap_resh;
+    a.cap_rcnt = (uint32_t)cap_rcnt;
+    a.cap_nodes = (uint32_t)cap_nodes;
+
+    // ---- setup: bootstrap compaction and roots
+    Ctl hc = {};
+    hc.n_nodes = (uint32_t)Tb;
+    HIPC(hipMemcpyAsync(ctl.p, &hc, sizeof(Ctl), hipMemcpyHostToDevice, st));
+    HIPC(hipMemcpyAsync(d_off.p, off.data(), (Tb + 1) * 8, hipMemcpyHostToDevice, st));
+    k_fill_u32<<<grid_for((size_t)Tb * K), 256, 0, st>>>(root_cnt.as<uint32_t>(), (size_t)Tb * K, 0u);
+    uint32_t src_root = 0;
+    if (boot) {
+      std::vector<uint32_t> ids(n_tot);
+      for (int32_t t = 0; t < Tb; ++t)
+        std::memcpy(ids.data() + off[t], boot_ids + boot_off[t0 + t], (off[t + 1] - off[t]) * 4);
+      d_ids.alloc(n_tot * 4);
+      HIPC(hipMemcpyAsync(d_ids.p, ids.data(), n_tot * 4, hipMemcpyHostToDevice, st));
+      k_fill_u8<<<grid_for((size_t)Tb * N), 256, 0, st>>>(mark.as<uint8_t>(), (size_t)Tb * N, 0);
+      k_boot_mark<<<dim3(grid_for(N, 256, 1024), Tb), 256, 0, st>>>(d_ids.as<uint32_t>(), d_off.as<uint64_t>(), Tb,
+                                                                    N, mark.as<uint8_t>());
+      k_root_counts<LabT><<<dim3(grid_for(N, 256, 256), Tb), 256, K * 4, st>>>(
+          a.y, d_ids.as<uint32_t>(), d_off.as<uint64_t>(), Tb, N, K, root_cnt.as<uint32_t>());
+      a.front = front[0].as<FNode>();
+      k_boot_tasks<IdxT, LabT><<<grid_for((size_t)Tb * 64), 256, 0, st>>>(a, d_off.as<uint64_t>(), Tb,
+                                                                          front[0].as<FNode>(), 0u);
+      {
+        Timed tm(ctx, "k_boot_count");
+        k_resh_count<IdxT, LabT><<<G, kBlock, 0, st>>>(a);
+      }
+      k_resh_carry<IdxT, LabT><<<G, kBlock, 0, st>>>(a);
+      {
+        Timed tm(ctx, "k_boot_compact");
+        k_resh<IdxT, LabT><<<G, kBlock, 0, st>>>(a);
+      }
+      src_root = 1;
+    } else {
+      k_root_counts<LabT><<<dim3(grid_for(N, 256, 256), Tb), 256, K * 4, st>>>(a.y, nullptr, nullptr, Tb, N, K,
+                                                                              root_cnt.as<uint32_t>());
+    }
+    a.next_front = front[1].as<FNode>();
+    a.next_cnt = cnt[1].as<uint32_t>();
+    k_roots<IdxT, LabT><<<grid_for((size_t)Tb * 64), 256, 0, st>>>(a, root_cnt.as<uint32_t>(),
+                                                                   boot ? d_off.as<uint64_t>() : nullptr, Tb, src_root);
+    k_ctl_next<<<1, 64, 0, st>>>(a.ctl);
+    int cur = 1;  // front[cur] holds the frontier
+    read_ctl(a, hc);
+    if (hc.err) fail(CTA_EINTERNAL, "device capacity check failed during setup (err %u)", hc.err);
+    if (boot) ctx->add_bytes("k_boot_compact", hc.bytes_boot);
+
+    const int mode = o.mode;
+    if (mode == CTA_MODE_BATCHED) {
+      uint32_t q = hc.q;
+      while (q > 0) {
+        a.front = front[cur].as<FNode>();
+        a.cnt = cnt[cur].as<uint32_t>();
+        a.next_front = front[cur ^ 1].as<FNode>();
+        a.next_cnt = cnt[cur ^ 1].as<uint32_t>();
+        level(a);
+        cur ^= 1;
+        Ctl prev = hc;
+        read_ctl(a, hc);
+        account(prev, hc);
+        if (hc.err) fail(CTA_EINTERNAL, "device capacity check failed (err %u)", hc.err);
+        q = hc.q;
+      }
+    } else {
+      // per-node (and hybrid) schedule: host stack of frontier entries
+      std::vector<HostEntry> stack, queue;
+      pull_frontier(front[cur], cnt[cur], hc.q, stack);
+      std::reverse(stack.begin(), stack.end());
+      const uint64_t thr = mode == CTA_MODE_HYBRID ? (uint64_t)std::max<int64_t>(o.bfs_threshold, 0) : 0;
+      while (!stack.empty()) {
+        HostEntry e = std::move(stack.back());
+        stack.pop_back();
+        if (mode == CTA_MODE_HYBRID && e.nd.n <= thr) {
+          queue.push_back(std::move(e));
+          continue;
+        }
+        push_frontier(a, front[0], cnt[0], {e});
+        a.front = front[0].as<FNode>();
+        a.cnt = cnt[0].as<uint32_t>();
+        a.next_front = front[1].as<FNode>();
+        a.next_cnt = cnt[1].as<uint32_t>();
+        level(a);
+        Ctl prev = hc;
+        read_ctl(a, hc);
+        account(prev, hc);
+        if (hc.err) fail(CTA_EINTERNAL, "device capacity check failed (err %u)", hc.err);
+        std::vector<HostEntry> kids;
+        pull_frontier(front[1], cnt[1], hc.q, kids);
+        for (auto it = kids.rbegin(); it != kids.rend(); ++it) stack.push_back(std::move(*it));
+      }
+      if (!queue.empty()) {
+        push_frontier(a, front[0], cnt[0], queue);
+        cur = 0;
+        uint32_t q = (uint32_t)queue.size();
+        while (q > 0) {
+          a.front = front[cur].as<FNode>();
+          a.cnt = cnt[cur].as<uint32_t>();
+          a.next_front = front[cur ^ 1].as<FNode>();
+          a.next_cnt = cnt[cur ^ 1].as<uint32_t>();
+          level(a);
+          cur ^= 1;
+          Ctl prev = hc;
+          read_ctl(a, hc);
+          account(prev, hc);
+          if (hc.err) fail(CTA_EINTERNAL, "device capacity check failed (err %u)", hc.err);
+          q = hc.q;
+        }
+      }
+    }
+    export_batch(a, hc.n_nodes, Tb, out);
+  }
+
+  void read_ctl(const FitArgs<IdxT, LabT> &a, Ctl &hc) {
+    HIPC(hipMemcpyAsync(&hc, a.ctl, sizeof(Ctl), hipMemcpyDeviceToHost, st));
+    ctx->sync();
+  }
+
+  void account(const Ctl &prev, const Ctl &now) {
+    ctx->add_bytes("k_scan", now.bytes_scan - prev.bytes_scan);
+    ctx->add_bytes("k_fill", now.bytes_fill - prev.bytes_fill);
+    ctx->add_bytes("k_resh", now.bytes_resh - prev.bytes_resh);
+  }
+
+  // One level over the frontier in a.front (count in ctl->q).
+  void level(const FitArgs<IdxT, LabT> &a) {
+    const size_t sm = smem();
+    {
+      Timed tm(ctx, "k_select");
+      k_select<IdxT, LabT><<<G, kBlock, 0, st>>>(a);
+    }
+    {
+      Timed tm(ctx, "k_scan_hist");
+      k_scan_hist<IdxT, LabT><<<G, kBlock, sm, st>>>(a);
+    }
+    k_scan_carry<IdxT, LabT><<<G, kBlock, 0, st>>>(a);
+    {
+      Timed tm(ctx, "k_scan");
+      k_scan<IdxT, LabT><<<G, kBlock, sm, st>>>(a);
+    }
+    {
+      Timed tm(ctx, "k_decide");
+      k_decide<IdxT, LabT><<<G, kBlock, 0, st>>>(a);
+    }
+    {
+      Timed tm(ctx, "k_fill");
+      k_fill<IdxT, LabT><<<G, kBlock, sm, st>>>(a);
+    }
+    {
+      Timed tm(ctx, "k_children");
+      k_children<IdxT, LabT><<<G, kBlock, sm, st>>>(a);
+    }
+    {
+      Timed tm(ctx, "k_resh_count");
+      k_resh_count<IdxT, LabT><<<G, kBlock, 0, st>>>(a);
+    }
+    k_resh_carry<IdxT, LabT><<<G, kBlock, 0, st>>>(a);
+    {
+      Timed tm(ctx, "k_resh");
+      k_resh<IdxT, LabT><<<G, kBlock, 0, st>>>(a);
+    }
+    k_ctl_next<<<1, 64, 0, st>>>(a.ctl);
+    HIPC(hipGetLastError());
+  }
+
+  void pull_frontier(const DevMem &fr, const DevMem &c, uint32_t q, std::vector<HostEntry> &out) {
+    if (q == 0) return;
+    std::vector<FNode> nodes(q);
+    std::vector<uint32_t> counts((size_t)q * K);
+    HIPC(hipMemcpyAsync(nodes.data(), fr.p, q * sizeof(FNode), hipMemcpyDeviceToHost, st));
+    HIPC(hipMemcpyAsync(counts.data(), c.p, (size_t)q * K * 4, hipMemcpyDeviceToHost, st));
+    ctx->sync();
+    for (uint32_t i = 0; i < q; ++i) {
+      HostEntry e;
+      e.nd = nodes[i];
+      e.cnt.assign(counts.begin() + (size_t)i * K, counts.begin() + (size_t)(i + 1) * K);
+      out.push_back(std::move(e));
+    }
+  }
+
+  void push_frontier(const FitArgs<IdxT, LabT> &a, DevMem &fr, DevMem &c, const std::vector<HostEntry> &es) {
+    std::vector<FNode> nodes;
+    std::vector<uint32_t> counts;
+    for (const auto &e : es) {
+      nodes.push_back(e.nd);
+      counts.insert(counts.end(), e.cnt.begin(), e.cnt.end());
+    }
+    HIPC(hipMemcpyAsync(fr.p, nodes.data(), nodes.size() * sizeof(FNode), hipMemcpyHostToDevice, st));
+    HIPC(hipMemcpyAsync(c.p, counts.data(), counts.size() * 4, hipMemcpyHostToDevice, st));
+    k_ctl_set<<<1, 64, 0, st>>>(a.ctl, (uint32_t)es.size());
+    // host vectors must outlive the asynchronous copies
+    ctx->sync();
+  }
+
+  void export_batch(const FitArgs<IdxT, LabT> &a, uint32_t n_nodes, int32_t Tb, cta_forest *out) {
+    std::vector<int32_t> feat(n_nodes);
+    std::vector<float> thr(n_nodes);
+    std::vector<uint32_t> lf(n_nodes), rt(n_nodes), val(n_nodes), ns(n_nodes), dp(n_nodes);
+    HIPC(hipMemcpyAsync(feat.data(), a.arena.feat, n_nodes * 4, hipMemcpyDeviceToHost, st));
+    HIPC(hipMemcpyAsync(thr.data(), a.arena.thr, n_nodes * 4, hipMemcpyDeviceToHost, st));
+    HIPC(hipMemcpyAsync(lf.data(), a.arena.left, n_nodes * 4, hipMemcpyDeviceToHost, st));
+    HIPC(hipMemcpyAsync(rt.data(), a.arena.right, n_nodes * 4, hipMemcpyDeviceToHost, st));
+    HIPC(hipMemcpyAsync(val.data(), a.arena.value, n_nodes * 4, hipMemcpyDeviceToHost, st));
+    HIPC(hipMemcpyAsync(ns.data(), a.arena.nsamp, n_nodes * 4, hipMemcpyDeviceToHost, st));
+    HIPC(hipMemcpyAsync(dp.data(), a.arena.depth, n_nodes * 4, hipMemcpyDeviceToHost, st));
+    ctx->sync();
+    std::vector<uint32_t> order, pos(n_nodes, 0xFFFFFFFFu), stk;
+    for (int32_t t = 0; t < Tb; ++t) {
+      order.clear();
+      stk.assign(1, (uint32_t)t);
+      while (!stk.empty()) {
+        const uint32_t id = stk.back();
+        stk.pop_back();
+        if (id >= n_nodes) fail(CTA_EINTERNAL, "node id %u out of range", id);
+        pos[id] = (uint32_t)order.size();
+        order.push_back(id);
+        if (feat[id] >= 0) {
+          stk.push_back(rt[id]);
+          stk.push_back(lf[id]);
+        }
+      }
+      for (uint32_t id : order) {
+        const bool leaf = feat[id] < 0;
+        out->feature.push_back(leaf ? 0 : feat[id]);
+        out->threshold.push_back(leaf ? 0.f : thr[id]);
+        out->left.push_back(leaf ? 0u : pos[lf[id]]);
+        out->right.push_back(leaf ? 0u : pos[rt[id]]);
+        out->value.push_back(leaf ? val[id] : 0u);
+        out->nsamp.push_back(ns[id]);
+        out->depth.push_back(dp[id]);
+      }
+      out->tree_off.push_back(out->tree_off.back() + (int64_t)order.size());
+      out->T += 1;
+    }
+  }
+};
+
+template <class IdxT, class LabT>
+void fit_impl(cta_ctx *ctx, cta_dataset *ds, int32_t T, const uint32_t *ids, const int64_t *off,
+              const cta_fit_opts &o, cta_forest *out) {
+  Fitter<IdxT, LabT> fz(ctx, ds, o);
+  int32_t per = o.trees_per_batch;
+  if (o.mode != CTA_MODE_BATCHED) per = 1;
+  if (per <= 0) {
+    size_t fr = 0, tot = 0;
+    HIPC(hipMemGetInfo(&fr, &tot));
+    const size_t bpt = fz.bytes_per_tree();
+    per = (int32_t)std::max<size_t>(1, (size_t)(0.6 * (double)fr) / bpt);
+  }
+  per = std::min(per, T);
+  for (int32_t t0 = 0; t0 < T; t0 += per) fz.run_batch(t0, std::min(per, T - t0), ids, off, out);
+}
+
+template <class IdxT>
+void presort(cta_ctx *ctx, cta_dataset *ds) {
+  const uint32_t N = ds->N, F = ds->F;
+  const size_t total = (size_t)N * F;
+  hipStream_t st = ctx->stream;
+  DevMem k0(total * 4), k1(total * 4), v0(total * 4), v1(total * 4);
+  const uint32_t ntiles = (N + kSortTile - 1) / kSortTile;
+  DevMem hist((size_t)F * kDigits * ntiles * 4);
+  k_sort_init<<<grid_for(total), 256, 0, st>>>(ds->XT.as<float>(), N, F, k0.as<uint32_t>(), v0.as<uint32_t>());
+  uint32_t *kin = k0.as<uint32_t>(), *vin = v0.as<uint32_t>(), *kout = k1.as<uint32_t>(), *vout = v1.as<uint32_t>();
+  {
+    Timed tm(ctx, "k_presort");
+    for (uint32_t shift = 0; shift < 32; shift += 4) {
+      k_sort_hist<<<dim3(ntiles, F), kSortThreads, 0, st>>>(kin, N, ntiles, shift, hist.as<uint32_t>());
+      k_sort_scan<<<F, kSortThreads, 0, st>>>(hist.as<uint32_t>(), ntiles);
+      k_sort_scatter<<<dim3(ntiles, F), kSortThreads, 0, st>>>(kin, vin, kout, vout, N, ntiles, shift,
+                                                               hist.as<uint32_t>());
+      std::swap(kin, kout);
+      std::swap(vin, vout);
+    }
+  }
+  ds->si0.alloc(total * sizeof(IdxT));
+  k_sort_finish<IdxT><<<grid_for(total), 256, 0, st>>>(vin, total, ds->si0.as<IdxT>());
+  HIPC(hipGetLastError());
+  ctx->sync();
+}
+
+void dataset_build(cta_ctx *ctx, cta_dataset *ds, const float *X_dev, const uint32_t *y_dev) {
+  const uint32_t N = ds->N, F = ds->F;
+  hipStream_t st = ctx->stream;
+  DevMem err(4);
+  HIPC(hipMemsetAsync(err.p, 0, 4, st));
+  ds->XT.alloc((size_t)N * F * 4);
+  k_transpose<<<dim3((N + kTT - 1) / kTT, (F + kTT - 1) / kTT), 256, 0, st>>>(X_dev, N, F, ds->XT.as<float>(),
+                                                                             err.as<uint32_t>());
+  if (ds->lab_bits == 8) {
+    ds->y.alloc(N);
+    k_labels<uint8_t><<<grid_for(N), 256, 0, st>>>(y_dev, N, ds->K, ds->y.as<uint8_t>(), err.as<uint32_t>());
+  } else {
+    ds->y.alloc((size_t)N * 2);
+    k_labels<uint16_t><<<grid_for(N), 256, 0, st>>>(y_dev, N, ds->K, ds->y.as<uint16_t>(), err.as<uint32_t>());
+  }
+  uint32_t herr = 0;
+  HIPC(hipMemcpyAsync(&herr, err.p, 4, hipMemcpyDeviceToHost, st));
+  ctx->sync();
+  if (herr & 1u) fail(CTA_EINVAL, "X contains NaN");
+  if (herr & 2u) fail(CTA_EINVAL, "a label is outside [0, K)");
+  if (ds->idx_bits == 16) presort<uint16_t>(ctx, ds);
+  else presort<uint32_t>(ctx, ds);
+}
+
+void check_sizes(int32_t N, int32_t F, int32_t K) {
+  if (N <= 0 || F <= 0 || K <= 0) fail(CTA_EINVAL, "N, F and K must be positive (got %d, %d, %d)", N, F, K);
+  if (N > CTA_MAX_SAMPLES) fail(CTA_EUNSUPPORTED, "N = %d exceeds %d samples", N, CTA_MAX_SAMPLES);
+  if (F > CTA_MAX_FEATURES) fail(CTA_EUNSUPPORTED, "F = %d exceeds %d features", F, CTA_MAX_FEATURES);
+  if (K > CTA_MAX_CLASSES) fail(CTA_EUNSUPPORTED, "K = %d exceeds %d classes", K, CTA_MAX_CLASSES);
+}
+
+}  // namespace
+
+// ---- C-ABI -------------------------------------------------------------------------
+extern "C" {
+
+const char *cta_last_error(void) { return g_err.c_str(); }
+
+int cta_version(void) { return 100; }
+
+int cta_ctx_create(int device, cta_ctx **out) {
+  return guarded([&] {
+    if (!out) fail(CTA_EINVAL, "out is NULL");
+    int n = 0;
+    HIPC(hipGetDeviceCount(&n));
+    if (device < 0 || device >= n) fail(CTA_EINVAL, "device %d not present (%d devices)", device, n);
+    std::unique_ptr<cta_ctx> c(new cta_ctx());
+    c->device = device;
+    HIPC(hipSetDevice(device));
+    HIPC(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    hipDeviceProp_t p;
+    HIPC(hipGetDeviceProperties(&p, device));
+    c->num_cu = p.multiProcessorCount > 0 ? p.multiProcessorCount : 256;
+    *out = c.release();
+  });
+}
+
+int cta_ctx_destroy(cta_ctx *ctx) {
+  return guarded([&] {
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    for (auto &p : ctx->pending) {
+      (void)hipEventDestroy(p.a);
+      (void)hipEventDestroy(p.b);
+    }
+    for (auto e : ctx->pool) (void)hipEventDestroy(e);
+    (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+  });
+}
+
+int cta_ctx_synchronize(cta_ctx *ctx) {
+  return guarded([&] {
+    if (!ctx) fail(CTA_EINVAL, "ctx is NULL");
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    HIPC(hipSetDevice(ctx->device));
+    ctx->sync();
+  });
+}
+
+int cta_ctx_mem_info(cta_ctx *ctx, uint64_t *free_bytes, uint64_t *total_bytes) {
+  return guarded([&] {
+    if (!ctx) fail(CTA_EINVAL, "ctx is NULL");
+    HIPC(hipSetDevice(ctx->device));
+    size_t f = 0, t = 0;
+    HIPC(hipMemGetInfo(&f, &t));
+    if (free_bytes) *free_bytes = f;
+    if (total_bytes) *total_bytes = t;
+  });
+}
+
+static int dataset_create_common(cta_ctx *ctx, const float *X, const uint32_t *y, int32_t N, int32_t F, int32_t K,
+                                 int32_t idx_bits, bool device, cta_dataset **out) {
+  return guarded([&] {
+    if (!ctx || !X || !y || !out) fail(CTA_EINVAL, "NULL argument");
+    check_sizes(N, F, K);
+    if (idx_bits == 0) idx_bits = N <= 65535 ? 16 : 32;
+    if (idx_bits != 16 && idx_bits != 32) fail(CTA_EINVAL, "idx_bits must be 0, 16 or 32");
+    if (idx_bits == 16 && N > 65535) fail(CTA_EINVAL, "16-bit indices need N <= 65535");
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    HIPC(hipSetDevice(ctx->device));
+    std::unique_ptr<cta_dataset> ds(new cta_dataset());
+    ds->ctx = ctx;
+    ds->N = N;
+    ds->F = F;
+    ds->K = K;
+    ds->idx_bits = idx_bits;
+    ds->lab_bits = K <= 255 ? 8 : 16;
+    if (device) {
+      dataset_build(ctx, ds.get(), X, y);
+    } else {
+      DevMem dX((size_t)N * F * 4), dy((size_t)N * 4);
+      HIPC(hipMemcpyAsync(dX.p, X, (size_t)N * F * 4, hipMemcpyHostToDevice, ctx->stream));
+      HIPC(hipMemcpyAsync(dy.p, y, (size_t)N * 4, hipMemcpyHostToDevice, ctx->stream));
+      dataset_build(ctx, ds.get(), dX.as<float>(), dy.as<uint32_t>());
+    }
+    *out = ds.release();
+  });
+}
+
+int cta_dataset_create(cta_ctx *ctx, const float *X, const uint32_t *y, int32_t N, int32_t F, int32_t K,
+                       int32_t idx_bits, cta_dataset **out) {
+  return dataset_create_common(ctx, X, y, N, F, K, idx_bits, false, out);
+}
+
+int cta_dataset_create_device(cta_ctx *ctx, const float *X_dev, const uint32_t *y_dev, int32_t N, int32_t F,
+                              int32_t K, int32_t idx_bits, cta_dataset **out) {
+  return dataset_create_common(ctx, X_dev, y_dev, N, F, K, idx_bits, true, out);
+}
+
+int cta_dataset_destroy(cta_dataset *ds) {
+  return guarded([&] {
+    if (!ds) return;
+    std::lock_guard<std::recursive_mutex> lk(ds->ctx->mu);
+    (void)hipSetDevice(ds->ctx->device);
+    (void)hipStreamSynchronize(ds->ctx->stream);
+    delete ds;
+  });
+}
+
+int cta_dataset_info(const cta_dataset *ds, int32_t *N, int32_t *F, int32_t *K, int32_t *idx_bits) {
+  return guarded([&] {
+    if (!ds) fail(CTA_EINVAL, "ds is NULL");
+    if (N) *N = ds->N;
+    if (F) *F = ds->F;
+    if (K) *K = ds->K;
+    if (idx_bits) *idx_bits = ds->idx_bits;
+  });
+}
+
+int cta_dataset_sorted(cta_dataset *ds, uint32_t *out) {
+  return guarded([&] {
+    if (!ds || !out) fail(CTA_EINVAL, "NULL argument");
+    cta_ctx *ctx = ds->ctx;
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    HIPC(hipSetDevice(ctx->device));
+    const size_t total = (size_t)ds->N * ds->F;
+    DevMem w(total * 4);
+    if (ds->idx_bits == 16)
+      k_widen<uint16_t><<<grid_for(total), 256, 0, ctx->stream>>>(ds->si0.as<uint16_t>(), total, w.as<uint32_t>());
+    else
+      k_widen<uint32_t><<<grid_for(total), 256, 0, ctx->stream>>>(ds->si0.as<uint32_t>(), total, w.as<uint32_t>());
+    HIPC(hipMemcpyAsync(out, w.p, total * 4, hipMemcpyDeviceToHost, ctx->stream));
+    ctx->sync();
+  });
+}
+
+void cta_fit_opts_default(cta_fit_opts *o) {
+  if (!o) return;
+  std::memset(o, 0, sizeof(*o));
+  o->seed = 0;
+  o->max_features = 0;
+  o->min_samples_split = 1;
+  o->max_depth = 0;
+  o->identity_perm = 0;
+  o->mode = CTA_MODE_BATCHED;
+  o->bfs_threshold = 0;
+}
+
+int cta_fit(cta_ctx *ctx, cta_dataset *ds, int32_t T, const uint32_t *boot_ids, const int64_t *boot_off,
+            const cta_fit_opts *opts, cta_forest **out) {
+  return guarded([&] {
+    if (!ctx || !ds || !opts || !out) fail(CTA_EINVAL, "NULL argument");
+    if (ds->ctx != ctx) fail(CTA_EINVAL, "dataset belongs to another context");
+    if (T <= 0) fail(CTA_EINVAL, "T must be positive");
+    cta_fit_opts o = *opts;
+    if (o.max_features <= 0 || o.max_features > ds->F)
+      fail(CTA_EINVAL, "max_features must be in [1, %d] (got %d)", ds->F, o.max_features);
+    if (o.mode < CTA_MODE_BATCHED || o.mode > CTA_MODE_PER_NODE) fail(CTA_EINVAL, "unknown mode %d", o.mode);
+    if (o.scan_chunk < 0 || o.resh_chunk < 0) fail(CTA_EINVAL, "chunk sizes must be >= 0");
+    if ((o.scan_chunk && o.scan_chunk < 64) || (o.resh_chunk && o.resh_chunk < 64))
+      fail(CTA_EINVAL, "chunk sizes must be 0 or >= 64");
+    if (boot_ids) {
+      if (!boot_off) fail(CTA_EINVAL, "boot_off is NULL");
+      for (int32_t t = 0; t < T; ++t) {
+        const int64_t a = boot_off[t], b = boot_off[t + 1];
+        if (b <= a || b - a > ds->N) fail(CTA_EINVAL, "tree %d has an empty or oversized in-bag set", t);
+        for (int64_t j = a; j < b; ++j)
+          if (boot_ids[j] >= (uint32_t)ds->N || (j > a && boot_ids[j] <= boot_ids[j - 1]))
+            fail(CTA_EINVAL, "in-bag ids of tree %d are not sorted unique ids below N", t);
+      }
+    }
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    HIPC(hipSetDevice(ctx->device));
+    std::unique_ptr<cta_forest> f(new cta_forest());
+    f->ctx = ctx;
+    f->F = ds->F;
+    f->K = ds->K;
+    const bool b16 = ds->idx_bits == 16, l8 = ds->lab_bits == 8;
+    if (b16 && l8) fit_impl<uint16_t, uint8_t>(ctx, ds, T, boot_ids, boot_off, o, f.get());
+    else if (b16) fit_impl<uint16_t, uint16_t>(ctx, ds, T, boot_ids, boot_off, o, f.get());
+    else if (l8) fit_impl<uint32_t, uint8_t>(ctx, ds, T, boot_ids, boot_off, o, f.get());
+    else fit_impl<uint32_t, uint16_t>(ctx, ds, T, boot_ids, boot_off, o, f.get());
+    *out = f.release();
+  });
+}
+
+int cta_forest_destroy(cta_forest *f) {
+  return guarded([&] {
+    if (!f) return;
+    std::lock_guard<std::recursive_mutex> lk(f->ctx->mu);
+    (void)hipSetDevice(f->ctx->device);
+    (void)hipStreamSynchronize(f->ctx->stream);
+    delete f;
+  });
+}
+
+int cta_forest_info(const cta_forest *f, int32_t *n_trees, int64_t *n_nodes, int32_t *F, int32_t *K) {
+  return guarded([&] {
+    if (!f) fail(CTA_EINVAL, "forest is NULL");
+    if (n_trees) *n_trees = f->T;
+    if (n_nodes) *n_nodes = f->tree_off.back();
+    if (F) *F = f->F;
+    if (K) *K = f->K;
+  });
+}
+
+int cta_forest_export(const cta_forest *f, int64_t *tree_off, int32_t *feature, float *threshold, uint32_t *left,
+                      uint32_t *right, uint32_t *value, uint32_t *n_samples, uint32_t *depth) {
+  return guarded([&] {
+    if (!f || !tree_off) fail(CTA_EINVAL, "NULL argument");
+    std::memcpy(tree_off, f->tree_off.data(), f->tree_off.size() * 8);
+    const size_t n = f->feature.size();
+    if (feature) std::memcpy(feature, f->feature.data(), n * 4);
+    if (threshold) std::memcpy(threshold, f->threshold.data(), n * 4);
+    if (left) std::memcpy(left, f->left.data(), n * 4);
+    if (right) std::memcpy(right, f->right.data(), n * 4);
+    if (value) std::memcpy(value, f->value.data(), n * 4);
+    if (n_samples) {
+      if (f->nsamp.size() == n) std::memcpy(n_samples, f->nsamp.data(), n * 4);
+      else std::memset(n_samples, 0, n * 4);
+    }
+    if (depth) {
+      if (f->depth.size() == n) std::memcpy(depth, f->depth.data(), n * 4);
+      else std::memset(depth, 0, n * 4);
+    }
+  });
+}
+
+int cta_forest_import(cta_ctx *ctx, int32_t T, int32_t F, int32_t K, const int64_t *tree_off, const int32_t *feature,
+                      const float *threshold, const uint32_t *left, const uint32_t *right, const uint32_t *value,
+                      cta_forest **out) {
+  return guarded([&] {
+    if (!ctx || !tree_off || !feature || !threshold || !left || !right || !value || !out)
+      fail(CTA_EINVAL, "NULL argument");
+    if (T <= 0 || F <= 0 || K <= 0 || F > CTA_MAX_FEATURES || K > CTA_MAX_CLASSES) fail(CTA_EINVAL, "bad sizes");
+    if (tree_off[0] != 0) fail(CTA_EINVAL, "tree_off[0] must be 0");
+    for (int32_t t = 0; t < T; ++t) {
+      const int64_t a = tree_off[t], b = tree_off[t + 1];
+      if (b <= a) fail(CTA_EINVAL, "tree %d is empty", t);
+      const int64_t n = b - a;
+      for (int64_t j = a; j < b; ++j) {
+        const bool leaf = left[j] == 0 || right[j] == 0;
+        if (leaf) {
+          if (value[j] >= (uint32_t)K) fail(CTA_EINVAL, "leaf label %u out of range", value[j]);
+        } else {
+          if (feature[j] < 0 || feature[j] >= F) fail(CTA_EINVAL, "feature %d out of range", feature[j]);
+          // pre-order: left child next, right child later, both inside the tree
+          if ((int64_t)left[j] != j - a + 1 || (int64_t)right[j] <= j - a + 1 || (int64_t)right[j] >= n)
+            fail(CTA_EINVAL, "tree %d node %lld is not in pre-order layout", t, (long long)(j - a));
+        }
+      }
+    }
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    HIPC(hipSetDevice(ctx->device));
+    std::unique_ptr<cta_forest> f(new cta_forest());
+    f->ctx = ctx;
+    f->T = T;
+    f->F = F;
+    f->K = K;
+    const int64_t n = tree_off[T];
+    f->tree_off.assign(tree_off, tree_off + T + 1);
+    f->feature.assign(feature, feature + n);
+    f->threshold.assign(threshold, threshold + n);
+    f->left.assign(left, left + n);
+    f->right.assign(right, right + n);
+    f->value.assign(value, value + n);
+    for (int64_t j = 0; j < n; ++j)
+      if (f->left[j] == 0 || f->right[j] == 0) f->left[j] = f->right[j] = 0;
+    upload_forest(f.get());
+    *out = f.release();
+  });
+}
+
+static void predict_dev(cta_ctx *ctx, cta_forest *f, const float *X, int64_t R, uint32_t *labels, uint32_t *votes) {
+  upload_forest(f);
+  hipStream_t st = ctx->stream;
+  const size_t vk = (size_t)R * f->K;
+  DevMem tmp;
+  uint32_t *v = votes;
+  if (!v) {
+    tmp.alloc(vk * 4);
+    v = tmp.as<uint32_t>();
+  }
+  k_fill_u32<<<grid_for(vk), 256, 0, st>>>(v, vk, 0u);
+  {
+    Timed tm(ctx, "k_predict");
+    k_predict<<<grid_for((size_t)R * f->T, 256, 65535u * 4), 256, 0, st>>>(
+        f->nodes.as<PNode>(), f->roots.as<uint32_t>(), (uint32_t)f->T, X, (uint64_t)R, (uint32_t)f->F,
+        (uint32_t)f->K, v);
+  }
+  k_vote<<<grid_for(R), 256, 0, st>>>(v, (uint64_t)R, (uint32_t)f->K, labels);
+  HIPC(hipGetLastError());
+}
+
+int cta_predict(cta_ctx *ctx, cta_forest *f, const float *X, int64_t R, int32_t F, uint32_t *labels,
+                uint32_t *votes) {
+  return guarded([&] {
+    if (!ctx || !f || !X || !labels) fail(CTA_EINVAL, "NULL argument");
+    if (F != f->F) fail(CTA_EINVAL, "X has %d features, the forest %d", F, f->F);
+    if (R < 0) fail(CTA_EINVAL, "R must be >= 0");
+    if (R == 0) return;
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    HIPC(hipSetDevice(ctx->device));
+    DevMem dX((size_t)R * F * 4), dl((size_t)R * 4), dv;
+    if (votes) dv.alloc((size_t)R * f->K * 4);
+    HIPC(hipMemcpyAsync(dX.p, X, (size_t)R * F * 4, hipMemcpyHostToDevice, ctx->stream));
+    predict_dev(ctx, f, dX.as<float>(), R, dl.as<uint32_t>(), votes ? dv.as<uint32_t>() : nullptr);
+    HIPC(hipMemcpyAsync(labels, dl.p, (size_t)R * 4, hipMemcpyDeviceToHost, ctx->stream));
+    if (votes) HIPC(hipMemcpyAsync(votes, dv.p, (size_t)R * f->K * 4, hipMemcpyDeviceToHost, ctx->stream));
+    ctx->sync();
+  });
+}
+
+int cta_predict_device(cta_ctx *ctx, cta_forest *f, const float *X_dev, int64_t R, int32_t F, uint32_t *labels_dev,
+                       uint32_t *votes_dev) {
+  return guarded([&] {
+    if (!ctx || !f || !X_dev || !labels_dev) fail(CTA_EINVAL, "NULL argument");
+    if (F != f->F) fail(CTA_EINVAL, "X has %d features, the forest %d", F, f->F);
+    if (R < 0) fail(CTA_EINVAL, "R must be >= 0");
+    if (R == 0) return;
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    HIPC(hipSetDevice(ctx->device));
+    predict_dev(ctx, f, X_dev, R, labels_dev, votes_dev);
+    ctx->sync();
+  });
+}
+
+int cta_ctx_set_profiling(cta_ctx *ctx, int enable) {
+  return guarded([&] {
+    if (!ctx) fail(CTA_EINVAL, "ctx is NULL");
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    ctx->sync();
+    ctx->profiling = enable != 0;
+  });
+}
+
+int cta_ctx_reset_stats(cta_ctx *ctx) {
+  return guarded([&] {
+    if (!ctx) fail(CTA_EINVAL, "ctx is NULL");
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    ctx->sync();
+    ctx->stats.clear();
+  });
+}
+
+int cta_ctx_kernel_stats(cta_ctx *ctx, cta_kernel_stat *out, int32_t max, int32_t *n) {
+  return guarded([&] {
+    if (!ctx || !n) fail(CTA_EINVAL, "NULL argument");
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    ctx->sync();
+    int32_t i = 0;
+    for (auto &kv : ctx->stats) {
+      if (out && i < max) {
+        cta_kernel_stat &s = out[i];
+        std::memset(&s, 0, sizeof(s));
+        std::strncpy(s.name, kv.first.c_str(), sizeof(s.name) - 1);
+        s.launches = kv.second.launches;
+        s.total_ms = kv.second.ms;
+        s.alg_bytes = kv.second.bytes;
+      }
+      ++i;
+    }
+    *n = i;
+  });
+}
+
+}  // extern "C"
+
+// ---- per-kernel parity hooks --------------------------------------------------------
+template <class IdxT, class LabT>
+static void hook_split_scan(cta_ctx *ctx, cta_dataset *ds, const uint32_t *rows, uint32_t n, const int32_t *features,
+                            int32_t m_eval, int32_t chunk, int32_t *bf, uint32_t *bp, uint64_t *num, uint64_t *den) {
+  const int32_t N = ds->N, F = ds->F, K = ds->K;
+  hipStream_t st = ctx->stream;
+  const uint32_t Cs = chunk > 0 ? (uint32_t)chunk : 8192u;
+  // node rows into tree 0's buffer 0
+  std::vector<IdxT> hrows((size_t)F * N, 0);
+  std::vector<uint32_t> counts(K, 0);
+  std::vector<LabT> hy(N);
+  HIPC(hipMemcpyAsync(hy.data(), ds->y.p, (size_t)N * sizeof(LabT), hipMemcpyDeviceToHost, st));
+  ctx->sync();
+  for (int32_t f = 0; f < F; ++f)
+    for (uint32_t i = 0; i < n; ++i) hrows[(size_t)f * N + i] = (IdxT)rows[(size_t)f * n + i];
+  for (uint32_t i = 0; i < n; ++i) counts[hy[rows[i]]]++;
+  const uint32_t nch = (n + Cs - 1) / Cs;
+  const uint32_t fpt = nch > 1 ? 1u : std::max(1u, Cs / n);
+  const uint32_t nf = (uint32_t)m_eval;
+  const uint32_t ntask = nf == 0 ? 0u : (nch > 1 ? nf * nch : (nf + fpt - 1) / fpt);
+  std::vector<Task> tasks(std::max(ntask, 1u));
+  for (uint32_t t = 0; t < ntask; ++t) {
+    if (nch > 1) tasks[t] = Task{0u, (t / nch) | (1u << 16), t % nch, nch};
+    else {
+      const uint32_t first = t * fpt;
+      tasks[t] = Task{0u, first | (std::min(fpt, nf - first) << 16), 0u, nch};
+    }
+  }
+  NAux ax = {};
+  ax.scan_base = 0;
+  ax.scan_ntask = ntask;
+  ax.nch_s = nch;
+  ax.fpt = fpt;
+  ax.hist_base = 0;
+  ax.best_f = -1;
+  FNode nd = {0u, 0u, n, 0u, 0u, 1u, 0u, 0u};
+  Ctl hc = {};
+  hc.q = 1;
+  hc.n_scan = ntask;
+  hc.n_nodes = 1;
+  DevMem d_rows(hrows.size() * sizeof(IdxT)), d_front(sizeof(FNode)), d_cnt(K * 4), d_aux(sizeof(NAux)),
+      d_feats(std::max(nf, 1u) * 4), d_tasks(tasks.size() * sizeof(Task)),
+      d_res(std::max(ntask, 1u) * sizeof(ScanRes)), d_hist((size_t)std::max(nf, 1u) * nch * K * 4), d_ctl(sizeof(Ctl)),
+      d_cntL(K * 4), d_ar(7 * 4 * 4), d_fill(((n + 8191) / 8192 + 1) * sizeof(Task));
+  HIPC(hipMemcpyAsync(d_rows.p, hrows.data(), hrows.size() * sizeof(IdxT), hipMemcpyHostToDevice, st));
+  HIPC(hipMemcpyAsync(d_front.p, &nd, sizeof nd, hipMemcpyHostToDevice, st));
+  HIPC(hipMemcpyAsync(d_cnt.p, counts.data(), K * 4, hipMemcpyHostToDevice, st));
+  HIPC(hipMemcpyAsync(d_aux.p, &ax, sizeof ax, hipMemcpyHostToDevice, st));
+  if (nf) HIPC(hipMemcpyAsync(d_feats.p, features, nf * 4, hipMemcpyHostToDevice, st));
+  HIPC(hipMemcpyAsync(d_tasks.p, tasks.data(), tasks.size() * sizeof(Task), hipMemcpyHostToDevice, st));
+  HIPC(hipMemcpyAsync(d_ctl.p, &hc, sizeof hc, hipMemcpyHostToDevice, st));
+  FitArgs<IdxT, LabT> a = {};
+  a.X_T = ds->XT.as<float>();
+  a.y = ds->y.as<LabT>();
+  a.si0 = ds->si0.as<IdxT>();
+  a.buf0 = d_rows.as<IdxT>();
+  a.buf1 = d_rows.as<IdxT>();
+  a.N = N;
+  a.F = F;
+  a.K = K;
+  a.m = (int32_t)std::max(nf, 1u);
+  a.Cs = Cs;
+  a.Cf = 8192;
+  a.Cr = 8192;
+  a.front = d_front.as<FNode>();
+  a.cnt = d_cnt.as<uint32_t>();
+  a.aux = d_aux.as<NAux>();
+  a.feats = d_feats.as<int32_t>();
+  a.scan_tasks = d_tasks.as<Task>();
+  a.res = d_res.as<ScanRes>();
+  a.hist = d_hist.as<uint32_t>();
+  a.cntL = d_cntL.as<uint32_t>();
+  a.fill_tasks = d_fill.as<Task>();
+  uint32_t *ar = d_ar.as<uint32_t>();
+  a.arena = Arena{(int32_t *)ar, (float *)(ar + 4), ar + 8, ar + 12, ar + 16, ar + 20, ar + 24};
+  a.ctl = d_ctl.as<Ctl>();
+  a.cap_q = 1;
+  a.cap_scan = std::max(ntask, 1u);
+  a.cap_hist = std::max(nf, 1u) * nch * K;
+  a.cap_fill = (n + 8191) / 8192 + 1;
+  a.cap_nodes = 3;
+  const size_t sm = (size_t)kWavesPerBlock * 2 * K * 4;
+  const uint32_t G = 64;
+  k_scan_hist<IdxT, LabT><<<G, kBlock, sm, st>>>(a);
+  k_scan_carry<IdxT, LabT><<<G, kBlock, 0, st>>>(a);
+  k_scan<IdxT, LabT><<<G, kBlock, sm, st>>>(a);
+  k_decide<IdxT, LabT><<<G, kBlock, 0, st>>>(a);
+  HIPC(hipGetLastError());
+  NAux out = {};
+  std::vector<ScanRes> res(std::max(ntask, 1u));
+  HIPC(hipMemcpyAsync(&out, d_aux.p, sizeof out, hipMemcpyDeviceToHost, st));
+  HIPC(hipMemcpyAsync(res.data(), d_res.p, res.size() * sizeof(ScanRes), hipMemcpyDeviceToHost, st));
+  ctx->sync();
+  *bf = out.best_f;
+  *bp = out.best_p;
+  *num = 0;
+  *den = 0;
+  for (uint32_t t = 0; t < ntask; ++t)
+    if (res[t].ok && res[t].feat == out.best_f && res[t].pos == out.best_p) {
+      *num = res[t].num;
+      *den = res[t].den;
+    }
+}
+
+extern "C" int cta_k_split_scan(cta_ctx *ctx, cta_dataset *ds, const uint32_t *rows, uint32_t n, const int32_t *features,
+                     int32_t m_eval, int32_t chunk, int32_t *best_feature, uint32_t *best_pos, uint64_t *num,
+                     uint64_t *den) {
+  return guarded([&] {
+    if (!ctx || !ds || !rows || !best_feature || !best_pos || !num || !den) fail(CTA_EINVAL, "NULL argument");
+    if (n < 1 || n > (uint32_t)ds->N) fail(CTA_EINVAL, "n out of range");
+    if (m_eval < 0 || m_eval > ds->F || (m_eval > 0 && !features)) fail(CTA_EINVAL, "bad feature list");
+    if (chunk != 0 && chunk < 64) fail(CTA_EINVAL, "chunk must be 0 or >= 64");
+    for (int32_t j = 0; j < m_eval; ++j)
+      if (features[j] < 0 || features[j] >= ds->F) fail(CTA_EINVAL, "feature out of range");
+    for (size_t i = 0; i < (size_t)ds->F * n; ++i)
+      if (rows[i] >= (uint32_t)ds->N) fail(CTA_EINVAL, "row index out of range");
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    HIPC(hipSetDevice(ctx->device));
+    const bool b16 = ds->idx_bits == 16, l8 = ds->lab_bits == 8;
+    if (b16 && l8) hook_split_scan<uint16_t, uint8_t>(ctx, ds, rows, n, features, m_eval, chunk, best_feature, best_pos, num, den);
+    else if (b16) hook_split_scan<uint16_t, uint16_t>(ctx, ds, rows, n, features, m_eval, chunk, best_feature, best_pos, num, den);
+    else if (l8) hook_split_scan<uint32_t, uint8_t>(ctx, ds, rows, n, features, m_eval, chunk, best_feature, best_pos, num, den);
+    else hook_split_scan<uint32_t, uint16_t>(ctx, ds, rows, n, features, m_eval, chunk, best_feature, best_pos, num, den);
+  });
+}
+
+template <class IdxT, class LabT>
+static void hook_reshuffle(cta_ctx *ctx, cta_dataset *ds, const uint32_t *rows, uint32_t n, const uint8_t *mark,
+                           int32_t chunk, uint32_t *out) {
+  const int32_t N = ds->N, F = ds->F, K = ds->K;
+  hipStream_t st = ctx->stream;
+  const uint32_t Cr = chunk > 0 ? (uint32_t)chunk : 8192u;
+  std::vector<IdxT> hrows((size_t)F * N, 0);
+  for (int32_t f = 0; f < F; ++f)
+    for (uint32_t i = 0; i < n; ++i) hrows[(size_t)f * N + i] = (IdxT)rows[(size_t)f * n + i];
+  uint32_t nL = 0;
+  for (uint32_t i = 0; i < n; ++i) nL += mark[rows[i]] ? 1 : 0;
+  const uint32_t nch = (n + Cr - 1) / Cr;
+  const uint32_t rpt = nch > 1 ? 1u : std::max(1u, Cr / n);
+  const uint32_t ntask = nch > 1 ? (uint32_t)F * nch : ((uint32_t)F + rpt - 1) / rpt;
+  std::vector<Task> tasks(ntask);
+  for (uint32_t t = 0; t < ntask; ++t) {
+    if (nch > 1) tasks[t] = Task{0u, (t / nch) | (1u << 16), t % nch, nch};
+    else {
+      const uint32_t first = t * rpt;
+      tasks[t] = Task{0u, first | (std::min(rpt, (uint32_t)F - first) << 16), 0u, nch};
+    }
+  }
+  NAux ax = {};
+  ax.nL = nL;
+  ax.flags = kSplit | kLiveL | kLiveR;
+  ax.dst = 2;
+  ax.resh_base = 0;
+  ax.resh_ntask = ntask;
+  ax.rcnt_base = 0;
+  ax.nch_r = nch;
+  ax.rpt = rpt;
+  FNode nd = {0u, 0u, n, 0u, 0u, 1u, 0u, 0u};
+  Ctl hc = {};
+  hc.q = 1;
+  hc.n_resh = ntask;
+  DevMem d_in(hrows.size() * sizeof(IdxT)), d_out(hrows.size() * sizeof(IdxT)), d_mark(N), d_front(sizeof(FNode)),
+      d_aux(sizeof(NAux)), d_tasks(tasks.size() * sizeof(Task)), d_rcnt((size_t)F * nch * 4 + 4), d_ctl(sizeof(Ctl));
+  HIPC(hipMemcpyAsync(d_in.p, hrows.data(), hrows.size() * sizeof(IdxT), hipMemcpyHostToDevice, st));
+  HIPC(hipMemcpyAsync(d_mark.p, mark, N, hipMemcpyHostToDevice, st));
+  HIPC(hipMemcpyAsync(d_front.p, &nd, sizeof nd, hipMemcpyHostToDevice, st));
+  HIPC(hipMemcpyAsync(d_aux.p, &ax, sizeof ax, hipMemcpyHostToDevice, st));
+  HIPC(hipMemcpyAsync(d_tasks.p, tasks.data(), tasks.size() * sizeof(Task), hipMemcpyHostToDevice, st));
+  HIPC(hipMemcpyAsync(d_ctl.p, &hc, sizeof hc, hipMemcpyHostToDevice, st));
+  FitArgs<IdxT, LabT> a = {};
+  a.X_T = ds->XT.as<float>();
+  a.y = ds->y.as<LabT>();
+  a.si0 = ds->si0.as<IdxT>();
+  a.buf0 = d_in.as<IdxT>();
+  a.buf1 = d_out.as<IdxT>();
+  a.mark = d_mark.as<uint8_t>();
+  a.N = N;
+  a.F = F;
+  a.K = K;
+  a.Cr = Cr;
+  a.front = d_front.as<FNode>();
+  a.aux = d_aux.as<NAux>();
+  a.resh_tasks = d_tasks.as<Task>();
+  a.rcnt = d_rcnt.as<uint32_t>();
+  a.ctl = d_ctl.as<Ctl>();
+  const uint32_t G = 64;
+  k_resh_count<IdxT, LabT><<<G, kBlock, 0, st>>>(a);
+  k_resh_carry<IdxT, LabT><<<G, kBlock, 0, st>>>(a);
+  k_resh<IdxT, LabT><<<G, kBlock, 0, st>>>(a);
+  HIPC(hipGetLastError());
+  std::vector<IdxT> hout(hrows.size());
+  HIPC(hipMemcpyAsync(hout.data(), d_out.p, hout.size() * sizeof(IdxT), hipMemcpyDeviceToHost, st));
+  ctx->sync();
+  for (int32_t f = 0; f < F; ++f)
+    for (uint32_t i = 0; i < n; ++i) out[(size_t)f * n + i] = hout[(size_t)f * N + i];
+}
+
+extern "C" int cta_k_reshuffle(cta_ctx *ctx, cta_dataset *ds, const uint32_t *rows, uint32_t n, const uint8_t *mark,
+                    int32_t chunk, uint32_t *out) {
+  return guarded([&] {
+    if (!ctx || !ds || !rows || !mark || !out) fail(CTA_EINVAL, "NULL argument");
+    if (n < 1 || n > (uint32_t)ds->N) fail(CTA_EINVAL, "n out of range");
+    if (chunk != 0 && chunk < 64) fail(CTA_EINVAL, "chunk must be 0 or >= 64");
+    for (size_t i = 0; i < (size_t)ds->F * n; ++i)
+      if (rows[i] >= (uint32_t)ds->N) fail(CTA_EINVAL, "row index out of range");
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    HIPC(hipSetDevice(ctx->device));
+    const bool b16 = ds->idx_bits == 16, l8 = ds->lab_bits == 8;
+    if (b16 && l8) hook_reshuffle<uint16_t, uint8_t>(ctx, ds, rows, n, mark, chunk, out);
+    else if (b16) hook_reshuffle<uint16_t, uint16_t>(ctx, ds, rows, n, mark, chunk, out);
+    else if (l8) hook_reshuffle<uint32_t, uint8_t>(ctx, ds, rows, n, mark, chunk, out);
+    else hook_reshuffle<uint32_t, uint16_t>(ctx, ds, rows, n, mark, chunk, out);
+  });
+}

@@ -1,0 +1,224 @@
+// aux_kernels.hip -- training-set preparation and prediction on gfx950.
+//
+//   k_transpose        X (N x F, row-major) -> X_T (F x N) through LDS tiles
+//                      (random_forest.py:233)
+//   k_sort_*           stable LSD radix sort of every feature row, 4-bit
+//                      digits, 8 passes; ties keep ascending sample id
+//                      (replaces the host np.argsort of random_forest.py:247)
+//   k_predict / k_vote batched traversal of a flattened pre-order forest,
+//                      one thread per (row, tree), votes by integer atomics
+//                      (predict.cu:12-69, random_forest.py:339-377)
+#include "device_util.h"
+#include "internal.h"
+
+namespace cta {
+
+// ---- transpose ------------------------------------------------------------
+constexpr int kTT = 64;
+
+__global__ __launch_bounds__(256) void k_transpose(const float *__restrict__ X, uint32_t N, uint32_t F,
+                                                   float *__restrict__ XT, uint32_t *err) {
+  __shared__ float tile[kTT][kTT + 1];
+  const uint32_t r0 = blockIdx.x * kTT, f0 = blockIdx.y * kTT;
+  const uint32_t tx = threadIdx.x % kTT, ty = threadIdx.x / kTT;  // 64 x 4
+  bool nan = false;
+  for (uint32_t j = ty; j < kTT; j += 4) {
+    const uint32_t r = r0 + j, f = f0 + tx;
+    float v = 0.f;
+    if (r < N && f < F) {
+      v = X[(size_t)r * F + f];
+      nan |= v != v;
+    }
+    tile[j][tx] = v;
+  }
+  __syncthreads();
+  for (uint32_t j = ty; j < kTT; j += 4) {
+    const uint32_t f = f0 + j, r = r0 + tx;
+    if (r < N && f < F) XT[(size_t)f * N + r] = tile[tx][j];
+  }
+  if (nan) atomicOr(err, 1u);
+}
+
+// ---- stable radix presort -------------------------------------------------
+constexpr int kSortThreads = 256;
+constexpr int kSortItems = 8;
+constexpr int kSortTile = kSortThreads * kSortItems;  // elements per block and pass
+constexpr int kDigits = 16;
+
+// Order-preserving key of a float: -0.0 and +0.0 map to the same key so that
+// they tie (and keep id order) as in a comparison sort.
+__device__ __forceinline__ uint32_t float_key(float v) {
+  uint32_t u = __float_as_uint(v);
+  if ((u & 0x7FFFFFFFu) == 0) u = 0;
+  return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+
+__global__ void k_sort_init(const float *XT, uint32_t N, uint32_t F, uint32_t *keys, uint32_t *vals) {
+  const size_t total = (size_t)N * F;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (size_t)gridDim.x * blockDim.x) {
+    keys[i] = float_key(XT[i]);
+    vals[i] = (uint32_t)(i % N);
+  }
+}
+
+// hist[(f * 16 + d) * ntiles + tile] = count of digit d in the tile.
+__global__ __launch_bounds__(kSortThreads) void k_sort_hist(const uint32_t *keys, uint32_t N, uint32_t ntiles,
+                                                            uint32_t shift, uint32_t *hist) {
+  __shared__ uint32_t h[kDigits];
+  const uint32_t tile = blockIdx.x, f = blockIdx.y;
+  if (threadIdx.x < kDigits) h[threadIdx.x] = 0;
+  __syncthreads();
+  const uint32_t *k = keys + (size_t)f * N;
+  const uint32_t base = tile * kSortTile;
+  for (uint32_t j = threadIdx.x; j < (uint32_t)kSortTile; j += kSortThreads) {
+    const uint32_t e = base + j;
+    if (e < N) atomicAdd(&h[(k[e] >> shift) & (kDigits - 1)], 1u);
+  }
+  __syncthreads();
+  if (threadIdx.x < kDigits) hist[((size_t)f * kDigits + threadIdx.x) * ntiles + tile] = h[threadIdx.x];
+}
+
+// Exclusive scan of each row's 16 x ntiles counts (digit-major): the output
+// offset of digit d's elements of each tile.
+__global__ __launch_bounds__(kSortThreads) void k_sort_scan(uint32_t *hist, uint32_t ntiles) {
+  __shared__ uint32_t part[kSortThreads];
+  uint32_t *h = hist + (size_t)blockIdx.x * kDigits * ntiles;
+  const uint32_t len = kDigits * ntiles;
+  const uint32_t per = (len + kSortThreads - 1) / kSortThreads;
+  const uint32_t a = threadIdx.x * per, b = min(len, a + per);
+  uint32_t s = 0;
+  for (uint32_t i = a; i < b; ++i) s += h[i];
+  part[threadIdx.x] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t run = 0;
+    for (int t = 0; t < kSortThreads; ++t) {
+      uint32_t v = part[t];
+      part[t] = run;
+      run += v;
+    }
+  }
+  __syncthreads();
+  uint32_t run = part[threadIdx.x];
+  for (uint32_t i = a; i < b; ++i) {
+    uint32_t v = h[i];
+    h[i] = run;
+    run += v;
+  }
+}
+
+// Stable scatter of one tile: rounds of 256 consecutive elements, ranks by
+// one ballot per digit inside each wave, wave and round offsets from LDS.
+__global__ __launch_bounds__(kSortThreads) void k_sort_scatter(const uint32_t *kin, const uint32_t *vin,
+                                                               uint32_t *kout, uint32_t *vout, uint32_t N,
+                                                               uint32_t ntiles, uint32_t shift,
+                                                               const uint32_t *hist) {
+  __shared__ uint32_t wcnt[kSortThreads / 64][kDigits];
+  __shared__ uint32_t run[kDigits];
+  __shared__ uint32_t off[kDigits];
+  const uint32_t tile = blockIdx.x, f = blockIdx.y;
+  const uint32_t lane = lane_id(), w = threadIdx.x / 64;
+  if (threadIdx.x < kDigits) {
+    run[threadIdx.x] = 0;
+    off[threadIdx.x] = hist[((size_t)f * kDigits + threadIdx.x) * ntiles + tile];
+  }
+  __syncthreads();
+  const size_t rb = (size_t)f * N;
+  for (uint32_t r = 0; r < (uint32_t)kSortItems; ++r) {
+    const uint32_t e = tile * kSortTile + r * kSortThreads + threadIdx.x;
+    const bool valid = e < N;
+    uint32_t key = 0, val = 0, d = kDigits;
+    if (valid) {
+      key = kin[rb + e];
+      val = vin[rb + e];
+      d = (key >> shift) & (kDigits - 1);
+    }
+    uint32_t rank = 0;
+#pragma unroll
+    for (uint32_t dd = 0; dd < (uint32_t)kDigits; ++dd) {
+      const uint64_t m = ballot(d == dd);
+      if (d == dd) rank = mbcnt(m);
+      if (lane == 0) wcnt[w][dd] = (uint32_t)__popcll(m);
+    }
+    __syncthreads();
+    if (valid) {
+      uint32_t pos = off[d] + run[d] + rank;
+      for (uint32_t ww = 0; ww < w; ++ww) pos += wcnt[ww][d];
+      kout[rb + pos] = key;
+      vout[rb + pos] = val;
+    }
+    __syncthreads();
+    if (threadIdx.x < kDigits) {
+      uint32_t s = 0;
+      for (int ww = 0; ww < kSortThreads / 64; ++ww) s += wcnt[ww][threadIdx.x];
+      run[threadIdx.x] += s;
+    }
+    __syncthreads();
+  }
+}
+
+template <class IdxT>
+__global__ void k_sort_finish(const uint32_t *vals, size_t total, IdxT *out) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (size_t)gridDim.x * blockDim.x)
+    out[i] = (IdxT)vals[i];
+}
+
+template <class IdxT>
+__global__ void k_widen(const IdxT *in, size_t total, uint32_t *out) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (size_t)gridDim.x * blockDim.x)
+    out[i] = (uint32_t)in[i];
+}
+
+template <class LabT>
+__global__ void k_labels(const uint32_t *y, uint32_t N, uint32_t K, LabT *out, uint32_t *err) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x) {
+    const uint32_t v = y[i];
+    if (v >= K) atomicOr(err, 2u);
+    out[i] = (LabT)(v < K ? v : 0);
+  }
+}
+
+// ---- predict ----------------------------------------------------------------
+// Flattened forest node: pre-order, so a split node's left child is the next
+// node; leaves have feat = -1 and carry their label in `value`.
+struct PNode {
+  float thr;
+  int32_t feat;
+  uint32_t right;
+  uint32_t value;
+};
+
+__global__ __launch_bounds__(256) void k_predict(const PNode *__restrict__ nodes, const uint32_t *__restrict__ roots,
+                                                 uint32_t T, const float *__restrict__ X, uint64_t R, uint32_t F,
+                                                 uint32_t K, uint32_t *__restrict__ votes) {
+  const uint64_t total = R * T;
+  for (uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; g < total; g += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t r = g % R;
+    const uint32_t t = (uint32_t)(g / R);
+    const float *x = X + r * F;
+    uint32_t node = roots[t];
+    PNode nd = nodes[node];
+    while (nd.feat >= 0) {
+      node = (x[nd.feat] < nd.thr) ? node + 1 : nd.right;
+      nd = nodes[node];
+    }
+    atomicAdd(&votes[r * K + nd.value], 1u);
+  }
+}
+
+// Winning label per row: most votes, lowest label on ties (np.argmax of
+// np.bincount, random_forest.py:358).
+__global__ void k_vote(const uint32_t *votes, uint64_t R, uint32_t K, uint32_t *labels) {
+  for (uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; r < R; r += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t *v = votes + r * K;
+    uint32_t best = 0, bv = v[0];
+    for (uint32_t k = 1; k < K; ++k)
+      if (v[k] > bv) {
+        bv = v[k];
+        best = k;
+      }
+    labels[r] = best;
+  }
+}
+
+}  // namespace cta

@@ -1,0 +1,821 @@
+// fit_kernels.hip -- level-synchronous random-forest growth on gfx950.
+//
+// One level of every resident tree is processed by a fixed sequence of
+// launches.  Every kernel is wave-granular: a 64-lane wave takes one frontier
+// node or one task (a chunk of a node's segment in one or more sorted rows)
+// at a time from a grid-stride loop whose bound is read from device memory,
+// so the host never has to size grids from device-side counts.
+//
+//   k_select      per node: feature visiting order, constant-feature skip,
+//                 split-scan task list           (bfs_module.cu:122-139)
+//   k_scan_hist   per multi-chunk task: class histogram of the chunk
+//                                                (dfs_module.cu:24-80)
+//   k_scan_carry  per multi-chunk feature: exclusive prefix over chunks
+//                                                (dfs_module.cu:82-106)
+//   k_scan        per task: Gini scan of the chunk, best position
+//                                                (dfs_module.cu:310-443, bfs_module.cu:65-203)
+//   k_decide      per node: best split over tasks, threshold, child ids
+//                                                (dfs_module.cu:446-491, 158-208, bfs_module.cu:206-235, 379-401)
+//   k_fill        per chunk of the split row: mark table, left class counts
+//                                                (dfs_module.cu:210-221, bfs_module.cu:238-264)
+//   k_children    per split node: leaf tests, next frontier, reshuffle tasks
+//                                                (random_tree.py:71-165)
+//   k_resh_count / k_resh_carry / k_resh
+//                 stable partition of every row (dfs_module.cu:224-307, bfs_module.cu:266-377)
+//
+// The split scan parallelises the reference's serial recurrence
+// S_L(p) = S_L(p-1) + 2 c_{y_p}(p-1) + 1 (dfs_module.cu:396-397): each lane's
+// c_{y_p}(p-1) is the chunk carry of its label plus its rank among earlier
+// lanes with the same label (one ballot per distinct label in the wave), and
+// S_L and D = sum_k tot_k c_k are then wave prefix sums.
+#include "device_util.h"
+#include "internal.h"
+
+namespace cta {
+
+#define WAVE_LOOP(count_expr)                                                        \
+  const uint32_t _wid = blockIdx.x * kWavesPerBlock + threadIdx.x / kWave;           \
+  const uint32_t _nw = gridDim.x * kWavesPerBlock;                                   \
+  const uint32_t _count = (count_expr);                                              \
+  for (uint32_t item = _wid; item < _count; item += _nw)
+
+__device__ __forceinline__ void wave_sync() { __builtin_amdgcn_wave_barrier(); }
+
+template <class IdxT, class LabT>
+__device__ __forceinline__ const IdxT *rows_of(const FitArgs<IdxT, LabT> &a, uint32_t src, uint32_t tree,
+                                               uint32_t f) {
+  if (src == 0) return a.si0 + (size_t)f * a.N;
+  const IdxT *b = (src == 1) ? a.buf0 : a.buf1;
+  return b + ((size_t)tree * a.F + f) * a.N;
+}
+
+template <class IdxT, class LabT>
+__device__ __forceinline__ IdxT *rows_out(const FitArgs<IdxT, LabT> &a, uint32_t dst, uint32_t tree,
+                                          uint32_t f) {
+  IdxT *b = (dst == 1) ? a.buf0 : a.buf1;
+  return b + ((size_t)tree * a.F + f) * a.N;
+}
+
+__device__ __forceinline__ uint32_t ceil_div(uint32_t a, uint32_t b) { return (a + b - 1) / b; }
+
+__device__ __forceinline__ uint32_t smem_wave_offset(int K) { return (threadIdx.x / kWave) * 2u * (uint32_t)K; }
+
+// ------------------------------------------------------------------------
+// k_select: visiting order and split-scan tasks of every frontier node.
+// ------------------------------------------------------------------------
+template <class IdxT, class LabT>
+__global__ __launch_bounds__(kBlock) void k_select(FitArgs<IdxT, LabT> a) {
+  const uint32_t lane = lane_id();
+  const uint32_t m = (uint32_t)a.m, F = (uint32_t)a.F;
+  WAVE_LOOP(a.ctl->q) {
+    const uint32_t i = item;
+    const FNode nd = a.front[i];
+    Perm pm = make_perm(a.seed, a.tree_base + nd.tree, nd.depth, nd.start, F, a.identity != 0);
+    uint32_t nf = 0;
+    for (uint32_t k0 = 0; k0 < F && nf < m; k0 += kWave) {
+      const uint32_t k = k0 + lane;
+      bool nonconst = false;
+      uint32_t f = 0;
+      if (k < F) {
+        f = perm_at(pm, k);
+        const IdxT *row = rows_of(a, nd.src, nd.tree, f) + nd.start;
+        const float *x = a.X_T + (size_t)f * a.N;
+        nonconst = x[row[0]] != x[row[nd.n - 1]];
+      }
+      const uint64_t mk = ballot(nonconst);
+      const uint32_t r = mbcnt(mk);
+      if (nonconst && nf + r < m) a.feats[(size_t)i * m + nf + r] = (int32_t)f;
+      nf += min((uint32_t)__popcll(mk), m - nf);
+    }
+    const uint32_t n = nd.n;
+    const uint32_t nch = ceil_div(n, a.Cs);
+    const uint32_t fpt = (nch > 1) ? 1u : max(1u, a.Cs / n);
+    const uint32_t ntask = (nf == 0) ? 0u : (nch > 1 ? nf * nch : ceil_div(nf, fpt));
+    uint32_t base = 0, hbase = 0;
+    if (lane == 0) {
+      base = atomicAdd(&a.ctl->n_scan, ntask);
+      if (nch > 1) hbase = atomicAdd(&a.ctl->n_hist, nf * nch * (uint32_t)a.K);
+      atomicAdd(&a.ctl->bytes_scan, (unsigned long long)nf * n * (sizeof(IdxT) + sizeof(LabT) + 4));
+    }
+    base = uniform(__shfl(base, 0));
+    hbase = uniform(__shfl(hbase, 0));
+    const bool fits = (uint64_t)base + ntask <= a.cap_scan &&
+                      (nch <= 1 || (uint64_t)hbase + (uint64_t)nf * nch * a.K <= a.cap_hist);
+    if (!fits) {
+      if (lane == 0) atomicOr(&a.ctl->err, kErrCapacity);
+      continue;
+    }
+    for (uint32_t t = lane; t < ntask; t += kWave) {
+      Task tk;
+      tk.node = i;
+      if (nch > 1) {
+        tk.first_count = (t / nch) | (1u << 16);
+        tk.chunk = t % nch;
+      } else {
+        const uint32_t first = t * fpt;
+        tk.first_count = first | (min(fpt, nf - first) << 16);
+        tk.chunk = 0;
+      }
+      tk.nch = nch;
+      a.scan_tasks[base + t] = tk;
+    }
+    if (lane == 0) {
+      NAux ax = {};
+      ax.scan_base = base;
+      ax.scan_ntask = ntask;
+      ax.nch_s = nch;
+      ax.fpt = fpt;
+      ax.hist_base = hbase;
+      ax.best_f = -1;
+      a.aux[i] = ax;
+    }
+  }
+}
+
+// ------------------------------------------------------------------------
+// k_scan_hist: class histogram of one chunk of one feature row (multi-chunk
+// nodes only).  The prefix over chunks gives each chunk its starting counts,
+// the decomposition of scan_gini_large + scan_reduce (dfs_module.cu:24-106).
+// ------------------------------------------------------------------------
+template <class IdxT, class LabT>
+__global__ __launch_bounds__(kBlock) void k_scan_hist(FitArgs<IdxT, LabT> a) {
+  extern __shared__ uint32_t smem[];
+  const uint32_t lane = lane_id();
+  const uint32_t K = (uint32_t)a.K;
+  uint32_t *h = smem + smem_wave_offset(a.K);
+  WAVE_LOOP(a.ctl->n_scan) {
+    const Task tk = a.scan_tasks[item];
+    if (tk.nch <= 1) continue;
+    const FNode nd = a.front[tk.node];
+    const NAux ax = a.aux[tk.node];
+    const uint32_t j = tk.first_count & 0xFFFFu;
+    const uint32_t f = (uint32_t)a.feats[(size_t)tk.node * a.m + j];
+    const IdxT *row = rows_of(a, nd.src, nd.tree, f) + nd.start;
+    for (uint32_t k = lane; k < K; k += kWave) h[k] = 0;
+    wave_sync();
+    const uint32_t p0 = tk.chunk * a.Cs, p1 = min(nd.n, p0 + a.Cs);
+    for (uint32_t p = p0 + lane; p < p1; p += kWave) atomicAdd(&h[a.y[row[p]]], 1u);
+    wave_sync();
+    uint32_t *dst = a.hist + ax.hist_base + ((size_t)j * tk.nch + tk.chunk) * K;
+    for (uint32_t k = lane; k < K; k += kWave) dst[k] = h[k];
+    wave_sync();
+  }
+}
+
+// Exclusive prefix over the chunks of each multi-chunk (node, feature).
+template <class IdxT, class LabT>
+__global__ __launch_bounds__(kBlock) void k_scan_carry(FitArgs<IdxT, LabT> a) {
+  const uint32_t lane = lane_id();
+  const uint32_t K = (uint32_t)a.K;
+  WAVE_LOOP(a.ctl->n_scan) {
+    const Task tk = a.scan_tasks[item];
+    if (tk.nch <= 1 || tk.chunk != 0) continue;
+    const NAux ax = a.aux[tk.node];
+    const uint32_t j = tk.first_count & 0xFFFFu;
+    uint32_t *base = a.hist + ax.hist_base + (size_t)j * tk.nch * K;
+    for (uint32_t k = lane; k < K; k += kWave) {
+      uint32_t run = 0;
+      for (uint32_t c = 0; c < tk.nch; ++c) {
+        uint32_t v = base[(size_t)c * K + k];
+        base[(size_t)c * K + k] = run;
+        run += v;
+      }
+    }
+  }
+}
+
+// ------------------------------------------------------------------------
+// k_scan: Gini scan of the task's chunk in each of its features.
+// ------------------------------------------------------------------------
+template <class IdxT, class LabT>
+__global__ __launch_bounds__(kBlock) void k_scan(FitArgs<IdxT, LabT> a) {
+  extern __shared__ uint32_t smem[];
+  const uint32_t lane = lane_id();
+  const uint32_t K = (uint32_t)a.K;
+  uint32_t *carry = smem + smem_wave_offset(a.K);
+  uint32_t *tot = carry + K;
+  WAVE_LOOP(a.ctl->n_scan) {
+    const Task tk = a.scan_tasks[item];
+    const uint32_t i = tk.node;
+    const FNode nd = a.front[i];
+    const NAux ax = a.aux[i];
+    const uint32_t n = nd.n;
+    uint64_t t2 = 0;
+    for (uint32_t k = lane; k < K; k += kWave) {
+      const uint32_t v = a.cnt[(size_t)i * K + k];
+      tot[k] = v;
+      t2 += (uint64_t)v * v;
+    }
+    const uint64_t T2 = uniform64(wave_sum64(t2));
+    const uint32_t first = tk.first_count & 0xFFFFu, count = tk.first_count >> 16;
+    const uint32_t p0 = tk.chunk * a.Cs, p1 = (tk.nch > 1) ? min(n, p0 + a.Cs) : n;
+    Cand best = {0, 1, 0, 0, 0};
+    for (uint32_t jj = 0; jj < count; ++jj) {
+      const uint32_t j = first + jj;
+      const uint32_t f = (uint32_t)a.feats[(size_t)i * a.m + j];
+      const IdxT *row = rows_of(a, nd.src, nd.tree, f) + nd.start;
+      const float *x = a.X_T + (size_t)f * a.N;
+      // chunk carry and the running sums at the chunk start
+      wave_sync();
+      uint64_t sl = 0, dd = 0;
+      const uint32_t *hc = (tk.nch > 1) ? a.hist + ax.hist_base + ((size_t)j * tk.nch + tk.chunk) * K : nullptr;
+      for (uint32_t k = lane; k < K; k += kWave) {
+        const uint32_t c = hc ? hc[k] : 0u;
+        carry[k] = c;
+        sl += (uint64_t)c * c;
+        dd += (uint64_t)tot[k] * c;
+      }
+      uint64_t SL = uniform64(wave_sum64(sl));
+      uint64_t D = uniform64(wave_sum64(dd));
+      wave_sync();
+      Cand lb = {0, 1, 0, (int32_t)f, 0};
+      for (uint32_t pb = p0; pb < p1; pb += kWave) {
+        const uint32_t p = pb + lane;
+        const bool valid = p < p1;
+        uint32_t lab = 0;
+        float val = 0.f;
+        if (valid) {
+          const uint32_t s = row[p];
+          lab = a.y[s];
+          val = x[s];
+        }
+        // rank of this lane among earlier lanes with the same label
+        uint32_t cprev = 0;
+        uint64_t rem = ballot(valid);
+        while (rem) {
+          const uint32_t leader = (uint32_t)__ffsll((unsigned long long)rem) - 1u;
+          const uint32_t L = readlane(lab, leader);
+          const uint64_t mm = ballot(valid && lab == L);
+          const uint32_t b0 = carry[L];
+          if (valid && lab == L) cprev = b0 + mbcnt(mm);
+          wave_sync();
+          if (lane == 0) carry[L] = b0 + (uint32_t)__popcll(mm);
+          wave_sync();
+          rem &= ~mm;
+        }
+        const uint32_t dS = valid ? 2u * cprev + 1u : 0u;
+        const uint32_t dD = valid ? tot[lab] : 0u;
+        const uint32_t iS = wave_incl_scan(dS);
+        const uint32_t iD = wave_incl_scan(dD);
+        const uint64_t SLp = SL + iS, Dp = D + iD;
+        SL += readlane(iS, kWave - 1);
+        D += readlane(iD, kWave - 1);
+        float nxt = __shfl_down(val, 1, kWave);
+        const bool need = valid && (lane == kWave - 1 || p + 1 == p1) && (p + 1 < n);
+        if (need) nxt = x[row[p + 1]];
+        const bool elig = valid && (p + 1 < n) && gap_ok(val, nxt);
+        if (elig) {
+          const uint64_t nL = (uint64_t)p + 1, nR = (uint64_t)n - nL;
+          const uint64_t SR = T2 + SLp - 2 * Dp;
+          Cand c = {SLp * nR + SR * nL, nL * nR, p, (int32_t)f, 1u};
+          if (!lb.ok || q_cmp(c.num, c.den, lb.num, lb.den) > 0) lb = c;
+        }
+      }
+      lb = wave_best(lb);
+      if (cand_better(lb, best)) best = lb;
+    }
+    if (lane == 0) {
+      ScanRes r;
+      r.num = best.num;
+      r.den = best.den;
+      r.pos = best.pos;
+      r.feat = best.feat;
+      r.ok = best.ok;
+      r.pad = 0;
+      a.res[item] = r;
+    }
+  }
+}
+
+// ------------------------------------------------------------------------
+// k_decide: best split of each node over its tasks; threshold; child ids.
+// ------------------------------------------------------------------------
+template <class IdxT, class LabT>
+__global__ __launch_bounds__(kBlock) void k_decide(FitArgs<IdxT, LabT> a) {
+  const uint32_t lane = lane_id();
+  const uint32_t K = (uint32_t)a.K;
+  WAVE_LOOP(a.ctl->q) {
+    const uint32_t i = item;
+    const FNode nd = a.front[i];
+    NAux ax = a.aux[i];
+    Cand best = {0, 1, 0, 0, 0};
+    for (uint32_t t = lane; t < ax.scan_ntask; t += kWave) {
+      const ScanRes r = a.res[ax.scan_base + t];
+      Cand c = {r.num, r.den, r.pos, r.feat, r.ok};
+      if (cand_better(c, best)) best = c;
+    }
+    best = wave_best(best);
+    const uint32_t id = nd.nid;
+    if (!best.ok) {
+      // no candidate position: leaf labelled by the first sample of the
+      // segment in feature 0's row (random_tree.py:628-634, 50-63)
+      if (lane == 0) {
+        const IdxT *row0 = rows_of(a, nd.src, nd.tree, 0) + nd.start;
+        a.arena.feat[id] = -1;
+        a.arena.thr[id] = 0.f;
+        a.arena.left[id] = kLeaf;
+        a.arena.right[id] = kLeaf;
+        a.arena.value[id] = a.y[row0[0]];
+        ax.best_f = -1;
+        ax.flags = 0;
+        a.aux[i] = ax;
+      }
+      continue;
+    }
+    const uint32_t f = (uint32_t)best.feat, p = best.pos;
+    for (uint32_t k = lane; k < K; k += kWave) a.cntL[(size_t)i * K + k] = 0;
+    if (lane == 0) {
+      const IdxT *row = rows_of(a, nd.src, nd.tree, f) + nd.start;
+      const float *x = a.X_T + (size_t)f * a.N;
+      // threshold midway in float, no contraction (bfs_module.cu:399-400)
+      const float thr = __fmul_rn(__fadd_rn(x[row[p]], x[row[p + 1]]), 0.5f);
+      const uint32_t ids = atomicAdd(&a.ctl->n_nodes, 2u);
+      const uint32_t nch = ceil_div(nd.n, a.Cf);
+      const uint32_t fb = atomicAdd(&a.ctl->n_fill, nch);
+      if ((uint64_t)ids + 2 > a.cap_nodes || (uint64_t)fb + nch > a.cap_fill) {
+        atomicOr(&a.ctl->err, kErrCapacity);
+        ax.flags = 0;
+        ax.best_f = -1;
+      } else {
+        a.arena.feat[id] = (int32_t)f;
+        a.arena.thr[id] = thr;
+        a.arena.left[id] = ids;
+        a.arena.right[id] = ids + 1;
+        a.arena.value[id] = 0;
+        ax.best_f = (int32_t)f;
+        ax.best_p = p;
+        ax.nL = p + 1;
+        ax.left_id = ids;
+        ax.right_id = ids + 1;
+        ax.flags = kSplit;
+        ax.dst = (nd.src == 2) ? 1u : 2u;
+        ax.fill_base = fb;
+        ax.nch_f = nch;
+        for (uint32_t c = 0; c < nch; ++c) a.fill_tasks[fb + c] = Task{i, 0u, c, nch};
+        atomicAdd(&a.ctl->bytes_fill, (unsigned long long)nd.n * (sizeof(IdxT) + sizeof(LabT) + 1));
+      }
+      a.aux[i] = ax;
+    }
+  }
+}
+
+// ------------------------------------------------------------------------
+// k_fill: mark every sample of the split row's segment with its side and
+// count the left child's classes.
+// ------------------------------------------------------------------------
+template <class IdxT, class LabT>
+__global__ __launch_bounds__(kBlock) void k_fill(FitArgs<IdxT, LabT> a) {
+  extern __shared__ uint32_t smem[];
+  const uint32_t lane = lane_id();
+  const uint32_t K = (uint32_t)a.K;
+  uint32_t *h = smem + smem_wave_offset(a.K);
+  WAVE_LOOP(a.ctl->n_fill) {
+    const Task tk = a.fill_tasks[item];
+    const FNode nd = a.front[tk.node];
+    const NAux ax = a.aux[tk.node];
+    const IdxT *row = rows_of(a, nd.src, nd.tree, (uint32_t)ax.best_f) + nd.start;
+    uint8_t *mark = a.mark + (size_t)nd.tree * a.N;
+    for (uint32_t k = lane; k < K; k += kWave) h[k] = 0;
+    wave_sync();
+    const uint32_t p0 = tk.chunk * a.Cf, p1 = min(nd.n, p0 + a.Cf);
+    for (uint32_t p = p0 + lane; p < p1; p += kWave) {
+      const uint32_t s = row[p];
+      const bool left = p <= ax.best_p;
+      mark[s] = left ? 1 : 0;
+      if (left) atomicAdd(&h[a.y[s]], 1u);
+    }
+    wave_sync();
+    for (uint32_t k = lane; k < K; k += kWave)
+      if (h[k]) atomicAdd(&a.cntL[(size_t)tk.node * K + k], h[k]);
+    wave_sync();
+  }
+}
+
+// Emits the reshuffle tasks of node i (n samples, F rows).
+template <class IdxT, class LabT>
+__device__ void emit_resh_tasks(const FitArgs<IdxT, LabT> &a, uint32_t i, uint32_t n, NAux &ax) {
+  const uint32_t lane = lane_id();
+  const uint32_t F = (uint32_t)a.F;
+  const uint32_t nch = ceil_div(n, a.Cr);
+  const uint32_t rpt = (nch > 1) ? 1u : max(1u, a.Cr / n);
+  const uint32_t ntask = (nch > 1) ? F * nch : ceil_div(F, rpt);
+  uint32_t base = 0, rbase = 0;
+  if (lane == 0) {
+    base = atomicAdd(&a.ctl->n_resh, ntask);
+    if (nch > 1) rbase = atomicAdd(&a.ctl->n_rcnt, F * nch);
+  }
+  base = uniform(__shfl(base, 0));
+  rbase = uniform(__shfl(rbase, 0));
+  if ((uint64_t)base + ntask > a.cap_resh || (nch > 1 && (uint64_t)rbase + (uint64_t)F * nch > a.cap_rcnt)) {
+    if (lane == 0) atomicOr(&a.ctl->err, kErrCapacity);
+    ax.resh_ntask = 0;
+    return;
+  }
+  for (uint32_t t = lane; t < ntask; t += kWave) {
+    Task tk;
+    tk.node = i;
+    if (nch > 1) {
+      tk.first_count = (t / nch) | (1u << 16);
+      tk.chunk = t % nch;
+    } else {
+      const uint32_t first = t * rpt;
+      tk.first_count = first | (min(rpt, F - first) << 16);
+      tk.chunk = 0;
+    }
+    tk.nch = nch;
+    a.resh_tasks[base + t] = tk;
+  }
+  ax.resh_base = base;
+  ax.resh_ntask = ntask;
+  ax.rcnt_base = rbase;
+  ax.nch_r = nch;
+  ax.rpt = rpt;
+}
+
+// Label of the majority class (lowest label on ties).
+__device__ uint32_t wave_majority(const uint32_t *c, uint32_t K) {
+  const uint32_t lane = lane_id();
+  uint32_t bv = 0, bk = 0xFFFFFFFFu;
+  for (uint32_t k = lane; k < K; k += kWave)
+    if (bk == 0xFFFFFFFFu || c[k] > bv) {
+      bv = c[k];
+      bk = k;
+    }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    uint32_t ov = __shfl_xor(bv, o, kWave), ok = __shfl_xor(bk, o, kWave);
+    if (ok != 0xFFFFFFFFu && (bk == 0xFFFFFFFFu || ov > bv || (ov == bv && ok < bk))) {
+      bv = ov;
+      bk = ok;
+    }
+  }
+  return bk;
+}
+
+// ------------------------------------------------------------------------
+// k_children: leaf tests of both children (random_tree.py:123-163),
+// the next frontier and the reshuffle tasks.
+// ------------------------------------------------------------------------
+template <class IdxT, class LabT>
+__global__ __launch_bounds__(kBlock) void k_children(FitArgs<IdxT, LabT> a) {
+  extern __shared__ uint32_t smem[];
+  const uint32_t lane = lane_id();
+  const uint32_t K = (uint32_t)a.K;
+  uint32_t *cl = smem + smem_wave_offset(a.K);
+  uint32_t *cr = cl + K;
+  WAVE_LOOP(a.ctl->q) {
+    const uint32_t i = item;
+    NAux ax = a.aux[i];
+    if (!(ax.flags & kSplit)) continue;
+    const FNode nd = a.front[i];
+    const uint32_t nL = ax.nL, nR = nd.n - ax.nL;
+    uint32_t nzl = 0, nzr = 0, onlyl = 0, onlyr = 0;
+    wave_sync();
+    for (uint32_t k = lane; k < K; k += kWave) {
+      const uint32_t l = a.cntL[(size_t)i * K + k];
+      const uint32_t r = a.cnt[(size_t)i * K + k] - l;
+      cl[k] = l;
+      cr[k] = r;
+      if (l) {
+        ++nzl;
+        onlyl = k;
+      }
+      if (r) {
+        ++nzr;
+        onlyr = k;
+      }
+    }
+    nzl = uniform(wave_sum32(nzl));
+    nzr = uniform(wave_sum32(nzr));
+    // the single present class (max over lanes: absent lanes hold 0)
+    uint32_t ol = onlyl, orr = onlyr;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      ol = max(ol, (uint32_t)__shfl_xor(ol, o, kWave));
+      orr = max(orr, (uint32_t)__shfl_xor(orr, o, kWave));
+    }
+    wave_sync();
+    const uint32_t cd = nd.depth + 1;
+    const bool capped = a.max_depth > 0 && (int64_t)cd >= (int64_t)a.max_depth;
+    // 0 live, 1 pure, 2 first-sample, 3 majority
+    auto kind = [&](uint32_t nz, uint32_t nc) -> uint32_t {
+      if (nz <= 1) return 1u;
+      if ((int64_t)nc < (int64_t)a.mss) return 2u;
+      if (capped) return 3u;
+      return 0u;
+    };
+    const uint32_t kl = kind(nzl, nL), kr = kind(nzr, nR);
+    uint32_t labl = ol, labr = orr;
+    if (kl == 3) labl = wave_majority(cl, K);
+    if (kr == 3) labr = wave_majority(cr, K);
+    if (kl == 2 || kr == 2) {
+      // first sample of each child in feature 0's row: the first marked
+      // (left) and first unmarked (right) sample of the parent's segment
+      const IdxT *row0 = rows_of(a, nd.src, nd.tree, 0) + nd.start;
+      const uint8_t *mark = a.mark + (size_t)nd.tree * a.N;
+      bool needl = kl == 2, needr = kr == 2;
+      for (uint32_t pb = 0; pb < nd.n && (needl || needr); pb += kWave) {
+        const uint32_t p = pb + lane;
+        uint32_t s = 0, side = 2;
+        if (p < nd.n) {
+          s = row0[p];
+          side = mark[s];
+        }
+        const uint64_t ml = ballot(side == 1), mr = ballot(side == 0);
+        if (needl && ml) {
+          labl = a.y[readlane(s, (uint32_t)__ffsll((unsigned long long)ml) - 1u)];
+          needl = false;
+        }
+        if (needr && mr) {
+          labr = a.y[readlane(s, (uint32_t)__ffsll((unsigned long long)mr) - 1u)];
+          needr = false;
+        }
+      }
+    }
+    uint32_t flags = ax.flags;
+    const uint32_t cids[2] = {ax.left_id, ax.right_id};
+    const uint32_t kinds[2] = {kl, kr};
+    const uint32_t labs[2] = {labl, labr};
+    const uint32_t cn[2] = {nL, nR};
+    const uint32_t cs[2] = {nd.start, nd.start + nL};
+    for (int c = 0; c < 2; ++c) {
+      const uint32_t id = cids[c];
+      if (lane == 0) {
+        a.arena.nsamp[id] = cn[c];
+        a.arena.depth[id] = cd;
+      }
+      if (kinds[c] != 0) {
+        if (lane == 0) {
+          a.arena.feat[id] = -1;
+          a.arena.thr[id] = 0.f;
+          a.arena.left[id] = kLeaf;
+          a.arena.right[id] = kLeaf;
+          a.arena.value[id] = labs[c];
+        }
+        continue;
+      }
+      uint32_t slot = 0;
+      if (lane == 0) slot = atomicAdd(&a.ctl->q_next, 1u);
+      slot = uniform(__shfl(slot, 0));
+      if (slot >= a.cap_q) {
+        if (lane == 0) atomicOr(&a.ctl->err, kErrCapacity);
+        continue;
+      }
+      if (lane == 0) {
+        FNode ch;
+        ch.tree = nd.tree;
+        ch.start = cs[c];
+        ch.n = cn[c];
+        ch.nid = id;
+        ch.depth = cd;
+        ch.src = ax.dst;
+        ch.pad0 = ch.pad1 = 0;
+        a.next_front[slot] = ch;
+      }
+      const uint32_t *cc = c ? cr : cl;
+      for (uint32_t k = lane; k < K; k += kWave) a.next_cnt[(size_t)slot * K + k] = cc[k];
+      flags |= c ? kLiveR : kLiveL;
+    }
+    ax.flags = flags;
+    if (flags & (kLiveL | kLiveR)) {
+      emit_resh_tasks(a, i, nd.n, ax);
+      if (lane == 0)
+        atomicAdd(&a.ctl->bytes_resh, (unsigned long long)a.F * nd.n * (2 * sizeof(IdxT) + 1));
+    } else {
+      ax.resh_ntask = 0;
+    }
+    if (lane == 0) a.aux[i] = ax;
+    wave_sync();
+  }
+}
+
+// ------------------------------------------------------------------------
+// Stable partition of every row of the split nodes (reshuffle).
+// ------------------------------------------------------------------------
+template <class IdxT, class LabT>
+__global__ __launch_bounds__(kBlock) void k_resh_count(FitArgs<IdxT, LabT> a) {
+  const uint32_t lane = lane_id();
+  WAVE_LOOP(a.ctl->n_resh) {
+    const Task tk = a.resh_tasks[item];
+    if (tk.nch <= 1) continue;
+    const FNode nd = a.front[tk.node];
+    const NAux ax = a.aux[tk.node];
+    const uint32_t f = tk.first_count & 0xFFFFu;
+    const IdxT *row = rows_of(a, nd.src, nd.tree, f) + nd.start;
+    const uint8_t *mark = a.mark + (size_t)nd.tree * a.N;
+    const uint32_t p0 = tk.chunk * a.Cr, p1 = min(nd.n, p0 + a.Cr);
+    uint32_t c = 0;
+    for (uint32_t p = p0 + lane; p < p1; p += kWave) c += mark[row[p]];
+    c = wave_sum32(c);
+    if (lane == 0) a.rcnt[ax.rcnt_base + f * tk.nch + tk.chunk] = c;
+  }
+}
+
+template <class IdxT, class LabT>
+__global__ __launch_bounds__(kBlock) void k_resh_carry(FitArgs<IdxT, LabT> a) {
+  const uint32_t lane = lane_id();
+  WAVE_LOOP(a.ctl->n_resh) {
+    const Task tk = a.resh_tasks[item];
+    if (tk.nch <= 1 || tk.chunk != 0) continue;
+    const NAux ax = a.aux[tk.node];
+    const uint32_t f = tk.first_count & 0xFFFFu;
+    uint32_t *base = a.rcnt + ax.rcnt_base + f * tk.nch;
+    uint32_t run = 0;
+    for (uint32_t c0 = 0; c0 < tk.nch; c0 += kWave) {
+      const uint32_t c = c0 + lane;
+      const uint32_t v = c < tk.nch ? base[c] : 0u;
+      const uint32_t inc = wave_incl_scan(v);
+      if (c < tk.nch) base[c] = run + inc - v;
+      run += readlane(inc, kWave - 1);
+    }
+  }
+}
+
+template <class IdxT, class LabT>
+__global__ __launch_bounds__(kBlock) void k_resh(FitArgs<IdxT, LabT> a) {
+  const uint32_t lane = lane_id();
+  WAVE_LOOP(a.ctl->n_resh) {
+    const Task tk = a.resh_tasks[item];
+    const FNode nd = a.front[tk.node];
+    const NAux ax = a.aux[tk.node];
+    const bool liveL = ax.flags & kLiveL, liveR = ax.flags & kLiveR;
+    const uint8_t *mark = a.mark + (size_t)nd.tree * a.N;
+    const uint32_t first = tk.first_count & 0xFFFFu, count = tk.first_count >> 16;
+    const uint32_t p0 = (tk.nch > 1) ? tk.chunk * a.Cr : 0u;
+    const uint32_t p1 = (tk.nch > 1) ? min(nd.n, p0 + a.Cr) : nd.n;
+    for (uint32_t f = first; f < first + count; ++f) {
+      const IdxT *in = rows_of(a, nd.src, nd.tree, f) + nd.start;
+      IdxT *out = rows_out(a, ax.dst, nd.tree, f) + nd.start;
+      uint32_t lb = (tk.nch > 1) ? a.rcnt[ax.rcnt_base + f * tk.nch + tk.chunk] : 0u;
+      for (uint32_t pb = p0; pb < p1; pb += kWave) {
+        const uint32_t p = pb + lane;
+        const bool valid = p < p1;
+        IdxT s = 0;
+        uint32_t side = 0;
+        if (valid) {
+          s = in[p];
+          side = mark[s];
+        }
+        const uint64_t ml = ballot(valid && side);
+        const uint32_t lefts = lb + mbcnt(ml);
+        if (valid) {
+          if (side) {
+            if (liveL) out[lefts] = s;
+          } else if (liveR) {
+            out[ax.nL + (p - lefts)] = s;
+          }
+        }
+        lb += (uint32_t)__popcll(ml);
+      }
+    }
+  }
+}
+
+// ------------------------------------------------------------------------
+// Bootstrap and roots (random_forest.py:113-144, bootstrap_fill.cu,
+// bootstrap_reshuffle.cu).
+// ------------------------------------------------------------------------
+__global__ void k_fill_u8(uint8_t *p, size_t n, uint8_t v) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = v;
+}
+
+__global__ void k_fill_u32(uint32_t *p, size_t n, uint32_t v) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = v;
+}
+
+// mark[t][id] = 1 for the in-bag ids of every batch tree.
+__global__ void k_boot_mark(const uint32_t *ids, const uint64_t *off, uint32_t T, uint32_t N, uint8_t *mark) {
+  const uint32_t t = blockIdx.y;
+  if (t >= T) return;
+  const uint64_t a = off[t], b = off[t + 1];
+  for (uint64_t j = a + (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; j < b; j += (uint64_t)gridDim.x * blockDim.x)
+    mark[(size_t)t * N + ids[j]] = 1;
+}
+
+// Class counts of every batch tree's root: over its ids, or over all N.
+template <class LabT>
+__global__ void k_root_counts(const LabT *y, const uint32_t *ids, const uint64_t *off, uint32_t T, uint32_t N,
+                              uint32_t K, uint32_t *cnt) {
+  extern __shared__ uint32_t h[];
+  const uint32_t t = blockIdx.y;
+  for (uint32_t k = threadIdx.x; k < K; k += blockDim.x) h[k] = 0;
+  __syncthreads();
+  const uint64_t a = ids ? off[t] : 0, b = ids ? off[t + 1] : N;
+  for (uint64_t j = a + (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; j < b; j += (uint64_t)gridDim.x * blockDim.x)
+    atomicAdd(&h[y[ids ? ids[j] : (uint32_t)j]], 1u);
+  __syncthreads();
+  for (uint32_t k = threadIdx.x; k < K; k += blockDim.x)
+    if (h[k]) atomicAdd(&cnt[(size_t)t * K + k], h[k]);
+}
+
+// Pseudo-frontier of the bootstrap compaction: tree t's presorted rows
+// (src 0, N samples) keep the in-bag samples (mark 1) in buffer 0.
+template <class IdxT, class LabT>
+__global__ __launch_bounds__(kBlock) void k_boot_tasks(FitArgs<IdxT, LabT> a, const uint64_t *off, uint32_t T,
+                                                       FNode *front, uint32_t dummy) {
+  const uint32_t lane = lane_id();
+  WAVE_LOOP(T) {
+    const uint32_t t = item;
+    if (lane == 0) {
+      FNode nd;
+      nd.tree = t;
+      nd.start = 0;
+      nd.n = (uint32_t)a.N;
+      nd.nid = 0;
+      nd.depth = 0;
+      nd.src = 0;
+      nd.pad0 = nd.pad1 = dummy;
+      front[t] = nd;
+    }
+    NAux ax = {};
+    ax.best_f = -1;
+    ax.nL = (uint32_t)(off[t + 1] - off[t]);
+    ax.flags = kSplit | kLiveL;
+    ax.dst = 1;
+    emit_resh_tasks(a, t, (uint32_t)a.N, ax);
+    if (lane == 0) {
+      a.aux[t] = ax;
+      atomicAdd(&a.ctl->bytes_boot, (unsigned long long)a.F * a.N * (sizeof(IdxT) + 1) +
+                                        (unsigned long long)a.F * ax.nL * sizeof(IdxT));
+    }
+  }
+}
+
+// Roots: tree t's root (arena id t) is a leaf if pure, else enters the frontier.
+template <class IdxT, class LabT>
+__global__ __launch_bounds__(kBlock) void k_roots(FitArgs<IdxT, LabT> a, const uint32_t *root_cnt,
+                                                  const uint64_t *off, uint32_t T, uint32_t src) {
+  const uint32_t lane = lane_id();
+  const uint32_t K = (uint32_t)a.K;
+  WAVE_LOOP(T) {
+    const uint32_t t = item;
+    const uint32_t n = off ? (uint32_t)(off[t + 1] - off[t]) : (uint32_t)a.N;
+    uint32_t nz = 0, only = 0;
+    for (uint32_t k = lane; k < K; k += kWave)
+      if (root_cnt[(size_t)t * K + k]) {
+        ++nz;
+        only = k;
+      }
+    nz = uniform(wave_sum32(nz));
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) only = max(only, (uint32_t)__shfl_xor(only, o, kWave));
+    if (lane == 0) {
+      a.arena.nsamp[t] = n;
+      a.arena.depth[t] = 0;
+    }
+    // The reference never tests the root for purity (random_tree.py:482-486);
+    // a pure root is a leaf here.  min_samples_split applies as at any node;
+    // a positive max_depth never caps the root.
+    uint32_t leaf_label = kLeaf;
+    if (nz <= 1) leaf_label = only;
+    else if ((int64_t)n < (int64_t)a.mss) {
+      const IdxT *row0 = rows_of(a, src, t, 0);
+      leaf_label = a.y[row0[0]];
+    }
+    if (leaf_label != kLeaf) {
+      if (lane == 0) {
+        a.arena.feat[t] = -1;
+        a.arena.thr[t] = 0.f;
+        a.arena.left[t] = kLeaf;
+        a.arena.right[t] = kLeaf;
+        a.arena.value[t] = leaf_label;
+      }
+      continue;
+    }
+    uint32_t slot = 0;
+    if (lane == 0) slot = atomicAdd(&a.ctl->q_next, 1u);
+    slot = uniform(__shfl(slot, 0));
+    if (lane == 0) {
+      FNode nd;
+      nd.tree = t;
+      nd.start = 0;
+      nd.n = n;
+      nd.nid = t;
+      nd.depth = 0;
+      nd.src = src;
+      nd.pad0 = nd.pad1 = 0;
+      a.next_front[slot] = nd;
+    }
+    for (uint32_t k = lane; k < K; k += kWave) a.next_cnt[(size_t)slot * K + k] = root_cnt[(size_t)t * K + k];
+  }
+}
+
+// Moves the next frontier's count into place and clears the level counters.
+__global__ void k_ctl_next(Ctl *ctl) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    ctl->q = ctl->q_next;
+    ctl->q_next = 0;
+    ctl->n_scan = ctl->n_fill = ctl->n_resh = ctl->n_hist = ctl->n_rcnt = 0;
+  }
+}
+
+// Sets the frontier size for a host-provided frontier.
+__global__ void k_ctl_set(Ctl *ctl, uint32_t q) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    ctl->q = q;
+    ctl->q_next = 0;
+    ctl->n_scan = ctl->n_fill = ctl->n_resh = ctl->n_hist = ctl->n_rcnt = 0;
+  }
+}
+
+}  // namespace cta

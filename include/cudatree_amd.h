@@ -1,0 +1,181 @@
+/*
+ * cudatree_amd.h -- C-ABI of the MI355X random-forest library
+ * (libcudatree_amd.so, HIP for gfx950).
+ *
+ * The reference (EasonLiao/CudaTree) has no C interface: its boundary is the
+ * Python class RandomForestClassifier (cudatree/random_forest.py:45-51, 292,
+ * 339, 364, 379) driving PyCUDA kernel launches (prepared_call on templated
+ * .cu modules, cudatree/util.py:24-37, random_forest.py:382-460).  This header
+ * is the boundary the Python facade (cudatree_amd/random_forest.py, loaded
+ * with ctypes) binds in place of those launches; each entry point names the
+ * reference code it replaces.
+ *
+ * Conventions
+ *   - Every call returns an int status (CTA_OK == 0); cta_last_error() gives
+ *     a thread-local message for the last failure.  No C++ exception crosses
+ *     the ABI.
+ *   - Host arrays are copied in and out; the library owns its device memory.
+ *     The *_device variants take device pointers (e.g. PyTorch-ROCm tensors)
+ *     that must stay valid until the call returns.
+ *   - A context owns one HIP stream on one device.  Calls on one context
+ *     are serialised by its internal mutex; use one context per thread for
+ *     concurrency.
+ *   - Labels are compact: y[i] in [0, K).  Up to 4,194,303 samples (exact
+ *     64-bit Gini arithmetic), 65,535 features and 1,024 classes.
+ */
+#ifndef CUDATREE_AMD_H
+#define CUDATREE_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CTA_OK 0
+#define CTA_EINVAL 1      /* bad argument */
+#define CTA_ENOMEM 2      /* device or host allocation failed */
+#define CTA_EHIP 3        /* HIP runtime error */
+#define CTA_EUNSUPPORTED 4 /* size beyond the library's limits */
+#define CTA_EINTERNAL 5   /* device-side consistency check failed */
+
+#define CTA_MAX_SAMPLES 4194303
+#define CTA_MAX_FEATURES 65535
+#define CTA_MAX_CLASSES 1024
+
+/* Fit scheduling modes.  All three build identical trees (nodes are
+ * canonically numbered in depth-first pre-order on export). */
+#define CTA_MODE_BATCHED 0  /* every level of every tree in one set of launches */
+#define CTA_MODE_HYBRID 1   /* per tree: nodes above bfs_threshold one at a time, the rest level-batched
+                               (random_tree.py:616-622, 309-458) */
+#define CTA_MODE_PER_NODE 2 /* every node in its own launch sequence, trees in series
+                               (random_tree.py:580-676 with bfs_threshold = 0) */
+
+typedef struct cta_ctx cta_ctx;
+typedef struct cta_dataset cta_dataset;
+typedef struct cta_forest cta_forest;
+
+/* Thread-local message of the last failed call ("" if none). */
+const char *cta_last_error(void);
+/* Library version, major * 10000 + minor * 100 + patch. */
+int cta_version(void);
+
+/* Context on HIP device `device` with its own stream.
+ * Replaces: pycuda.autoinit's implicit context (random_tree.py:1). */
+int cta_ctx_create(int device, cta_ctx **out);
+int cta_ctx_destroy(cta_ctx *ctx);
+int cta_ctx_synchronize(cta_ctx *ctx);
+/* Free / total device memory in bytes as seen by the context's device. */
+int cta_ctx_mem_info(cta_ctx *ctx, uint64_t *free_bytes, uint64_t *total_bytes);
+
+/* Training set: uploads X (row-major N x F, host) and compact labels y (N),
+ * transposes X to feature-major on the device and presorts every feature row
+ * with a stable device radix sort (ties keep ascending sample id).  idx_bits
+ * 0 picks 16-bit sample indices when N <= 65,535 and 32-bit otherwise.
+ * Replaces: fit_init's transpose, uploads and np.argsort
+ * (random_forest.py:215-266, 233, 244-248) and get_best_dtype
+ * (util.py:40-49). */
+int cta_dataset_create(cta_ctx *ctx, const float *X, const uint32_t *y, int32_t N, int32_t F, int32_t K,
+                       int32_t idx_bits, cta_dataset **out);
+/* As cta_dataset_create with X (N x F, float32) and y (N, uint32) in device memory. */
+int cta_dataset_create_device(cta_ctx *ctx, const float *X_dev, const uint32_t *y_dev, int32_t N, int32_t F,
+                              int32_t K, int32_t idx_bits, cta_dataset **out);
+int cta_dataset_destroy(cta_dataset *ds);
+int cta_dataset_info(const cta_dataset *ds, int32_t *N, int32_t *F, int32_t *K, int32_t *idx_bits);
+/* Copies the presorted index rows (F x N, widened to uint32) to host memory.
+ * Test hook for the presort. */
+int cta_dataset_sorted(cta_dataset *ds, uint32_t *out);
+
+typedef struct {
+  uint64_t seed;             /* keys the per-node feature permutation */
+  int32_t max_features;      /* m, 1..F (the facade applies ceil(sqrt(F)), random_forest.py:250-251) */
+  int32_t min_samples_split; /* random_forest.py:48 (default 1) */
+  int32_t max_depth;         /* <= 0: unlimited */
+  int32_t identity_perm;     /* 1: features in id order (reference debug mode, random_tree.py:193-200) */
+  int32_t mode;              /* CTA_MODE_* */
+  int64_t bfs_threshold;     /* CTA_MODE_HYBRID: nodes with n > bfs_threshold go one at a time */
+  int32_t scan_chunk;        /* samples per split-scan task (0: default) */
+  int32_t resh_chunk;        /* samples per reshuffle task (0: default) */
+  int32_t trees_per_batch;   /* trees resident at once (0: as many as memory allows) */
+  int32_t reserved[5];
+} cta_fit_opts;
+
+/* Fills opts with the defaults (batched mode, m = ceil(sqrt(F)) left to the caller). */
+void cta_fit_opts_default(cta_fit_opts *opts);
+
+/* Grows T trees.  boot_ids/boot_off: tree t's in-bag set is the sorted unique
+ * ids boot_ids[boot_off[t] .. boot_off[t+1]) (host memory); boot_ids == NULL
+ * grows every tree on all N samples (bootstrap=False).
+ * Replaces: RandomForestClassifier.fit's per-tree loop (random_forest.py:292-336),
+ * _get_sorted_indices with bootstrap_fill/bootstrap_reshuffle
+ * (random_forest.py:113-144, bootstrap_fill.cu, bootstrap_reshuffle.cu) and
+ * RandomClassifierTree.fit with its DFS/BFS kernels (random_tree.py:461-676,
+ * dfs_module.cu, bfs_module.cu). */
+int cta_fit(cta_ctx *ctx, cta_dataset *ds, int32_t T, const uint32_t *boot_ids, const int64_t *boot_off,
+            const cta_fit_opts *opts, cta_forest **out);
+
+int cta_forest_destroy(cta_forest *forest);
+int cta_forest_info(const cta_forest *forest, int32_t *n_trees, int64_t *n_nodes, int32_t *F, int32_t *K);
+/* Copies the trees out in depth-first pre-order, tree t's nodes at
+ * [tree_off[t], tree_off[t+1]).  Leaves have left == right == 0, feature 0,
+ * threshold 0 and value = label; internal nodes have value 0.  Any output
+ * pointer except tree_off may be NULL.
+ * Replaces: the tree arrays of random_tree.py:286-289, 493-513 and the pickle
+ * state of random_tree.py:203-216. */
+int cta_forest_export(const cta_forest *forest, int64_t *tree_off, int32_t *feature, float *threshold,
+                      uint32_t *left, uint32_t *right, uint32_t *value, uint32_t *n_samples, uint32_t *depth);
+/* Builds a device-resident forest from exported arrays (same layout as
+ * cta_forest_export; n_samples/depth not needed).
+ * Replaces: restore_forest / restore_tree (random_forest.py:17-23, random_tree.py:20-36),
+ * which left the restored forest unable to predict. */
+int cta_forest_import(cta_ctx *ctx, int32_t T, int32_t F, int32_t K, const int64_t *tree_off,
+                      const int32_t *feature, const float *threshold, const uint32_t *left,
+                      const uint32_t *right, const uint32_t *value, cta_forest **out);
+
+/* Majority vote of every tree for R row-major rows (host X, R x F).  Writes
+ * the winning label per row (lowest label on ties) and, if votes != NULL,
+ * the per-label tree counts (R x K).
+ * Replaces: predict / predict_proba (random_forest.py:339-377) with
+ * BaseTree.gpu_predict and the predict kernel (base_tree.py:44-81, predict.cu). */
+int cta_predict(cta_ctx *ctx, cta_forest *forest, const float *X, int64_t R, int32_t F, uint32_t *labels,
+                uint32_t *votes);
+/* As cta_predict with X (R x F float32), labels (R uint32) and votes (R x K
+ * uint32 or NULL) in device memory. */
+int cta_predict_device(cta_ctx *ctx, cta_forest *forest, const float *X_dev, int64_t R, int32_t F,
+                       uint32_t *labels_dev, uint32_t *votes_dev);
+
+/* ---- measurement ------------------------------------------------------- */
+typedef struct {
+  char name[32];
+  uint64_t launches;      /* timed launches */
+  double total_ms;        /* sum of HIP-event durations on the context stream */
+  uint64_t alg_bytes;     /* algorithmic bytes of the timed launches */
+} cta_kernel_stat;
+
+/* Enables per-launch HIP-event timing of the fit and predict kernels. */
+int cta_ctx_set_profiling(cta_ctx *ctx, int enable);
+int cta_ctx_reset_stats(cta_ctx *ctx);
+/* Copies up to max entries; *n gets the number of kernels with stats. */
+int cta_ctx_kernel_stats(cta_ctx *ctx, cta_kernel_stat *out, int32_t max, int32_t *n);
+
+/* ---- per-kernel parity hooks (tests) ------------------------------------ */
+/* Split scan of one node: rows (F x n uint32, host) hold the node's sorted
+ * segment of every feature; the m_eval features in `features` are scanned
+ * with the device kernel.  Writes the winning feature (-1 if none), position
+ * and exact score numerator/denominator.
+ * Replaces: scan_gini_large / scan_reduce / compute_2d / reduce_2d /
+ * find_min_imp (dfs_module.cu:24-208) and scan_bfs / compute_2d / reduce
+ * (bfs_module.cu:25-235) for one node. */
+int cta_k_split_scan(cta_ctx *ctx, cta_dataset *ds, const uint32_t *rows, uint32_t n, const int32_t *features,
+                     int32_t m_eval, int32_t chunk, int32_t *best_feature, uint32_t *best_pos, uint64_t *num,
+                     uint64_t *den);
+/* Stable partition of every row of one node (rows F x n uint32, host) by
+ * left-membership of each sample (mark, N bytes, 1 = left) into out (F x n).
+ * Replaces: fill_table + scan_reshuffle (dfs_module.cu:210-307, bfs_module.cu:238-377). */
+int cta_k_reshuffle(cta_ctx *ctx, cta_dataset *ds, const uint32_t *rows, uint32_t n, const uint8_t *mark,
+                    int32_t chunk, uint32_t *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

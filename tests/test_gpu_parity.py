@@ -1,0 +1,266 @@
+"""Bit-exact parity of the HIP path with the CPU oracle (needs an MI355X).
+
+Every comparison is exact: integer arrays equal, thresholds equal bit for
+bit, predictions equal.  Small chunk sizes force the multi-chunk scan and
+reshuffle paths on small inputs.
+"""
+import numpy as np
+import pytest
+
+import helpers
+import oracle
+from cudatree_amd import _lib, datasets
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    return _lib.default_context(0)
+
+
+@pytest.fixture(scope="module")
+def orc():
+    return oracle.get()
+
+
+def gpu_forest(ctx, case, mode="batched", scan_chunk=0, resh_chunk=0, idx_bits=0, tpb=0):
+    X, y, K = case["X"], case["y"], case["K"]
+    ds = _lib.Dataset(ctx, X, y, K, idx_bits=idx_bits)
+    o = _lib.default_opts()
+    o.seed = case["seed"]
+    o.max_features = case["m"]
+    o.min_samples_split = case["mss"]
+    o.max_depth = case["max_depth"]
+    o.identity_perm = int(case["identity"])
+    o.mode = _lib.MODES[mode]
+    o.bfs_threshold = case.get("bfs_threshold", 0)
+    o.scan_chunk = scan_chunk
+    o.resh_chunk = resh_chunk
+    o.trees_per_batch = tpb
+    f = _lib.Forest.fit(ctx, ds, case["n_trees"], o, helpers.case_boot_ids(case))
+    return f, ds
+
+
+# ---- presort -------------------------------------------------------------
+@pytest.mark.parametrize("N,idx_bits", [(1, 0), (37, 0), (5000, 0), (65535, 0), (70001, 0), (3000, 32)])
+def test_presort_stable(ctx, N, idx_bits):
+    print("presort N=%d idx_bits=%d" % (N, idx_bits), flush=True)
+    rng = np.random.default_rng(N)
+    F = 5
+    X = np.empty((N, F), np.float32)
+    X[:, 0] = rng.integers(0, 7, N)
+    X[:, 1] = rng.normal(size=N)
+    X[:, 2] = rng.choice([-0.0, 0.0, 1.0, -1.0, np.inf, -np.inf, 1e-38, -3e38], size=N)
+    X[:, 3] = 5.0
+    X[:, 4] = -rng.exponential(size=N)
+    y = rng.integers(0, 3, N)
+    ds = _lib.Dataset(ctx, X, y, 3, idx_bits=idx_bits)
+    assert ds.idx_bits == (idx_bits or (16 if N <= 65535 else 32))
+    got = ds.sorted_rows()
+    for f in range(F):
+        assert np.array_equal(got[f], np.argsort(X[:, f], kind="stable")), f
+
+
+def test_dataset_rejects_nan_and_bad_labels(ctx):
+    X = np.zeros((10, 2), np.float32)
+    X[3, 1] = np.nan
+    with pytest.raises(_lib.CtaError):
+        _lib.Dataset(ctx, X, np.zeros(10), 2)
+    with pytest.raises(_lib.CtaError):
+        _lib.Dataset(ctx, np.zeros((10, 2), np.float32), np.full(10, 5), 2)
+
+
+# ---- kernel hooks ----------------------------------------------------------
+def _node_rows(X, samples):
+    """Sorted rows of a node: each feature's samples ordered by (value, id)."""
+    F = X.shape[1]
+    return np.stack([np.array(sorted(samples, key=lambda i: (float(X[i, f]), i)), np.uint32) for f in range(F)])
+
+
+def _oracle_node_best(orc, X, y, K, rows, feats):
+    best = None
+    for f in feats:
+        r = orc.scan_feature(X[rows[f], f], y[rows[f]], K)
+        if r is None:
+            continue
+        p, num, den = r
+        key = (num, den, f, p)
+        if best is None:
+            best = key
+        else:
+            a = num * best[1]
+            b = best[0] * den
+            if a > b or (a == b and f < best[2]):
+                best = key
+    return None if best is None else (best[2], best[3], best[0], best[1])
+
+
+@pytest.mark.parametrize("chunk", [0, 64, 128, 200])
+def test_split_scan_hook(ctx, orc, chunk):
+    rng = np.random.default_rng(chunk + 1)
+    for trial in range(12):
+        N = int(rng.integers(50, 3000))
+        F = int(rng.integers(1, 9))
+        K = int(rng.choice([2, 3, 7, 70, 300]))
+        X = rng.integers(0, int(rng.choice([3, 50, 1000])), size=(N, F)).astype(np.float32)
+        y = rng.integers(0, K, N)
+        ds = _lib.Dataset(ctx, X, y, K)
+        samples = sorted(rng.choice(N, size=int(rng.integers(2, N + 1)), replace=False).tolist())
+        rows = _node_rows(X, samples)
+        feats = rng.permutation(F)[: int(rng.integers(1, F + 1))]
+        print("split_scan chunk=%d trial=%d N=%d F=%d K=%d n=%d" % (chunk, trial, N, F, K, len(samples)),
+              flush=True)
+        got = _lib.split_scan(ctx, ds, rows, feats, chunk=chunk)
+        want = _oracle_node_best(orc, X, y, K, rows, feats)
+        assert got == want, (trial, got, want)
+
+
+@pytest.mark.parametrize("chunk", [0, 64, 100])
+def test_reshuffle_hook(ctx, chunk):
+    rng = np.random.default_rng(chunk + 11)
+    for trial in range(8):
+        N = int(rng.integers(10, 5000))
+        F = int(rng.integers(1, 6))
+        X = rng.normal(size=(N, F)).astype(np.float32)
+        ds = _lib.Dataset(ctx, X, np.zeros(N, np.int64), 1)
+        samples = sorted(rng.choice(N, size=int(rng.integers(1, N + 1)), replace=False).tolist())
+        rows = _node_rows(X, samples)
+        mark = (rng.random(N) < rng.random()).astype(np.uint8)
+        print("reshuffle chunk=%d trial=%d N=%d n=%d" % (chunk, trial, N, len(samples)), flush=True)
+        got = _lib.reshuffle(ctx, ds, rows, mark, chunk=chunk)
+        for f in range(F):
+            r = rows[f]
+            want = np.concatenate([r[mark[r] == 1], r[mark[r] == 0]])
+            assert np.array_equal(got[f], want), (trial, f)
+
+
+# ---- whole forests -----------------------------------------------------------
+TINY = list(range(60))
+
+
+@pytest.mark.parametrize("mode,chunk", [("batched", 0), ("batched", 64), ("per_node", 64), ("hybrid", 64)])
+def test_forest_tiny_cases(ctx, orc, mode, chunk):
+    for s in TINY:
+        case = datasets.tiny_case(s)
+        case["bfs_threshold"] = 8
+        print("tiny forest mode=%s chunk=%d seed=%d" % (mode, chunk, s), flush=True)
+        f, _ = gpu_forest(ctx, case, mode=mode, scan_chunk=chunk, resh_chunk=chunk)
+        d = helpers.forest_diff(f.export(), helpers.oracle_forest(case, orc))
+        assert d is None, "seed %d: %s" % (s, d)
+
+
+def test_forest_many_classes_u16_labels(ctx, orc):
+    rng = np.random.default_rng(9)
+    N, F, K = 3000, 6, 300
+    X = rng.integers(0, 40, size=(N, F)).astype(np.float32)
+    y = rng.integers(0, K, N)
+    case = {"X": X, "y": y, "K": K, "m": 3, "mss": 1, "max_depth": 0, "identity": False, "seed": 5,
+            "boot": True, "n_trees": 3}
+    print("forest K=300", flush=True)
+    f, _ = gpu_forest(ctx, case, scan_chunk=256, resh_chunk=256)
+    assert helpers.forest_diff(f.export(), helpers.oracle_forest(case, orc)) is None
+
+
+def test_forest_digits_like_config(ctx, orc):
+    """1,797 x 64, 10 classes, bootstrap=False, m=8, 50 trees."""
+    X, y = datasets.digits_like()
+    case = {"X": X, "y": y, "K": 10, "m": 8, "mss": 1, "max_depth": 0, "identity": False, "seed": 2024,
+            "boot": False, "n_trees": 50}
+    print("digits-like 50 trees", flush=True)
+    f, _ = gpu_forest(ctx, case)
+    got = f.export()
+    want = helpers.oracle_forest(case, orc)
+    assert helpers.forest_diff(got, want) is None
+    assert np.array_equal(f.predict(X), orc.predict(want, X, 10))
+    assert np.array_equal(f.predict(X), y)   # memorised (bootstrap=False)
+
+
+@pytest.mark.parametrize("idx_bits", [0, 32])
+def test_forest_tabular_u32_indices(ctx, orc, idx_bits):
+    """covtype-shaped data above 65,535 rows (32-bit indices), bootstrap."""
+    X, y, _, _ = datasets.covtype_like(seed=7, n_train=70000, n_test=1)
+    case = {"X": X, "y": y, "K": 7, "m": 7, "mss": 1, "max_depth": 0, "identity": False, "seed": 99,
+            "boot": True, "n_trees": 3}
+    print("tabular 70000x54 idx_bits=%d" % idx_bits, flush=True)
+    f, ds = gpu_forest(ctx, case, idx_bits=idx_bits)
+    assert ds.idx_bits == 32
+    got = f.export()
+    assert helpers.forest_diff(got, helpers.oracle_forest(case, orc)) is None
+
+
+def test_modes_and_batching_identical(ctx, orc):
+    X, y, _, _ = datasets.covtype_like(seed=3, n_train=20000, n_test=1)
+    case = {"X": X, "y": y, "K": 7, "m": 5, "mss": 2, "max_depth": 0, "identity": False, "seed": 17,
+            "boot": True, "n_trees": 4, "bfs_threshold": 3000}
+    want = helpers.oracle_forest(case, orc)
+    for mode, tpb, chunk in [("batched", 0, 0), ("batched", 1, 512), ("batched", 3, 0), ("hybrid", 0, 1024),
+                             ("per_node", 0, 0)]:
+        print("modes: %s tpb=%d chunk=%d" % (mode, tpb, chunk), flush=True)
+        f, _ = gpu_forest(ctx, case, mode=mode, tpb=tpb, scan_chunk=chunk, resh_chunk=chunk)
+        assert helpers.forest_diff(f.export(), want) is None, (mode, tpb, chunk)
+
+
+def test_max_depth_and_min_samples_split(ctx, orc):
+    X, y, _, _ = datasets.covtype_like(seed=5, n_train=8000, n_test=1)
+    for md, mss in [(1, 1), (4, 1), (6, 20), (0, 50)]:
+        case = {"X": X, "y": y, "K": 7, "m": 7, "mss": mss, "max_depth": md, "identity": False, "seed": 1,
+                "boot": True, "n_trees": 2}
+        print("max_depth=%d mss=%d" % (md, mss), flush=True)
+        f, _ = gpu_forest(ctx, case)
+        assert helpers.forest_diff(f.export(), helpers.oracle_forest(case, orc)) is None
+
+
+# ---- prediction ----------------------------------------------------------------
+def test_predict_matches_oracle_and_import(ctx, orc):
+    X, y, Xt, yt = datasets.covtype_like(seed=11, n_train=10000, n_test=5000)
+    case = {"X": X, "y": y, "K": 7, "m": 7, "mss": 1, "max_depth": 0, "identity": False, "seed": 8,
+            "boot": True, "n_trees": 7}
+    f, _ = gpu_forest(ctx, case)
+    arrays = f.export()
+    want = orc.predict(arrays, Xt, 7, return_votes=True)
+    got = f.predict(Xt, votes=True)
+    assert np.array_equal(got[0], want[0]) and np.array_equal(got[1], want[1])
+    g = _lib.Forest.from_arrays(ctx, arrays, 54, 7)
+    got2 = g.predict(Xt, votes=True)
+    assert np.array_equal(got2[0], want[0]) and np.array_equal(got2[1], want[1])
+    # a forest of oracle-built trees predicts identically through the device
+    o_arr = helpers.oracle_forest(case, orc)
+    h = _lib.Forest.from_arrays(ctx, o_arr, 54, 7)
+    assert np.array_equal(h.predict(Xt), orc.predict(o_arr, Xt, 7))
+
+
+def test_import_rejects_malformed(ctx):
+    bad = {"tree_off": np.array([0, 3]), "feature": np.array([0, 0, 0]), "threshold": np.zeros(3, np.float32),
+           "left": np.array([2, 0, 0]), "right": np.array([1, 0, 0]), "value": np.array([0, 1, 0])}
+    with pytest.raises(_lib.CtaError):
+        _lib.Forest.from_arrays(ctx, bad, 1, 2)
+
+
+# ---- facade ----------------------------------------------------------------------
+@pytest.mark.parametrize("name", ["digits", "iris", "diabetes"])
+def test_facade_memorize(name):
+    """The reference's memorize tests through the CudaTree-style API."""
+    import cudatree_amd
+
+    x, y = cudatree_amd.load_data(name)
+    forest = cudatree_amd.RandomForestClassifier(n_estimators=6, bootstrap=False, random_state=3)
+    forest.fit(x, y)
+    assert np.array_equal(forest.predict(x), y)
+
+
+def test_facade_pickle_and_proba():
+    import pickle
+
+    import cudatree_amd
+
+    x, y = cudatree_amd.load_data("digits")
+    labels = np.array(["c%d" % v for v in y])
+    f = cudatree_amd.RandomForestClassifier(n_estimators=9, random_state=4).fit(x, labels)
+    p = f.predict(x)
+    assert p.dtype.kind == "U" and np.mean(p == labels) > 0.97
+    g = pickle.loads(pickle.dumps(f))
+    assert np.array_equal(g.predict(x), p)
+    proba = g.predict_proba(x[:100])
+    assert proba.shape == (100, 10) and np.allclose(proba.sum(axis=1), 1.0)
+    assert np.array_equal(f.compt_table[np.argmax(proba, axis=1)], p[:100])
