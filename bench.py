@@ -1,0 +1,216 @@
+"""Benchmark: 50-tree random-forest fit at 581,012 x 54 and batched predict on
+one MI355X, with the split-scan / reshuffle roofline and CPU baselines.
+
+    python bench.py [--gpus 1] [--steps 3] [--warmup 1]
+
+A step is one complete fit of 50 trees (device transpose + presort, bootstrap
+compaction, every level of every tree, export of the canonical trees) on the
+covtype-shaped synthetic set with its inputs already resident in HBM.  Prints
+one JSON line.  Only one GPU is used; --gpus above 1 exits with a message.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "fit s/50 trees @581k×54; predict rows/s + p99 latency; HBM GB/s fraction"
+HBM_PEAK_GBS = 8000.0
+N_TREES = 50
+MAX_FEATURES = 7
+SEED = 20240601
+
+
+def log(msg):
+    print("[bench] " + msg, file=sys.stderr, flush=True)
+
+
+def fit_once(ctx, lib, Xd, yd, N, F, K, ids, mode="batched", n_trees=N_TREES):
+    ds_h = lib.load()
+    import ctypes
+
+    h = ctypes.c_void_p()
+    lib.check(ds_h.cta_dataset_create_device(ctx.h, ctypes.c_void_p(Xd.data_ptr()), ctypes.c_void_p(yd.data_ptr()),
+                                             N, F, K, 0, ctypes.byref(h)))
+    ds = lib.Dataset.__new__(lib.Dataset)
+    ds.ctx, ds.lib, ds.h, ds.N, ds.F, ds.K = ctx, ds_h, h, N, F, K
+    o = lib.default_opts()
+    o.seed = SEED
+    o.max_features = MAX_FEATURES
+    o.mode = lib.MODES[mode]
+    forest = lib.Forest.fit(ctx, ds, n_trees, o, ids)
+    arrays = forest.export()
+    ds.close()
+    return forest, arrays
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--cpu-trees", type=int, default=3, help="trees in the bounded CPU-oracle sample")
+    ap.add_argument("--per-node-trees", type=int, default=1, help="trees in the per-node GPU sample (0: skip)")
+    ap.add_argument("--sklearn-trees", type=int, default=8, help="trees in the scikit-learn sample (0: skip)")
+    ap.add_argument("--no-profile", action="store_true", help="skip per-kernel HIP-event timing")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one MI355X: this build has no multi-GPU path (trees would shard across GPUs "
+              "with one gather of the node tables); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    from cudatree_amd import _lib as lib
+    from cudatree_amd import datasets
+
+    t0 = time.time()
+    X, y, Xt, yt = datasets.covtype_like()
+    N, F = X.shape
+    K = int(y.max()) + 1
+    ids = datasets.bootstrap_ids(N, N_TREES, SEED)
+    log("data %.1fs: N=%d F=%d K=%d, mean in-bag %.0f" % (time.time() - t0, N, F, K, np.mean([len(i) for i in ids])))
+
+    ctx = lib.default_context(0)
+    dev = torch.device("cuda:0")
+    Xd = torch.from_numpy(X).to(dev)
+    yd = torch.from_numpy(y.astype(np.uint32).view(np.int32)).to(dev)
+    Xtd = torch.from_numpy(Xt).to(dev)
+    torch.cuda.synchronize()
+
+    for w in range(args.warmup):
+        t = time.time()
+        forest, arrays = fit_once(ctx, lib, Xd, yd, N, F, K, ids)
+        log("warmup %d: %.3fs, %d nodes" % (w, time.time() - t, len(arrays["feature"])))
+        del forest
+
+    if not args.no_profile:
+        ctx.set_profiling(True)
+    ctx.reset_stats()
+    ctx.synchronize()
+    torch.cuda.synchronize()
+    t_start = time.perf_counter()
+    for s in range(args.steps):
+        forest, arrays = fit_once(ctx, lib, Xd, yd, N, F, K, ids)
+    ctx.synchronize()
+    torch.cuda.synchronize()
+    t_fit = (time.perf_counter() - t_start) / args.steps
+    stats = ctx.kernel_stats()
+    ctx.set_profiling(False)
+    log("fit: %.4f s per 50 trees (%d nodes)" % (t_fit, len(arrays["feature"])))
+    for k, v in sorted(stats.items(), key=lambda kv: -kv[1]["ms"]):
+        gbs = v["alg_bytes"] / (v["ms"] * 1e-3) / 1e9 if v["ms"] > 0 and v["alg_bytes"] else 0.0
+        log("  %-16s %8d launches %10.3f ms  %8.1f GB/s alg" % (k, v["launches"], v["ms"] / args.steps, gbs))
+
+    # roofline of the dominant HBM kernel (reshuffle), HIP events over the timed steps
+    roof = None
+    if "k_resh" in stats and stats["k_resh"]["ms"] > 0:
+        s = stats["k_resh"]
+        per_launch_bytes = s["alg_bytes"] / s["launches"]
+        per_launch_s = s["ms"] * 1e-3 / s["launches"]
+        ach = per_launch_bytes / per_launch_s / 1e9
+        roof = {"bound": "hbm", "kernel": "k_resh", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
+                "launches": s["launches"], "avg_launch_us": round(per_launch_s * 1e6, 2),
+                "alg_bytes_per_launch": int(per_launch_bytes)}
+        if "k_scan" in stats and stats["k_scan"]["ms"] > 0:
+            sc = stats["k_scan"]
+            roof["scan_achieved"] = round(sc["alg_bytes"] / (sc["ms"] * 1e-3) / 1e9, 1)
+
+    # ---- predict: 100,000 held-out rows resident in HBM
+    R = Xt.shape[0]
+    labels = torch.empty(R, dtype=torch.int32, device=dev)
+    import ctypes
+
+    def pred(nrows):
+        lib.check(lib.load().cta_predict_device(ctx.h, forest.h, ctypes.c_void_p(Xtd.data_ptr()), nrows, F,
+                                                ctypes.c_void_p(labels.data_ptr()), None))
+
+    pred(R)
+    t = time.perf_counter()
+    reps = 10
+    for _ in range(reps):
+        pred(R)
+    t_pred = (time.perf_counter() - t) / reps
+    got = labels.cpu().numpy().astype(np.uint32)
+    acc = float(np.mean(got == yt))
+    # latency per request, log-uniform 1..65,536 rows, one request per launch
+    rng = np.random.default_rng(7)
+    sizes = np.exp(rng.uniform(0, np.log(65536), size=200)).astype(int)
+    lat = []
+    for n in sizes:
+        t = time.perf_counter()
+        pred(int(n))
+        lat.append((time.perf_counter() - t) * 1e3)
+    lat = np.array(lat)
+    predict = {"rows": R, "rows_per_s": round(R / t_pred, 1), "ms_per_batch": round(t_pred * 1e3, 3),
+               "accuracy": round(acc, 4), "request_rows": "log-uniform 1..65536, 200 requests",
+               "p50_ms": round(float(np.percentile(lat, 50)), 3), "p99_ms": round(float(np.percentile(lat, 99)), 3)}
+    log("predict: %.0f rows/s, acc %.4f, p50 %.3f ms p99 %.3f ms" % (predict["rows_per_s"], acc, predict["p50_ms"],
+                                                                    predict["p99_ms"]))
+
+    # ---- same kernels, one node per launch sequence (bounded sample)
+    per_node = None
+    if args.per_node_trees > 0:
+        nt = args.per_node_trees
+        t = time.perf_counter()
+        f2, a2 = fit_once(ctx, lib, Xd, yd, N, F, K, ids[:nt], mode="per_node", n_trees=nt)
+        dt = time.perf_counter() - t
+        same = all(np.array_equal(a2[k], arrays[k][: len(a2[k])]) for k in ("feature", "left", "right", "value"))
+        per_node = {"sample": "%d of %d trees, per-node launches, scaled x%g" % (nt, N_TREES, N_TREES / nt),
+                    "s_per_50_trees": round(dt * N_TREES / nt, 3), "speedup_batched": round(dt * N_TREES / nt / t_fit, 2),
+                    "identical_trees": bool(same)}
+        log("per-node: %.2fs for %d tree(s) -> %.1fs/50, batched speedup %.1fx" % (dt, nt, dt * N_TREES / nt,
+                                                                                 dt * N_TREES / nt / t_fit))
+        del f2
+
+    # ---- CPU baselines (bounded samples on the host cores)
+    cpu = None
+    if args.cpu_trees > 0:
+        import oracle
+
+        o = oracle.get()
+        nt = args.cpu_trees
+        t = time.perf_counter()
+        fo = o.fit(X, y, K, nt, MAX_FEATURES, boot_ids=ids[:nt], seed=SEED)
+        dt = time.perf_counter() - t
+        same = all(np.array_equal(fo[k], arrays[k][: len(fo[k])]) for k in ("feature", "left", "right", "value"))
+        cpu = {"value": round(dt * N_TREES / nt, 2), "unit": "s/50 trees", "cores": 1, "kind": "port",
+               "sample": "%d of %d trees of the same workload, scaled x%.3g" % (nt, N_TREES, N_TREES / nt),
+               "identical_trees": bool(same)}
+        log("cpu oracle: %.2fs for %d trees -> %.1fs/50 (identical: %s)" % (dt, nt, dt * N_TREES / nt, same))
+    sk = None
+    if args.sklearn_trees > 0:
+        from sklearn.ensemble import RandomForestClassifier
+
+        cores = len(os.sched_getaffinity(0))
+        nt = args.sklearn_trees
+        t = time.perf_counter()
+        rf = RandomForestClassifier(n_estimators=nt, max_features=MAX_FEATURES, n_jobs=-1, random_state=0).fit(X, y)
+        dt = time.perf_counter() - t
+        t = time.perf_counter()
+        rf.predict(Xt)
+        dp = time.perf_counter() - t
+        sk = {"fit_s_per_50_trees": round(dt * N_TREES / nt, 2), "cores": cores,
+              "sample": "%d trees scaled x%.3g" % (nt, N_TREES / nt),
+              "predict_rows_per_s_%d_trees" % nt: round(R / dp, 1)}
+        log("sklearn: %.2fs for %d trees on %d cores -> %.1fs/50" % (dt, nt, cores, dt * N_TREES / nt))
+
+    out = {"metric": METRIC, "value": round(t_fit, 4), "unit": "s/50 trees", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": round(t_fit * 1e3, 2), "higher_is_better": False,
+           "scaling": "weak", "vs_baseline": None, "dtype": "u64", "data": "synthetic",
+           "config": {"workload": "covtype-shaped 581,012 x 54 float32, 7 classes; 50 trees, bootstrap, "
+                                  "max_features=7, full depth; predict 100,000 held-out rows",
+                      "n_samples": N, "n_features": F, "n_classes": K, "n_trees": N_TREES,
+                      "nodes": int(len(arrays["feature"])), "mode": "batched"},
+           "roofline": roof, "cpu_baseline": cpu, "predict": predict, "per_node_gpu": per_node, "sklearn": sk}
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()
