@@ -42,7 +42,8 @@ class FitOpts(ctypes.Structure):
         ("scan_chunk", ctypes.c_int32),
         ("resh_chunk", ctypes.c_int32),
         ("trees_per_batch", ctypes.c_int32),
-        ("reserved", ctypes.c_int32 * 5),
+        ("resh_global", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 4),
     ]
 
 
@@ -85,7 +86,8 @@ SIGNATURES = {
     "cta_ctx_kernel_stats": (ctypes.c_int, [_vp, ctypes.POINTER(KernelStat), ctypes.c_int32, _i32p]),
     "cta_k_split_scan": (ctypes.c_int, [_vp, _vp, _u32p, ctypes.c_uint32, _i32p, ctypes.c_int32, ctypes.c_int32,
                                         _i32p, _u32p, _u64p, _u64p]),
-    "cta_k_reshuffle": (ctypes.c_int, [_vp, _vp, _u32p, ctypes.c_uint32, _u8p, ctypes.c_int32, _u32p]),
+    "cta_k_reshuffle": (ctypes.c_int, [_vp, _vp, _u32p, ctypes.c_uint32, _u8p, ctypes.c_int32, ctypes.c_int32,
+                                       _u32p]),
 }
 
 
@@ -315,11 +317,11 @@ def split_scan(ctx, ds, rows, features, chunk=0):
     return bf.value, bp.value, num.value, den.value
 
 
-def reshuffle(ctx, ds, rows, mark, chunk=0):
+def reshuffle(ctx, ds, rows, mark, chunk=0, global_bits=False):
     """Device stable partition of one node's rows (test hook)."""
     rows = np.ascontiguousarray(rows, dtype=np.uint32)
     mark = np.ascontiguousarray(mark, dtype=np.uint8)
     out = np.zeros_like(rows)
     check(ctx.lib.cta_k_reshuffle(ctx.h, ds.h, ptr(rows, _u32p), rows.shape[1], ptr(mark, _u8p), chunk,
-                                  ptr(out, _u32p)))
+                                  int(bool(global_bits)), ptr(out, _u32p)))
     return out

@@ -114,6 +114,7 @@ struct cta_ctx {
   hipStream_t stream = nullptr;
   std::recursive_mutex mu;
   int num_cu = 256;
+  size_t lds_per_block = 65536;
   bool profiling = false;
   std::map<std::string, KStat> stats;
   struct Pending {
@@ -245,7 +246,7 @@ struct Fitter {
     K = d->K;
     m = opts.max_features;
     Cs = opts.scan_chunk > 0 ? (uint32_t)opts.scan_chunk : 8192u;
-    Cr = opts.resh_chunk > 0 ? (uint32_t)opts.resh_chunk : 8192u;
+    Cr = opts.resh_chunk > 0 ? (uint32_t)opts.resh_chunk : 2048u;
     Cf = 8192u;
     st = c->stream;
     G = (uint32_t)std::max(64, ctx->num_cu * 8);
@@ -255,7 +256,7 @@ struct Fitter {
 
   // Bytes of device memory one resident tree needs (rows, marks, level state).
   size_t bytes_per_tree() const {
-    const size_t rows = 2 * (size_t)F * N * sizeof(IdxT) + N;
+    const size_t rows = 2 * (size_t)F * N * sizeof(IdxT) + N / 8 + 64;
     const size_t per_sample = 2 * sizeof(FNode) + 3 * 4 * (size_t)K + sizeof(NAux) + 4 * (size_t)m +
                               2 * 7 * 4 + (sizeof(Task) + sizeof(ScanRes)) + 2 * sizeof(Task);
     return rows + per_sample * (size_t)N / 2 + 4096;
@@ -274,20 +275,23 @@ struct Fitter {
     const uint64_t cap_q = n_tot / 2 + Tb + 64;
     const uint64_t cap_nodes = 2 * n_tot + Tb + 64;
     const uint64_t cap_scan = cap_q + 3ull * m * n_tot / Cs + 64;
-    const uint64_t cap_resh = std::max<uint64_t>(cap_q + 3ull * F * n_tot / Cr + 64, (uint64_t)Tb * F * (N / Cr + 2));
+    const uint64_t cap_big = (uint64_t)F * (n_tot / kBigN + Tb + 1);
+    const uint64_t cap_small = cap_q + 2ull * F * n_tot / Cr + 64;
     const uint64_t cap_fill = cap_q + n_tot / Cf + 64;
     const uint64_t cap_hist = (uint64_t)m * K * (2 * n_tot / Cs + 64);
-    const uint64_t cap_rcnt = std::max<uint64_t>((uint64_t)F * (2 * n_tot / Cr + 64), (uint64_t)Tb * F * (N / Cr + 2));
-    if (cap_scan > 0xFFFFFFF0ull || cap_resh > 0xFFFFFFF0ull || cap_nodes > 0xFFFFFFF0ull || cap_hist > 0xFFFFFFF0ull)
+    if (cap_scan > 0xFFFFFFF0ull || cap_small > 0xFFFFFFF0ull || cap_big > 0xFFFFFFF0ull ||
+        cap_nodes > 0xFFFFFFF0ull || cap_hist > 0xFFFFFFF0ull)
       fail(CTA_EUNSUPPORTED, "batch of %d trees too large for 32-bit task indices", Tb);
 
     DevMem buf0((size_t)Tb * F * N * sizeof(IdxT)), buf1((size_t)Tb * F * N * sizeof(IdxT));
-    DevMem mark((size_t)Tb * N);
+    const uint32_t W = ((uint32_t)N + 31u) / 32u;
+    DevMem bits((size_t)Tb * W * 4);
+    DevMem plan_cnt((size_t)Tb * 4 * 2), plan_elems((size_t)Tb * 8), plan_off((size_t)(Tb + 1) * 4 * 3);
     DevMem front[2] = {DevMem(cap_q * sizeof(FNode)), DevMem(cap_q * sizeof(FNode))};
     DevMem cnt[2] = {DevMem(cap_q * K * 4), DevMem(cap_q * K * 4)};
     DevMem cntL(cap_q * K * 4), aux(cap_q * sizeof(NAux)), feats(cap_q * m * 4);
     DevMem scan_tasks(cap_scan * sizeof(Task)), res(cap_scan * sizeof(ScanRes)), hist(cap_hist * 4);
-    DevMem fill_tasks(cap_fill * sizeof(Task)), resh_tasks(cap_resh * sizeof(Task)), rcnt(cap_rcnt * 4);
+    DevMem fill_tasks(cap_fill * sizeof(Task)), big_tasks(cap_big * sizeof(Task)), small_tasks(cap_small * sizeof(Task));
     DevMem a_feat(cap_nodes * 4), a_thr(cap_nodes * 4), a_left(cap_nodes * 4), a_right(cap_nodes * 4),
         a_value(cap_nodes * 4), a_nsamp(cap_nodes * 4), a_depth(cap_nodes * 4);
     DevMem ctl(sizeof(Ctl)), root_cnt((size_t)Tb * K * 4), d_off((Tb + 1) * 8);
@@ -299,7 +303,9 @@ struct Fitter {
     a.si0 = ds->si0.as<IdxT>();
     a.buf0 = buf0.as<IdxT>();
     a.buf1 = buf1.as<IdxT>();
-    a.mark = mark.as<uint8_t>();
+    a.bits = bits.as<uint32_t>();
+    a.W = W;
+    a.T = (uint32_t)Tb;
     a.N = N;
     a.F = F;
     a.K = K;
@@ -319,8 +325,14 @@ struct Fitter {
     a.res = res.as<ScanRes>();
     a.hist = hist.as<uint32_t>();
     a.fill_tasks = fill_tasks.as<Task>();
-    a.resh_tasks = resh_tasks.as<Task>();
-    a.rcnt = rcnt.as<uint32_t>();
+    a.big_tasks = big_tasks.as<Task>();
+    a.small_tasks = small_tasks.as<Task>();
+    a.cnt_big = plan_cnt.as<uint32_t>();
+    a.cnt_small = plan_cnt.as<uint32_t>() + Tb;
+    a.elems = plan_elems.as<unsigned long long>();
+    a.big_off = plan_off.as<uint32_t>();
+    a.small_off = plan_off.as<uint32_t>() + (Tb + 1);
+    a.wg_off = plan_off.as<uint32_t>() + 2 * (Tb + 1);
     a.arena = Arena{a_feat.as<int32_t>(), a_thr.as<float>(), a_left.as<uint32_t>(), a_right.as<uint32_t>(),
                     a_value.as<uint32_t>(), a_nsamp.as<uint32_t>(), a_depth.as<uint32_t>()};
     a.ctl = ctl.as<Ctl>();
@@ -328,8 +340,8 @@ struct Fitter {
     a.cap_scan = (uint32_t)cap_scan;
     a.cap_hist = (uint32_t)cap_hist;
     a.cap_fill = (uint32_t)cap_fill;
-    a.cap_resh = (uint32_t)cap_resh;
-    a.cap_rcnt = (uint32_t)cap_rcnt;
+    a.cap_big = (uint32_t)cap_big;
+    a.cap_small = (uint32_t)cap_small;
     a.cap_nodes = (uint32_t)cap_nodes;
 
     // ---- setup: bootstrap compaction and roots
@@ -338,6 +350,8 @@ struct Fitter {
     HIPC(hipMemcpyAsync(ctl.p, &hc, sizeof(Ctl), hipMemcpyHostToDevice, st));
     HIPC(hipMemcpyAsync(d_off.p, off.data(), (Tb + 1) * 8, hipMemcpyHostToDevice, st));
     k_fill_u32<<<grid_for((size_t)Tb * K), 256, 0, st>>>(root_cnt.as<uint32_t>(), (size_t)Tb * K, 0u);
+    k_fill_u32<<<1, 256, 0, st>>>(plan_cnt.as<uint32_t>(), (size_t)Tb * 2, 0u);
+    k_fill_u32<<<1, 256, 0, st>>>(plan_elems.as<uint32_t>(), (size_t)Tb * 2, 0u);
     uint32_t src_root = 0;
     if (boot) {
       std::vector<uint32_t> ids(n_tot);
@@ -345,23 +359,16 @@ struct Fitter {
         std::memcpy(ids.data() + off[t], boot_ids + boot_off[t0 + t], (off[t + 1] - off[t]) * 4);
       d_ids.alloc(n_tot * 4);
       HIPC(hipMemcpyAsync(d_ids.p, ids.data(), n_tot * 4, hipMemcpyHostToDevice, st));
-      k_fill_u8<<<grid_for((size_t)Tb * N), 256, 0, st>>>(mark.as<uint8_t>(), (size_t)Tb * N, 0);
+      k_fill_u32<<<grid_for((size_t)Tb * W), 256, 0, st>>>(bits.as<uint32_t>(), (size_t)Tb * W, 0u);
       k_boot_mark<<<dim3(grid_for(N, 256, 1024), Tb), 256, 0, st>>>(d_ids.as<uint32_t>(), d_off.as<uint64_t>(), Tb,
-                                                                    N, mark.as<uint8_t>());
+                                                                    W, bits.as<uint32_t>());
       k_root_counts<LabT><<<dim3(grid_for(N, 256, 256), Tb), 256, K * 4, st>>>(
           a.y, d_ids.as<uint32_t>(), d_off.as<uint64_t>(), Tb, N, K, root_cnt.as<uint32_t>());
       a.front = front[0].as<FNode>();
+      k_ctl_set<<<1, 64, 0, st>>>(a.ctl, (uint32_t)Tb);
       k_boot_tasks<IdxT, LabT><<<grid_for((size_t)Tb * 64), 256, 0, st>>>(a, d_off.as<uint64_t>(), Tb,
                                                                           front[0].as<FNode>(), 0u);
-      {
-        Timed tm(ctx, "k_boot_count");
-        k_resh_count<IdxT, LabT><<<G, kBlock, 0, st>>>(a);
-      }
-      k_resh_carry<IdxT, LabT><<<G, kBlock, 0, st>>>(a);
-      {
-        Timed tm(ctx, "k_boot_compact");
-        k_resh<IdxT, LabT><<<G, kBlock, 0, st>>>(a);
-      }
+      reshuffle(a, "k_boot_compact");
       src_root = 1;
     } else {
       k_root_counts<LabT><<<dim3(grid_for(N, 256, 256), Tb), 256, K * 4, st>>>(a.y, nullptr, nullptr, Tb, N, K,
@@ -481,17 +488,29 @@ struct Fitter {
       Timed tm(ctx, "k_children");
       k_children<IdxT, LabT><<<G, kBlock, sm, st>>>(a);
     }
-    {
-      Timed tm(ctx, "k_resh_count");
-      k_resh_count<IdxT, LabT><<<G, kBlock, 0, st>>>(a);
-    }
-    k_resh_carry<IdxT, LabT><<<G, kBlock, 0, st>>>(a);
-    {
-      Timed tm(ctx, "k_resh");
-      k_resh<IdxT, LabT><<<G, kBlock, 0, st>>>(a);
-    }
+    reshuffle(a, "k_resh");
     k_ctl_next<<<1, 64, 0, st>>>(a.ctl);
     HIPC(hipGetLastError());
+  }
+
+  // Plan, task lists and the partition of every row of the split nodes.
+  void reshuffle(const FitArgs<IdxT, LabT> &a, const char *name) {
+    k_plan<IdxT, LabT><<<1, 64, 0, st>>>(a);
+    k_emit<IdxT, LabT><<<G, kBlock, 0, st>>>(a);
+    const size_t lds = (((size_t)a.W + 3) & ~(size_t)3) * 4 + 2 * 64 * 4;
+    const uint32_t GR = (uint32_t)std::max(8, ctx->num_cu * 2);
+    Timed tm(ctx, name);
+    if (!o.resh_global && lds <= ctx->lds_per_block) {
+      static bool attr = false;
+      if (!attr) {
+        HIPC(hipFuncSetAttribute((const void *)k_resh<IdxT, LabT, true>,
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)ctx->lds_per_block));
+        attr = true;
+      }
+      k_resh<IdxT, LabT, true><<<GR, kReshThreads, lds, st>>>(a);
+    } else {
+      k_resh<IdxT, LabT, false><<<GR, kReshThreads, 2 * 64 * 4, st>>>(a);
+    }
   }
 
   void pull_frontier(const DevMem &fr, const DevMem &c, uint32_t q, std::vector<HostEntry> &out) {
@@ -662,6 +681,7 @@ int cta_ctx_create(int device, cta_ctx **out) {
     hipDeviceProp_t p;
     HIPC(hipGetDeviceProperties(&p, device));
     c->num_cu = p.multiProcessorCount > 0 ? p.multiProcessorCount : 256;
+    c->lds_per_block = p.sharedMemPerBlock > 0 ? p.sharedMemPerBlock : 65536;
     *out = c.release();
   });
 }
@@ -1137,46 +1157,46 @@ extern "C" int cta_k_split_scan(cta_ctx *ctx, cta_dataset *ds, const uint32_t *r
 
 template <class IdxT, class LabT>
 static void hook_reshuffle(cta_ctx *ctx, cta_dataset *ds, const uint32_t *rows, uint32_t n, const uint8_t *mark,
-                           int32_t chunk, uint32_t *out) {
+                           int32_t chunk, int32_t global_bits, uint32_t *out) {
   const int32_t N = ds->N, F = ds->F, K = ds->K;
   hipStream_t st = ctx->stream;
-  const uint32_t Cr = chunk > 0 ? (uint32_t)chunk : 8192u;
+  const uint32_t Cr = chunk > 0 ? (uint32_t)chunk : 2048u;
+  const uint32_t W = ((uint32_t)N + 31u) / 32u;
   std::vector<IdxT> hrows((size_t)F * N, 0);
   for (int32_t f = 0; f < F; ++f)
     for (uint32_t i = 0; i < n; ++i) hrows[(size_t)f * N + i] = (IdxT)rows[(size_t)f * n + i];
+  std::vector<uint32_t> hbits(W, 0);
+  for (int32_t s = 0; s < N; ++s)
+    if (mark[s]) hbits[s >> 5] |= 1u << (s & 31);
   uint32_t nL = 0;
   for (uint32_t i = 0; i < n; ++i) nL += mark[rows[i]] ? 1 : 0;
-  const uint32_t nch = (n + Cr - 1) / Cr;
-  const uint32_t rpt = nch > 1 ? 1u : std::max(1u, Cr / n);
-  const uint32_t ntask = nch > 1 ? (uint32_t)F * nch : ((uint32_t)F + rpt - 1) / rpt;
-  std::vector<Task> tasks(ntask);
-  for (uint32_t t = 0; t < ntask; ++t) {
-    if (nch > 1) tasks[t] = Task{0u, (t / nch) | (1u << 16), t % nch, nch};
-    else {
-      const uint32_t first = t * rpt;
-      tasks[t] = Task{0u, first | (std::min(rpt, (uint32_t)F - first) << 16), 0u, nch};
-    }
-  }
+  // the counting step of k_children for one node of tree 0
+  const bool big = n >= kBigN;
+  const uint32_t rpt = big ? 1u : std::max(1u, Cr / n);
+  const uint32_t ntask = big ? (uint32_t)F : ((uint32_t)F + rpt - 1) / rpt;
   NAux ax = {};
   ax.nL = nL;
   ax.flags = kSplit | kLiveL | kLiveR;
   ax.dst = 2;
-  ax.resh_base = 0;
+  ax.resh_big = big ? 1u : 0u;
+  ax.resh_off = 0;
   ax.resh_ntask = ntask;
-  ax.rcnt_base = 0;
-  ax.nch_r = nch;
   ax.rpt = rpt;
   FNode nd = {0u, 0u, n, 0u, 0u, 1u, 0u, 0u};
   Ctl hc = {};
   hc.q = 1;
-  hc.n_resh = ntask;
-  DevMem d_in(hrows.size() * sizeof(IdxT)), d_out(hrows.size() * sizeof(IdxT)), d_mark(N), d_front(sizeof(FNode)),
-      d_aux(sizeof(NAux)), d_tasks(tasks.size() * sizeof(Task)), d_rcnt((size_t)F * nch * 4 + 4), d_ctl(sizeof(Ctl));
+  uint32_t cnts[2] = {big ? ntask : 0u, big ? 0u : ntask};
+  unsigned long long elems = (unsigned long long)F * n;
+  DevMem d_in(hrows.size() * sizeof(IdxT)), d_out(hrows.size() * sizeof(IdxT)), d_bits((size_t)W * 4),
+      d_front(sizeof(FNode)), d_aux(sizeof(NAux)), d_big((size_t)F * sizeof(Task)), d_small((size_t)ntask * sizeof(Task)),
+      d_cnt(8), d_elems(8), d_off(3 * 2 * 4), d_ctl(sizeof(Ctl));
   HIPC(hipMemcpyAsync(d_in.p, hrows.data(), hrows.size() * sizeof(IdxT), hipMemcpyHostToDevice, st));
-  HIPC(hipMemcpyAsync(d_mark.p, mark, N, hipMemcpyHostToDevice, st));
+  HIPC(hipMemcpyAsync(d_out.p, hrows.data(), hrows.size() * sizeof(IdxT), hipMemcpyHostToDevice, st));
+  HIPC(hipMemcpyAsync(d_bits.p, hbits.data(), (size_t)W * 4, hipMemcpyHostToDevice, st));
   HIPC(hipMemcpyAsync(d_front.p, &nd, sizeof nd, hipMemcpyHostToDevice, st));
   HIPC(hipMemcpyAsync(d_aux.p, &ax, sizeof ax, hipMemcpyHostToDevice, st));
-  HIPC(hipMemcpyAsync(d_tasks.p, tasks.data(), tasks.size() * sizeof(Task), hipMemcpyHostToDevice, st));
+  HIPC(hipMemcpyAsync(d_cnt.p, cnts, 8, hipMemcpyHostToDevice, st));
+  HIPC(hipMemcpyAsync(d_elems.p, &elems, 8, hipMemcpyHostToDevice, st));
   HIPC(hipMemcpyAsync(d_ctl.p, &hc, sizeof hc, hipMemcpyHostToDevice, st));
   FitArgs<IdxT, LabT> a = {};
   a.X_T = ds->XT.as<float>();
@@ -1184,30 +1204,49 @@ static void hook_reshuffle(cta_ctx *ctx, cta_dataset *ds, const uint32_t *rows, 
   a.si0 = ds->si0.as<IdxT>();
   a.buf0 = d_in.as<IdxT>();
   a.buf1 = d_out.as<IdxT>();
-  a.mark = d_mark.as<uint8_t>();
+  a.bits = d_bits.as<uint32_t>();
+  a.W = W;
+  a.T = 1;
   a.N = N;
   a.F = F;
   a.K = K;
   a.Cr = Cr;
   a.front = d_front.as<FNode>();
   a.aux = d_aux.as<NAux>();
-  a.resh_tasks = d_tasks.as<Task>();
-  a.rcnt = d_rcnt.as<uint32_t>();
+  a.big_tasks = d_big.as<Task>();
+  a.small_tasks = d_small.as<Task>();
+  a.cnt_big = d_cnt.as<uint32_t>();
+  a.cnt_small = d_cnt.as<uint32_t>() + 1;
+  a.elems = d_elems.as<unsigned long long>();
+  a.big_off = d_off.as<uint32_t>();
+  a.small_off = d_off.as<uint32_t>() + 2;
+  a.wg_off = d_off.as<uint32_t>() + 4;
   a.ctl = d_ctl.as<Ctl>();
-  const uint32_t G = 64;
-  k_resh_count<IdxT, LabT><<<G, kBlock, 0, st>>>(a);
-  k_resh_carry<IdxT, LabT><<<G, kBlock, 0, st>>>(a);
-  k_resh<IdxT, LabT><<<G, kBlock, 0, st>>>(a);
+  a.cap_big = (uint32_t)F;
+  a.cap_small = ntask;
+  k_plan<IdxT, LabT><<<1, 64, 0, st>>>(a);
+  k_emit<IdxT, LabT><<<8, kBlock, 0, st>>>(a);
+  const size_t lds = (((size_t)W + 3) & ~(size_t)3) * 4 + 2 * 64 * 4;
+  if (!global_bits && lds <= ctx->lds_per_block) {
+    HIPC(hipFuncSetAttribute((const void *)k_resh<IdxT, LabT, true>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                             (int)ctx->lds_per_block));
+    k_resh<IdxT, LabT, true><<<16, kReshThreads, lds, st>>>(a);
+  } else {
+    k_resh<IdxT, LabT, false><<<16, kReshThreads, 2 * 64 * 4, st>>>(a);
+  }
   HIPC(hipGetLastError());
   std::vector<IdxT> hout(hrows.size());
+  Ctl back = {};
   HIPC(hipMemcpyAsync(hout.data(), d_out.p, hout.size() * sizeof(IdxT), hipMemcpyDeviceToHost, st));
+  HIPC(hipMemcpyAsync(&back, d_ctl.p, sizeof back, hipMemcpyDeviceToHost, st));
   ctx->sync();
+  if (back.err) fail(CTA_EINTERNAL, "reshuffle hook: device capacity check failed");
   for (int32_t f = 0; f < F; ++f)
     for (uint32_t i = 0; i < n; ++i) out[(size_t)f * n + i] = hout[(size_t)f * N + i];
 }
 
 extern "C" int cta_k_reshuffle(cta_ctx *ctx, cta_dataset *ds, const uint32_t *rows, uint32_t n, const uint8_t *mark,
-                    int32_t chunk, uint32_t *out) {
+                               int32_t chunk, int32_t global_bits, uint32_t *out) {
   return guarded([&] {
     if (!ctx || !ds || !rows || !mark || !out) fail(CTA_EINVAL, "NULL argument");
     if (n < 1 || n > (uint32_t)ds->N) fail(CTA_EINVAL, "n out of range");
@@ -1217,9 +1256,9 @@ extern "C" int cta_k_reshuffle(cta_ctx *ctx, cta_dataset *ds, const uint32_t *ro
     std::lock_guard<std::recursive_mutex> lk(ctx->mu);
     HIPC(hipSetDevice(ctx->device));
     const bool b16 = ds->idx_bits == 16, l8 = ds->lab_bits == 8;
-    if (b16 && l8) hook_reshuffle<uint16_t, uint8_t>(ctx, ds, rows, n, mark, chunk, out);
-    else if (b16) hook_reshuffle<uint16_t, uint16_t>(ctx, ds, rows, n, mark, chunk, out);
-    else if (l8) hook_reshuffle<uint32_t, uint8_t>(ctx, ds, rows, n, mark, chunk, out);
-    else hook_reshuffle<uint32_t, uint16_t>(ctx, ds, rows, n, mark, chunk, out);
+    if (b16 && l8) hook_reshuffle<uint16_t, uint8_t>(ctx, ds, rows, n, mark, chunk, global_bits, out);
+    else if (b16) hook_reshuffle<uint16_t, uint16_t>(ctx, ds, rows, n, mark, chunk, global_bits, out);
+    else if (l8) hook_reshuffle<uint32_t, uint8_t>(ctx, ds, rows, n, mark, chunk, global_bits, out);
+    else hook_reshuffle<uint32_t, uint16_t>(ctx, ds, rows, n, mark, chunk, global_bits, out);
   });
 }

@@ -20,7 +20,7 @@
 //                                                (dfs_module.cu:210-221, bfs_module.cu:238-264)
 //   k_children    per split node: leaf tests, next frontier, reshuffle tasks
 //                                                (random_tree.py:71-165)
-//   k_resh_count / k_resh_carry / k_resh
+//   k_plan / k_emit / k_resh
 //                 stable partition of every row (dfs_module.cu:224-307, bfs_module.cu:266-377)
 //
 // The split scan parallelises the reference's serial recurrence
@@ -360,8 +360,8 @@ __global__ __launch_bounds__(kBlock) void k_decide(FitArgs<IdxT, LabT> a) {
 }
 
 // ------------------------------------------------------------------------
-// k_fill: mark every sample of the split row's segment with its side and
-// count the left child's classes.
+// k_fill: set every sample of the split row's segment to its side in the
+// tree's bitmap (bit set = left) and count the left child's classes.
 // ------------------------------------------------------------------------
 template <class IdxT, class LabT>
 __global__ __launch_bounds__(kBlock) void k_fill(FitArgs<IdxT, LabT> a) {
@@ -374,15 +374,19 @@ __global__ __launch_bounds__(kBlock) void k_fill(FitArgs<IdxT, LabT> a) {
     const FNode nd = a.front[tk.node];
     const NAux ax = a.aux[tk.node];
     const IdxT *row = rows_of(a, nd.src, nd.tree, (uint32_t)ax.best_f) + nd.start;
-    uint8_t *mark = a.mark + (size_t)nd.tree * a.N;
+    uint32_t *bits = a.bits + (size_t)nd.tree * a.W;
     for (uint32_t k = lane; k < K; k += kWave) h[k] = 0;
     wave_sync();
     const uint32_t p0 = tk.chunk * a.Cf, p1 = min(nd.n, p0 + a.Cf);
     for (uint32_t p = p0 + lane; p < p1; p += kWave) {
       const uint32_t s = row[p];
-      const bool left = p <= ax.best_p;
-      mark[s] = left ? 1 : 0;
-      if (left) atomicAdd(&h[a.y[s]], 1u);
+      const uint32_t bit = 1u << (s & 31u);
+      if (p <= ax.best_p) {
+        atomicOr(&bits[s >> 5], bit);
+        atomicAdd(&h[a.y[s]], 1u);
+      } else {
+        atomicAnd(&bits[s >> 5], ~bit);
+      }
     }
     wave_sync();
     for (uint32_t k = lane; k < K; k += kWave)
@@ -391,44 +395,31 @@ __global__ __launch_bounds__(kBlock) void k_fill(FitArgs<IdxT, LabT> a) {
   }
 }
 
-// Emits the reshuffle tasks of node i (n samples, F rows).
+__device__ __forceinline__ uint32_t bit_of(const uint32_t *bits, uint32_t s) { return (bits[s >> 5] >> (s & 31u)) & 1u; }
+
+// Reshuffle bookkeeping of split node i (n samples) of tree t: rows of a
+// node with at least kBigN samples are moved by whole workgroups (one task
+// per row), smaller nodes by single waves (one task per group of rows).
+// Counts the node's tasks into its tree's lists; k_plan turns the counts
+// into offsets and k_emit writes the task descriptors.
+constexpr uint32_t kBigN = 4096;
+
 template <class IdxT, class LabT>
-__device__ void emit_resh_tasks(const FitArgs<IdxT, LabT> &a, uint32_t i, uint32_t n, NAux &ax) {
+__device__ void count_resh_tasks(const FitArgs<IdxT, LabT> &a, uint32_t t, uint32_t n, NAux &ax) {
   const uint32_t lane = lane_id();
   const uint32_t F = (uint32_t)a.F;
-  const uint32_t nch = ceil_div(n, a.Cr);
-  const uint32_t rpt = (nch > 1) ? 1u : max(1u, a.Cr / n);
-  const uint32_t ntask = (nch > 1) ? F * nch : ceil_div(F, rpt);
-  uint32_t base = 0, rbase = 0;
+  const bool big = n >= kBigN;
+  const uint32_t rpt = big ? 1u : max(1u, a.Cr / n);
+  const uint32_t ntask = big ? F : ceil_div(F, rpt);
+  uint32_t off = 0;
   if (lane == 0) {
-    base = atomicAdd(&a.ctl->n_resh, ntask);
-    if (nch > 1) rbase = atomicAdd(&a.ctl->n_rcnt, F * nch);
+    off = atomicAdd(big ? &a.cnt_big[t] : &a.cnt_small[t], ntask);
+    atomicAdd(&a.elems[t], (unsigned long long)F * n);
   }
-  base = uniform(__shfl(base, 0));
-  rbase = uniform(__shfl(rbase, 0));
-  if ((uint64_t)base + ntask > a.cap_resh || (nch > 1 && (uint64_t)rbase + (uint64_t)F * nch > a.cap_rcnt)) {
-    if (lane == 0) atomicOr(&a.ctl->err, kErrCapacity);
-    ax.resh_ntask = 0;
-    return;
-  }
-  for (uint32_t t = lane; t < ntask; t += kWave) {
-    Task tk;
-    tk.node = i;
-    if (nch > 1) {
-      tk.first_count = (t / nch) | (1u << 16);
-      tk.chunk = t % nch;
-    } else {
-      const uint32_t first = t * rpt;
-      tk.first_count = first | (min(rpt, F - first) << 16);
-      tk.chunk = 0;
-    }
-    tk.nch = nch;
-    a.resh_tasks[base + t] = tk;
-  }
-  ax.resh_base = base;
+  off = uniform(__shfl(off, 0));
+  ax.resh_big = big ? 1u : 0u;
+  ax.resh_off = off;
   ax.resh_ntask = ntask;
-  ax.rcnt_base = rbase;
-  ax.nch_r = nch;
   ax.rpt = rpt;
 }
 
@@ -512,14 +503,14 @@ __global__ __launch_bounds__(kBlock) void k_children(FitArgs<IdxT, LabT> a) {
       // first sample of each child in feature 0's row: the first marked
       // (left) and first unmarked (right) sample of the parent's segment
       const IdxT *row0 = rows_of(a, nd.src, nd.tree, 0) + nd.start;
-      const uint8_t *mark = a.mark + (size_t)nd.tree * a.N;
+      const uint32_t *bits = a.bits + (size_t)nd.tree * a.W;
       bool needl = kl == 2, needr = kr == 2;
       for (uint32_t pb = 0; pb < nd.n && (needl || needr); pb += kWave) {
         const uint32_t p = pb + lane;
         uint32_t s = 0, side = 2;
         if (p < nd.n) {
           s = row0[p];
-          side = mark[s];
+          side = bit_of(bits, s);
         }
         const uint64_t ml = ballot(side == 1), mr = ballot(side == 0);
         if (needl && ml) {
@@ -578,7 +569,7 @@ __global__ __launch_bounds__(kBlock) void k_children(FitArgs<IdxT, LabT> a) {
     }
     ax.flags = flags;
     if (flags & (kLiveL | kLiveR)) {
-      emit_resh_tasks(a, i, nd.n, ax);
+      count_resh_tasks(a, nd.tree, nd.n, ax);
       if (lane == 0)
         atomicAdd(&a.ctl->bytes_resh, (unsigned long long)a.F * nd.n * (2 * sizeof(IdxT) + 1));
     } else {
@@ -590,84 +581,210 @@ __global__ __launch_bounds__(kBlock) void k_children(FitArgs<IdxT, LabT> a) {
 }
 
 // ------------------------------------------------------------------------
-// Stable partition of every row of the split nodes (reshuffle).
+// Reshuffle: stable partition of every row of the split nodes.
+//
+// k_plan turns the per-tree task counts into offsets and gives each tree
+// ceil(elements / kWgElems) workgroups.  k_emit writes the task lists.
+// k_resh runs 1,024-thread workgroups, each bound to one tree: it stages the
+// tree's side bitmap in LDS (when it fits), then moves its share of the
+// tree's big tasks cooperatively (4,096-element tiles, one barrier per tile,
+// the left count carried across tiles) and its share of the small tasks one
+// wave per task.  Loads and stores are striped (lane-consecutive), and the
+// stable ranks come from one ballot per 64 elements.
 // ------------------------------------------------------------------------
+constexpr int kReshThreads = 1024;
+constexpr int kReshWaves = kReshThreads / kWave;  // 16
+constexpr int kReshE = 4;                         // elements per thread per tile
+constexpr uint32_t kReshTile = kReshThreads * kReshE;
+constexpr uint64_t kWgElems = 1ull << 19;
+
 template <class IdxT, class LabT>
-__global__ __launch_bounds__(kBlock) void k_resh_count(FitArgs<IdxT, LabT> a) {
-  const uint32_t lane = lane_id();
-  WAVE_LOOP(a.ctl->n_resh) {
-    const Task tk = a.resh_tasks[item];
-    if (tk.nch <= 1) continue;
-    const FNode nd = a.front[tk.node];
-    const NAux ax = a.aux[tk.node];
-    const uint32_t f = tk.first_count & 0xFFFFu;
-    const IdxT *row = rows_of(a, nd.src, nd.tree, f) + nd.start;
-    const uint8_t *mark = a.mark + (size_t)nd.tree * a.N;
-    const uint32_t p0 = tk.chunk * a.Cr, p1 = min(nd.n, p0 + a.Cr);
-    uint32_t c = 0;
-    for (uint32_t p = p0 + lane; p < p1; p += kWave) c += mark[row[p]];
-    c = wave_sum32(c);
-    if (lane == 0) a.rcnt[ax.rcnt_base + f * tk.nch + tk.chunk] = c;
+__global__ void k_plan(FitArgs<IdxT, LabT> a) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  uint32_t b = 0, sm = 0, w = 0;
+  for (uint32_t t = 0; t < a.T; ++t) {
+    a.big_off[t] = b;
+    a.small_off[t] = sm;
+    a.wg_off[t] = w;
+    const unsigned long long e = a.elems[t];
+    b += a.cnt_big[t];
+    sm += a.cnt_small[t];
+    if (e) w += (uint32_t)min((unsigned long long)4096, (e + kWgElems - 1) / kWgElems);
+  }
+  a.big_off[a.T] = b;
+  a.small_off[a.T] = sm;
+  a.wg_off[a.T] = w;
+  if (b > a.cap_big || sm > a.cap_small) {
+    atomicOr(&a.ctl->err, kErrCapacity);
+    w = 0;
+    a.wg_off[a.T] = 0;
+  }
+  a.ctl->n_plan = w;
+  for (uint32_t t = 0; t < a.T; ++t) {
+    a.cnt_big[t] = 0;
+    a.cnt_small[t] = 0;
+    a.elems[t] = 0;
   }
 }
 
+// Writes the task descriptors of every split node with a live child.
 template <class IdxT, class LabT>
-__global__ __launch_bounds__(kBlock) void k_resh_carry(FitArgs<IdxT, LabT> a) {
+__global__ __launch_bounds__(kBlock) void k_emit(FitArgs<IdxT, LabT> a) {
   const uint32_t lane = lane_id();
-  WAVE_LOOP(a.ctl->n_resh) {
-    const Task tk = a.resh_tasks[item];
-    if (tk.nch <= 1 || tk.chunk != 0) continue;
-    const NAux ax = a.aux[tk.node];
-    const uint32_t f = tk.first_count & 0xFFFFu;
-    uint32_t *base = a.rcnt + ax.rcnt_base + f * tk.nch;
-    uint32_t run = 0;
-    for (uint32_t c0 = 0; c0 < tk.nch; c0 += kWave) {
-      const uint32_t c = c0 + lane;
-      const uint32_t v = c < tk.nch ? base[c] : 0u;
-      const uint32_t inc = wave_incl_scan(v);
-      if (c < tk.nch) base[c] = run + inc - v;
-      run += readlane(inc, kWave - 1);
-    }
-  }
-}
-
-template <class IdxT, class LabT>
-__global__ __launch_bounds__(kBlock) void k_resh(FitArgs<IdxT, LabT> a) {
-  const uint32_t lane = lane_id();
-  WAVE_LOOP(a.ctl->n_resh) {
-    const Task tk = a.resh_tasks[item];
-    const FNode nd = a.front[tk.node];
-    const NAux ax = a.aux[tk.node];
-    const bool liveL = ax.flags & kLiveL, liveR = ax.flags & kLiveR;
-    const uint8_t *mark = a.mark + (size_t)nd.tree * a.N;
-    const uint32_t first = tk.first_count & 0xFFFFu, count = tk.first_count >> 16;
-    const uint32_t p0 = (tk.nch > 1) ? tk.chunk * a.Cr : 0u;
-    const uint32_t p1 = (tk.nch > 1) ? min(nd.n, p0 + a.Cr) : nd.n;
-    for (uint32_t f = first; f < first + count; ++f) {
-      const IdxT *in = rows_of(a, nd.src, nd.tree, f) + nd.start;
-      IdxT *out = rows_out(a, ax.dst, nd.tree, f) + nd.start;
-      uint32_t lb = (tk.nch > 1) ? a.rcnt[ax.rcnt_base + f * tk.nch + tk.chunk] : 0u;
-      for (uint32_t pb = p0; pb < p1; pb += kWave) {
-        const uint32_t p = pb + lane;
-        const bool valid = p < p1;
-        IdxT s = 0;
-        uint32_t side = 0;
-        if (valid) {
-          s = in[p];
-          side = mark[s];
-        }
-        const uint64_t ml = ballot(valid && side);
-        const uint32_t lefts = lb + mbcnt(ml);
-        if (valid) {
-          if (side) {
-            if (liveL) out[lefts] = s;
-          } else if (liveR) {
-            out[ax.nL + (p - lefts)] = s;
-          }
-        }
-        lb += (uint32_t)__popcll(ml);
+  if (a.ctl->n_plan == 0) return;
+  WAVE_LOOP(a.ctl->q) {
+    const NAux ax = a.aux[item];
+    if (!(ax.flags & kSplit) || !(ax.flags & (kLiveL | kLiveR)) || ax.resh_ntask == 0) continue;
+    const FNode nd = a.front[item];
+    const uint32_t F = (uint32_t)a.F;
+    if (ax.resh_big) {
+      Task *out = a.big_tasks + a.big_off[nd.tree] + ax.resh_off;
+      for (uint32_t f = lane; f < F; f += kWave) out[f] = Task{item, f | (1u << 16), 0u, 1u};
+    } else {
+      Task *out = a.small_tasks + a.small_off[nd.tree] + ax.resh_off;
+      for (uint32_t t = lane; t < ax.resh_ntask; t += kWave) {
+        const uint32_t first = t * ax.rpt;
+        out[t] = Task{item, first | (min(ax.rpt, F - first) << 16), 0u, 1u};
       }
     }
+  }
+}
+
+// One big task: all rows' segment [0, n) of one row, moved by the workgroup.
+// `par` selects the half of wcnt a tile writes; it alternates across tiles
+// and across consecutive tasks, so a tile never overwrites counts that a
+// slower wave of the previous tile may still be reading.
+template <class IdxT, class LabT, class Side>
+__device__ void resh_big(const FitArgs<IdxT, LabT> &a, const Task &tk, const Side &side, uint32_t *wcnt,
+                         uint32_t &par) {
+  const FNode nd = a.front[tk.node];
+  const NAux ax = a.aux[tk.node];
+  const uint32_t f = tk.first_count & 0xFFFFu;
+  const IdxT *in = rows_of(a, nd.src, nd.tree, f) + nd.start;
+  IdxT *out = rows_out(a, ax.dst, nd.tree, f) + nd.start;
+  const bool liveL = ax.flags & kLiveL, liveR = ax.flags & kLiveR;
+  const uint32_t n = nd.n, nL = ax.nL;
+  const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
+  uint32_t carry = 0;
+  for (uint32_t base = 0; base < n; base += kReshTile) {
+    IdxT e[kReshE];
+    uint32_t sd[kReshE], rk[kReshE];
+#pragma unroll
+    for (int k = 0; k < kReshE; ++k) {
+      const uint32_t p = base + (uint32_t)k * kReshThreads + tid;
+      e[k] = 0;
+      if (p < n) e[k] = in[p];
+    }
+#pragma unroll
+    for (int k = 0; k < kReshE; ++k) {
+      const uint32_t p = base + (uint32_t)k * kReshThreads + tid;
+      sd[k] = (p < n) ? side(e[k]) : 0u;
+      const uint64_t m = ballot(sd[k] != 0);
+      rk[k] = mbcnt(m);
+      if (lane == 0) wcnt[par * 64 + k * kReshWaves + wave] = (uint32_t)__popcll(m);
+    }
+    __syncthreads();
+    // exclusive prefix over the 64 (sub-tile, wave) counts in position order
+    const uint32_t v = wcnt[par * 64 + lane];
+    const uint32_t vi = wave_incl_scan(v);
+    const uint32_t total = readlane(vi, 63);
+#pragma unroll
+    for (int k = 0; k < kReshE; ++k) {
+      const uint32_t slot = (uint32_t)k * kReshWaves + wave;
+      const uint32_t before = carry + readlane(vi, slot) - readlane(v, slot);
+      const uint32_t p = base + (uint32_t)k * kReshThreads + tid;
+      if (p < n) {
+        const uint32_t lb = before + rk[k];
+        if (sd[k]) {
+          if (liveL) out[lb] = e[k];
+        } else if (liveR) {
+          out[nL + (p - lb)] = e[k];
+        }
+      }
+    }
+    carry += total;
+    par ^= 1u;
+  }
+}
+
+// One small task: rows [first, first+count) of a node, moved by one wave.
+template <class IdxT, class LabT, class Side>
+__device__ void resh_small(const FitArgs<IdxT, LabT> &a, const Task &tk, const Side &side) {
+  const FNode nd = a.front[tk.node];
+  const NAux ax = a.aux[tk.node];
+  const bool liveL = ax.flags & kLiveL, liveR = ax.flags & kLiveR;
+  const uint32_t first = tk.first_count & 0xFFFFu, count = tk.first_count >> 16;
+  const uint32_t n = nd.n, nL = ax.nL;
+  const uint32_t lane = lane_id();
+  for (uint32_t f = first; f < first + count; ++f) {
+    const IdxT *in = rows_of(a, nd.src, nd.tree, f) + nd.start;
+    IdxT *out = rows_out(a, ax.dst, nd.tree, f) + nd.start;
+    uint32_t carry = 0;
+    for (uint32_t base = 0; base < n; base += kWave * kReshE) {
+      IdxT e[kReshE];
+      uint32_t sd[kReshE];
+#pragma unroll
+      for (int k = 0; k < kReshE; ++k) {
+        const uint32_t p = base + (uint32_t)k * kWave + lane;
+        e[k] = 0;
+        if (p < n) e[k] = in[p];
+      }
+#pragma unroll
+      for (int k = 0; k < kReshE; ++k) {
+        const uint32_t p = base + (uint32_t)k * kWave + lane;
+        sd[k] = (p < n) ? side(e[k]) : 0u;
+        const uint64_t m = ballot(sd[k] != 0);
+        if (p < n) {
+          const uint32_t lb = carry + mbcnt(m);
+          if (sd[k]) {
+            if (liveL) out[lb] = e[k];
+          } else if (liveR) {
+            out[nL + (p - lb)] = e[k];
+          }
+        }
+        carry += (uint32_t)__popcll(m);
+      }
+    }
+  }
+}
+
+// kLds: the tree's bitmap is staged in LDS (N/8 bytes); otherwise it is read
+// from global memory.
+template <class IdxT, class LabT, bool kLds>
+__global__ __launch_bounds__(kReshThreads) void k_resh(FitArgs<IdxT, LabT> a) {
+  extern __shared__ uint32_t lds[];
+  uint32_t *wcnt = lds + (kLds ? ((a.W + 3u) & ~3u) : 0u);  // 2 x 64 counts
+  const uint32_t P = a.ctl->n_plan;
+  const uint32_t T = a.T;
+  const uint32_t wave = threadIdx.x / kWave;
+  uint32_t loaded = 0xFFFFFFFFu, par = 0;
+  for (uint32_t pe = blockIdx.x; pe < P; pe += gridDim.x) {
+    // tree of this plan entry: last t with wg_off[t] <= pe
+    uint32_t lo = 0, hi = T;
+    while (hi - lo > 1) {
+      const uint32_t mid = (lo + hi) / 2;
+      if (a.wg_off[mid] <= pe) lo = mid;
+      else hi = mid;
+    }
+    const uint32_t t = lo;
+    const uint32_t j = pe - a.wg_off[t], wgs = a.wg_off[t + 1] - a.wg_off[t];
+    const uint32_t *gbits = a.bits + (size_t)t * a.W;
+    if (kLds && t != loaded) {
+      __syncthreads();
+      for (uint32_t w = threadIdx.x; w < a.W; w += kReshThreads) lds[w] = gbits[w];
+      __syncthreads();
+      loaded = t;
+    }
+    auto side = [&](IdxT s) -> uint32_t {
+      const uint32_t u = (uint32_t)s;
+      if constexpr (kLds) return (lds[u >> 5] >> (u & 31u)) & 1u;
+      else return (gbits[u >> 5] >> (u & 31u)) & 1u;
+    };
+    const uint32_t nb = a.big_off[t + 1] - a.big_off[t];
+    for (uint32_t bt = j; bt < nb; bt += wgs) resh_big(a, a.big_tasks[a.big_off[t] + bt], side, wcnt, par);
+    const uint32_t ns = a.small_off[t + 1] - a.small_off[t];
+    for (uint32_t st = j * kReshWaves + wave; st < ns; st += wgs * kReshWaves)
+      resh_small(a, a.small_tasks[a.small_off[t] + st], side);
   }
 }
 
@@ -683,13 +800,13 @@ __global__ void k_fill_u32(uint32_t *p, size_t n, uint32_t v) {
   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = v;
 }
 
-// mark[t][id] = 1 for the in-bag ids of every batch tree.
-__global__ void k_boot_mark(const uint32_t *ids, const uint64_t *off, uint32_t T, uint32_t N, uint8_t *mark) {
+// Sets the bit of every in-bag id of every batch tree (bitmaps zeroed first).
+__global__ void k_boot_mark(const uint32_t *ids, const uint64_t *off, uint32_t T, uint32_t W, uint32_t *bits) {
   const uint32_t t = blockIdx.y;
   if (t >= T) return;
   const uint64_t a = off[t], b = off[t + 1];
   for (uint64_t j = a + (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; j < b; j += (uint64_t)gridDim.x * blockDim.x)
-    mark[(size_t)t * N + ids[j]] = 1;
+    atomicOr(&bits[(size_t)t * W + (ids[j] >> 5)], 1u << (ids[j] & 31u));
 }
 
 // Class counts of every batch tree's root: over its ids, or over all N.
@@ -732,7 +849,7 @@ __global__ __launch_bounds__(kBlock) void k_boot_tasks(FitArgs<IdxT, LabT> a, co
     ax.nL = (uint32_t)(off[t + 1] - off[t]);
     ax.flags = kSplit | kLiveL;
     ax.dst = 1;
-    emit_resh_tasks(a, t, (uint32_t)a.N, ax);
+    count_resh_tasks(a, t, (uint32_t)a.N, ax);
     if (lane == 0) {
       a.aux[t] = ax;
       atomicAdd(&a.ctl->bytes_boot, (unsigned long long)a.F * a.N * (sizeof(IdxT) + 1) +
@@ -805,7 +922,7 @@ __global__ void k_ctl_next(Ctl *ctl) {
   if (threadIdx.x == 0 && blockIdx.x == 0) {
     ctl->q = ctl->q_next;
     ctl->q_next = 0;
-    ctl->n_scan = ctl->n_fill = ctl->n_resh = ctl->n_hist = ctl->n_rcnt = 0;
+    ctl->n_scan = ctl->n_fill = ctl->n_hist = ctl->n_plan = 0;
   }
 }
 
@@ -814,7 +931,7 @@ __global__ void k_ctl_set(Ctl *ctl, uint32_t q) {
   if (threadIdx.x == 0 && blockIdx.x == 0) {
     ctl->q = q;
     ctl->q_next = 0;
-    ctl->n_scan = ctl->n_fill = ctl->n_resh = ctl->n_hist = ctl->n_rcnt = 0;
+    ctl->n_scan = ctl->n_fill = ctl->n_hist = ctl->n_plan = 0;
   }
 }
 

@@ -31,8 +31,10 @@ struct NAux {
   uint32_t flags;                              // kSplit | kLiveL | kLiveR
   uint32_t dst;                                // rows receiving the children
   uint32_t fill_base, nch_f;
-  uint32_t resh_base, resh_ntask, rcnt_base, nch_r, rpt;
-  uint32_t pad[3];
+  uint32_t resh_big;                           // 1: rows reshuffled by whole workgroups
+  uint32_t resh_off;                           // first task in the tree's big or small list
+  uint32_t resh_ntask, rpt;                    // small tasks: count and rows per task
+  uint32_t pad[2];
 };
 constexpr uint32_t kSplit = 1u, kLiveL = 2u, kLiveR = 4u;
 
@@ -52,10 +54,10 @@ struct ScanRes {
 // Counters of one level, in device memory.
 struct Ctl {
   uint32_t q, q_next;
-  uint32_t n_scan, n_fill, n_resh, n_hist, n_rcnt;
+  uint32_t n_scan, n_fill, n_hist, n_plan;
   uint32_t n_nodes;
   uint32_t err;
-  uint32_t pad[7];
+  uint32_t pad[8];
   unsigned long long bytes_scan, bytes_fill, bytes_resh, bytes_boot;
 };
 constexpr uint32_t kErrCapacity = 1u, kErrBadNode = 2u;
@@ -73,7 +75,9 @@ struct FitArgs {
   const LabT *y;      // N compact labels
   const IdxT *si0;    // F x N presorted rows
   IdxT *buf0, *buf1;  // per tree F x N ping-pong rows
-  uint8_t *mark;      // per tree N bytes: 1 = goes left
+  uint32_t *bits;     // per tree W words: bit s set = sample s goes left
+  uint32_t W;         // words per tree bitmap, (N + 31) / 32
+  uint32_t T;         // trees in the batch
   int32_t N, F, K, m, mss, max_depth, identity;
   uint32_t tree_base;  // global index of batch tree 0 (keys the permutation)
   uint64_t seed;
@@ -90,11 +94,15 @@ struct FitArgs {
   ScanRes *res;
   uint32_t *hist;  // class histograms / carries of multi-chunk scans
   Task *fill_tasks;
-  Task *resh_tasks;
-  uint32_t *rcnt;  // left counts / carries of multi-chunk reshuffles
+  // reshuffle plan: per-tree task lists and workgroup ranges
+  Task *big_tasks, *small_tasks;
+  uint32_t *cnt_big, *cnt_small;  // [T] per-level counters
+  unsigned long long *elems;      // [T] elements to move per tree
+  uint32_t *big_off, *small_off;  // [T + 1]
+  uint32_t *wg_off;               // [T + 1] workgroups per tree (prefix)
   Arena arena;
   Ctl *ctl;
-  uint32_t cap_q, cap_scan, cap_hist, cap_fill, cap_resh, cap_rcnt, cap_nodes;
+  uint32_t cap_q, cap_scan, cap_hist, cap_fill, cap_big, cap_small, cap_nodes;
 };
 
 }  // namespace cta

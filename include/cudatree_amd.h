@@ -95,9 +95,10 @@ typedef struct {
   int32_t mode;              /* CTA_MODE_* */
   int64_t bfs_threshold;     /* CTA_MODE_HYBRID: nodes with n > bfs_threshold go one at a time */
   int32_t scan_chunk;        /* samples per split-scan task (0: default) */
-  int32_t resh_chunk;        /* samples per reshuffle task (0: default) */
+  int32_t resh_chunk;        /* samples per small-node reshuffle task (0: default) */
   int32_t trees_per_batch;   /* trees resident at once (0: as many as memory allows) */
-  int32_t reserved[5];
+  int32_t resh_global;       /* 1: reshuffle reads the side bitmap from global memory instead of LDS */
+  int32_t reserved[4];
 } cta_fit_opts;
 
 /* Fills opts with the defaults (batched mode, m = ceil(sqrt(F)) left to the caller). */
@@ -171,9 +172,11 @@ int cta_k_split_scan(cta_ctx *ctx, cta_dataset *ds, const uint32_t *rows, uint32
                      uint64_t *den);
 /* Stable partition of every row of one node (rows F x n uint32, host) by
  * left-membership of each sample (mark, N bytes, 1 = left) into out (F x n).
+ * chunk: samples per small-node task (0: default); global_bits: 1 reads the
+ * side bitmap from global memory instead of staging it in LDS.
  * Replaces: fill_table + scan_reshuffle (dfs_module.cu:210-307, bfs_module.cu:238-377). */
 int cta_k_reshuffle(cta_ctx *ctx, cta_dataset *ds, const uint32_t *rows, uint32_t n, const uint8_t *mark,
-                    int32_t chunk, uint32_t *out);
+                    int32_t chunk, int32_t global_bits, uint32_t *out);
 
 #ifdef __cplusplus
 }

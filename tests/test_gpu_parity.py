@@ -24,7 +24,7 @@ def orc():
     return oracle.get()
 
 
-def gpu_forest(ctx, case, mode="batched", scan_chunk=0, resh_chunk=0, idx_bits=0, tpb=0):
+def gpu_forest(ctx, case, mode="batched", scan_chunk=0, resh_chunk=0, idx_bits=0, tpb=0, resh_global=0):
     X, y, K = case["X"], case["y"], case["K"]
     ds = _lib.Dataset(ctx, X, y, K, idx_bits=idx_bits)
     o = _lib.default_opts()
@@ -38,6 +38,7 @@ def gpu_forest(ctx, case, mode="batched", scan_chunk=0, resh_chunk=0, idx_bits=0
     o.scan_chunk = scan_chunk
     o.resh_chunk = resh_chunk
     o.trees_per_batch = tpb
+    o.resh_global = resh_global
     f = _lib.Forest.fit(ctx, ds, case["n_trees"], o, helpers.case_boot_ids(case))
     return f, ds
 
@@ -116,19 +117,22 @@ def test_split_scan_hook(ctx, orc, chunk):
         assert got == want, (trial, got, want)
 
 
-@pytest.mark.parametrize("chunk", [0, 64, 100])
-def test_reshuffle_hook(ctx, chunk):
-    rng = np.random.default_rng(chunk + 11)
-    for trial in range(8):
-        N = int(rng.integers(10, 5000))
+@pytest.mark.parametrize("chunk,global_bits", [(0, False), (64, False), (100, False), (0, True)])
+def test_reshuffle_hook(ctx, chunk, global_bits):
+    """Small nodes (one wave per row group) and nodes of >= 4,096 samples
+    (whole workgroups per row, several 4,096-element tiles)."""
+    rng = np.random.default_rng(chunk + 11 + int(global_bits))
+    for trial in range(10):
+        N = int(rng.integers(10, 30000))
         F = int(rng.integers(1, 6))
         X = rng.normal(size=(N, F)).astype(np.float32)
         ds = _lib.Dataset(ctx, X, np.zeros(N, np.int64), 1)
-        samples = sorted(rng.choice(N, size=int(rng.integers(1, N + 1)), replace=False).tolist())
-        rows = _node_rows(X, samples)
+        samples = np.sort(rng.choice(N, size=int(rng.integers(1, N + 1)), replace=False))
+        rows = np.stack([samples[np.argsort(X[samples, f], kind="stable")] for f in range(F)]).astype(np.uint32)
         mark = (rng.random(N) < rng.random()).astype(np.uint8)
-        print("reshuffle chunk=%d trial=%d N=%d n=%d" % (chunk, trial, N, len(samples)), flush=True)
-        got = _lib.reshuffle(ctx, ds, rows, mark, chunk=chunk)
+        print("reshuffle chunk=%d global=%d trial=%d N=%d n=%d" % (chunk, global_bits, trial, N, len(samples)),
+              flush=True)
+        got = _lib.reshuffle(ctx, ds, rows, mark, chunk=chunk, global_bits=global_bits)
         for f in range(F):
             r = rows[f]
             want = np.concatenate([r[mark[r] == 1], r[mark[r] == 0]])
@@ -194,11 +198,11 @@ def test_modes_and_batching_identical(ctx, orc):
     case = {"X": X, "y": y, "K": 7, "m": 5, "mss": 2, "max_depth": 0, "identity": False, "seed": 17,
             "boot": True, "n_trees": 4, "bfs_threshold": 3000}
     want = helpers.oracle_forest(case, orc)
-    for mode, tpb, chunk in [("batched", 0, 0), ("batched", 1, 512), ("batched", 3, 0), ("hybrid", 0, 1024),
-                             ("per_node", 0, 0)]:
-        print("modes: %s tpb=%d chunk=%d" % (mode, tpb, chunk), flush=True)
-        f, _ = gpu_forest(ctx, case, mode=mode, tpb=tpb, scan_chunk=chunk, resh_chunk=chunk)
-        assert helpers.forest_diff(f.export(), want) is None, (mode, tpb, chunk)
+    for mode, tpb, chunk, glob in [("batched", 0, 0, 0), ("batched", 1, 512, 0), ("batched", 3, 0, 1),
+                                   ("hybrid", 0, 1024, 0), ("per_node", 0, 0, 0)]:
+        print("modes: %s tpb=%d chunk=%d global=%d" % (mode, tpb, chunk, glob), flush=True)
+        f, _ = gpu_forest(ctx, case, mode=mode, tpb=tpb, scan_chunk=chunk, resh_chunk=chunk, resh_global=glob)
+        assert helpers.forest_diff(f.export(), want) is None, (mode, tpb, chunk, glob)
 
 
 def test_max_depth_and_min_samples_split(ctx, orc):
