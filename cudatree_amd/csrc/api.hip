@@ -6,6 +6,7 @@
 
 #include <algorithm>
 #include <atomic>
+#include <chrono>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -123,6 +124,43 @@ struct cta_ctx {
   };
   std::vector<Pending> pending;
   std::vector<hipEvent_t> pool;
+  // grow-only device workspaces and pinned host staging, reused across calls
+  std::map<std::string, std::unique_ptr<DevMem>> wspace;
+  std::map<std::string, std::pair<void *, size_t>> pinned;
+
+  void *ws(const std::string &key, size_t bytes) {
+    auto &slot = wspace[key];
+    if (!slot) slot.reset(new DevMem());
+    if (slot->bytes < bytes || !slot->p) slot->alloc(bytes);
+    return slot->p;
+  }
+  template <class T>
+  T *ws_as(const std::string &key, size_t count) {
+    return static_cast<T *>(ws(key, count * sizeof(T)));
+  }
+  size_t ws_bytes() const {
+    size_t b = 0;
+    for (auto &kv : wspace) b += kv.second ? kv.second->bytes : 0;
+    return b;
+  }
+  void ws_release() { wspace.clear(); }
+  void *host(const std::string &key, size_t bytes) {
+    auto &slot = pinned[key];
+    if (slot.second < bytes) {
+      if (slot.first) (void)hipHostFree(slot.first);
+      slot.first = nullptr;
+      slot.second = 0;
+      HIPC(hipHostMalloc(&slot.first, std::max<size_t>(bytes, 64), hipHostMallocDefault));
+      slot.second = bytes;
+    }
+    return slot.first;
+  }
+  void host_time(const char *name, double ms) {
+    if (!profiling) return;
+    KStat &s = stats[name];
+    s.launches += 1;
+    s.ms += ms;
+  }
 
   hipEvent_t ev() {
     if (!pool.empty()) {
@@ -283,27 +321,41 @@ struct Fitter {
         cap_nodes > 0xFFFFFFF0ull || cap_hist > 0xFFFFFFF0ull)
       fail(CTA_EUNSUPPORTED, "batch of %d trees too large for 32-bit task indices", Tb);
 
-    DevMem buf0((size_t)Tb * F * N * sizeof(IdxT)), buf1((size_t)Tb * F * N * sizeof(IdxT));
+    const auto t_setup = std::chrono::steady_clock::now();
     const uint32_t W = ((uint32_t)N + 31u) / 32u;
-    DevMem bits((size_t)Tb * W * 4);
-    DevMem plan_cnt((size_t)Tb * 4 * 2), plan_elems((size_t)Tb * 8), plan_off((size_t)(Tb + 1) * 4 * 3);
-    DevMem front[2] = {DevMem(cap_q * sizeof(FNode)), DevMem(cap_q * sizeof(FNode))};
-    DevMem cnt[2] = {DevMem(cap_q * K * 4), DevMem(cap_q * K * 4)};
-    DevMem cntL(cap_q * K * 4), aux(cap_q * sizeof(NAux)), feats(cap_q * m * 4);
-    DevMem scan_tasks(cap_scan * sizeof(Task)), res(cap_scan * sizeof(ScanRes)), hist(cap_hist * 4);
-    DevMem fill_tasks(cap_fill * sizeof(Task)), big_tasks(cap_big * sizeof(Task)), small_tasks(cap_small * sizeof(Task));
-    DevMem a_feat(cap_nodes * 4), a_thr(cap_nodes * 4), a_left(cap_nodes * 4), a_right(cap_nodes * 4),
-        a_value(cap_nodes * 4), a_nsamp(cap_nodes * 4), a_depth(cap_nodes * 4);
-    DevMem ctl(sizeof(Ctl)), root_cnt((size_t)Tb * K * 4), d_off((Tb + 1) * 8);
-    DevMem d_ids;
+    IdxT *buf0 = ctx->ws_as<IdxT>("buf0", (size_t)Tb * F * N), *buf1 = ctx->ws_as<IdxT>("buf1", (size_t)Tb * F * N);
+    uint32_t *bits = ctx->ws_as<uint32_t>("bits", (size_t)Tb * W);
+    uint32_t *plan_cnt = ctx->ws_as<uint32_t>("plan_cnt", (size_t)Tb * 2);
+    unsigned long long *plan_elems = ctx->ws_as<unsigned long long>("plan_elems", Tb);
+    uint32_t *plan_off = ctx->ws_as<uint32_t>("plan_off", (size_t)(Tb + 1) * 3);
+    FNode *front[2] = {ctx->ws_as<FNode>("front0", cap_q), ctx->ws_as<FNode>("front1", cap_q)};
+    uint32_t *cnt[2] = {ctx->ws_as<uint32_t>("cnt0", cap_q * K), ctx->ws_as<uint32_t>("cnt1", cap_q * K)};
+    uint32_t *cntL = ctx->ws_as<uint32_t>("cntL", cap_q * K);
+    NAux *aux = ctx->ws_as<NAux>("aux", cap_q);
+    int32_t *feats = ctx->ws_as<int32_t>("feats", cap_q * m);
+    Task *scan_tasks = ctx->ws_as<Task>("scan_tasks", cap_scan);
+    ScanRes *res = ctx->ws_as<ScanRes>("res", cap_scan);
+    uint32_t *hist = ctx->ws_as<uint32_t>("hist", cap_hist);
+    Task *fill_tasks = ctx->ws_as<Task>("fill_tasks", cap_fill);
+    Task *big_tasks = ctx->ws_as<Task>("big_tasks", cap_big);
+    Task *small_tasks = ctx->ws_as<Task>("small_tasks", cap_small);
+    int32_t *a_feat = ctx->ws_as<int32_t>("a_feat", cap_nodes);
+    float *a_thr = ctx->ws_as<float>("a_thr", cap_nodes);
+    uint32_t *a_left = ctx->ws_as<uint32_t>("a_left", cap_nodes), *a_right = ctx->ws_as<uint32_t>("a_right", cap_nodes);
+    uint32_t *a_value = ctx->ws_as<uint32_t>("a_value", cap_nodes), *a_nsamp = ctx->ws_as<uint32_t>("a_nsamp", cap_nodes);
+    uint32_t *a_depth = ctx->ws_as<uint32_t>("a_depth", cap_nodes);
+    Ctl *ctl = ctx->ws_as<Ctl>("ctl", 1);
+    uint32_t *root_cnt = ctx->ws_as<uint32_t>("root_cnt", (size_t)Tb * K);
+    uint64_t *d_off = ctx->ws_as<uint64_t>("boot_off", Tb + 1);
+    uint32_t *d_ids = nullptr;
 
     FitArgs<IdxT, LabT> a = {};
     a.X_T = ds->XT.as<float>();
     a.y = ds->y.as<LabT>();
     a.si0 = ds->si0.as<IdxT>();
-    a.buf0 = buf0.as<IdxT>();
-    a.buf1 = buf1.as<IdxT>();
-    a.bits = bits.as<uint32_t>();
+    a.buf0 = buf0;
+    a.buf1 = buf1;
+    a.bits = bits;
     a.W = W;
     a.T = (uint32_t)Tb;
     a.N = N;
@@ -318,24 +370,23 @@ struct Fitter {
     a.Cs = Cs;
     a.Cf = Cf;
     a.Cr = Cr;
-    a.cntL = cntL.as<uint32_t>();
-    a.aux = aux.as<NAux>();
-    a.feats = feats.as<int32_t>();
-    a.scan_tasks = scan_tasks.as<Task>();
-    a.res = res.as<ScanRes>();
-    a.hist = hist.as<uint32_t>();
-    a.fill_tasks = fill_tasks.as<Task>();
-    a.big_tasks = big_tasks.as<Task>();
-    a.small_tasks = small_tasks.as<Task>();
-    a.cnt_big = plan_cnt.as<uint32_t>();
-    a.cnt_small = plan_cnt.as<uint32_t>() + Tb;
-    a.elems = plan_elems.as<unsigned long long>();
-    a.big_off = plan_off.as<uint32_t>();
-    a.small_off = plan_off.as<uint32_t>() + (Tb + 1);
-    a.wg_off = plan_off.as<uint32_t>() + 2 * (Tb + 1);
-    a.arena = Arena{a_feat.as<int32_t>(), a_thr.as<float>(), a_left.as<uint32_t>(), a_right.as<uint32_t>(),
-                    a_value.as<uint32_t>(), a_nsamp.as<uint32_t>(), a_depth.as<uint32_t>()};
-    a.ctl = ctl.as<Ctl>();
+    a.cntL = cntL;
+    a.aux = aux;
+    a.feats = feats;
+    a.scan_tasks = scan_tasks;
+    a.res = res;
+    a.hist = hist;
+    a.fill_tasks = fill_tasks;
+    a.big_tasks = big_tasks;
+    a.small_tasks = small_tasks;
+    a.cnt_big = plan_cnt;
+    a.cnt_small = plan_cnt + Tb;
+    a.elems = plan_elems;
+    a.big_off = plan_off;
+    a.small_off = plan_off + (Tb + 1);
+    a.wg_off = plan_off + 2 * (Tb + 1);
+    a.arena = Arena{a_feat, a_thr, a_left, a_right, a_value, a_nsamp, a_depth};
+    a.ctl = ctl;
     a.cap_q = (uint32_t)cap_q;
     a.cap_scan = (uint32_t)cap_scan;
     a.cap_hist = (uint32_t)cap_hist;
@@ -347,51 +398,54 @@ struct Fitter {
     // ---- setup: bootstrap compaction and roots
     Ctl hc = {};
     hc.n_nodes = (uint32_t)Tb;
-    HIPC(hipMemcpyAsync(ctl.p, &hc, sizeof(Ctl), hipMemcpyHostToDevice, st));
-    HIPC(hipMemcpyAsync(d_off.p, off.data(), (Tb + 1) * 8, hipMemcpyHostToDevice, st));
-    k_fill_u32<<<grid_for((size_t)Tb * K), 256, 0, st>>>(root_cnt.as<uint32_t>(), (size_t)Tb * K, 0u);
-    k_fill_u32<<<1, 256, 0, st>>>(plan_cnt.as<uint32_t>(), (size_t)Tb * 2, 0u);
-    k_fill_u32<<<1, 256, 0, st>>>(plan_elems.as<uint32_t>(), (size_t)Tb * 2, 0u);
+    Ctl *hctl = static_cast<Ctl *>(ctx->host("ctl", sizeof(Ctl)));
+    *hctl = hc;
+    uint64_t *hoff = static_cast<uint64_t *>(ctx->host("boot_off", (Tb + 1) * 8));
+    std::memcpy(hoff, off.data(), (Tb + 1) * 8);
+    HIPC(hipMemcpyAsync(ctl, hctl, sizeof(Ctl), hipMemcpyHostToDevice, st));
+    HIPC(hipMemcpyAsync(d_off, hoff, (Tb + 1) * 8, hipMemcpyHostToDevice, st));
+    k_fill_u32<<<grid_for((size_t)Tb * K), 256, 0, st>>>(root_cnt, (size_t)Tb * K, 0u);
+    k_fill_u32<<<1, 256, 0, st>>>(plan_cnt, (size_t)Tb * 2, 0u);
+    k_fill_u32<<<1, 256, 0, st>>>(reinterpret_cast<uint32_t *>(plan_elems), (size_t)Tb * 2, 0u);
     uint32_t src_root = 0;
     if (boot) {
-      std::vector<uint32_t> ids(n_tot);
+      uint32_t *ids = static_cast<uint32_t *>(ctx->host("boot_ids", n_tot * 4));
       for (int32_t t = 0; t < Tb; ++t)
-        std::memcpy(ids.data() + off[t], boot_ids + boot_off[t0 + t], (off[t + 1] - off[t]) * 4);
-      d_ids.alloc(n_tot * 4);
-      HIPC(hipMemcpyAsync(d_ids.p, ids.data(), n_tot * 4, hipMemcpyHostToDevice, st));
-      k_fill_u32<<<grid_for((size_t)Tb * W), 256, 0, st>>>(bits.as<uint32_t>(), (size_t)Tb * W, 0u);
-      k_boot_mark<<<dim3(grid_for(N, 256, 1024), Tb), 256, 0, st>>>(d_ids.as<uint32_t>(), d_off.as<uint64_t>(), Tb,
-                                                                    W, bits.as<uint32_t>());
-      k_root_counts<LabT><<<dim3(grid_for(N, 256, 256), Tb), 256, K * 4, st>>>(
-          a.y, d_ids.as<uint32_t>(), d_off.as<uint64_t>(), Tb, N, K, root_cnt.as<uint32_t>());
-      a.front = front[0].as<FNode>();
+        std::memcpy(ids + off[t], boot_ids + boot_off[t0 + t], (off[t + 1] - off[t]) * 4);
+      d_ids = ctx->ws_as<uint32_t>("boot_ids", n_tot);
+      HIPC(hipMemcpyAsync(d_ids, ids, n_tot * 4, hipMemcpyHostToDevice, st));
+      k_fill_u32<<<grid_for((size_t)Tb * W), 256, 0, st>>>(bits, (size_t)Tb * W, 0u);
+      k_boot_mark<<<dim3(grid_for(N, 256, 1024), Tb), 256, 0, st>>>(d_ids, d_off, Tb, W, bits);
+      k_root_counts<LabT><<<dim3(grid_for(N, 256, 256), Tb), 256, K * 4, st>>>(a.y, d_ids, d_off, Tb, N, K,
+                                                                              root_cnt);
+      a.front = front[0];
       k_ctl_set<<<1, 64, 0, st>>>(a.ctl, (uint32_t)Tb);
-      k_boot_tasks<IdxT, LabT><<<grid_for((size_t)Tb * 64), 256, 0, st>>>(a, d_off.as<uint64_t>(), Tb,
-                                                                          front[0].as<FNode>(), 0u);
+      k_boot_tasks<IdxT, LabT><<<grid_for((size_t)Tb * 64), 256, 0, st>>>(a, d_off, Tb, front[0], 0u);
       reshuffle(a, "k_boot_compact");
       src_root = 1;
     } else {
       k_root_counts<LabT><<<dim3(grid_for(N, 256, 256), Tb), 256, K * 4, st>>>(a.y, nullptr, nullptr, Tb, N, K,
-                                                                              root_cnt.as<uint32_t>());
+                                                                              root_cnt);
     }
-    a.next_front = front[1].as<FNode>();
-    a.next_cnt = cnt[1].as<uint32_t>();
-    k_roots<IdxT, LabT><<<grid_for((size_t)Tb * 64), 256, 0, st>>>(a, root_cnt.as<uint32_t>(),
-                                                                   boot ? d_off.as<uint64_t>() : nullptr, Tb, src_root);
+    a.next_front = front[1];
+    a.next_cnt = cnt[1];
+    k_roots<IdxT, LabT><<<grid_for((size_t)Tb * 64), 256, 0, st>>>(a, root_cnt, boot ? d_off : nullptr, Tb, src_root);
     k_ctl_next<<<1, 64, 0, st>>>(a.ctl);
     int cur = 1;  // front[cur] holds the frontier
     read_ctl(a, hc);
     if (hc.err) fail(CTA_EINTERNAL, "device capacity check failed during setup (err %u)", hc.err);
     if (boot) ctx->add_bytes("k_boot_compact", hc.bytes_boot);
+    const auto t_levels = std::chrono::steady_clock::now();
+    ctx->host_time("host_setup", std::chrono::duration<double, std::milli>(t_levels - t_setup).count());
 
     const int mode = o.mode;
     if (mode == CTA_MODE_BATCHED) {
       uint32_t q = hc.q;
       while (q > 0) {
-        a.front = front[cur].as<FNode>();
-        a.cnt = cnt[cur].as<uint32_t>();
-        a.next_front = front[cur ^ 1].as<FNode>();
-        a.next_cnt = cnt[cur ^ 1].as<uint32_t>();
+        a.front = front[cur];
+        a.cnt = cnt[cur];
+        a.next_front = front[cur ^ 1];
+        a.next_cnt = cnt[cur ^ 1];
         level(a);
         cur ^= 1;
         Ctl prev = hc;
@@ -414,10 +468,10 @@ struct Fitter {
           continue;
         }
         push_frontier(a, front[0], cnt[0], {e});
-        a.front = front[0].as<FNode>();
-        a.cnt = cnt[0].as<uint32_t>();
-        a.next_front = front[1].as<FNode>();
-        a.next_cnt = cnt[1].as<uint32_t>();
+        a.front = front[0];
+        a.cnt = cnt[0];
+        a.next_front = front[1];
+        a.next_cnt = cnt[1];
         level(a);
         Ctl prev = hc;
         read_ctl(a, hc);
@@ -432,10 +486,10 @@ struct Fitter {
         cur = 0;
         uint32_t q = (uint32_t)queue.size();
         while (q > 0) {
-          a.front = front[cur].as<FNode>();
-          a.cnt = cnt[cur].as<uint32_t>();
-          a.next_front = front[cur ^ 1].as<FNode>();
-          a.next_cnt = cnt[cur ^ 1].as<uint32_t>();
+          a.front = front[cur];
+          a.cnt = cnt[cur];
+          a.next_front = front[cur ^ 1];
+          a.next_cnt = cnt[cur ^ 1];
           level(a);
           cur ^= 1;
           Ctl prev = hc;
@@ -446,12 +500,18 @@ struct Fitter {
         }
       }
     }
+    const auto t_export = std::chrono::steady_clock::now();
+    ctx->host_time("host_levels", std::chrono::duration<double, std::milli>(t_export - t_levels).count());
     export_batch(a, hc.n_nodes, Tb, out);
+    ctx->host_time("host_export",
+                   std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_export).count());
   }
 
   void read_ctl(const FitArgs<IdxT, LabT> &a, Ctl &hc) {
-    HIPC(hipMemcpyAsync(&hc, a.ctl, sizeof(Ctl), hipMemcpyDeviceToHost, st));
+    Ctl *h = static_cast<Ctl *>(ctx->host("ctl", sizeof(Ctl)));
+    HIPC(hipMemcpyAsync(h, a.ctl, sizeof(Ctl), hipMemcpyDeviceToHost, st));
     ctx->sync();
+    hc = *h;
   }
 
   void account(const Ctl &prev, const Ctl &now) {
@@ -513,12 +573,12 @@ struct Fitter {
     }
   }
 
-  void pull_frontier(const DevMem &fr, const DevMem &c, uint32_t q, std::vector<HostEntry> &out) {
+  void pull_frontier(const FNode *fr, const uint32_t *c, uint32_t q, std::vector<HostEntry> &out) {
     if (q == 0) return;
     std::vector<FNode> nodes(q);
     std::vector<uint32_t> counts((size_t)q * K);
-    HIPC(hipMemcpyAsync(nodes.data(), fr.p, q * sizeof(FNode), hipMemcpyDeviceToHost, st));
-    HIPC(hipMemcpyAsync(counts.data(), c.p, (size_t)q * K * 4, hipMemcpyDeviceToHost, st));
+    HIPC(hipMemcpyAsync(nodes.data(), fr, q * sizeof(FNode), hipMemcpyDeviceToHost, st));
+    HIPC(hipMemcpyAsync(counts.data(), c, (size_t)q * K * 4, hipMemcpyDeviceToHost, st));
     ctx->sync();
     for (uint32_t i = 0; i < q; ++i) {
       HostEntry e;
@@ -528,31 +588,33 @@ struct Fitter {
     }
   }
 
-  void push_frontier(const FitArgs<IdxT, LabT> &a, DevMem &fr, DevMem &c, const std::vector<HostEntry> &es) {
+  void push_frontier(const FitArgs<IdxT, LabT> &a, FNode *fr, uint32_t *c, const std::vector<HostEntry> &es) {
     std::vector<FNode> nodes;
     std::vector<uint32_t> counts;
     for (const auto &e : es) {
       nodes.push_back(e.nd);
       counts.insert(counts.end(), e.cnt.begin(), e.cnt.end());
     }
-    HIPC(hipMemcpyAsync(fr.p, nodes.data(), nodes.size() * sizeof(FNode), hipMemcpyHostToDevice, st));
-    HIPC(hipMemcpyAsync(c.p, counts.data(), counts.size() * 4, hipMemcpyHostToDevice, st));
+    HIPC(hipMemcpyAsync(fr, nodes.data(), nodes.size() * sizeof(FNode), hipMemcpyHostToDevice, st));
+    HIPC(hipMemcpyAsync(c, counts.data(), counts.size() * 4, hipMemcpyHostToDevice, st));
     k_ctl_set<<<1, 64, 0, st>>>(a.ctl, (uint32_t)es.size());
     // host vectors must outlive the asynchronous copies
     ctx->sync();
   }
 
   void export_batch(const FitArgs<IdxT, LabT> &a, uint32_t n_nodes, int32_t Tb, cta_forest *out) {
-    std::vector<int32_t> feat(n_nodes);
-    std::vector<float> thr(n_nodes);
-    std::vector<uint32_t> lf(n_nodes), rt(n_nodes), val(n_nodes), ns(n_nodes), dp(n_nodes);
-    HIPC(hipMemcpyAsync(feat.data(), a.arena.feat, n_nodes * 4, hipMemcpyDeviceToHost, st));
-    HIPC(hipMemcpyAsync(thr.data(), a.arena.thr, n_nodes * 4, hipMemcpyDeviceToHost, st));
-    HIPC(hipMemcpyAsync(lf.data(), a.arena.left, n_nodes * 4, hipMemcpyDeviceToHost, st));
-    HIPC(hipMemcpyAsync(rt.data(), a.arena.right, n_nodes * 4, hipMemcpyDeviceToHost, st));
-    HIPC(hipMemcpyAsync(val.data(), a.arena.value, n_nodes * 4, hipMemcpyDeviceToHost, st));
-    HIPC(hipMemcpyAsync(ns.data(), a.arena.nsamp, n_nodes * 4, hipMemcpyDeviceToHost, st));
-    HIPC(hipMemcpyAsync(dp.data(), a.arena.depth, n_nodes * 4, hipMemcpyDeviceToHost, st));
+    uint32_t *stage = static_cast<uint32_t *>(ctx->host("export", (size_t)n_nodes * 4 * 7));
+    int32_t *feat = reinterpret_cast<int32_t *>(stage);
+    float *thr = reinterpret_cast<float *>(stage + (size_t)n_nodes);
+    uint32_t *lf = stage + 2 * (size_t)n_nodes, *rt = stage + 3 * (size_t)n_nodes, *val = stage + 4 * (size_t)n_nodes;
+    uint32_t *ns = stage + 5 * (size_t)n_nodes, *dp = stage + 6 * (size_t)n_nodes;
+    HIPC(hipMemcpyAsync(feat, a.arena.feat, n_nodes * 4, hipMemcpyDeviceToHost, st));
+    HIPC(hipMemcpyAsync(thr, a.arena.thr, n_nodes * 4, hipMemcpyDeviceToHost, st));
+    HIPC(hipMemcpyAsync(lf, a.arena.left, n_nodes * 4, hipMemcpyDeviceToHost, st));
+    HIPC(hipMemcpyAsync(rt, a.arena.right, n_nodes * 4, hipMemcpyDeviceToHost, st));
+    HIPC(hipMemcpyAsync(val, a.arena.value, n_nodes * 4, hipMemcpyDeviceToHost, st));
+    HIPC(hipMemcpyAsync(ns, a.arena.nsamp, n_nodes * 4, hipMemcpyDeviceToHost, st));
+    HIPC(hipMemcpyAsync(dp, a.arena.depth, n_nodes * 4, hipMemcpyDeviceToHost, st));
     ctx->sync();
     std::vector<uint32_t> order, pos(n_nodes, 0xFFFFFFFFu), stk;
     for (int32_t t = 0; t < Tb; ++t) {
@@ -594,6 +656,7 @@ void fit_impl(cta_ctx *ctx, cta_dataset *ds, int32_t T, const uint32_t *ids, con
   if (per <= 0) {
     size_t fr = 0, tot = 0;
     HIPC(hipMemGetInfo(&fr, &tot));
+    fr += ctx->ws_bytes();  // workspaces are reused by this fit
     const size_t bpt = fz.bytes_per_tree();
     per = (int32_t)std::max<size_t>(1, (size_t)(0.6 * (double)fr) / bpt);
   }
@@ -606,18 +669,18 @@ void presort(cta_ctx *ctx, cta_dataset *ds) {
   const uint32_t N = ds->N, F = ds->F;
   const size_t total = (size_t)N * F;
   hipStream_t st = ctx->stream;
-  DevMem k0(total * 4), k1(total * 4), v0(total * 4), v1(total * 4);
+  uint32_t *k0 = ctx->ws_as<uint32_t>("sort_k0", total), *k1 = ctx->ws_as<uint32_t>("sort_k1", total);
+  uint32_t *v0 = ctx->ws_as<uint32_t>("sort_v0", total), *v1 = ctx->ws_as<uint32_t>("sort_v1", total);
   const uint32_t ntiles = (N + kSortTile - 1) / kSortTile;
-  DevMem hist((size_t)F * kDigits * ntiles * 4);
-  k_sort_init<<<grid_for(total), 256, 0, st>>>(ds->XT.as<float>(), N, F, k0.as<uint32_t>(), v0.as<uint32_t>());
-  uint32_t *kin = k0.as<uint32_t>(), *vin = v0.as<uint32_t>(), *kout = k1.as<uint32_t>(), *vout = v1.as<uint32_t>();
+  uint32_t *hist = ctx->ws_as<uint32_t>("sort_hist", (size_t)F * kDigits * ntiles);
+  k_sort_init<<<grid_for(total), 256, 0, st>>>(ds->XT.as<float>(), N, F, k0, v0);
+  uint32_t *kin = k0, *vin = v0, *kout = k1, *vout = v1;
   {
     Timed tm(ctx, "k_presort");
     for (uint32_t shift = 0; shift < 32; shift += 4) {
-      k_sort_hist<<<dim3(ntiles, F), kSortThreads, 0, st>>>(kin, N, ntiles, shift, hist.as<uint32_t>());
-      k_sort_scan<<<F, kSortThreads, 0, st>>>(hist.as<uint32_t>(), ntiles);
-      k_sort_scatter<<<dim3(ntiles, F), kSortThreads, 0, st>>>(kin, vin, kout, vout, N, ntiles, shift,
-                                                               hist.as<uint32_t>());
+      k_sort_hist<<<dim3(ntiles, F), kSortThreads, 0, st>>>(kin, N, ntiles, shift, hist);
+      k_sort_scan<<<F, kSortThreads, 0, st>>>(hist, ntiles);
+      k_sort_scatter<<<dim3(ntiles, F), kSortThreads, 0, st>>>(kin, vin, kout, vout, N, ntiles, shift, hist);
       std::swap(kin, kout);
       std::swap(vin, vout);
     }
@@ -631,21 +694,21 @@ void presort(cta_ctx *ctx, cta_dataset *ds) {
 void dataset_build(cta_ctx *ctx, cta_dataset *ds, const float *X_dev, const uint32_t *y_dev) {
   const uint32_t N = ds->N, F = ds->F;
   hipStream_t st = ctx->stream;
-  DevMem err(4);
-  HIPC(hipMemsetAsync(err.p, 0, 4, st));
+  uint32_t *err = ctx->ws_as<uint32_t>("ds_err", 1);
+  k_fill_u32<<<1, 64, 0, st>>>(err, 1, 0u);
   ds->XT.alloc((size_t)N * F * 4);
-  k_transpose<<<dim3((N + kTT - 1) / kTT, (F + kTT - 1) / kTT), 256, 0, st>>>(X_dev, N, F, ds->XT.as<float>(),
-                                                                             err.as<uint32_t>());
+  k_transpose<<<dim3((N + kTT - 1) / kTT, (F + kTT - 1) / kTT), 256, 0, st>>>(X_dev, N, F, ds->XT.as<float>(), err);
   if (ds->lab_bits == 8) {
     ds->y.alloc(N);
-    k_labels<uint8_t><<<grid_for(N), 256, 0, st>>>(y_dev, N, ds->K, ds->y.as<uint8_t>(), err.as<uint32_t>());
+    k_labels<uint8_t><<<grid_for(N), 256, 0, st>>>(y_dev, N, ds->K, ds->y.as<uint8_t>(), err);
   } else {
     ds->y.alloc((size_t)N * 2);
-    k_labels<uint16_t><<<grid_for(N), 256, 0, st>>>(y_dev, N, ds->K, ds->y.as<uint16_t>(), err.as<uint32_t>());
+    k_labels<uint16_t><<<grid_for(N), 256, 0, st>>>(y_dev, N, ds->K, ds->y.as<uint16_t>(), err);
   }
-  uint32_t herr = 0;
-  HIPC(hipMemcpyAsync(&herr, err.p, 4, hipMemcpyDeviceToHost, st));
+  uint32_t *hp = static_cast<uint32_t *>(ctx->host("ds_err", 4));
+  HIPC(hipMemcpyAsync(hp, err, 4, hipMemcpyDeviceToHost, st));
   ctx->sync();
+  const uint32_t herr = *hp;
   if (herr & 1u) fail(CTA_EINVAL, "X contains NaN");
   if (herr & 2u) fail(CTA_EINVAL, "a label is outside [0, K)");
   if (ds->idx_bits == 16) presort<uint16_t>(ctx, ds);
@@ -696,6 +759,9 @@ int cta_ctx_destroy(cta_ctx *ctx) {
       (void)hipEventDestroy(p.b);
     }
     for (auto e : ctx->pool) (void)hipEventDestroy(e);
+    ctx->wspace.clear();
+    for (auto &kv : ctx->pinned)
+      if (kv.second.first) (void)hipHostFree(kv.second.first);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
   });

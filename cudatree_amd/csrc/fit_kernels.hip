@@ -154,7 +154,20 @@ __global__ __launch_bounds__(kBlock) void k_scan_hist(FitArgs<IdxT, LabT> a) {
     for (uint32_t k = lane; k < K; k += kWave) h[k] = 0;
     wave_sync();
     const uint32_t p0 = tk.chunk * a.Cs, p1 = min(nd.n, p0 + a.Cs);
-    for (uint32_t p = p0 + lane; p < p1; p += kWave) atomicAdd(&h[a.y[row[p]]], 1u);
+    for (uint32_t pb = p0; pb < p1; pb += 4 * kWave) {
+      uint32_t sid[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const uint32_t p = pb + (uint32_t)u * kWave + lane;
+        sid[u] = (p < p1) ? (uint32_t)row[p] : 0xFFFFFFFFu;
+      }
+      uint32_t l[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) l[u] = (sid[u] != 0xFFFFFFFFu) ? (uint32_t)a.y[sid[u]] : 0xFFFFFFFFu;
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+        if (l[u] != 0xFFFFFFFFu) atomicAdd(&h[l[u]], 1u);
+    }
     wave_sync();
     uint32_t *dst = a.hist + ax.hist_base + ((size_t)j * tk.nch + tk.chunk) * K;
     for (uint32_t k = lane; k < K; k += kWave) dst[k] = h[k];
@@ -185,8 +198,12 @@ __global__ __launch_bounds__(kBlock) void k_scan_carry(FitArgs<IdxT, LabT> a) {
 }
 
 // ------------------------------------------------------------------------
-// k_scan: Gini scan of the task's chunk in each of its features.
+// k_scan: Gini scan of the task's chunk in each of its features.  A wave
+// keeps kScanU 64-position tiles in flight: their index loads, then their
+// label and value gathers, are issued together before the tiles are scanned
+// in order.
 // ------------------------------------------------------------------------
+constexpr int kScanU = 4;
 template <class IdxT, class LabT>
 __global__ __launch_bounds__(kBlock) void k_scan(FitArgs<IdxT, LabT> a) {
   extern __shared__ uint32_t smem[];
@@ -229,46 +246,71 @@ __global__ __launch_bounds__(kBlock) void k_scan(FitArgs<IdxT, LabT> a) {
       uint64_t D = uniform64(wave_sum64(dd));
       wave_sync();
       Cand lb = {0, 1, 0, (int32_t)f, 0};
-      for (uint32_t pb = p0; pb < p1; pb += kWave) {
-        const uint32_t p = pb + lane;
-        const bool valid = p < p1;
-        uint32_t lab = 0;
-        float val = 0.f;
-        if (valid) {
-          const uint32_t s = row[p];
-          lab = a.y[s];
-          val = x[s];
+      // value just past the chunk end (the next chunk's first sample)
+      const float chunk_next = (p1 < n) ? x[row[p1]] : 0.f;
+      for (uint32_t pb = p0; pb < p1; pb += kWave * kScanU) {
+        // kScanU tiles in flight: index loads, then label and value gathers
+        uint32_t sid[kScanU], lab[kScanU];
+        float val[kScanU];
+#pragma unroll
+        for (int u = 0; u < kScanU; ++u) {
+          const uint32_t p = pb + (uint32_t)u * kWave + lane;
+          sid[u] = (p < p1) ? (uint32_t)row[p] : 0xFFFFFFFFu;
         }
-        // rank of this lane among earlier lanes with the same label
-        uint32_t cprev = 0;
-        uint64_t rem = ballot(valid);
-        while (rem) {
-          const uint32_t leader = (uint32_t)__ffsll((unsigned long long)rem) - 1u;
-          const uint32_t L = readlane(lab, leader);
-          const uint64_t mm = ballot(valid && lab == L);
-          const uint32_t b0 = carry[L];
-          if (valid && lab == L) cprev = b0 + mbcnt(mm);
-          wave_sync();
-          if (lane == 0) carry[L] = b0 + (uint32_t)__popcll(mm);
-          wave_sync();
-          rem &= ~mm;
+        const uint32_t pe = pb + kWave * kScanU;  // first position after the batch
+        const float batch_next = (pe < p1 && lane == kWave - 1) ? x[row[pe]] : 0.f;
+#pragma unroll
+        for (int u = 0; u < kScanU; ++u) {
+          lab[u] = 0;
+          val[u] = 0.f;
+          if (sid[u] != 0xFFFFFFFFu) {
+            lab[u] = a.y[sid[u]];
+            val[u] = x[sid[u]];
+          }
         }
-        const uint32_t dS = valid ? 2u * cprev + 1u : 0u;
-        const uint32_t dD = valid ? tot[lab] : 0u;
-        const uint32_t iS = wave_incl_scan(dS);
-        const uint32_t iD = wave_incl_scan(dD);
-        const uint64_t SLp = SL + iS, Dp = D + iD;
-        SL += readlane(iS, kWave - 1);
-        D += readlane(iD, kWave - 1);
-        float nxt = __shfl_down(val, 1, kWave);
-        const bool need = valid && (lane == kWave - 1 || p + 1 == p1) && (p + 1 < n);
-        if (need) nxt = x[row[p + 1]];
-        const bool elig = valid && (p + 1 < n) && gap_ok(val, nxt);
-        if (elig) {
-          const uint64_t nL = (uint64_t)p + 1, nR = (uint64_t)n - nL;
-          const uint64_t SR = T2 + SLp - 2 * Dp;
-          Cand c = {SLp * nR + SR * nL, nL * nR, p, (int32_t)f, 1u};
-          if (!lb.ok || q_cmp(c.num, c.den, lb.num, lb.den) > 0) lb = c;
+#pragma unroll
+        for (int u = 0; u < kScanU; ++u) {
+          const uint32_t p = pb + (uint32_t)u * kWave + lane;
+          const bool valid = p < p1;
+          if (!ballot(valid)) break;
+          // rank of this lane among earlier lanes with the same label
+          uint32_t cprev = 0;
+          uint64_t rem = ballot(valid);
+          while (rem) {
+            const uint32_t leader = (uint32_t)__ffsll((unsigned long long)rem) - 1u;
+            const uint32_t L = readlane(lab[u], leader);
+            const uint64_t mm = ballot(valid && lab[u] == L);
+            const uint32_t b0 = carry[L];
+            if (valid && lab[u] == L) cprev = b0 + mbcnt(mm);
+            wave_sync();
+            if (lane == 0) carry[L] = b0 + (uint32_t)__popcll(mm);
+            wave_sync();
+            rem &= ~mm;
+          }
+          const uint32_t dS = valid ? 2u * cprev + 1u : 0u;
+          const uint32_t dD = valid ? tot[lab[u]] : 0u;
+          const uint32_t iS = wave_incl_scan(dS);
+          const uint32_t iD = wave_incl_scan(dD);
+          const uint64_t SLp = SL + iS, Dp = D + iD;
+          SL += readlane(iS, kWave - 1);
+          D += readlane(iD, kWave - 1);
+          // value at p + 1: the next lane, the next tile's lane 0, the next
+          // batch, or the next chunk
+          float nxt = __shfl_down(val[u], 1, kWave);
+          if (u + 1 < kScanU) {
+            const float first_next = __shfl(val[u + 1 < kScanU ? u + 1 : u], 0, kWave);
+            if (lane == kWave - 1) nxt = first_next;
+          } else if (lane == kWave - 1) {
+            nxt = batch_next;
+          }
+          if (p + 1 == p1) nxt = chunk_next;
+          const bool elig = valid && (p + 1 < n) && gap_ok(val[u], nxt);
+          if (elig) {
+            const uint64_t nL = (uint64_t)p + 1, nR = (uint64_t)n - nL;
+            const uint64_t SR = T2 + SLp - 2 * Dp;
+            Cand c = {SLp * nR + SR * nL, nL * nR, p, (int32_t)f, 1u};
+            if (!lb.ok || q_cmp(c.num, c.den, lb.num, lb.den) > 0) lb = c;
+          }
         }
       }
       lb = wave_best(lb);
@@ -378,14 +420,30 @@ __global__ __launch_bounds__(kBlock) void k_fill(FitArgs<IdxT, LabT> a) {
     for (uint32_t k = lane; k < K; k += kWave) h[k] = 0;
     wave_sync();
     const uint32_t p0 = tk.chunk * a.Cf, p1 = min(nd.n, p0 + a.Cf);
-    for (uint32_t p = p0 + lane; p < p1; p += kWave) {
-      const uint32_t s = row[p];
-      const uint32_t bit = 1u << (s & 31u);
-      if (p <= ax.best_p) {
-        atomicOr(&bits[s >> 5], bit);
-        atomicAdd(&h[a.y[s]], 1u);
-      } else {
-        atomicAnd(&bits[s >> 5], ~bit);
+    for (uint32_t pb = p0; pb < p1; pb += 4 * kWave) {
+      uint32_t sid[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const uint32_t p = pb + (uint32_t)u * kWave + lane;
+        sid[u] = (p < p1) ? (uint32_t)row[p] : 0xFFFFFFFFu;
+      }
+      uint32_t l[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const uint32_t p = pb + (uint32_t)u * kWave + lane;
+        l[u] = (sid[u] != 0xFFFFFFFFu && p <= ax.best_p) ? (uint32_t)a.y[sid[u]] : 0xFFFFFFFFu;
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        if (sid[u] == 0xFFFFFFFFu) continue;
+        const uint32_t s = sid[u];
+        const uint32_t bit = 1u << (s & 31u);
+        if (l[u] != 0xFFFFFFFFu) {
+          atomicOr(&bits[s >> 5], bit);
+          atomicAdd(&h[l[u]], 1u);
+        } else {
+          atomicAnd(&bits[s >> 5], ~bit);
+        }
       }
     }
     wave_sync();
