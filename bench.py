@@ -30,7 +30,7 @@ def log(msg):
     print("[bench] " + msg, file=sys.stderr, flush=True)
 
 
-def fit_once(ctx, lib, Xd, yd, N, F, K, ids, mode="batched", n_trees=N_TREES):
+def fit_once(ctx, lib, Xd, yd, N, F, K, mode="batched", n_trees=N_TREES):
     ds_h = lib.load()
     import ctypes
 
@@ -43,10 +43,86 @@ def fit_once(ctx, lib, Xd, yd, N, F, K, ids, mode="batched", n_trees=N_TREES):
     o.seed = SEED
     o.max_features = MAX_FEATURES
     o.mode = lib.MODES[mode]
-    forest = lib.Forest.fit(ctx, ds, n_trees, o, ids)
+    o.bootstrap = 1  # in-bag sets drawn on the device from (seed, tree)
+    forest = lib.Forest.fit(ctx, ds, n_trees, o, None)
     arrays = forest.export()
     ds.close()
     return forest, arrays
+
+
+def serving_bench(lib, X, y, Xt, args):
+    """Resident forest of the tabular type (serving_trees trees, depth <= 24)
+    behind the serving loop; a Poisson stream of predict requests
+    (log-uniform 1..65,536 rows) mixed with small fit jobs (1,000-200,000 rows,
+    10-50 trees), replayed with coalesced and with per-request launches."""
+    import threading
+
+    ctx = lib.default_context(0)
+    ds = lib.Dataset(ctx, X, y, int(y.max()) + 1)
+    o = lib.default_opts()
+    o.seed = SEED + 1
+    o.max_features = MAX_FEATURES
+    o.max_depth = 24
+    o.bootstrap = 1
+    t = time.perf_counter()
+    forest = lib.Forest.fit(ctx, ds, args.serving_trees, o, None)
+    t_fit = time.perf_counter() - t
+    ds.close()
+    ctx.release_workspace()
+    log("serving forest: %d trees, %d nodes, fit %.2fs" % (forest.n_trees, forest.n_nodes, t_fit))
+    rng = np.random.default_rng(11)
+    dur = args.serving_seconds
+    n_req = int(args.serving_qps * dur)
+    arrivals = np.cumsum(rng.exponential(1.0 / args.serving_qps, size=n_req))
+    sizes = np.minimum(np.exp(rng.uniform(0, np.log(65536), size=n_req)).astype(int), 65536)
+    starts = rng.integers(0, len(Xt) - 65536, size=n_req)
+    fits = []
+    for k in range(int(dur // 2)):
+        n = int(np.exp(rng.uniform(np.log(1000), np.log(200000))))
+        fits.append((2.0 * k + 1.0, n, int(rng.integers(10, 51)), int(rng.integers(0, len(X) - n))))
+    result = {"forest_trees": forest.n_trees, "forest_nodes": forest.n_nodes, "forest_fit_s": round(t_fit, 2),
+              "stream": "Poisson %.0f req/s for %.0fs, predict rows log-uniform 1..65536, %d fit jobs "
+                        "(1k-200k rows, 10-50 trees)" % (args.serving_qps, dur, len(fits))}
+    for coalesce in (True, False):
+        srv = lib.Server(coalesce=coalesce, fit_workers=1)
+        fid = srv.add_forest(forest)
+        tickets, fit_tickets = [], []
+        t0 = time.perf_counter()
+
+        def producer():
+            fi = 0
+            for i in range(n_req):
+                while fi < len(fits) and fits[fi][0] <= arrivals[i]:
+                    _, n, T, a = fits[fi]
+                    fo = lib.default_opts()
+                    fo.seed = 5 + fi
+                    fo.max_features = MAX_FEATURES
+                    fo.bootstrap = 1
+                    fit_tickets.append(srv.submit_fit(X[a:a + n], y[a:a + n], int(y.max()) + 1, T, fo))
+                    fi += 1
+                delay = arrivals[i] - (time.perf_counter() - t0)
+                if delay > 0:
+                    time.sleep(delay)
+                tickets.append(srv.submit_predict(fid, Xt[starts[i]:starts[i] + sizes[i]]))
+
+        th = threading.Thread(target=producer)
+        th.start()
+        th.join()
+        for tk in tickets:
+            st, _ = srv.wait(tk)
+            assert st == 0
+        for tk in fit_tickets:
+            srv.wait(tk)
+        s = srv.stats()
+        srv.close()
+        key = "coalesced" if coalesce else "per_request"
+        result[key] = {"p50_ms": round(s["p50_ms"], 3), "p99_ms": round(s["p99_ms"], 3),
+                       "rows_per_s": round(s["rows_per_s"], 1), "requests": s["predict_requests"],
+                       "launches": s["predict_launches"], "fit_jobs": s["fit_jobs"],
+                       "hbm_peak_gb": round(s["hbm_peak"] / 1e9, 2)}
+        log("serving %s: p50 %.2f ms p99 %.2f ms, %.0f rows/s, %d launches for %d requests" %
+            (key, s["p50_ms"], s["p99_ms"], s["rows_per_s"], s["predict_launches"], s["predict_requests"]))
+    return result
 
 
 def main():
@@ -58,6 +134,9 @@ def main():
     ap.add_argument("--per-node-trees", type=int, default=1, help="trees in the per-node GPU sample (0: skip)")
     ap.add_argument("--sklearn-trees", type=int, default=8, help="trees in the scikit-learn sample (0: skip)")
     ap.add_argument("--no-profile", action="store_true", help="skip per-kernel HIP-event timing")
+    ap.add_argument("--serving-seconds", type=float, default=6.0, help="length of each serving stream (0: skip)")
+    ap.add_argument("--serving-trees", type=int, default=500, help="trees of the resident serving forest")
+    ap.add_argument("--serving-qps", type=float, default=100.0, help="predict requests per second")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one MI355X: this build has no multi-GPU path (trees would shard across GPUs "
@@ -73,8 +152,7 @@ def main():
     X, y, Xt, yt = datasets.covtype_like()
     N, F = X.shape
     K = int(y.max()) + 1
-    ids = datasets.bootstrap_ids(N, N_TREES, SEED)
-    log("data %.1fs: N=%d F=%d K=%d, mean in-bag %.0f" % (time.time() - t0, N, F, K, np.mean([len(i) for i in ids])))
+    log("data %.1fs: N=%d F=%d K=%d" % (time.time() - t0, N, F, K))
 
     ctx = lib.default_context(0)
     dev = torch.device("cuda:0")
@@ -85,7 +163,7 @@ def main():
 
     for w in range(args.warmup):
         t = time.time()
-        forest, arrays = fit_once(ctx, lib, Xd, yd, N, F, K, ids)
+        forest, arrays = fit_once(ctx, lib, Xd, yd, N, F, K)
         log("warmup %d: %.3fs, %d nodes" % (w, time.time() - t, len(arrays["feature"])))
         del forest
 
@@ -96,7 +174,7 @@ def main():
     torch.cuda.synchronize()
     t_start = time.perf_counter()
     for s in range(args.steps):
-        forest, arrays = fit_once(ctx, lib, Xd, yd, N, F, K, ids)
+        forest, arrays = fit_once(ctx, lib, Xd, yd, N, F, K)
     ctx.synchronize()
     torch.cuda.synchronize()
     t_fit = (time.perf_counter() - t_start) / args.steps
@@ -159,7 +237,7 @@ def main():
     if args.per_node_trees > 0:
         nt = args.per_node_trees
         t = time.perf_counter()
-        f2, a2 = fit_once(ctx, lib, Xd, yd, N, F, K, ids[:nt], mode="per_node", n_trees=nt)
+        f2, a2 = fit_once(ctx, lib, Xd, yd, N, F, K, mode="per_node", n_trees=nt)
         dt = time.perf_counter() - t
         same = all(np.array_equal(a2[k], arrays[k][: len(a2[k])]) for k in ("feature", "left", "right", "value"))
         per_node = {"sample": "%d of %d trees, per-node launches, scaled x%g" % (nt, N_TREES, N_TREES / nt),
@@ -176,8 +254,9 @@ def main():
 
         o = oracle.get()
         nt = args.cpu_trees
+        ids = o.bootstrap_forest_ids(SEED, nt, N)
         t = time.perf_counter()
-        fo = o.fit(X, y, K, nt, MAX_FEATURES, boot_ids=ids[:nt], seed=SEED)
+        fo = o.fit(X, y, K, nt, MAX_FEATURES, boot_ids=ids, seed=SEED)
         dt = time.perf_counter() - t
         same = all(np.array_equal(fo[k], arrays[k][: len(fo[k])]) for k in ("feature", "left", "right", "value"))
         cpu = {"value": round(dt * N_TREES / nt, 2), "unit": "s/50 trees", "cores": 1, "kind": "port",
@@ -201,6 +280,10 @@ def main():
               "predict_rows_per_s_%d_trees" % nt: round(R / dp, 1)}
         log("sklearn: %.2fs for %d trees on %d cores -> %.1fs/50" % (dt, nt, cores, dt * N_TREES / nt))
 
+    serving = None
+    if args.serving_seconds > 0:
+        serving = serving_bench(lib, X, y, Xt, args)
+
     out = {"metric": METRIC, "value": round(t_fit, 4), "unit": "s/50 trees", "n_gpus": 1, "steps": args.steps,
            "warmup": args.warmup, "ms_per_step": round(t_fit * 1e3, 2), "higher_is_better": False,
            "scaling": "weak", "vs_baseline": None, "dtype": "u64", "data": "synthetic",
@@ -208,7 +291,8 @@ def main():
                                   "max_features=7, full depth; predict 100,000 held-out rows",
                       "n_samples": N, "n_features": F, "n_classes": K, "n_trees": N_TREES,
                       "nodes": int(len(arrays["feature"])), "mode": "batched"},
-           "roofline": roof, "cpu_baseline": cpu, "predict": predict, "per_node_gpu": per_node, "sklearn": sk}
+           "roofline": roof, "cpu_baseline": cpu, "predict": predict, "per_node_gpu": per_node, "sklearn": sk,
+           "serving": serving}
     print(json.dumps(out))
 
 

@@ -43,7 +43,40 @@ class FitOpts(ctypes.Structure):
         ("resh_chunk", ctypes.c_int32),
         ("trees_per_batch", ctypes.c_int32),
         ("resh_global", ctypes.c_int32),
-        ("reserved", ctypes.c_int32 * 4),
+        ("bootstrap", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 3),
+    ]
+
+
+class ServerOpts(ctypes.Structure):
+    _fields_ = [
+        ("device", ctypes.c_int32),
+        ("coalesce", ctypes.c_int32),
+        ("max_batch_rows", ctypes.c_int64),
+        ("hbm_budget", ctypes.c_uint64),
+        ("fit_workers", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 7),
+    ]
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [
+        ("predict_requests", ctypes.c_uint64),
+        ("predict_rows", ctypes.c_uint64),
+        ("predict_launches", ctypes.c_uint64),
+        ("fit_jobs", ctypes.c_uint64),
+        ("fit_rejected", ctypes.c_uint64),
+        ("p50_ms", ctypes.c_double),
+        ("p99_ms", ctypes.c_double),
+        ("mean_ms", ctypes.c_double),
+        ("rows_per_s", ctypes.c_double),
+        ("hbm_budget", ctypes.c_uint64),
+        ("hbm_in_use", ctypes.c_uint64),
+        ("hbm_peak", ctypes.c_uint64),
+        ("resident_forests", ctypes.c_int32),
+        ("queued_predict", ctypes.c_int32),
+        ("queued_fit", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
     ]
 
 
@@ -81,6 +114,22 @@ SIGNATURES = {
                                          _u32p, _u32p, _u32p, ctypes.POINTER(_vp)]),
     "cta_predict": (ctypes.c_int, [_vp, _vp, _f32p, ctypes.c_int64, ctypes.c_int32, _u32p, _u32p]),
     "cta_predict_device": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int64, ctypes.c_int32, _vp, _vp]),
+    "cta_ctx_release_workspace": (ctypes.c_int, [_vp]),
+    "cta_ctx_workspace_bytes": (ctypes.c_int, [_vp, _u64p]),
+    "cta_server_opts_default": (None, [ctypes.POINTER(ServerOpts)]),
+    "cta_server_last_error": (ctypes.c_char_p, []),
+    "cta_server_create": (ctypes.c_int, [ctypes.POINTER(ServerOpts), ctypes.POINTER(_vp)]),
+    "cta_server_destroy": (ctypes.c_int, [_vp]),
+    "cta_server_add_forest": (ctypes.c_int, [_vp, _vp, _i32p]),
+    "cta_server_drop_forest": (ctypes.c_int, [_vp, ctypes.c_int32]),
+    "cta_server_get_forest": (ctypes.c_int, [_vp, ctypes.c_int32, ctypes.POINTER(_vp)]),
+    "cta_server_submit_predict": (ctypes.c_int, [_vp, ctypes.c_int32, _f32p, ctypes.c_int64, ctypes.c_int32, _u32p,
+                                                 _u64p]),
+    "cta_server_submit_fit": (ctypes.c_int, [_vp, _f32p, _u32p, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+                                             ctypes.c_int32, ctypes.POINTER(FitOpts), _u64p]),
+    "cta_server_wait": (ctypes.c_int, [_vp, ctypes.c_uint64, ctypes.c_int64, _i32p, _i32p]),
+    "cta_server_get_stats": (ctypes.c_int, [_vp, ctypes.POINTER(ServerStats)]),
+    "cta_server_reset_stats": (ctypes.c_int, [_vp]),
     "cta_ctx_set_profiling": (ctypes.c_int, [_vp, ctypes.c_int]),
     "cta_ctx_reset_stats": (ctypes.c_int, [_vp]),
     "cta_ctx_kernel_stats": (ctypes.c_int, [_vp, ctypes.POINTER(KernelStat), ctypes.c_int32, _i32p]),
@@ -164,6 +213,9 @@ class Context:
 
     def reset_stats(self):
         check(self.lib.cta_ctx_reset_stats(self.h))
+
+    def release_workspace(self):
+        check(self.lib.cta_ctx_release_workspace(self.h))
 
     def kernel_stats(self):
         n = ctypes.c_int32()
@@ -325,3 +377,99 @@ def reshuffle(ctx, ds, rows, mark, chunk=0, global_bits=False):
     check(ctx.lib.cta_k_reshuffle(ctx.h, ds.h, ptr(rows, _u32p), rows.shape[1], ptr(mark, _u8p), chunk,
                                   int(bool(global_bits)), ptr(out, _u32p)))
     return out
+
+
+def check_server(rc):
+    if rc != 0:
+        msg = load().cta_server_last_error()
+        raise CtaError(rc, msg.decode() if msg else "")
+
+
+class Server:
+    """Serving loop (cta_server): resident forests, coalesced predict
+    requests, fit jobs admitted by HBM accounting."""
+
+    def __init__(self, device=0, coalesce=True, max_batch_rows=0, hbm_budget=0, fit_workers=1):
+        self.lib = load()
+        o = ServerOpts()
+        self.lib.cta_server_opts_default(ctypes.byref(o))
+        o.device = device
+        o.coalesce = int(bool(coalesce))
+        if max_batch_rows:
+            o.max_batch_rows = max_batch_rows
+        o.hbm_budget = hbm_budget
+        o.fit_workers = fit_workers
+        h = _vp()
+        check_server(self.lib.cta_server_create(ctypes.byref(o), ctypes.byref(h)))
+        self.h = h
+        self._keep = {}
+        self._lock = threading.Lock()
+
+    def add_forest(self, forest):
+        fid = ctypes.c_int32()
+        check_server(self.lib.cta_server_add_forest(self.h, forest.h, ctypes.byref(fid)))
+        return fid.value
+
+    def drop_forest(self, fid):
+        check_server(self.lib.cta_server_drop_forest(self.h, fid))
+
+    def export_forest(self, fid):
+        f = _vp()
+        check_server(self.lib.cta_server_get_forest(self.h, fid, ctypes.byref(f)))
+        borrowed = Forest.__new__(Forest)
+        borrowed.lib = self.lib
+        borrowed.h = f
+        T, n, F, K = ctypes.c_int32(), ctypes.c_int64(), ctypes.c_int32(), ctypes.c_int32()
+        check(self.lib.cta_forest_info(f, ctypes.byref(T), ctypes.byref(n), ctypes.byref(F), ctypes.byref(K)))
+        borrowed.n_trees, borrowed.n_nodes, borrowed.F, borrowed.K = T.value, n.value, F.value, K.value
+        out = Forest.export(borrowed)
+        borrowed.h = None  # borrowed: never destroyed from here
+        return out
+
+    def submit_predict(self, fid, X):
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        out = np.zeros(X.shape[0], np.uint32)
+        t = ctypes.c_uint64()
+        check_server(self.lib.cta_server_submit_predict(self.h, fid, ptr(X, _f32p), X.shape[0], X.shape[1],
+                                                        ptr(out, _u32p), ctypes.byref(t)))
+        with self._lock:
+            self._keep[t.value] = (X, out)
+        return t.value
+
+    def submit_fit(self, X, y, K, n_trees, opts):
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        y = np.ascontiguousarray(y, dtype=np.uint32)
+        t = ctypes.c_uint64()
+        check_server(self.lib.cta_server_submit_fit(self.h, ptr(X, _f32p), ptr(y, _u32p), X.shape[0], X.shape[1],
+                                                    K, n_trees, ctypes.byref(opts), ctypes.byref(t)))
+        return t.value
+
+    def wait(self, ticket, timeout_us=-1):
+        """(status, result): result is the labels of a predict ticket or the
+        forest id of a fit ticket; status -1 means still pending."""
+        st, fid = ctypes.c_int32(), ctypes.c_int32()
+        check_server(self.lib.cta_server_wait(self.h, ticket, timeout_us, ctypes.byref(st), ctypes.byref(fid)))
+        if st.value < 0:
+            return -1, None
+        with self._lock:
+            kept = self._keep.pop(ticket, None)
+        return st.value, (kept[1] if kept is not None else fid.value)
+
+    def stats(self):
+        s = ServerStats()
+        check_server(self.lib.cta_server_get_stats(self.h, ctypes.byref(s)))
+        return {k: getattr(s, k) for k, _ in ServerStats._fields_ if k != "reserved"}
+
+    def reset_stats(self):
+        check_server(self.lib.cta_server_reset_stats(self.h))
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.cta_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

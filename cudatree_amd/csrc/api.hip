@@ -301,12 +301,32 @@ struct Fitter {
   }
 
   // Grows trees [t0, t0 + Tb) and appends them to `out` in pre-order.
-  void run_batch(int32_t t0, int32_t Tb, const uint32_t *boot_ids, const int64_t *boot_off, cta_forest *out) {
-    const bool boot = boot_ids != nullptr;
+  void run_batch(int32_t t0, int32_t Tb, int32_t total_trees, const uint32_t *boot_ids, const int64_t *boot_off,
+                 cta_forest *out) {
+    const auto t_setup = std::chrono::steady_clock::now();
+    const bool boot_gen = boot_ids == nullptr && o.bootstrap != 0;
+    const bool boot = boot_ids != nullptr || boot_gen;
+    const uint32_t W = ((uint32_t)N + 31u) / 32u;
+    uint32_t *bits = ctx->ws_as<uint32_t>("bits", (size_t)Tb * W);
     std::vector<uint64_t> off(Tb + 1, 0);
     uint64_t n_tot = 0;
+    std::vector<uint32_t> gen_n;
+    if (boot_gen) {
+      // in-bag bitmaps drawn on the device; their counts size the batch
+      uint32_t *cnts = ctx->ws_as<uint32_t>("boot_counts", Tb);
+      k_fill_u32<<<grid_for((size_t)Tb * W), 256, 0, st>>>(bits, (size_t)Tb * W, 0u);
+      k_fill_u32<<<grid_for(Tb), 256, 0, st>>>(cnts, Tb, 0u);
+      k_boot_gen<<<dim3(grid_for((N + 3) / 4, 256, 256), Tb), 256, 0, st>>>(o.seed, (uint32_t)t0, N, W, bits);
+      k_bits_count<<<dim3(grid_for(W, 256, 64), Tb), 256, 0, st>>>(bits, W, cnts);
+      uint32_t *h = static_cast<uint32_t *>(ctx->host("boot_counts", (size_t)Tb * 4));
+      HIPC(hipMemcpyAsync(h, cnts, (size_t)Tb * 4, hipMemcpyDeviceToHost, st));
+      ctx->sync();
+      gen_n.assign(h, h + Tb);
+    }
     for (int32_t t = 0; t < Tb; ++t) {
-      const uint64_t n = boot ? (uint64_t)(boot_off[t0 + t + 1] - boot_off[t0 + t]) : (uint64_t)N;
+      const uint64_t n = boot_gen ? gen_n[t]
+                         : boot ? (uint64_t)(boot_off[t0 + t + 1] - boot_off[t0 + t])
+                                : (uint64_t)N;
       off[t + 1] = off[t] + n;
       n_tot += n;
     }
@@ -321,10 +341,7 @@ struct Fitter {
         cap_nodes > 0xFFFFFFF0ull || cap_hist > 0xFFFFFFF0ull)
       fail(CTA_EUNSUPPORTED, "batch of %d trees too large for 32-bit task indices", Tb);
 
-    const auto t_setup = std::chrono::steady_clock::now();
-    const uint32_t W = ((uint32_t)N + 31u) / 32u;
     IdxT *buf0 = ctx->ws_as<IdxT>("buf0", (size_t)Tb * F * N), *buf1 = ctx->ws_as<IdxT>("buf1", (size_t)Tb * F * N);
-    uint32_t *bits = ctx->ws_as<uint32_t>("bits", (size_t)Tb * W);
     uint32_t *plan_cnt = ctx->ws_as<uint32_t>("plan_cnt", (size_t)Tb * 2);
     unsigned long long *plan_elems = ctx->ws_as<unsigned long long>("plan_elems", Tb);
     uint32_t *plan_off = ctx->ws_as<uint32_t>("plan_off", (size_t)(Tb + 1) * 3);
@@ -409,15 +426,19 @@ struct Fitter {
     k_fill_u32<<<1, 256, 0, st>>>(reinterpret_cast<uint32_t *>(plan_elems), (size_t)Tb * 2, 0u);
     uint32_t src_root = 0;
     if (boot) {
-      uint32_t *ids = static_cast<uint32_t *>(ctx->host("boot_ids", n_tot * 4));
-      for (int32_t t = 0; t < Tb; ++t)
-        std::memcpy(ids + off[t], boot_ids + boot_off[t0 + t], (off[t + 1] - off[t]) * 4);
-      d_ids = ctx->ws_as<uint32_t>("boot_ids", n_tot);
-      HIPC(hipMemcpyAsync(d_ids, ids, n_tot * 4, hipMemcpyHostToDevice, st));
-      k_fill_u32<<<grid_for((size_t)Tb * W), 256, 0, st>>>(bits, (size_t)Tb * W, 0u);
-      k_boot_mark<<<dim3(grid_for(N, 256, 1024), Tb), 256, 0, st>>>(d_ids, d_off, Tb, W, bits);
-      k_root_counts<LabT><<<dim3(grid_for(N, 256, 256), Tb), 256, K * 4, st>>>(a.y, d_ids, d_off, Tb, N, K,
-                                                                              root_cnt);
+      if (boot_gen) {
+        k_root_counts_bits<LabT><<<dim3(grid_for(N, 256, 256), Tb), 256, K * 4, st>>>(a.y, bits, N, W, K, root_cnt);
+      } else {
+        uint32_t *ids = static_cast<uint32_t *>(ctx->host("boot_ids", n_tot * 4));
+        for (int32_t t = 0; t < Tb; ++t)
+          std::memcpy(ids + off[t], boot_ids + boot_off[t0 + t], (off[t + 1] - off[t]) * 4);
+        d_ids = ctx->ws_as<uint32_t>("boot_ids", n_tot);
+        HIPC(hipMemcpyAsync(d_ids, ids, n_tot * 4, hipMemcpyHostToDevice, st));
+        k_fill_u32<<<grid_for((size_t)Tb * W), 256, 0, st>>>(bits, (size_t)Tb * W, 0u);
+        k_boot_mark<<<dim3(grid_for(N, 256, 1024), Tb), 256, 0, st>>>(d_ids, d_off, Tb, W, bits);
+        k_root_counts<LabT><<<dim3(grid_for(N, 256, 256), Tb), 256, K * 4, st>>>(a.y, d_ids, d_off, Tb, N, K,
+                                                                                root_cnt);
+      }
       a.front = front[0];
       k_ctl_set<<<1, 64, 0, st>>>(a.ctl, (uint32_t)Tb);
       k_boot_tasks<IdxT, LabT><<<grid_for((size_t)Tb * 64), 256, 0, st>>>(a, d_off, Tb, front[0], 0u);
@@ -502,7 +523,7 @@ struct Fitter {
     }
     const auto t_export = std::chrono::steady_clock::now();
     ctx->host_time("host_levels", std::chrono::duration<double, std::milli>(t_export - t_levels).count());
-    export_batch(a, hc.n_nodes, Tb, out);
+    export_batch(a, hc, Tb, Tb == total_trees, out);
     ctx->host_time("host_export",
                    std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_export).count());
   }
@@ -540,6 +561,7 @@ struct Fitter {
       Timed tm(ctx, "k_decide");
       k_decide<IdxT, LabT><<<G, kBlock, 0, st>>>(a);
     }
+    k_fill_u32<<<grid_for((size_t)a.T * a.W), 256, 0, st>>>(a.bits, (size_t)a.T * a.W, 0u);
     {
       Timed tm(ctx, "k_fill");
       k_fill<IdxT, LabT><<<G, kBlock, sm, st>>>(a);
@@ -557,7 +579,7 @@ struct Fitter {
   void reshuffle(const FitArgs<IdxT, LabT> &a, const char *name) {
     k_plan<IdxT, LabT><<<1, 64, 0, st>>>(a);
     k_emit<IdxT, LabT><<<G, kBlock, 0, st>>>(a);
-    const size_t lds = (((size_t)a.W + 3) & ~(size_t)3) * 4 + 2 * 64 * 4;
+    const size_t lds = (((size_t)a.W + 3) & ~(size_t)3) * 4 + 2 * kSlots * 4;
     const uint32_t GR = (uint32_t)std::max(8, ctx->num_cu * 2);
     Timed tm(ctx, name);
     if (!o.resh_global && lds <= ctx->lds_per_block) {
@@ -569,7 +591,7 @@ struct Fitter {
       }
       k_resh<IdxT, LabT, true><<<GR, kReshThreads, lds, st>>>(a);
     } else {
-      k_resh<IdxT, LabT, false><<<GR, kReshThreads, 2 * 64 * 4, st>>>(a);
+      k_resh<IdxT, LabT, false><<<GR, kReshThreads, 2 * kSlots * 4, st>>>(a);
     }
   }
 
@@ -602,48 +624,80 @@ struct Fitter {
     ctx->sync();
   }
 
-  void export_batch(const FitArgs<IdxT, LabT> &a, uint32_t n_nodes, int32_t Tb, cta_forest *out) {
-    uint32_t *stage = static_cast<uint32_t *>(ctx->host("export", (size_t)n_nodes * 4 * 7));
-    int32_t *feat = reinterpret_cast<int32_t *>(stage);
-    float *thr = reinterpret_cast<float *>(stage + (size_t)n_nodes);
-    uint32_t *lf = stage + 2 * (size_t)n_nodes, *rt = stage + 3 * (size_t)n_nodes, *val = stage + 4 * (size_t)n_nodes;
-    uint32_t *ns = stage + 5 * (size_t)n_nodes, *dp = stage + 6 * (size_t)n_nodes;
-    HIPC(hipMemcpyAsync(feat, a.arena.feat, n_nodes * 4, hipMemcpyDeviceToHost, st));
-    HIPC(hipMemcpyAsync(thr, a.arena.thr, n_nodes * 4, hipMemcpyDeviceToHost, st));
-    HIPC(hipMemcpyAsync(lf, a.arena.left, n_nodes * 4, hipMemcpyDeviceToHost, st));
-    HIPC(hipMemcpyAsync(rt, a.arena.right, n_nodes * 4, hipMemcpyDeviceToHost, st));
-    HIPC(hipMemcpyAsync(val, a.arena.value, n_nodes * 4, hipMemcpyDeviceToHost, st));
-    HIPC(hipMemcpyAsync(ns, a.arena.nsamp, n_nodes * 4, hipMemcpyDeviceToHost, st));
-    HIPC(hipMemcpyAsync(dp, a.arena.depth, n_nodes * 4, hipMemcpyDeviceToHost, st));
+  // Canonical pre-order trees of the batch, numbered on the device, appended
+  // to `out`; a single-batch fit also keeps the device predict table.
+  void export_batch(const FitArgs<IdxT, LabT> &a, const Ctl &hc, int32_t Tb, bool only_batch, cta_forest *out) {
+    const uint32_t n = hc.n_nodes, D = hc.max_d;
+    const Arena &ar = a.arena;
+    uint32_t *dcnt = ctx->ws_as<uint32_t>("canon_dcnt", D + 2);
+    uint32_t *cursor = ctx->ws_as<uint32_t>("canon_cursor", D + 2);
+    uint32_t *order = ctx->ws_as<uint32_t>("canon_order", n);
+    uint32_t *size = ctx->ws_as<uint32_t>("canon_size", n);
+    uint32_t *pos = ctx->ws_as<uint32_t>("canon_pos", n);
+    uint32_t *tr = ctx->ws_as<uint32_t>("canon_tr", n);
+    uint32_t *toff = ctx->ws_as<uint32_t>("canon_toff", (size_t)Tb + 1);
+    CanonOut co;
+    co.feat = ctx->ws_as<int32_t>("canon_feat", n);
+    co.thr = ctx->ws_as<float>("canon_thr", n);
+    co.left = ctx->ws_as<uint32_t>("canon_left", n);
+    co.right = ctx->ws_as<uint32_t>("canon_right", n);
+    co.value = ctx->ws_as<uint32_t>("canon_value", n);
+    co.nsamp = ctx->ws_as<uint32_t>("canon_nsamp", n);
+    co.depth = ctx->ws_as<uint32_t>("canon_depth", n);
+    co.pnode = ctx->ws_as<PNode>("canon_pnode", n);
+    k_fill_u32<<<grid_for(D + 2), 256, 0, st>>>(dcnt, D + 2, 0u);
+    k_depth_hist<<<grid_for(n), 256, 0, st>>>(ar.depth, n, dcnt);
+    k_prefix_serial<<<1, 64, 0, st>>>(dcnt, D + 1);
+    HIPC(hipMemcpyAsync(cursor, dcnt, (D + 2) * 4, hipMemcpyDeviceToDevice, st));
+    k_depth_scatter<<<grid_for(n), 256, 0, st>>>(ar.depth, n, cursor, order);
+    uint32_t *hoff = static_cast<uint32_t *>(ctx->host("canon_dcnt", (D + 2) * 4));
+    HIPC(hipMemcpyAsync(hoff, dcnt, (D + 2) * 4, hipMemcpyDeviceToHost, st));
     ctx->sync();
-    std::vector<uint32_t> order, pos(n_nodes, 0xFFFFFFFFu), stk;
-    for (int32_t t = 0; t < Tb; ++t) {
-      order.clear();
-      stk.assign(1, (uint32_t)t);
-      while (!stk.empty()) {
-        const uint32_t id = stk.back();
-        stk.pop_back();
-        if (id >= n_nodes) fail(CTA_EINTERNAL, "node id %u out of range", id);
-        pos[id] = (uint32_t)order.size();
-        order.push_back(id);
-        if (feat[id] >= 0) {
-          stk.push_back(rt[id]);
-          stk.push_back(lf[id]);
-        }
-      }
-      for (uint32_t id : order) {
-        const bool leaf = feat[id] < 0;
-        out->feature.push_back(leaf ? 0 : feat[id]);
-        out->threshold.push_back(leaf ? 0.f : thr[id]);
-        out->left.push_back(leaf ? 0u : pos[lf[id]]);
-        out->right.push_back(leaf ? 0u : pos[rt[id]]);
-        out->value.push_back(leaf ? val[id] : 0u);
-        out->nsamp.push_back(ns[id]);
-        out->depth.push_back(dp[id]);
-      }
-      out->tree_off.push_back(out->tree_off.back() + (int64_t)order.size());
-      out->T += 1;
+    const std::vector<uint32_t> doff(hoff, hoff + D + 2);
+    for (int64_t d = D; d >= 0; --d) {
+      const uint32_t b = doff[d], e = doff[d + 1];
+      if (e > b) k_subtree_size<<<grid_for(e - b), 256, 0, st>>>(order, b, e, ar, size);
     }
+    k_root_sizes<<<grid_for(Tb), 256, 0, st>>>(size, (uint32_t)Tb, toff);
+    k_prefix_serial<<<1, 64, 0, st>>>(toff, (uint32_t)Tb);
+    k_root_pos<<<grid_for(Tb), 256, 0, st>>>((uint32_t)Tb, pos, tr);
+    for (uint32_t d = 0; d <= D; ++d) {
+      const uint32_t b = doff[d], e = doff[d + 1];
+      if (e > b) k_preorder<<<grid_for(e - b), 256, 0, st>>>(order, b, e, ar, size, pos, tr);
+    }
+    k_canon_scatter<<<grid_for(n), 256, 0, st>>>(n, ar, size, pos, tr, toff, co);
+    HIPC(hipGetLastError());
+    // copy out through pinned staging
+    uint32_t *stage = static_cast<uint32_t *>(ctx->host("export", (size_t)n * 4 * 7 + ((size_t)Tb + 1) * 4));
+    const void *srcs[7] = {co.feat, co.thr, co.left, co.right, co.value, co.nsamp, co.depth};
+    for (int k = 0; k < 7; ++k)
+      HIPC(hipMemcpyAsync(stage + (size_t)k * n, srcs[k], (size_t)n * 4, hipMemcpyDeviceToHost, st));
+    uint32_t *htoff = stage + (size_t)7 * n;
+    HIPC(hipMemcpyAsync(htoff, toff, ((size_t)Tb + 1) * 4, hipMemcpyDeviceToHost, st));
+    if (only_batch) {
+      out->nodes.alloc((size_t)n * sizeof(PNode));
+      HIPC(hipMemcpyAsync(out->nodes.p, co.pnode, (size_t)n * sizeof(PNode), hipMemcpyDeviceToDevice, st));
+      out->roots.alloc(((size_t)Tb + 1) * 4);
+      HIPC(hipMemcpyAsync(out->roots.p, toff, (size_t)Tb * 4, hipMemcpyDeviceToDevice, st));
+    }
+    ctx->sync();
+    if (htoff[Tb] != n) fail(CTA_EINTERNAL, "canonical export counted %u of %u nodes", htoff[Tb], n);
+    auto app = [&](auto &vec, int k) {
+      using V = typename std::decay_t<decltype(vec)>::value_type;
+      const V *p = reinterpret_cast<const V *>(stage + (size_t)k * n);
+      vec.insert(vec.end(), p, p + n);
+    };
+    app(out->feature, 0);
+    app(out->threshold, 1);
+    app(out->left, 2);
+    app(out->right, 3);
+    app(out->value, 4);
+    app(out->nsamp, 5);
+    app(out->depth, 6);
+    const int64_t base = out->tree_off.back();
+    for (int32_t t = 1; t <= Tb; ++t) out->tree_off.push_back(base + htoff[t]);
+    out->T += Tb;
+    out->device_ready = only_batch;
   }
 };
 
@@ -661,7 +715,7 @@ void fit_impl(cta_ctx *ctx, cta_dataset *ds, int32_t T, const uint32_t *ids, con
     per = (int32_t)std::max<size_t>(1, (size_t)(0.6 * (double)fr) / bpt);
   }
   per = std::min(per, T);
-  for (int32_t t0 = 0; t0 < T; t0 += per) fz.run_batch(t0, std::min(per, T - t0), ids, off, out);
+  for (int32_t t0 = 0; t0 < T; t0 += per) fz.run_batch(t0, std::min(per, T - t0), T, ids, off, out);
 }
 
 template <class IdxT>
@@ -909,6 +963,7 @@ int cta_fit(cta_ctx *ctx, cta_dataset *ds, int32_t T, const uint32_t *boot_ids, 
     else if (b16) fit_impl<uint16_t, uint16_t>(ctx, ds, T, boot_ids, boot_off, o, f.get());
     else if (l8) fit_impl<uint32_t, uint8_t>(ctx, ds, T, boot_ids, boot_off, o, f.get());
     else fit_impl<uint32_t, uint16_t>(ctx, ds, T, boot_ids, boot_off, o, f.get());
+    upload_forest(f.get());  // the predict table is resident when the fit returns
     *out = f.release();
   });
 }
@@ -1004,12 +1059,7 @@ static void predict_dev(cta_ctx *ctx, cta_forest *f, const float *X, int64_t R, 
   upload_forest(f);
   hipStream_t st = ctx->stream;
   const size_t vk = (size_t)R * f->K;
-  DevMem tmp;
-  uint32_t *v = votes;
-  if (!v) {
-    tmp.alloc(vk * 4);
-    v = tmp.as<uint32_t>();
-  }
+  uint32_t *v = votes ? votes : ctx->ws_as<uint32_t>("pred_votes", vk);
   k_fill_u32<<<grid_for(vk), 256, 0, st>>>(v, vk, 0u);
   {
     Timed tm(ctx, "k_predict");
@@ -1030,12 +1080,13 @@ int cta_predict(cta_ctx *ctx, cta_forest *f, const float *X, int64_t R, int32_t 
     if (R == 0) return;
     std::lock_guard<std::recursive_mutex> lk(ctx->mu);
     HIPC(hipSetDevice(ctx->device));
-    DevMem dX((size_t)R * F * 4), dl((size_t)R * 4), dv;
-    if (votes) dv.alloc((size_t)R * f->K * 4);
-    HIPC(hipMemcpyAsync(dX.p, X, (size_t)R * F * 4, hipMemcpyHostToDevice, ctx->stream));
-    predict_dev(ctx, f, dX.as<float>(), R, dl.as<uint32_t>(), votes ? dv.as<uint32_t>() : nullptr);
-    HIPC(hipMemcpyAsync(labels, dl.p, (size_t)R * 4, hipMemcpyDeviceToHost, ctx->stream));
-    if (votes) HIPC(hipMemcpyAsync(votes, dv.p, (size_t)R * f->K * 4, hipMemcpyDeviceToHost, ctx->stream));
+    float *dX = ctx->ws_as<float>("pred_x", (size_t)R * F);
+    uint32_t *dl = ctx->ws_as<uint32_t>("pred_labels", (size_t)R);
+    uint32_t *dv = votes ? ctx->ws_as<uint32_t>("pred_vout", (size_t)R * f->K) : nullptr;
+    HIPC(hipMemcpyAsync(dX, X, (size_t)R * F * 4, hipMemcpyHostToDevice, ctx->stream));
+    predict_dev(ctx, f, dX, R, dl, dv);
+    HIPC(hipMemcpyAsync(labels, dl, (size_t)R * 4, hipMemcpyDeviceToHost, ctx->stream));
+    if (votes) HIPC(hipMemcpyAsync(votes, dv, (size_t)R * f->K * 4, hipMemcpyDeviceToHost, ctx->stream));
     ctx->sync();
   });
 }
@@ -1051,6 +1102,24 @@ int cta_predict_device(cta_ctx *ctx, cta_forest *f, const float *X_dev, int64_t 
     HIPC(hipSetDevice(ctx->device));
     predict_dev(ctx, f, X_dev, R, labels_dev, votes_dev);
     ctx->sync();
+  });
+}
+
+int cta_ctx_release_workspace(cta_ctx *ctx) {
+  return guarded([&] {
+    if (!ctx) fail(CTA_EINVAL, "ctx is NULL");
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    HIPC(hipSetDevice(ctx->device));
+    ctx->sync();
+    ctx->ws_release();
+  });
+}
+
+int cta_ctx_workspace_bytes(cta_ctx *ctx, uint64_t *bytes) {
+  return guarded([&] {
+    if (!ctx || !bytes) fail(CTA_EINVAL, "NULL argument");
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    *bytes = ctx->ws_bytes();
   });
 }
 
@@ -1292,13 +1361,13 @@ static void hook_reshuffle(cta_ctx *ctx, cta_dataset *ds, const uint32_t *rows, 
   a.cap_small = ntask;
   k_plan<IdxT, LabT><<<1, 64, 0, st>>>(a);
   k_emit<IdxT, LabT><<<8, kBlock, 0, st>>>(a);
-  const size_t lds = (((size_t)W + 3) & ~(size_t)3) * 4 + 2 * 64 * 4;
+  const size_t lds = (((size_t)W + 3) & ~(size_t)3) * 4 + 2 * kSlots * 4;
   if (!global_bits && lds <= ctx->lds_per_block) {
     HIPC(hipFuncSetAttribute((const void *)k_resh<IdxT, LabT, true>, hipFuncAttributeMaxDynamicSharedMemorySize,
                              (int)ctx->lds_per_block));
     k_resh<IdxT, LabT, true><<<16, kReshThreads, lds, st>>>(a);
   } else {
-    k_resh<IdxT, LabT, false><<<16, kReshThreads, 2 * 64 * 4, st>>>(a);
+    k_resh<IdxT, LabT, false><<<16, kReshThreads, 2 * kSlots * 4, st>>>(a);
   }
   HIPC(hipGetLastError());
   std::vector<IdxT> hout(hrows.size());

@@ -179,15 +179,6 @@ __global__ void k_labels(const uint32_t *y, uint32_t N, uint32_t K, LabT *out, u
 }
 
 // ---- predict ----------------------------------------------------------------
-// Flattened forest node: pre-order, so a split node's left child is the next
-// node; leaves have feat = -1 and carry their label in `value`.
-struct PNode {
-  float thr;
-  int32_t feat;
-  uint32_t right;
-  uint32_t value;
-};
-
 __global__ __launch_bounds__(256) void k_predict(const PNode *__restrict__ nodes, const uint32_t *__restrict__ roots,
                                                  uint32_t T, const float *__restrict__ X, uint64_t R, uint32_t F,
                                                  uint32_t K, uint32_t *__restrict__ votes) {

@@ -178,6 +178,21 @@ __device__ __forceinline__ uint32_t feistel(uint32_t x, const Perm &p) {
   return (L << p.h) | R;
 }
 
+// Canonical bootstrap draws of tree `tree`: block `blk` gives draws
+// 4*blk .. 4*blk+3 as the words of Philox4x32-10(counter = (blk, tree, "BOOT",
+// 0), key = seed); draw u maps to sample floor(u * N / 2^32).
+__device__ __forceinline__ void boot_block(uint64_t seed, uint32_t tree, uint32_t blk, uint32_t w[4]) {
+  w[0] = blk;
+  w[1] = tree;
+  w[2] = 0x424F4F54u;
+  w[3] = 0u;
+  philox4x32_10(w, (uint32_t)seed, (uint32_t)(seed >> 32));
+}
+
+__device__ __forceinline__ uint32_t draw_to_sample(uint32_t u, uint32_t N) {
+  return (uint32_t)(((uint64_t)u * N) >> 32);
+}
+
 // pi(k) for k < F.
 __device__ __forceinline__ uint32_t perm_at(const Perm &p, uint32_t k) {
   if (p.identity) return k;

@@ -402,8 +402,9 @@ __global__ __launch_bounds__(kBlock) void k_decide(FitArgs<IdxT, LabT> a) {
 }
 
 // ------------------------------------------------------------------------
-// k_fill: set every sample of the split row's segment to its side in the
-// tree's bitmap (bit set = left) and count the left child's classes.
+// k_fill: set the bit of every left-going sample of the split row's segment
+// in the tree's bitmap (the bitmaps are cleared at the start of the level, so
+// right-going samples read 0) and count the left child's classes.
 // ------------------------------------------------------------------------
 template <class IdxT, class LabT>
 __global__ __launch_bounds__(kBlock) void k_fill(FitArgs<IdxT, LabT> a) {
@@ -435,15 +436,10 @@ __global__ __launch_bounds__(kBlock) void k_fill(FitArgs<IdxT, LabT> a) {
       }
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
-        if (sid[u] == 0xFFFFFFFFu) continue;
+        if (l[u] == 0xFFFFFFFFu) continue;  // right side: its bit stays clear
         const uint32_t s = sid[u];
-        const uint32_t bit = 1u << (s & 31u);
-        if (l[u] != 0xFFFFFFFFu) {
-          atomicOr(&bits[s >> 5], bit);
-          atomicAdd(&h[l[u]], 1u);
-        } else {
-          atomicAnd(&bits[s >> 5], ~bit);
-        }
+        atomicOr(&bits[s >> 5], 1u << (s & 31u));
+        atomicAdd(&h[l[u]], 1u);
       }
     }
     wave_sync();
@@ -545,6 +541,7 @@ __global__ __launch_bounds__(kBlock) void k_children(FitArgs<IdxT, LabT> a) {
     }
     wave_sync();
     const uint32_t cd = nd.depth + 1;
+    if (lane == 0) atomicMax(&a.ctl->max_d, cd);
     const bool capped = a.max_depth > 0 && (int64_t)cd >= (int64_t)a.max_depth;
     // 0 live, 1 pure, 2 first-sample, 3 majority
     auto kind = [&](uint32_t nz, uint32_t nc) -> uint32_t {
@@ -652,8 +649,10 @@ __global__ __launch_bounds__(kBlock) void k_children(FitArgs<IdxT, LabT> a) {
 // ------------------------------------------------------------------------
 constexpr int kReshThreads = 1024;
 constexpr int kReshWaves = kReshThreads / kWave;  // 16
-constexpr int kReshE = 4;                         // elements per thread per tile
+constexpr int kReshE = 8;                         // elements per thread per tile
 constexpr uint32_t kReshTile = kReshThreads * kReshE;
+constexpr uint32_t kSlots = kReshE * kReshWaves;  // 128 (sub-tile, wave) counts per tile
+static_assert(kSlots == 2 * kWave, "the tile prefix takes two slots per lane");
 constexpr uint64_t kWgElems = 1ull << 19;
 
 template <class IdxT, class LabT>
@@ -739,17 +738,20 @@ __device__ void resh_big(const FitArgs<IdxT, LabT> &a, const Task &tk, const Sid
       sd[k] = (p < n) ? side(e[k]) : 0u;
       const uint64_t m = ballot(sd[k] != 0);
       rk[k] = mbcnt(m);
-      if (lane == 0) wcnt[par * 64 + k * kReshWaves + wave] = (uint32_t)__popcll(m);
+      if (lane == 0) wcnt[par * kSlots + k * kReshWaves + wave] = (uint32_t)__popcll(m);
     }
     __syncthreads();
-    // exclusive prefix over the 64 (sub-tile, wave) counts in position order
-    const uint32_t v = wcnt[par * 64 + lane];
-    const uint32_t vi = wave_incl_scan(v);
-    const uint32_t total = readlane(vi, 63);
+    // exclusive prefix over the 128 (sub-tile, wave) counts in position
+    // order, two slots per lane
+    const uint32_t v0 = wcnt[par * kSlots + 2 * lane], v1 = wcnt[par * kSlots + 2 * lane + 1];
+    const uint32_t inc = wave_incl_scan(v0 + v1);
+    const uint32_t e0 = inc - v0 - v1, e1 = e0 + v0;
+    const uint32_t total = readlane(inc, 63);
+    const uint32_t esel = (wave & 1u) ? e1 : e0;  // slot parity = wave parity
 #pragma unroll
     for (int k = 0; k < kReshE; ++k) {
       const uint32_t slot = (uint32_t)k * kReshWaves + wave;
-      const uint32_t before = carry + readlane(vi, slot) - readlane(v, slot);
+      const uint32_t before = carry + readlane(esel, slot >> 1);
       const uint32_t p = base + (uint32_t)k * kReshThreads + tid;
       if (p < n) {
         const uint32_t lb = before + rk[k];
@@ -809,9 +811,9 @@ __device__ void resh_small(const FitArgs<IdxT, LabT> &a, const Task &tk, const S
 // kLds: the tree's bitmap is staged in LDS (N/8 bytes); otherwise it is read
 // from global memory.
 template <class IdxT, class LabT, bool kLds>
-__global__ __launch_bounds__(kReshThreads) void k_resh(FitArgs<IdxT, LabT> a) {
+__global__ __launch_bounds__(kReshThreads, 8) void k_resh(FitArgs<IdxT, LabT> a) {
   extern __shared__ uint32_t lds[];
-  uint32_t *wcnt = lds + (kLds ? ((a.W + 3u) & ~3u) : 0u);  // 2 x 64 counts
+  uint32_t *wcnt = lds + (kLds ? ((a.W + 3u) & ~3u) : 0u);  // 2 x kSlots counts
   const uint32_t P = a.ctl->n_plan;
   const uint32_t T = a.T;
   const uint32_t wave = threadIdx.x / kWave;
@@ -865,6 +867,50 @@ __global__ void k_boot_mark(const uint32_t *ids, const uint64_t *off, uint32_t T
   const uint64_t a = off[t], b = off[t + 1];
   for (uint64_t j = a + (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; j < b; j += (uint64_t)gridDim.x * blockDim.x)
     atomicOr(&bits[(size_t)t * W + (ids[j] >> 5)], 1u << (ids[j] & 31u));
+}
+
+// In-bag bitmaps from the canonical generator: bit s of tree t is set when
+// some draw of tree (tree_base + t) lands on s (bitmaps zeroed first).
+__global__ void k_boot_gen(uint64_t seed, uint32_t tree_base, uint32_t N, uint32_t W, uint32_t *bits) {
+  const uint32_t t = blockIdx.y;
+  uint32_t *b = bits + (size_t)t * W;
+  const uint32_t nblk = (N + 3) / 4;
+  for (uint32_t blk = blockIdx.x * blockDim.x + threadIdx.x; blk < nblk; blk += gridDim.x * blockDim.x) {
+    uint32_t w[4];
+    boot_block(seed, tree_base + t, blk, w);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      if (blk * 4 + (uint32_t)k >= N) break;
+      const uint32_t s = draw_to_sample(w[k], N);
+      atomicOr(&b[s >> 5], 1u << (s & 31u));
+    }
+  }
+}
+
+// Set bits of every tree's bitmap (the in-bag counts).
+__global__ void k_bits_count(const uint32_t *bits, uint32_t W, uint32_t *counts) {
+  const uint32_t t = blockIdx.y;
+  const uint32_t *b = bits + (size_t)t * W;
+  uint32_t c = 0;
+  for (uint32_t w = blockIdx.x * blockDim.x + threadIdx.x; w < W; w += gridDim.x * blockDim.x) c += __popc(b[w]);
+  c = wave_sum32(c);
+  if (lane_id() == 0 && c) atomicAdd(&counts[t], c);
+}
+
+// Class counts of every tree's root from its in-bag bitmap.
+template <class LabT>
+__global__ void k_root_counts_bits(const LabT *y, const uint32_t *bits, uint32_t N, uint32_t W, uint32_t K,
+                                   uint32_t *cnt) {
+  extern __shared__ uint32_t h[];
+  const uint32_t t = blockIdx.y;
+  const uint32_t *b = bits + (size_t)t * W;
+  for (uint32_t k = threadIdx.x; k < K; k += blockDim.x) h[k] = 0;
+  __syncthreads();
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x)
+    if ((b[i >> 5] >> (i & 31u)) & 1u) atomicAdd(&h[y[i]], 1u);
+  __syncthreads();
+  for (uint32_t k = threadIdx.x; k < K; k += blockDim.x)
+    if (h[k]) atomicAdd(&cnt[(size_t)t * K + k], h[k]);
 }
 
 // Class counts of every batch tree's root: over its ids, or over all N.
@@ -972,6 +1018,97 @@ __global__ __launch_bounds__(kBlock) void k_roots(FitArgs<IdxT, LabT> a, const u
       a.next_front[slot] = nd;
     }
     for (uint32_t k = lane; k < K; k += kWave) a.next_cnt[(size_t)slot * K + k] = root_cnt[(size_t)t * K + k];
+  }
+}
+
+// ------------------------------------------------------------------------
+// Canonical export: pre-order numbering of every tree on the device.
+// Nodes are bucketed by depth; subtree sizes are summed bottom-up one depth
+// at a time, pre-order positions assigned top-down (left child = parent + 1,
+// right child = parent + 1 + size of the left subtree), then one scatter
+// writes the canonical arrays and the flattened predict table.
+// ------------------------------------------------------------------------
+__global__ void k_depth_hist(const uint32_t *depth, uint32_t n, uint32_t *cnt) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    atomicAdd(&cnt[depth[i]], 1u);
+}
+
+// In-place exclusive prefix of v[0..n) with the total in v[n] (one thread).
+__global__ void k_prefix_serial(uint32_t *v, uint32_t n) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  uint32_t run = 0;
+  for (uint32_t i = 0; i < n; ++i) {
+    const uint32_t x = v[i];
+    v[i] = run;
+    run += x;
+  }
+  v[n] = run;
+}
+
+__global__ void k_depth_scatter(const uint32_t *depth, uint32_t n, uint32_t *cursor, uint32_t *order) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    order[atomicAdd(&cursor[depth[i]], 1u)] = i;
+}
+
+__global__ void k_subtree_size(const uint32_t *order, uint32_t b, uint32_t e, Arena ar, uint32_t *size) {
+  for (uint32_t k = b + blockIdx.x * blockDim.x + threadIdx.x; k < e; k += gridDim.x * blockDim.x) {
+    const uint32_t id = order[k];
+    size[id] = ar.feat[id] < 0 ? 1u : 1u + size[ar.left[id]] + size[ar.right[id]];
+  }
+}
+
+__global__ void k_root_sizes(const uint32_t *size, uint32_t T, uint32_t *tsz) {
+  for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < T; t += gridDim.x * blockDim.x) tsz[t] = size[t];
+}
+
+// pos: position inside the tree; tr: tree of the node.  Roots are set first.
+__global__ void k_preorder(const uint32_t *order, uint32_t b, uint32_t e, Arena ar, const uint32_t *size,
+                           uint32_t *pos, uint32_t *tr) {
+  for (uint32_t k = b + blockIdx.x * blockDim.x + threadIdx.x; k < e; k += gridDim.x * blockDim.x) {
+    const uint32_t id = order[k];
+    if (ar.feat[id] < 0) continue;
+    const uint32_t l = ar.left[id], r = ar.right[id];
+    pos[l] = pos[id] + 1;
+    pos[r] = pos[id] + 1 + size[l];
+    tr[l] = tr[id];
+    tr[r] = tr[id];
+  }
+}
+
+__global__ void k_root_pos(uint32_t T, uint32_t *pos, uint32_t *tr) {
+  for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < T; t += gridDim.x * blockDim.x) {
+    pos[t] = 0;
+    tr[t] = t;
+  }
+}
+
+struct CanonOut {
+  int32_t *feat;
+  float *thr;
+  uint32_t *left, *right, *value, *nsamp, *depth;
+  PNode *pnode;
+};
+
+__global__ void k_canon_scatter(uint32_t n, Arena ar, const uint32_t *size, const uint32_t *pos, const uint32_t *tr,
+                                const uint32_t *tree_off, CanonOut o) {
+  for (uint32_t id = blockIdx.x * blockDim.x + threadIdx.x; id < n; id += gridDim.x * blockDim.x) {
+    const uint32_t base = tree_off[tr[id]];
+    const uint32_t p = pos[id], g = base + p;
+    const bool leaf = ar.feat[id] < 0;
+    const uint32_t rl = leaf ? 0u : p + 1 + size[ar.left[id]];
+    o.feat[g] = leaf ? 0 : ar.feat[id];
+    o.thr[g] = leaf ? 0.f : ar.thr[id];
+    o.left[g] = leaf ? 0u : p + 1;
+    o.right[g] = rl;
+    o.value[g] = leaf ? ar.value[id] : 0u;
+    o.nsamp[g] = ar.nsamp[id];
+    o.depth[g] = ar.depth[id];
+    PNode pn;
+    pn.thr = leaf ? 0.f : ar.thr[id];
+    pn.feat = leaf ? -1 : ar.feat[id];
+    pn.right = leaf ? 0u : base + rl;
+    pn.value = leaf ? ar.value[id] : 0u;
+    o.pnode[g] = pn;
   }
 }
 

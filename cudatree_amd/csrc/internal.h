@@ -57,10 +57,20 @@ struct Ctl {
   uint32_t n_scan, n_fill, n_hist, n_plan;
   uint32_t n_nodes;
   uint32_t err;
-  uint32_t pad[8];
+  uint32_t max_d;  // deepest node created so far
+  uint32_t pad[7];
   unsigned long long bytes_scan, bytes_fill, bytes_resh, bytes_boot;
 };
 constexpr uint32_t kErrCapacity = 1u, kErrBadNode = 2u;
+
+// Flattened forest node for prediction: pre-order, so a split node's left
+// child is the next node; leaves have feat = -1 and carry their label.
+struct PNode {
+  float thr;
+  int32_t feat;
+  uint32_t right;  // global index of the right child
+  uint32_t value;
+};
 
 // Node storage of a resident batch of trees, indexed by arena id.
 struct Arena {

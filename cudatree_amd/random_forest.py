@@ -4,8 +4,9 @@ Mirrors cudatree/random_forest.py: the constructor arguments
 (random_forest.py:45-51), ``fit(samples, target, bfs_threshold=None)``
 (:292), ``predict`` (:339), ``predict_proba`` (:364), ``score`` (:379) and
 pickling (:42-43).  Keyword-only extensions: ``max_depth``, ``random_state``,
-``mode`` (batched / hybrid / per_node scheduling), ``device`` and the scan /
-reshuffle chunk sizes.
+``mode`` (batched / hybrid / per_node scheduling), ``device``, the scan /
+reshuffle chunk sizes and ``autotune`` (time a probe of the problem to pick
+the schedule).
 """
 import math
 import time
@@ -13,7 +14,6 @@ import time
 import numpy as np
 
 from . import _lib
-from .datasets import bootstrap_ids
 
 
 def _best_bfs_threshold(n_labels, n_samples, max_features):
@@ -56,7 +56,7 @@ class RandomForestClassifier(object):
 
     def __init__(self, n_estimators=10, max_features=None, min_samples_split=1, bootstrap=True, verbose=False,
                  debug=False, *, max_depth=None, random_state=None, mode="batched", device=0, scan_chunk=0,
-                 resh_chunk=0, trees_per_batch=0):
+                 resh_chunk=0, trees_per_batch=0, autotune=False):
         self.n_estimators = n_estimators
         self.max_features = max_features
         self.min_samples_split = min_samples_split
@@ -70,6 +70,8 @@ class RandomForestClassifier(object):
         self.scan_chunk = scan_chunk
         self.resh_chunk = resh_chunk
         self.trees_per_batch = trees_per_batch
+        self.autotune = autotune
+        self.tuned_ = None
         self._forest = None
         self._arrays = None
 
@@ -106,8 +108,11 @@ class RandomForestClassifier(object):
         self.seed_ = seed
         ctx = _lib.default_context(self.device)
         ds = _lib.Dataset(ctx, samples.astype(np.float32), y.astype(np.uint32), self.n_labels)
-        ids = bootstrap_ids(N, self.n_estimators, seed) if self.bootstrap else None
         opts = _lib.default_opts()
+        # in-bag sets drawn on the device from (seed, tree): the sorted unique
+        # ids of N uniform draws, as np.unique(randint(0, N, N)) in the
+        # reference (random_forest.py:123-126)
+        opts.bootstrap = int(bool(self.bootstrap))
         opts.seed = seed
         opts.max_features = m
         opts.min_samples_split = int(self.min_samples_split)
@@ -118,13 +123,22 @@ class RandomForestClassifier(object):
         opts.scan_chunk = int(self.scan_chunk)
         opts.resh_chunk = int(self.resh_chunk)
         opts.trees_per_batch = int(self.trees_per_batch)
+        if self.autotune:
+            # schedule picked by timing a probe of this problem; every setting
+            # grows the same trees
+            from . import autotune as _autotune
+
+            best, _ = _autotune.tune(samples, y.astype(np.uint32), self.n_labels, m, bootstrap=self.bootstrap,
+                                     max_depth=self.max_depth, seed=seed, device=self.device)
+            _autotune.apply(opts, best)
+            self.tuned_ = best
         if self.verbose:
             print("bfs_threshold : %d; bootstrap : %r; min_samples_split : %d; mode : %s"
                   % (self.bfs_threshold, self.bootstrap, self.min_samples_split, self.mode))
             print("n_samples : %d; n_features : %d; n_labels : %d; max_features : %d"
                   % (N, F, self.n_labels, m))
         t0 = time.time()
-        self._forest = _lib.Forest.fit(ctx, ds, self.n_estimators, opts, ids)
+        self._forest = _lib.Forest.fit(ctx, ds, self.n_estimators, opts, None)
         ds.close()
         self._arrays = None
         if self.verbose:

@@ -98,7 +98,10 @@ typedef struct {
   int32_t resh_chunk;        /* samples per small-node reshuffle task (0: default) */
   int32_t trees_per_batch;   /* trees resident at once (0: as many as memory allows) */
   int32_t resh_global;       /* 1: reshuffle reads the side bitmap from global memory instead of LDS */
-  int32_t reserved[4];
+  int32_t bootstrap;         /* 1 with boot_ids == NULL: draw every tree's in-bag set on the device from
+                                (seed, tree) -- the sorted unique values of N draws
+                                floor(u * N / 2^32), u the words of Philox4x32-10((j/4, tree, "BOOT", 0), seed) */
+  int32_t reserved[3];
 } cta_fit_opts;
 
 /* Fills opts with the defaults (batched mode, m = ceil(sqrt(F)) left to the caller). */
@@ -106,7 +109,9 @@ void cta_fit_opts_default(cta_fit_opts *opts);
 
 /* Grows T trees.  boot_ids/boot_off: tree t's in-bag set is the sorted unique
  * ids boot_ids[boot_off[t] .. boot_off[t+1]) (host memory); boot_ids == NULL
- * grows every tree on all N samples (bootstrap=False).
+ * with opts->bootstrap == 1 draws the in-bag sets on the device; boot_ids ==
+ * NULL with opts->bootstrap == 0 grows every tree on all N samples
+ * (bootstrap=False).
  * Replaces: RandomForestClassifier.fit's per-tree loop (random_forest.py:292-336),
  * _get_sorted_indices with bootstrap_fill/bootstrap_reshuffle
  * (random_forest.py:113-144, bootstrap_fill.cu, bootstrap_reshuffle.cu) and
@@ -144,6 +149,68 @@ int cta_predict(cta_ctx *ctx, cta_forest *forest, const float *X, int64_t R, int
  * uint32 or NULL) in device memory. */
 int cta_predict_device(cta_ctx *ctx, cta_forest *forest, const float *X_dev, int64_t R, int32_t F,
                        uint32_t *labels_dev, uint32_t *votes_dev);
+
+/* Frees the context's cached device workspaces (they are reused across calls
+ * and grow to the largest request); *bytes reports their current size. */
+int cta_ctx_release_workspace(cta_ctx *ctx);
+int cta_ctx_workspace_bytes(cta_ctx *ctx, uint64_t *bytes);
+
+/* ---- serving loop -------------------------------------------------------
+ * No reference counterpart: the reference re-uploads rows and trees on every
+ * predict call (base_tree.py:57-62).  A server keeps forests resident, runs a
+ * predict worker and fit workers on their own contexts and streams, coalesces
+ * the rows of queued predict requests for one forest into one traversal
+ * launch (requests arriving while a batch runs join the next batch), and
+ * admits fit jobs only when their estimated device footprint fits the HBM
+ * budget (a job larger than the whole budget fails with CTA_ENOMEM). */
+typedef struct cta_server cta_server;
+
+typedef struct {
+  int32_t device;
+  int32_t coalesce;        /* 1: one launch for the queued requests of a forest; 0: one launch per request */
+  int64_t max_batch_rows;  /* rows per launch at most (0: 1,048,576); also the largest request */
+  uint64_t hbm_budget;     /* device bytes the server may hold (0: 90% of free memory at creation) */
+  int32_t fit_workers;     /* concurrent fit jobs (0: 1) */
+  int32_t reserved[7];
+} cta_server_opts;
+
+typedef struct {
+  uint64_t predict_requests, predict_rows, predict_launches;
+  uint64_t fit_jobs, fit_rejected;
+  double p50_ms, p99_ms, mean_ms;  /* predict latency, submission to completion */
+  double rows_per_s;               /* predict rows over first submission .. last completion */
+  uint64_t hbm_budget, hbm_in_use, hbm_peak;
+  int32_t resident_forests, queued_predict, queued_fit, reserved;
+} cta_server_stats;
+
+void cta_server_opts_default(cta_server_opts *opts);
+/* Thread-local message of the last failed cta_server_* call. */
+const char *cta_server_last_error(void);
+int cta_server_create(const cta_server_opts *opts, cta_server **out);
+/* Drains queued work, stops the workers and frees every resident forest. */
+int cta_server_destroy(cta_server *server);
+/* Makes a resident copy of `forest` on the server's device. */
+int cta_server_add_forest(cta_server *server, const cta_forest *forest, int32_t *forest_id);
+/* Borrowed handle of a resident forest (valid until it is dropped), e.g. to
+ * export a forest fitted by the server. */
+int cta_server_get_forest(cta_server *server, int32_t forest_id, cta_forest **forest);
+/* Frees a resident forest once its queued requests are done. */
+int cta_server_drop_forest(cta_server *server, int32_t forest_id);
+/* Queues R rows (host, R x F, valid until the ticket completes); the labels
+ * (R uint32) are written before the ticket completes. */
+int cta_server_submit_predict(cta_server *server, int32_t forest_id, const float *X, int64_t R, int32_t F,
+                              uint32_t *labels, uint64_t *ticket);
+/* Queues a fit job (X, y copied at submission; boot_ids come from
+ * opts->bootstrap).  On success the new forest becomes resident and its id is
+ * reported by cta_server_wait. */
+int cta_server_submit_fit(cta_server *server, const float *X, const uint32_t *y, int32_t N, int32_t F, int32_t K,
+                          int32_t T, const cta_fit_opts *opts, uint64_t *ticket);
+/* Waits up to timeout_us (< 0: forever).  *status: -1 still pending, else the
+ * job's CTA_* status; *forest_id: the fitted forest of a fit job.  A
+ * completed ticket is forgotten after it has been reported. */
+int cta_server_wait(cta_server *server, uint64_t ticket, int64_t timeout_us, int32_t *status, int32_t *forest_id);
+int cta_server_get_stats(cta_server *server, cta_server_stats *out);
+int cta_server_reset_stats(cta_server *server);
 
 /* ---- measurement ------------------------------------------------------- */
 typedef struct {

@@ -72,6 +72,8 @@ class Oracle:
         lib.orc_feature_perm.argtypes = [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32,
                                          ctypes.c_uint32, ctypes.c_int32, ctypes.c_int32, _i32p]
         lib.orc_feature_perm.restype = None
+        lib.orc_bootstrap_ids.argtypes = [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_int32, _u32p]
+        lib.orc_bootstrap_ids.restype = ctypes.c_int64
         lib.orc_presort.argtypes = [_f32p, ctypes.c_int32, ctypes.c_int32, _u32p]
         lib.orc_presort.restype = ctypes.c_int
         lib.orc_scan_feature.argtypes = [_f32p, _u32p, ctypes.c_uint32, ctypes.c_uint32,
@@ -101,6 +103,17 @@ class Oracle:
         out = np.zeros(F, np.int32)
         self.lib.orc_feature_perm(seed, tree, depth, start, F, int(identity), _ptr(out, _i32p))
         return out
+
+    def bootstrap_ids(self, seed, tree, N):
+        out = np.zeros(N, np.uint32)
+        n = self.lib.orc_bootstrap_ids(seed, tree, N, _ptr(out, _u32p))
+        if n < 0:
+            raise MemoryError("orc_bootstrap_ids")
+        return out[:n].copy()
+
+    def bootstrap_forest_ids(self, seed, n_trees, N):
+        """In-bag ids of trees 0..n_trees-1 under the canonical generator."""
+        return [self.bootstrap_ids(seed, t, N) for t in range(n_trees)]
 
     def presort(self, X_T):
         X_T = np.ascontiguousarray(X_T, dtype=np.float32)

@@ -134,6 +134,36 @@ void orc_feature_perm(uint64_t seed, uint32_t tree, uint32_t depth, uint32_t sta
 }
 
 /* ------------------------------------------------------------------------ */
+/* Bootstrap (canonical D3).  The reference draws np.unique(np.random.randint */
+/* (0, N, N)) from numpy's global generator (random_forest.py:123-126): the  */
+/* set of distinct ids among N uniform draws.  Here the draws come from      */
+/* Philox keyed by (seed, tree), four per block, mapped to [0, N) by the     */
+/* high half of u * N.                                                       */
+/* ------------------------------------------------------------------------ */
+#define BOOT_TAG 0x424F4F54u /* "BOOT" */
+
+int64_t orc_bootstrap_ids(uint64_t seed, uint32_t tree, int32_t N, uint32_t *out) {
+  if (N <= 0) return 0;
+  uint8_t *seen = (uint8_t *)calloc((size_t)N, 1);
+  if (!seen) return -1;
+  uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+  uint32_t w[4];
+  for (int64_t j = 0; j < N; ++j) {
+    if ((j & 3) == 0) {
+      uint32_t ctr[4] = {(uint32_t)(j >> 2), tree, BOOT_TAG, 0u};
+      orc_philox4x32_10(ctr, key, w);
+    }
+    uint32_t id = (uint32_t)(((uint64_t)w[j & 3] * (uint64_t)N) >> 32);
+    seen[id] = 1;
+  }
+  int64_t n = 0;
+  for (int32_t i = 0; i < N; ++i)
+    if (seen[i]) out[n++] = (uint32_t)i;
+  free(seen);
+  return n;
+}
+
+/* ------------------------------------------------------------------------ */
 /* Presort (canonical D4).  The reference sorts each feature row of the     */
 /* transposed samples with numpy's default argsort (random_forest.py:233,  */
 /* 247); ties there are unordered.  Here ties keep ascending sample id.     */

@@ -14,8 +14,10 @@
  * function.
  *
  * Pinning status (which independent check fixes each function):
- *   orc_philox4x32_10   -- Random123 known-answer vectors (tests/test_oracle_rng.py)
+ *   orc_philox4x32_10   -- Random123 known-answer vectors (tests/test_oracle.py)
  *   orc_feature_perm    -- permutation/identity/determinism invariants
+ *   orc_bootstrap_ids   -- sorted/unique/range invariants, expected in-bag
+ *                          fraction 1 - (1 - 1/N)^N, uniform coverage
  *   orc_presort         -- numpy.argsort(kind="stable") (closed form)
  *   orc_scan_feature    -- brute-force exact-fraction Gini on tiny inputs
  *   orc_fit             -- brute-force tree builder on tiny inputs; memorize
@@ -73,6 +75,12 @@ void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t ou
  * (seed, tree, depth, start); identity when identity != 0. */
 void orc_feature_perm(uint64_t seed, uint32_t tree, uint32_t depth, uint32_t start,
                       int32_t F, int32_t identity, int32_t *out);
+
+/* Canonical bootstrap of tree `tree`: the sorted unique values of N draws
+ * u_j -> floor(u_j * N / 2^32), u_j the (j mod 4)-th word of
+ * Philox4x32-10(counter = (j / 4, tree, "BOOT", 0), key = seed).  Writes the
+ * ids to out (capacity N) and returns their count. */
+int64_t orc_bootstrap_ids(uint64_t seed, uint32_t tree, int32_t N, uint32_t *out);
 
 /* Stable ascending argsort of every feature row of X_T (F x N, feature-major).
  * Ties (including -0.0 == +0.0) keep ascending sample id.  NaN -> ORC_EINVAL. */

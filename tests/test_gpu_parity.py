@@ -205,6 +205,28 @@ def test_modes_and_batching_identical(ctx, orc):
         assert helpers.forest_diff(f.export(), want) is None, (mode, tpb, chunk, glob)
 
 
+@pytest.mark.parametrize("tpb", [0, 1, 2])
+def test_device_bootstrap_matches_oracle_generator(ctx, orc, tpb):
+    """In-bag sets drawn on the device equal the oracle's canonical generator
+    (keyed by the global tree index, whatever the batch split)."""
+    X, y, _, _ = datasets.covtype_like(seed=13, n_train=12000, n_test=1)
+    case = {"X": X, "y": y, "K": 7, "m": 7, "mss": 1, "max_depth": 0, "identity": False,
+            "seed": 0x1234567890ABCDEF, "boot": False, "n_trees": 5}
+    ds = _lib.Dataset(ctx, X, y, 7)
+    o = _lib.default_opts()
+    o.seed = case["seed"]
+    o.max_features = 7
+    o.trees_per_batch = tpb
+    o.bootstrap = 1
+    print("device bootstrap tpb=%d" % tpb, flush=True)
+    f = _lib.Forest.fit(ctx, ds, 5, o, None)
+    ids = orc.bootstrap_forest_ids(case["seed"], 5, len(y))
+    want = orc.fit(X, y, 7, 5, 7, boot_ids=ids, seed=case["seed"])
+    got = f.export()
+    assert helpers.forest_diff(got, want) is None
+    assert [int(got["n_samples"][got["tree_off"][t]]) for t in range(5)] == [len(i) for i in ids]
+
+
 def test_max_depth_and_min_samples_split(ctx, orc):
     X, y, _, _ = datasets.covtype_like(seed=5, n_train=8000, n_test=1)
     for md, mss in [(1, 1), (4, 1), (6, 20), (0, 50)]:

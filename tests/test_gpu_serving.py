@@ -1,0 +1,141 @@
+"""Serving loop on the GPU: each result is the same alone and in the mix,
+admission by HBM accounting, coalesced and unbatched launches."""
+import threading
+
+import numpy as np
+import pytest
+
+import helpers
+from cudatree_amd import _lib, datasets
+
+pytestmark = pytest.mark.gpu
+
+
+def fit_opts(seed, m=7, max_depth=0):
+    o = _lib.default_opts()
+    o.seed = seed
+    o.max_features = m
+    o.max_depth = max_depth
+    o.bootstrap = 1
+    return o
+
+
+@pytest.fixture(scope="module")
+def data():
+    X, y, Xt, yt = datasets.covtype_like(seed=21, n_train=20000, n_test=20000)
+    return X, y, Xt
+
+
+@pytest.fixture(scope="module")
+def base_forest(data):
+    X, y, _ = data
+    ctx = _lib.default_context(0)
+    ds = _lib.Dataset(ctx, X, y, 7)
+    return _lib.Forest.fit(ctx, ds, 20, fit_opts(3), None)
+
+
+def _requests(Xt, n, seed):
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n):
+        r = int(np.exp(rng.uniform(0, np.log(3000))))
+        a = int(rng.integers(0, len(Xt) - r))
+        out.append(Xt[a:a + r])
+    return out
+
+
+@pytest.mark.parametrize("coalesce", [True, False])
+def test_predict_alone_vs_in_mix(data, base_forest, coalesce):
+    X, y, Xt = data
+    reqs = _requests(Xt, 60, 5)
+    alone = [base_forest.predict(r) for r in reqs]
+    fit_jobs = [(X[:5000 * (k + 1)], y[:5000 * (k + 1)], 10 + 5 * k, 100 + k) for k in range(3)]
+    fit_alone = []
+    ctx = _lib.default_context(0)
+    for Xf, yf, T, seed in fit_jobs:
+        ds = _lib.Dataset(ctx, Xf, yf, 7)
+        fit_alone.append(_lib.Forest.fit(ctx, ds, T, fit_opts(seed), None).export())
+    srv = _lib.Server(coalesce=coalesce, fit_workers=1)
+    try:
+        fid = srv.add_forest(base_forest)
+        tickets = [None] * len(reqs)
+        fit_tickets = []
+
+        def client(part):
+            for i in range(part, len(reqs), 4):
+                tickets[i] = srv.submit_predict(fid, reqs[i])
+
+        print("serving mix coalesce=%d" % coalesce, flush=True)
+        for Xf, yf, T, seed in fit_jobs:
+            fit_tickets.append(srv.submit_fit(Xf, yf, 7, T, fit_opts(seed)))
+        threads = [threading.Thread(target=client, args=(p,)) for p in range(4)]
+        for t in threads:
+            t.start()
+        for t in threads:
+            t.join()
+        for i, t in enumerate(tickets):
+            st, labels = srv.wait(t)
+            assert st == 0
+            assert np.array_equal(labels, alone[i]), "request %d differs in the mix" % i
+        for k, t in enumerate(fit_tickets):
+            st, new_fid = srv.wait(t)
+            assert st == 0 and new_fid >= 0
+            assert helpers.forest_diff(srv.export_forest(new_fid), fit_alone[k]) is None
+        s = srv.stats()
+        assert s["predict_requests"] == len(reqs) and s["fit_jobs"] == 3
+        if coalesce:
+            assert s["predict_launches"] <= s["predict_requests"]
+        else:
+            assert s["predict_launches"] == s["predict_requests"]
+        assert s["p99_ms"] >= s["p50_ms"] > 0 and s["hbm_peak"] <= s["hbm_budget"]
+    finally:
+        srv.close()
+
+
+def test_fit_admission_by_hbm_budget(data):
+    X, y, _ = data
+    srv = _lib.Server(hbm_budget=256 << 20)
+    try:
+        t = srv.submit_fit(X, y, 7, 50, fit_opts(1))       # needs far more than 256 MB
+        st, _ = srv.wait(t)
+        assert st == 2  # CTA_ENOMEM: rejected, never started
+        t = srv.submit_fit(X[:2000], y[:2000], 7, 5, fit_opts(1))   # small job fits
+        st, fid = srv.wait(t)
+        assert st == 0 and fid >= 0
+        s = srv.stats()
+        assert s["fit_rejected"] == 1 and s["fit_jobs"] == 1 and s["hbm_peak"] <= 256 << 20
+    finally:
+        srv.close()
+
+
+def test_jobs_queue_until_memory_frees(data):
+    X, y, _ = data
+    import ctypes
+
+    # budget for about one of these jobs at a time
+    one = 20000 * 54 * (4 + 2 + 16) + 8 * (2 * 20000 * 54 * 2 + 20000 // 8 + 20000 * (220 + 42)) + (64 << 20)
+    srv = _lib.Server(hbm_budget=int(one * 1.5), fit_workers=2)
+    try:
+        ts = [srv.submit_fit(X, y, 7, 8, fit_opts(s)) for s in (1, 2, 3)]
+        for t in ts:
+            st, fid = srv.wait(t)
+            assert st == 0
+        s = srv.stats()
+        assert s["fit_jobs"] == 3 and s["hbm_peak"] <= int(one * 1.5)
+    finally:
+        srv.close()
+
+
+def test_bad_requests_are_rejected(base_forest):
+    srv = _lib.Server()
+    try:
+        fid = srv.add_forest(base_forest)
+        with pytest.raises(_lib.CtaError):
+            srv.submit_predict(fid + 7, np.zeros((3, 54), np.float32))
+        with pytest.raises(_lib.CtaError):
+            srv.submit_predict(fid, np.zeros((3, 5), np.float32))
+        srv.drop_forest(fid)
+        with pytest.raises(_lib.CtaError):
+            srv.submit_predict(fid, np.zeros((3, 54), np.float32))
+    finally:
+        srv.close()

@@ -55,6 +55,32 @@ def test_feature_perm_first_slot_roughly_uniform(orc):
     assert counts.min() > 0.4 * len(firsts) / F and counts.max() < 1.8 * len(firsts) / F
 
 
+@pytest.mark.parametrize("N", [1, 2, 7, 100, 5000, 70001])
+def test_bootstrap_ids_invariants(orc, N):
+    ids = orc.bootstrap_ids(123, 4, N)
+    assert ids.dtype == np.uint32 and len(ids) >= 1
+    assert np.all(np.diff(ids.astype(np.int64)) > 0) and ids[-1] < N
+    assert np.array_equal(ids, orc.bootstrap_ids(123, 4, N))
+    if N >= 100:
+        # the in-bag fraction of N draws with replacement: 1 - (1 - 1/N)^N
+        expect = 1 - (1 - 1.0 / N) ** N
+        sd = np.sqrt(N * expect * (1 - expect))
+        assert abs(len(ids) - N * expect) < 6 * sd + 2
+        assert not np.array_equal(ids, orc.bootstrap_ids(123, 5, N)[: len(ids)])
+        assert not np.array_equal(ids, orc.bootstrap_ids(124, 4, N)[: len(ids)])
+
+
+def test_bootstrap_ids_uniform_coverage(orc):
+    N, T = 1000, 400
+    hits = np.zeros(N)
+    for t in range(T):
+        hits[orc.bootstrap_ids(9, t, N)] += 1
+    p = 1 - (1 - 1.0 / N) ** N
+    # every sample is in-bag in about p * T trees (binomial, 6 sigma)
+    assert np.all(np.abs(hits - p * T) < 6 * np.sqrt(T * p * (1 - p)))
+    assert abs(hits[: N // 2].mean() - hits[N // 2:].mean()) < 3.0
+
+
 def test_presort_matches_stable_argsort(orc):
     rng = np.random.default_rng(3)
     X_T = rng.integers(0, 5, size=(6, 500)).astype(np.float32)
