@@ -10,6 +10,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -462,19 +463,33 @@ struct Fitter {
     const int mode = o.mode;
     if (mode == CTA_MODE_BATCHED) {
       uint32_t q = hc.q;
+      const char *lvl_path = ctx->profiling ? std::getenv("CTA_LEVEL_LOG") : nullptr;
+      FILE *lvl = lvl_path ? std::fopen(lvl_path, "a") : nullptr;
+      if (lvl) std::fprintf(lvl, "level,q,scan_bytes,resh_bytes,fill_bytes,k_scan_ms,k_resh_ms,k_fill_ms,k_scan_hist_ms\n");
+      int depth = 0;
       while (q > 0) {
         a.front = front[cur];
         a.cnt = cnt[cur];
         a.next_front = front[cur ^ 1];
         a.next_cnt = cnt[cur ^ 1];
+        std::map<std::string, KStat> before;
+        if (lvl) before = ctx->stats;
         level(a);
         cur ^= 1;
         Ctl prev = hc;
         read_ctl(a, hc);
         account(prev, hc);
+        if (lvl) {
+          auto d = [&](const char *k) { return ctx->stats[k].ms - before[k].ms; };
+          std::fprintf(lvl, "%d,%u,%llu,%llu,%llu,%.4f,%.4f,%.4f,%.4f\n", depth, q, hc.bytes_scan - prev.bytes_scan,
+                       hc.bytes_resh - prev.bytes_resh, hc.bytes_fill - prev.bytes_fill, d("k_scan"), d("k_resh"),
+                       d("k_fill"), d("k_scan_hist"));
+        }
+        ++depth;
         if (hc.err) fail(CTA_EINTERNAL, "device capacity check failed (err %u)", hc.err);
         q = hc.q;
       }
+      if (lvl) std::fclose(lvl);
     } else {
       // per-node (and hybrid) schedule: host stack of frontier entries
       std::vector<HostEntry> stack, queue;
