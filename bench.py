@@ -136,7 +136,9 @@ def main():
     ap.add_argument("--no-profile", action="store_true", help="skip per-kernel HIP-event timing")
     ap.add_argument("--serving-seconds", type=float, default=6.0, help="length of each serving stream (0: skip)")
     ap.add_argument("--serving-trees", type=int, default=500, help="trees of the resident serving forest")
-    ap.add_argument("--serving-qps", type=float, default=100.0, help="predict requests per second")
+    ap.add_argument("--serving-qps", type=float, default=1000.0, help="predict requests per second")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="also run one plain 1 GiB device-to-device copy (calibrates FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one MI355X: this build has no multi-GPU path (trees would shard across GPUs "
@@ -279,6 +281,15 @@ def main():
               "sample": "%d trees scaled x%.3g" % (nt, N_TREES / nt),
               "predict_rows_per_s_%d_trees" % nt: round(R / dp, 1)}
         log("sklearn: %.2fs for %d trees on %d cores -> %.1fs/50" % (dt, nt, cores, dt * N_TREES / nt))
+
+    if args.calibrate_copy:
+        src = torch.empty(1 << 28, dtype=torch.float32, device=dev).fill_(1.0)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log("calibration copy: 1 GiB read + 1 GiB written")
+        del src, dst
 
     serving = None
     if args.serving_seconds > 0:

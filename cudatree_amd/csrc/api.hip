@@ -461,11 +461,16 @@ struct Fitter {
     ctx->host_time("host_setup", std::chrono::duration<double, std::milli>(t_levels - t_setup).count());
 
     const int mode = o.mode;
+    uint32_t depth_bound = 0;  // deepest node depth that can exist in the batch
+    std::vector<uint32_t> level_off;  // batched mode: ids of depth d are [level_off[d], level_off[d+1])
     if (mode == CTA_MODE_BATCHED) {
+      level_off = {0u, (uint32_t)Tb};
       uint32_t q = hc.q;
       const char *lvl_path = ctx->profiling ? std::getenv("CTA_LEVEL_LOG") : nullptr;
       FILE *lvl = lvl_path ? std::fopen(lvl_path, "a") : nullptr;
-      if (lvl) std::fprintf(lvl, "level,q,scan_bytes,resh_bytes,fill_bytes,k_scan_ms,k_resh_ms,k_fill_ms,k_scan_hist_ms\n");
+      if (lvl)
+        std::fprintf(lvl, "level,q,scan_bytes,resh_bytes,fill_bytes,k_scan_ms,k_resh_ms,k_fill_ms,k_scan_hist_ms,"
+                          "k_select_ms,k_decide_ms,k_children_ms\n");
       int depth = 0;
       while (q > 0) {
         a.front = front[cur];
@@ -481,14 +486,17 @@ struct Fitter {
         account(prev, hc);
         if (lvl) {
           auto d = [&](const char *k) { return ctx->stats[k].ms - before[k].ms; };
-          std::fprintf(lvl, "%d,%u,%llu,%llu,%llu,%.4f,%.4f,%.4f,%.4f\n", depth, q, hc.bytes_scan - prev.bytes_scan,
-                       hc.bytes_resh - prev.bytes_resh, hc.bytes_fill - prev.bytes_fill, d("k_scan"), d("k_resh"),
-                       d("k_fill"), d("k_scan_hist"));
+          std::fprintf(lvl, "%d,%u,%llu,%llu,%llu,%.4f,%.4f,%.4f,%.4f,%.4f,%.4f,%.4f\n", depth, q,
+                       hc.bytes_scan - prev.bytes_scan, hc.bytes_resh - prev.bytes_resh, hc.bytes_fill - prev.bytes_fill,
+                       d("k_scan"), d("k_resh"), d("k_fill"), d("k_scan_hist"), d("k_select"), d("k_decide"),
+                       d("k_children"));
         }
         ++depth;
         if (hc.err) fail(CTA_EINTERNAL, "device capacity check failed (err %u)", hc.err);
+        level_off.push_back(hc.n_nodes);
         q = hc.q;
       }
+      depth_bound = (uint32_t)depth;
       if (lvl) std::fclose(lvl);
     } else {
       // per-node (and hybrid) schedule: host stack of frontier entries
@@ -503,6 +511,7 @@ struct Fitter {
           queue.push_back(std::move(e));
           continue;
         }
+        depth_bound = std::max(depth_bound, e.nd.depth + 1);
         push_frontier(a, front[0], cnt[0], {e});
         a.front = front[0];
         a.cnt = cnt[0];
@@ -518,10 +527,13 @@ struct Fitter {
         for (auto it = kids.rbegin(); it != kids.rend(); ++it) stack.push_back(std::move(*it));
       }
       if (!queue.empty()) {
+        uint32_t qd = 0;
+        for (const auto &e : queue) qd = std::max(qd, e.nd.depth);
         push_frontier(a, front[0], cnt[0], queue);
         cur = 0;
         uint32_t q = (uint32_t)queue.size();
         while (q > 0) {
+          ++qd;
           a.front = front[cur];
           a.cnt = cnt[cur];
           a.next_front = front[cur ^ 1];
@@ -534,11 +546,12 @@ struct Fitter {
           if (hc.err) fail(CTA_EINTERNAL, "device capacity check failed (err %u)", hc.err);
           q = hc.q;
         }
+        depth_bound = std::max(depth_bound, qd);
       }
     }
     const auto t_export = std::chrono::steady_clock::now();
     ctx->host_time("host_levels", std::chrono::duration<double, std::milli>(t_export - t_levels).count());
-    export_batch(a, hc, Tb, Tb == total_trees, out);
+    export_batch(a, hc, depth_bound, level_off, Tb, Tb == total_trees, out);
     ctx->host_time("host_export",
                    std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_export).count());
   }
@@ -641,12 +654,15 @@ struct Fitter {
 
   // Canonical pre-order trees of the batch, numbered on the device, appended
   // to `out`; a single-batch fit also keeps the device predict table.
-  void export_batch(const FitArgs<IdxT, LabT> &a, const Ctl &hc, int32_t Tb, bool only_batch, cta_forest *out) {
-    const uint32_t n = hc.n_nodes, D = hc.max_d;
+  // level_off: when node ids were allocated one depth at a time (batched
+  // mode), depth d holds ids [level_off[d], level_off[d+1]) and no bucketing
+  // is needed.
+  void export_batch(const FitArgs<IdxT, LabT> &a, const Ctl &hc, uint32_t D, const std::vector<uint32_t> &level_off,
+                    int32_t Tb, bool only_batch, cta_forest *out) {
+    const uint32_t n = hc.n_nodes;
     const Arena &ar = a.arena;
-    uint32_t *dcnt = ctx->ws_as<uint32_t>("canon_dcnt", D + 2);
-    uint32_t *cursor = ctx->ws_as<uint32_t>("canon_cursor", D + 2);
-    uint32_t *order = ctx->ws_as<uint32_t>("canon_order", n);
+    const bool contiguous = !level_off.empty();
+    uint32_t *order = contiguous ? nullptr : ctx->ws_as<uint32_t>("canon_order", n);
     uint32_t *size = ctx->ws_as<uint32_t>("canon_size", n);
     uint32_t *pos = ctx->ws_as<uint32_t>("canon_pos", n);
     uint32_t *tr = ctx->ws_as<uint32_t>("canon_tr", n);
@@ -660,15 +676,23 @@ struct Fitter {
     co.nsamp = ctx->ws_as<uint32_t>("canon_nsamp", n);
     co.depth = ctx->ws_as<uint32_t>("canon_depth", n);
     co.pnode = ctx->ws_as<PNode>("canon_pnode", n);
-    k_fill_u32<<<grid_for(D + 2), 256, 0, st>>>(dcnt, D + 2, 0u);
-    k_depth_hist<<<grid_for(n), 256, 0, st>>>(ar.depth, n, dcnt);
-    k_prefix_serial<<<1, 64, 0, st>>>(dcnt, D + 1);
-    HIPC(hipMemcpyAsync(cursor, dcnt, (D + 2) * 4, hipMemcpyDeviceToDevice, st));
-    k_depth_scatter<<<grid_for(n), 256, 0, st>>>(ar.depth, n, cursor, order);
-    uint32_t *hoff = static_cast<uint32_t *>(ctx->host("canon_dcnt", (D + 2) * 4));
-    HIPC(hipMemcpyAsync(hoff, dcnt, (D + 2) * 4, hipMemcpyDeviceToHost, st));
-    ctx->sync();
-    const std::vector<uint32_t> doff(hoff, hoff + D + 2);
+    std::vector<uint32_t> doff;
+    if (contiguous) {
+      doff = level_off;
+      D = (uint32_t)doff.size() - 2;
+    } else {
+      uint32_t *dcnt = ctx->ws_as<uint32_t>("canon_dcnt", D + 2);
+      uint32_t *cursor = ctx->ws_as<uint32_t>("canon_cursor", D + 2);
+      k_fill_u32<<<grid_for(D + 2), 256, 0, st>>>(dcnt, D + 2, 0u);
+      k_depth_hist<<<grid_for(n), 256, 0, st>>>(ar.depth, n, dcnt);
+      k_prefix_serial<<<1, 64, 0, st>>>(dcnt, D + 1);
+      HIPC(hipMemcpyAsync(cursor, dcnt, (D + 2) * 4, hipMemcpyDeviceToDevice, st));
+      k_depth_scatter<<<grid_for(n), 256, 0, st>>>(ar.depth, n, cursor, order);
+      uint32_t *hoff = static_cast<uint32_t *>(ctx->host("canon_dcnt", (D + 2) * 4));
+      HIPC(hipMemcpyAsync(hoff, dcnt, (D + 2) * 4, hipMemcpyDeviceToHost, st));
+      ctx->sync();
+      doff.assign(hoff, hoff + D + 2);
+    }
     for (int64_t d = D; d >= 0; --d) {
       const uint32_t b = doff[d], e = doff[d + 1];
       if (e > b) k_subtree_size<<<grid_for(e - b), 256, 0, st>>>(order, b, e, ar, size);

@@ -541,7 +541,6 @@ __global__ __launch_bounds__(kBlock) void k_children(FitArgs<IdxT, LabT> a) {
     }
     wave_sync();
     const uint32_t cd = nd.depth + 1;
-    if (lane == 0) atomicMax(&a.ctl->max_d, cd);
     const bool capped = a.max_depth > 0 && (int64_t)cd >= (int64_t)a.max_depth;
     // 0 live, 1 pure, 2 first-sample, 3 majority
     auto kind = [&](uint32_t nz, uint32_t nc) -> uint32_t {
@@ -1050,9 +1049,10 @@ __global__ void k_depth_scatter(const uint32_t *depth, uint32_t n, uint32_t *cur
     order[atomicAdd(&cursor[depth[i]], 1u)] = i;
 }
 
+// order == nullptr: the ids of a depth are the contiguous range [b, e).
 __global__ void k_subtree_size(const uint32_t *order, uint32_t b, uint32_t e, Arena ar, uint32_t *size) {
   for (uint32_t k = b + blockIdx.x * blockDim.x + threadIdx.x; k < e; k += gridDim.x * blockDim.x) {
-    const uint32_t id = order[k];
+    const uint32_t id = order ? order[k] : k;
     size[id] = ar.feat[id] < 0 ? 1u : 1u + size[ar.left[id]] + size[ar.right[id]];
   }
 }
@@ -1065,7 +1065,7 @@ __global__ void k_root_sizes(const uint32_t *size, uint32_t T, uint32_t *tsz) {
 __global__ void k_preorder(const uint32_t *order, uint32_t b, uint32_t e, Arena ar, const uint32_t *size,
                            uint32_t *pos, uint32_t *tr) {
   for (uint32_t k = b + blockIdx.x * blockDim.x + threadIdx.x; k < e; k += gridDim.x * blockDim.x) {
-    const uint32_t id = order[k];
+    const uint32_t id = order ? order[k] : k;
     if (ar.feat[id] < 0) continue;
     const uint32_t l = ar.left[id], r = ar.right[id];
     pos[l] = pos[id] + 1;

@@ -57,8 +57,7 @@ struct Ctl {
   uint32_t n_scan, n_fill, n_hist, n_plan;
   uint32_t n_nodes;
   uint32_t err;
-  uint32_t max_d;  // deepest node created so far
-  uint32_t pad[7];
+  uint32_t pad[8];
   unsigned long long bytes_scan, bytes_fill, bytes_resh, bytes_boot;
 };
 constexpr uint32_t kErrCapacity = 1u, kErrBadNode = 2u;
