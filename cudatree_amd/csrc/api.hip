@@ -278,6 +278,10 @@ struct Fitter {
   uint32_t Cs, Cf, Cr;
   hipStream_t st;
   uint32_t G;  // blocks of the wave-loop kernels
+  uint64_t n_tot = 0;      // samples over the batch's trees
+  bool fill_lds = false;   // side bitmaps built in LDS (N / 8 bytes fit a workgroup)
+  uint32_t Jcap = 1;       // most fill workgroups per tree
+  uint32_t *part = nullptr, *bsum = nullptr;
 
   Fitter(cta_ctx *c, cta_dataset *d, const cta_fit_opts &opts) : ctx(c), ds(d), o(opts) {
     N = d->N;
@@ -286,10 +290,12 @@ struct Fitter {
     m = opts.max_features;
     Cs = opts.scan_chunk > 0 ? (uint32_t)opts.scan_chunk : 8192u;
     Cr = opts.resh_chunk > 0 ? (uint32_t)opts.resh_chunk : 2048u;
-    Cf = 8192u;
+    Cf = 2048u;
     st = c->stream;
     G = (uint32_t)std::max(64, ctx->num_cu * 8);
   }
+
+  size_t fill_lds_bytes() const { return (((size_t)N + 31) / 32) * 4; }
 
   size_t smem() const { return (size_t)kWavesPerBlock * 2 * K * sizeof(uint32_t); }
 
@@ -297,8 +303,9 @@ struct Fitter {
   size_t bytes_per_tree() const {
     const size_t rows = 2 * (size_t)F * N * sizeof(IdxT) + N / 8 + 64;
     const size_t per_sample = 2 * sizeof(FNode) + 3 * 4 * (size_t)K + sizeof(NAux) + 4 * (size_t)m +
-                              2 * 7 * 4 + (sizeof(Task) + sizeof(ScanRes)) + 2 * sizeof(Task);
-    return rows + per_sample * (size_t)N / 2 + 4096;
+                              2 * 7 * 4 + (sizeof(Task) + sizeof(ScanRes)) + 2 * sizeof(Task) +
+                              2 * kChFields * 4;
+    return rows + per_sample * (size_t)N / 2 + 16 * ((size_t)N / 8) + 4096;
   }
 
   // Grows trees [t0, t0 + Tb) and appends them to `out` in pre-order.
@@ -310,7 +317,7 @@ struct Fitter {
     const uint32_t W = ((uint32_t)N + 31u) / 32u;
     uint32_t *bits = ctx->ws_as<uint32_t>("bits", (size_t)Tb * W);
     std::vector<uint64_t> off(Tb + 1, 0);
-    uint64_t n_tot = 0;
+    n_tot = 0;
     std::vector<uint32_t> gen_n;
     if (boot_gen) {
       // in-bag bitmaps drawn on the device; their counts size the batch
@@ -343,9 +350,17 @@ struct Fitter {
       fail(CTA_EUNSUPPORTED, "batch of %d trees too large for 32-bit task indices", Tb);
 
     IdxT *buf0 = ctx->ws_as<IdxT>("buf0", (size_t)Tb * F * N), *buf1 = ctx->ws_as<IdxT>("buf1", (size_t)Tb * F * N);
-    uint32_t *plan_cnt = ctx->ws_as<uint32_t>("plan_cnt", (size_t)Tb * 2);
-    unsigned long long *plan_elems = ctx->ws_as<unsigned long long>("plan_elems", Tb);
-    uint32_t *plan_off = ctx->ws_as<uint32_t>("plan_off", (size_t)(Tb + 1) * 3);
+    uint32_t *plan = ctx->ws_as<uint32_t>("plan", (size_t)Tb * 4);
+    uint32_t *wg_off = ctx->ws_as<uint32_t>("wg_off", (size_t)Tb + 1);
+    uint32_t *tlo = ctx->ws_as<uint32_t>("tlo", (size_t)Tb * 2);
+    uint32_t *vals = ctx->ws_as<uint32_t>("scan_vals", cap_q * kChFields);
+    uint32_t *offs = ctx->ws_as<uint32_t>("scan_offs", cap_q * kChFields);
+    bsum = ctx->ws_as<uint32_t>("scan_bsum", (cap_q / kOffTile + 2) * kChFields);
+    const size_t nbslot = 4 * (size_t)kByteSlots * kByteStride;
+    unsigned long long *bslot = ctx->ws_as<unsigned long long>("byte_slots", nbslot);
+    fill_lds = fill_lds_bytes() <= (size_t)ctx->lds_per_block;
+    Jcap = fill_lds ? (uint32_t)std::max<int64_t>(1, std::min<int64_t>(16, (3 * (int64_t)ctx->num_cu) / Tb)) : 1u;
+    part = Jcap > 1 ? ctx->ws_as<uint32_t>("fill_part", (size_t)Tb * Jcap * W) : nullptr;
     FNode *front[2] = {ctx->ws_as<FNode>("front0", cap_q), ctx->ws_as<FNode>("front1", cap_q)};
     uint32_t *cnt[2] = {ctx->ws_as<uint32_t>("cnt0", cap_q * K), ctx->ws_as<uint32_t>("cnt1", cap_q * K)};
     uint32_t *cntL = ctx->ws_as<uint32_t>("cntL", cap_q * K);
@@ -397,12 +412,13 @@ struct Fitter {
     a.fill_tasks = fill_tasks;
     a.big_tasks = big_tasks;
     a.small_tasks = small_tasks;
-    a.cnt_big = plan_cnt;
-    a.cnt_small = plan_cnt + Tb;
-    a.elems = plan_elems;
-    a.big_off = plan_off;
-    a.small_off = plan_off + (Tb + 1);
-    a.wg_off = plan_off + 2 * (Tb + 1);
+    a.plan = plan;
+    a.wg_off = wg_off;
+    a.tlo = tlo;
+    a.thi = tlo + Tb;
+    a.vals = vals;
+    a.offs = offs;
+    a.bslot = bslot;
     a.arena = Arena{a_feat, a_thr, a_left, a_right, a_value, a_nsamp, a_depth};
     a.ctl = ctl;
     a.cap_q = (uint32_t)cap_q;
@@ -423,8 +439,8 @@ struct Fitter {
     HIPC(hipMemcpyAsync(ctl, hctl, sizeof(Ctl), hipMemcpyHostToDevice, st));
     HIPC(hipMemcpyAsync(d_off, hoff, (Tb + 1) * 8, hipMemcpyHostToDevice, st));
     k_fill_u32<<<grid_for((size_t)Tb * K), 256, 0, st>>>(root_cnt, (size_t)Tb * K, 0u);
-    k_fill_u32<<<1, 256, 0, st>>>(plan_cnt, (size_t)Tb * 2, 0u);
-    k_fill_u32<<<1, 256, 0, st>>>(reinterpret_cast<uint32_t *>(plan_elems), (size_t)Tb * 2, 0u);
+    k_fill_u32<<<grid_for((size_t)Tb * 2), 256, 0, st>>>(tlo, (size_t)Tb * 2, 0u);
+    k_fill_u32<<<grid_for(nbslot * 2), 256, 0, st>>>(reinterpret_cast<uint32_t *>(bslot), nbslot * 2, 0u);
     uint32_t src_root = 0;
     if (boot) {
       if (boot_gen) {
@@ -442,7 +458,9 @@ struct Fitter {
       }
       a.front = front[0];
       k_ctl_set<<<1, 64, 0, st>>>(a.ctl, (uint32_t)Tb);
-      k_boot_tasks<IdxT, LabT><<<grid_for((size_t)Tb * 64), 256, 0, st>>>(a, d_off, Tb, front[0], 0u);
+      k_boot_tasks<IdxT, LabT><<<grid_for((size_t)Tb * 64), 256, 0, st>>>(a, d_off, Tb, front[0]);
+      offsets<kChFields>(a, (uint32_t)Tb, &a.ctl->c_ids);
+      k_emit_resh<IdxT, LabT><<<G, kBlock, 0, st>>>(a);
       reshuffle(a, "k_boot_compact");
       src_root = 1;
     } else {
@@ -451,8 +469,8 @@ struct Fitter {
     }
     a.next_front = front[1];
     a.next_cnt = cnt[1];
-    k_roots<IdxT, LabT><<<grid_for((size_t)Tb * 64), 256, 0, st>>>(a, root_cnt, boot ? d_off : nullptr, Tb, src_root);
-    k_ctl_next<<<1, 64, 0, st>>>(a.ctl);
+    k_roots<IdxT, LabT><<<1, kPlanThreads, 0, st>>>(a, root_cnt, boot ? d_off : nullptr, Tb, src_root);
+    k_ctl_next<<<1, 256, 0, st>>>(a.ctl, a.bslot);
     int cur = 1;  // front[cur] holds the frontier
     read_ctl(a, hc);
     if (hc.err) fail(CTA_EINTERNAL, "device capacity check failed during setup (err %u)", hc.err);
@@ -479,7 +497,7 @@ struct Fitter {
         a.next_cnt = cnt[cur ^ 1];
         std::map<std::string, KStat> before;
         if (lvl) before = ctx->stats;
-        level(a);
+        level(a, q);
         cur ^= 1;
         Ctl prev = hc;
         read_ctl(a, hc);
@@ -517,7 +535,7 @@ struct Fitter {
         a.cnt = cnt[0];
         a.next_front = front[1];
         a.next_cnt = cnt[1];
-        level(a);
+        level(a, 1);
         Ctl prev = hc;
         read_ctl(a, hc);
         account(prev, hc);
@@ -529,6 +547,9 @@ struct Fitter {
       if (!queue.empty()) {
         uint32_t qd = 0;
         for (const auto &e : queue) qd = std::max(qd, e.nd.depth);
+        // frontiers are kept sorted by tree
+        std::stable_sort(queue.begin(), queue.end(),
+                         [](const HostEntry &x, const HostEntry &y) { return x.nd.tree < y.nd.tree; });
         push_frontier(a, front[0], cnt[0], queue);
         cur = 0;
         uint32_t q = (uint32_t)queue.size();
@@ -538,7 +559,7 @@ struct Fitter {
           a.cnt = cnt[cur];
           a.next_front = front[cur ^ 1];
           a.next_cnt = cnt[cur ^ 1];
-          level(a);
+          level(a, q);
           cur ^= 1;
           Ctl prev = hc;
           read_ctl(a, hc);
@@ -569,12 +590,53 @@ struct Fitter {
     ctx->add_bytes("k_resh", now.bytes_resh - prev.bytes_resh);
   }
 
-  // One level over the frontier in a.front (count in ctl->q).
-  void level(const FitArgs<IdxT, LabT> &a) {
+  // Exclusive offsets and totals of the per-node counts in a.vals (NF
+  // fields, q nodes) into a.offs and tot[0..NF).
+  template <int NF>
+  void offsets(const FitArgs<IdxT, LabT> &a, uint32_t q, uint32_t *tot) {
+    if (q <= 8 * kOffTile) {
+      k_off_single<NF><<<1, kOffSingleThreads, 0, st>>>(a.vals, q, a.offs, tot);
+    } else {
+      const uint32_t nb = (q + kOffTile - 1) / kOffTile;
+      k_off_sums<NF><<<nb, kOffThreads, 0, st>>>(a.vals, q, bsum);
+      k_off_single<NF><<<1, kOffSingleThreads, 0, st>>>(bsum, nb, bsum, tot);
+      k_off_apply<NF><<<nb, kOffThreads, 0, st>>>(a.vals, q, bsum, a.offs);
+    }
+  }
+
+  // Side bitmaps of the split nodes of a level of q frontier nodes.
+  void fill(const FitArgs<IdxT, LabT> &a, uint32_t q) {
+    Timed tm(ctx, "k_fill");
+    if (fill_lds) {
+      // fill tasks per tree (at most one per node plus one per Cf samples),
+      // about 32 per workgroup, at most Jcap workgroups per tree
+      const uint64_t est = ((uint64_t)q + n_tot / Cf) / std::max<uint32_t>(1, a.T);
+      const uint32_t J = (uint32_t)std::min<uint64_t>(Jcap, std::max<uint64_t>(1, (est + 31) / 32));
+      static bool attr = false;
+      if (!attr) {
+        HIPC(hipFuncSetAttribute((const void *)k_fill_lds<IdxT, LabT>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                 (int)ctx->lds_per_block));
+        attr = true;
+      }
+      k_fill_lds<IdxT, LabT><<<a.T * J, kFillThreads, fill_lds_bytes(), st>>>(a, J, part);
+      if (J > 1)
+        k_merge_bits<<<dim3(std::min<uint32_t>(64, (a.W + 255) / 256), a.T), 256, 0, st>>>(a.tlo, a.thi, a.W, J,
+                                                                                            part, a.bits);
+    } else {
+      k_fill_u32<<<grid_for((size_t)a.T * a.W), 256, 0, st>>>(a.bits, (size_t)a.T * a.W, 0u);
+      k_fill_global<IdxT, LabT><<<G, kBlock, 0, st>>>(a);
+    }
+  }
+
+  // One level over the frontier in a.front (q nodes, sorted by tree; count
+  // also in ctl->q).
+  void level(const FitArgs<IdxT, LabT> &a, uint32_t q) {
     const size_t sm = smem();
     {
       Timed tm(ctx, "k_select");
       k_select<IdxT, LabT><<<G, kBlock, 0, st>>>(a);
+      offsets<kSelFields>(a, q, &a.ctl->n_scan);
+      k_select_emit<IdxT, LabT><<<G, kBlock, 0, st>>>(a);
     }
     {
       Timed tm(ctx, "k_scan_hist");
@@ -587,29 +649,26 @@ struct Fitter {
     }
     {
       Timed tm(ctx, "k_decide");
-      k_decide<IdxT, LabT><<<G, kBlock, 0, st>>>(a);
+      k_decide<IdxT, LabT><<<G, kBlock, sm, st>>>(a);
     }
-    k_fill_u32<<<grid_for((size_t)a.T * a.W), 256, 0, st>>>(a.bits, (size_t)a.T * a.W, 0u);
-    {
-      Timed tm(ctx, "k_fill");
-      k_fill<IdxT, LabT><<<G, kBlock, sm, st>>>(a);
-    }
+    fill(a, q);
     {
       Timed tm(ctx, "k_children");
       k_children<IdxT, LabT><<<G, kBlock, sm, st>>>(a);
+      offsets<kChFields>(a, q, &a.ctl->c_ids);
+      k_children_emit<IdxT, LabT><<<G, kBlock, 0, st>>>(a);
     }
     reshuffle(a, "k_resh");
-    k_ctl_next<<<1, 64, 0, st>>>(a.ctl);
+    k_ctl_next<<<1, 256, 0, st>>>(a.ctl, a.bslot);
     HIPC(hipGetLastError());
   }
 
-  // Plan, task lists and the partition of every row of the split nodes.
+  // Plan and partition of every row of the split nodes (task lists written).
   void reshuffle(const FitArgs<IdxT, LabT> &a, const char *name) {
-    k_plan<IdxT, LabT><<<1, 64, 0, st>>>(a);
-    k_emit<IdxT, LabT><<<G, kBlock, 0, st>>>(a);
+    Timed tm(ctx, name);
+    k_plan<IdxT, LabT><<<1, kPlanThreads, 0, st>>>(a, (uint32_t)std::max(8, ctx->num_cu * 4));
     const size_t lds = (((size_t)a.W + 3) & ~(size_t)3) * 4 + 2 * kSlots * 4;
     const uint32_t GR = (uint32_t)std::max(8, ctx->num_cu * 2);
-    Timed tm(ctx, name);
     if (!o.resh_global && lds <= ctx->lds_per_block) {
       static bool attr = false;
       if (!attr) {
@@ -1238,15 +1297,21 @@ static void hook_split_scan(cta_ctx *ctx, cta_dataset *ds, const uint32_t *rows,
   ax.fpt = fpt;
   ax.hist_base = 0;
   ax.best_f = -1;
+  ax.nf = nf;
+  const uint32_t Cf = 2048;
+  ax.fill_base = 0;
+  ax.nch_f = (n + Cf - 1) / Cf;
   FNode nd = {0u, 0u, n, 0u, 0u, 1u, 0u, 0u};
   Ctl hc = {};
   hc.q = 1;
   hc.n_scan = ntask;
   hc.n_nodes = 1;
+  const size_t nbslot = 4 * (size_t)kByteSlots * kByteStride;
   DevMem d_rows(hrows.size() * sizeof(IdxT)), d_front(sizeof(FNode)), d_cnt(K * 4), d_aux(sizeof(NAux)),
       d_feats(std::max(nf, 1u) * 4), d_tasks(tasks.size() * sizeof(Task)),
       d_res(std::max(ntask, 1u) * sizeof(ScanRes)), d_hist((size_t)std::max(nf, 1u) * nch * K * 4), d_ctl(sizeof(Ctl)),
-      d_cntL(K * 4), d_ar(7 * 4 * 4), d_fill(((n + 8191) / 8192 + 1) * sizeof(Task));
+      d_cntL(K * 4), d_ar(7 * 4 * 4), d_fill(((size_t)ax.nch_f + 1) * sizeof(Task)), d_bslot(nbslot * 8);
+  k_fill_u32<<<grid_for(nbslot * 2), 256, 0, st>>>(d_bslot.as<uint32_t>(), nbslot * 2, 0u);
   HIPC(hipMemcpyAsync(d_rows.p, hrows.data(), hrows.size() * sizeof(IdxT), hipMemcpyHostToDevice, st));
   HIPC(hipMemcpyAsync(d_front.p, &nd, sizeof nd, hipMemcpyHostToDevice, st));
   HIPC(hipMemcpyAsync(d_cnt.p, counts.data(), K * 4, hipMemcpyHostToDevice, st));
@@ -1265,7 +1330,7 @@ static void hook_split_scan(cta_ctx *ctx, cta_dataset *ds, const uint32_t *rows,
   a.K = K;
   a.m = (int32_t)std::max(nf, 1u);
   a.Cs = Cs;
-  a.Cf = 8192;
+  a.Cf = Cf;
   a.Cr = 8192;
   a.front = d_front.as<FNode>();
   a.cnt = d_cnt.as<uint32_t>();
@@ -1276,20 +1341,21 @@ static void hook_split_scan(cta_ctx *ctx, cta_dataset *ds, const uint32_t *rows,
   a.hist = d_hist.as<uint32_t>();
   a.cntL = d_cntL.as<uint32_t>();
   a.fill_tasks = d_fill.as<Task>();
+  a.bslot = d_bslot.as<unsigned long long>();
   uint32_t *ar = d_ar.as<uint32_t>();
   a.arena = Arena{(int32_t *)ar, (float *)(ar + 4), ar + 8, ar + 12, ar + 16, ar + 20, ar + 24};
   a.ctl = d_ctl.as<Ctl>();
   a.cap_q = 1;
   a.cap_scan = std::max(ntask, 1u);
   a.cap_hist = std::max(nf, 1u) * nch * K;
-  a.cap_fill = (n + 8191) / 8192 + 1;
+  a.cap_fill = ax.nch_f + 1;
   a.cap_nodes = 3;
   const size_t sm = (size_t)kWavesPerBlock * 2 * K * 4;
   const uint32_t G = 64;
   k_scan_hist<IdxT, LabT><<<G, kBlock, sm, st>>>(a);
   k_scan_carry<IdxT, LabT><<<G, kBlock, 0, st>>>(a);
   k_scan<IdxT, LabT><<<G, kBlock, sm, st>>>(a);
-  k_decide<IdxT, LabT><<<G, kBlock, 0, st>>>(a);
+  k_decide<IdxT, LabT><<<G, kBlock, sm, st>>>(a);
   HIPC(hipGetLastError());
   NAux out = {};
   std::vector<ScanRes> res(std::max(ntask, 1u));
@@ -1344,33 +1410,43 @@ static void hook_reshuffle(cta_ctx *ctx, cta_dataset *ds, const uint32_t *rows, 
     if (mark[s]) hbits[s >> 5] |= 1u << (s & 31);
   uint32_t nL = 0;
   for (uint32_t i = 0; i < n; ++i) nL += mark[rows[i]] ? 1 : 0;
-  // the counting step of k_children for one node of tree 0
-  const bool big = n >= kBigN;
-  const uint32_t rpt = big ? 1u : std::max(1u, Cr / n);
-  const uint32_t ntask = big ? (uint32_t)F : ((uint32_t)F + rpt - 1) / rpt;
+  // the counting step of k_children for one node of tree 0, whose frontier
+  // range is [0, 1) and whose tasks start at offset 0
   NAux ax = {};
   ax.nL = nL;
   ax.flags = kSplit | kLiveL | kLiveR;
   ax.dst = 2;
-  ax.resh_big = big ? 1u : 0u;
-  ax.resh_off = 0;
-  ax.resh_ntask = ntask;
-  ax.rpt = rpt;
+  uint32_t v[kChFields] = {0u, 0u, 0u, 0u, 0u};
+  {
+    const bool big = n >= kBigN;
+    const uint32_t rpt = big ? 1u : std::max(1u, Cr / n);
+    const uint32_t ntask = big ? (uint32_t)F : ((uint32_t)F + rpt - 1) / rpt;
+    ax.resh_big = big ? 1u : 0u;
+    ax.rpt = rpt;
+    ax.resh_ntask = ntask;
+    v[2] = big ? (uint32_t)F : 0u;
+    v[3] = big ? 0u : ntask;
+    v[4] = big ? (uint32_t)F * ((n + kReshTile - 1) / kReshTile) : 0u;
+  }
   FNode nd = {0u, 0u, n, 0u, 0u, 1u, 0u, 0u};
   Ctl hc = {};
   hc.q = 1;
-  uint32_t cnts[2] = {big ? ntask : 0u, big ? 0u : ntask};
-  unsigned long long elems = (unsigned long long)F * n;
+  hc.c_big = v[2];
+  hc.c_small = v[3];
+  hc.c_tiles = v[4];
+  const uint32_t zeros[kChFields] = {0u, 0u, 0u, 0u, 0u};
+  const uint32_t range[2] = {0u, 1u};
   DevMem d_in(hrows.size() * sizeof(IdxT)), d_out(hrows.size() * sizeof(IdxT)), d_bits((size_t)W * 4),
-      d_front(sizeof(FNode)), d_aux(sizeof(NAux)), d_big((size_t)F * sizeof(Task)), d_small((size_t)ntask * sizeof(Task)),
-      d_cnt(8), d_elems(8), d_off(3 * 2 * 4), d_ctl(sizeof(Ctl));
+      d_front(sizeof(FNode)), d_aux(sizeof(NAux)), d_big((size_t)F * sizeof(Task)),
+      d_small((size_t)std::max(v[3], 1u) * sizeof(Task)), d_offs(sizeof zeros), d_range(sizeof range), d_plan(4 * 4),
+      d_wg(2 * 4), d_ctl(sizeof(Ctl));
   HIPC(hipMemcpyAsync(d_in.p, hrows.data(), hrows.size() * sizeof(IdxT), hipMemcpyHostToDevice, st));
   HIPC(hipMemcpyAsync(d_out.p, hrows.data(), hrows.size() * sizeof(IdxT), hipMemcpyHostToDevice, st));
   HIPC(hipMemcpyAsync(d_bits.p, hbits.data(), (size_t)W * 4, hipMemcpyHostToDevice, st));
   HIPC(hipMemcpyAsync(d_front.p, &nd, sizeof nd, hipMemcpyHostToDevice, st));
   HIPC(hipMemcpyAsync(d_aux.p, &ax, sizeof ax, hipMemcpyHostToDevice, st));
-  HIPC(hipMemcpyAsync(d_cnt.p, cnts, 8, hipMemcpyHostToDevice, st));
-  HIPC(hipMemcpyAsync(d_elems.p, &elems, 8, hipMemcpyHostToDevice, st));
+  HIPC(hipMemcpyAsync(d_offs.p, zeros, sizeof zeros, hipMemcpyHostToDevice, st));
+  HIPC(hipMemcpyAsync(d_range.p, range, sizeof range, hipMemcpyHostToDevice, st));
   HIPC(hipMemcpyAsync(d_ctl.p, &hc, sizeof hc, hipMemcpyHostToDevice, st));
   FitArgs<IdxT, LabT> a = {};
   a.X_T = ds->XT.as<float>();
@@ -1389,17 +1465,16 @@ static void hook_reshuffle(cta_ctx *ctx, cta_dataset *ds, const uint32_t *rows, 
   a.aux = d_aux.as<NAux>();
   a.big_tasks = d_big.as<Task>();
   a.small_tasks = d_small.as<Task>();
-  a.cnt_big = d_cnt.as<uint32_t>();
-  a.cnt_small = d_cnt.as<uint32_t>() + 1;
-  a.elems = d_elems.as<unsigned long long>();
-  a.big_off = d_off.as<uint32_t>();
-  a.small_off = d_off.as<uint32_t>() + 2;
-  a.wg_off = d_off.as<uint32_t>() + 4;
+  a.offs = d_offs.as<uint32_t>();
+  a.tlo = d_range.as<uint32_t>();
+  a.thi = d_range.as<uint32_t>() + 1;
+  a.plan = d_plan.as<uint32_t>();
+  a.wg_off = d_wg.as<uint32_t>();
   a.ctl = d_ctl.as<Ctl>();
   a.cap_big = (uint32_t)F;
-  a.cap_small = ntask;
-  k_plan<IdxT, LabT><<<1, 64, 0, st>>>(a);
-  k_emit<IdxT, LabT><<<8, kBlock, 0, st>>>(a);
+  a.cap_small = std::max(v[3], 1u);
+  k_emit_resh<IdxT, LabT><<<8, kBlock, 0, st>>>(a);
+  k_plan<IdxT, LabT><<<1, kPlanThreads, 0, st>>>(a, 16u);
   const size_t lds = (((size_t)W + 3) & ~(size_t)3) * 4 + 2 * kSlots * 4;
   if (!global_bits && lds <= ctx->lds_per_block) {
     HIPC(hipFuncSetAttribute((const void *)k_resh<IdxT, LabT, true>, hipFuncAttributeMaxDynamicSharedMemorySize,

@@ -1,26 +1,32 @@
 // fit_kernels.hip -- level-synchronous random-forest growth on gfx950.
 //
 // One level of every resident tree is processed by a fixed sequence of
-// launches.  Every kernel is wave-granular: a 64-lane wave takes one frontier
-// node or one task (a chunk of a node's segment in one or more sorted rows)
-// at a time from a grid-stride loop whose bound is read from device memory,
-// so the host never has to size grids from device-side counts.
+// launches.  The per-node kernels are wave-granular: a 64-lane wave takes one
+// frontier node or one task (a chunk of a node's segment in one or more
+// sorted rows) at a time from a grid-stride loop whose bound is read from
+// device memory, so the host never has to size grids from device-side
+// counts.  Lists are laid out by exclusive scans over per-node counts, never
+// by atomics, so frontiers stay sorted by tree and every tree's nodes and
+// tasks are contiguous.
 //
 //   k_select      per node: feature visiting order, constant-feature skip,
-//                 split-scan task list           (bfs_module.cu:122-139)
+//                 task / histogram / fill-slot counts  (bfs_module.cu:122-139)
+//   k_off_*       exclusive offsets of per-node counts
+//   k_select_emit per node: split-scan task list
 //   k_scan_hist   per multi-chunk task: class histogram of the chunk
 //                                                (dfs_module.cu:24-80)
 //   k_scan_carry  per multi-chunk feature: exclusive prefix over chunks
 //                                                (dfs_module.cu:82-106)
 //   k_scan        per task: Gini scan of the chunk, best position
 //                                                (dfs_module.cu:310-443, bfs_module.cu:65-203)
-//   k_decide      per node: best split over tasks, threshold, child ids
-//                                                (dfs_module.cu:446-491, 158-208, bfs_module.cu:206-235, 379-401)
-//   k_fill        per chunk of the split row: mark table, left class counts
+//   k_decide      per node: best split over tasks, threshold, left-child
+//                 class counts, fill tasks      (dfs_module.cu:446-491, 158-208, bfs_module.cu:206-235, 379-401)
+//   k_fill_lds    per tree: side bitmap built in LDS
 //                                                (dfs_module.cu:210-221, bfs_module.cu:238-264)
-//   k_children    per split node: leaf tests, next frontier, reshuffle tasks
-//                                                (random_tree.py:71-165)
-//   k_plan / k_emit / k_resh
+//   k_children    per split node: leaf tests (random_tree.py:71-165), child
+//                 and reshuffle counts; k_children_emit: ids, next frontier,
+//                 reshuffle tasks
+//   k_plan / k_resh
 //                 stable partition of every row (dfs_module.cu:224-307, bfs_module.cu:266-377)
 //
 // The split scan parallelises the reference's serial recurrence
@@ -60,14 +66,149 @@ __device__ __forceinline__ uint32_t ceil_div(uint32_t a, uint32_t b) { return (a
 
 __device__ __forceinline__ uint32_t smem_wave_offset(int K) { return (threadIdx.x / kWave) * 2u * (uint32_t)K; }
 
+// Adds a wave's algorithmic byte count to the counter of `kind` (one
+// atomic per wave and launch, spread over kByteSlots lines).
+template <class IdxT, class LabT>
+__device__ __forceinline__ void flush_bytes(const FitArgs<IdxT, LabT> &a, ByteKind kind, unsigned long long v) {
+  if (lane_id() == 0 && v)
+    atomicAdd(&a.bslot[((uint32_t)kind * kByteSlots + blockIdx.x % kByteSlots) * kByteStride], v);
+}
+
 // ------------------------------------------------------------------------
-// k_select: visiting order and split-scan tasks of every frontier node.
+// Offset scans.  Per-node counts (NF fields, node-major) become exclusive
+// offsets and totals, so every list a level builds (split-scan tasks,
+// histograms, fill tasks, child ids, next frontier, reshuffle tasks) is laid
+// out in frontier order without atomics.  Frontiers are sorted by tree and
+// children are emitted in parent order, so each tree's nodes and tasks stay
+// contiguous.
+// ------------------------------------------------------------------------
+constexpr int kOffThreads = 256, kOffItems = 4;
+constexpr uint32_t kOffTile = kOffThreads * kOffItems;  // items per block of the multi-block scan
+constexpr int kOffSingleThreads = 1024;
+
+// Per-block sums of each field.
+template <int NF>
+__global__ __launch_bounds__(kOffThreads) void k_off_sums(const uint32_t *v, uint32_t q, uint32_t *bsum) {
+  __shared__ uint32_t ws[kOffThreads / kWave][NF];
+  const uint32_t tid = threadIdx.x, lane = tid % kWave, wave = tid / kWave;
+  const uint32_t i0 = blockIdx.x * kOffTile + tid * kOffItems;
+  uint32_t s[NF];
+#pragma unroll
+  for (int f = 0; f < NF; ++f) s[f] = 0;
+#pragma unroll
+  for (int k = 0; k < kOffItems; ++k)
+    if (i0 + k < q)
+#pragma unroll
+      for (int f = 0; f < NF; ++f) s[f] += v[(size_t)(i0 + k) * NF + f];
+#pragma unroll
+  for (int f = 0; f < NF; ++f) {
+    s[f] = wave_sum32(s[f]);
+    if (lane == 0) ws[wave][f] = s[f];
+  }
+  __syncthreads();
+  if (tid < (uint32_t)NF) {
+    uint32_t t = 0;
+    for (int w = 0; w < kOffThreads / kWave; ++w) t += ws[w][tid];
+    bsum[(size_t)blockIdx.x * NF + tid] = t;
+  }
+}
+
+// One block: exclusive scan of n items (in place allowed), totals to tot.
+template <int NF>
+__global__ __launch_bounds__(kOffSingleThreads) void k_off_single(const uint32_t *v, uint32_t n, uint32_t *out,
+                                                                 uint32_t *tot) {
+  constexpr int kW = kOffSingleThreads / kWave;
+  __shared__ uint32_t ws[kW][NF];
+  __shared__ uint32_t carry[NF];
+  const uint32_t tid = threadIdx.x, lane = tid % kWave, wave = tid / kWave;
+  if (tid < (uint32_t)NF) carry[tid] = 0;
+  __syncthreads();
+  for (uint32_t base = 0; base < n; base += kOffSingleThreads * kOffItems) {
+    const uint32_t i0 = base + tid * kOffItems;
+    uint32_t x[kOffItems][NF], ts[NF], inc[NF];
+#pragma unroll
+    for (int f = 0; f < NF; ++f) ts[f] = 0;
+#pragma unroll
+    for (int k = 0; k < kOffItems; ++k)
+#pragma unroll
+      for (int f = 0; f < NF; ++f) {
+        x[k][f] = (i0 + k < n) ? v[(size_t)(i0 + k) * NF + f] : 0u;
+        ts[f] += x[k][f];
+      }
+#pragma unroll
+    for (int f = 0; f < NF; ++f) {
+      inc[f] = wave_incl_scan(ts[f]);
+      if (lane == kWave - 1) ws[wave][f] = inc[f];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int f = 0; f < NF; ++f) {
+      uint32_t run = carry[f] + inc[f] - ts[f];
+      for (uint32_t w = 0; w < wave; ++w) run += ws[w][f];
+#pragma unroll
+      for (int k = 0; k < kOffItems; ++k) {
+        if (i0 + k < n) out[(size_t)(i0 + k) * NF + f] = run;
+        run += x[k][f];
+      }
+    }
+    __syncthreads();
+    if (tid < (uint32_t)NF) {
+      uint32_t t = 0;
+      for (int w = 0; w < kW; ++w) t += ws[w][tid];
+      carry[tid] += t;
+    }
+    __syncthreads();
+  }
+  if (tid < (uint32_t)NF) tot[tid] = carry[tid];
+}
+
+// Exclusive offsets of one block's items given the scanned block sums.
+template <int NF>
+__global__ __launch_bounds__(kOffThreads) void k_off_apply(const uint32_t *v, uint32_t q, const uint32_t *bsum,
+                                                           uint32_t *out) {
+  __shared__ uint32_t ws[kOffThreads / kWave][NF];
+  const uint32_t tid = threadIdx.x, lane = tid % kWave, wave = tid / kWave;
+  const uint32_t i0 = blockIdx.x * kOffTile + tid * kOffItems;
+  uint32_t x[kOffItems][NF], ts[NF], inc[NF];
+#pragma unroll
+  for (int f = 0; f < NF; ++f) ts[f] = 0;
+#pragma unroll
+  for (int k = 0; k < kOffItems; ++k)
+#pragma unroll
+    for (int f = 0; f < NF; ++f) {
+      x[k][f] = (i0 + k < q) ? v[(size_t)(i0 + k) * NF + f] : 0u;
+      ts[f] += x[k][f];
+    }
+#pragma unroll
+  for (int f = 0; f < NF; ++f) {
+    inc[f] = wave_incl_scan(ts[f]);
+    if (lane == kWave - 1) ws[wave][f] = inc[f];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int f = 0; f < NF; ++f) {
+    uint32_t run = bsum[(size_t)blockIdx.x * NF + f] + inc[f] - ts[f];
+    for (uint32_t w = 0; w < wave; ++w) run += ws[w][f];
+#pragma unroll
+    for (int k = 0; k < kOffItems; ++k) {
+      if (i0 + k < q) out[(size_t)(i0 + k) * NF + f] = run;
+      run += x[k][f];
+    }
+  }
+}
+
+// ------------------------------------------------------------------------
+// k_select: visiting order of every frontier node, its task and slot counts
+// (scan tasks, histogram words, fill-task slots) and the frontier range of
+// each tree.
 // ------------------------------------------------------------------------
 template <class IdxT, class LabT>
 __global__ __launch_bounds__(kBlock) void k_select(FitArgs<IdxT, LabT> a) {
   const uint32_t lane = lane_id();
   const uint32_t m = (uint32_t)a.m, F = (uint32_t)a.F;
-  WAVE_LOOP(a.ctl->q) {
+  const uint32_t q = a.ctl->q;
+  unsigned long long bytes = 0;
+  WAVE_LOOP(q) {
     const uint32_t i = item;
     const FNode nd = a.front[i];
     Perm pm = make_perm(a.seed, a.tree_base + nd.tree, nd.depth, nd.start, F, a.identity != 0);
@@ -91,18 +232,47 @@ __global__ __launch_bounds__(kBlock) void k_select(FitArgs<IdxT, LabT> a) {
     const uint32_t nch = ceil_div(n, a.Cs);
     const uint32_t fpt = (nch > 1) ? 1u : max(1u, a.Cs / n);
     const uint32_t ntask = (nf == 0) ? 0u : (nch > 1 ? nf * nch : ceil_div(nf, fpt));
-    uint32_t base = 0, hbase = 0;
     if (lane == 0) {
-      base = atomicAdd(&a.ctl->n_scan, ntask);
-      if (nch > 1) hbase = atomicAdd(&a.ctl->n_hist, nf * nch * (uint32_t)a.K);
-      atomicAdd(&a.ctl->bytes_scan, (unsigned long long)nf * n * (sizeof(IdxT) + sizeof(LabT) + 4));
+      uint32_t *v = a.vals + (size_t)i * kSelFields;
+      v[0] = ntask;
+      v[1] = (nch > 1) ? nf * nch * (uint32_t)a.K : 0u;
+      v[2] = ceil_div(n, a.Cf);
+      NAux ax = {};
+      ax.scan_ntask = ntask;
+      ax.nch_s = nch;
+      ax.fpt = fpt;
+      ax.nf = nf;
+      ax.best_f = -1;
+      a.aux[i] = ax;
+      if (i == 0 || a.front[i - 1].tree != nd.tree) a.tlo[nd.tree] = i;
+      if (i + 1 == q || a.front[i + 1].tree != nd.tree) a.thi[nd.tree] = i + 1;
+      bytes += (unsigned long long)nf * n * (sizeof(IdxT) + sizeof(LabT) + 4);
     }
-    base = uniform(__shfl(base, 0));
-    hbase = uniform(__shfl(hbase, 0));
-    const bool fits = (uint64_t)base + ntask <= a.cap_scan &&
-                      (nch <= 1 || (uint64_t)hbase + (uint64_t)nf * nch * a.K <= a.cap_hist);
+  }
+  flush_bytes(a, kBytesScan, bytes);
+}
+
+// Writes the split-scan tasks of every node at its scanned offset.
+template <class IdxT, class LabT>
+__global__ __launch_bounds__(kBlock) void k_select_emit(FitArgs<IdxT, LabT> a) {
+  if (a.ctl->err) return;  // a list overflowed: the fit fails, nothing more is read
+  const uint32_t lane = lane_id();
+  WAVE_LOOP(a.ctl->q) {
+    const uint32_t i = item;
+    NAux ax = a.aux[i];
+    const uint32_t *o = a.offs + (size_t)i * kSelFields;
+    const uint32_t *v = a.vals + (size_t)i * kSelFields;
+    const uint32_t base = o[0], hbase = o[1], fbase = o[2];
+    const uint32_t ntask = ax.scan_ntask, nch = ax.nch_s, fpt = ax.fpt, nf = ax.nf;
+    const bool fits = (uint64_t)base + ntask <= a.cap_scan && (uint64_t)hbase + v[1] <= a.cap_hist &&
+                      (uint64_t)fbase + v[2] <= a.cap_fill;
     if (!fits) {
-      if (lane == 0) atomicOr(&a.ctl->err, kErrCapacity);
+      if (lane == 0) {
+        atomicOr(&a.ctl->err, kErrCapacity);
+        ax.scan_ntask = 0;
+        ax.nch_f = 0;
+        a.aux[i] = ax;
+      }
       continue;
     }
     for (uint32_t t = lane; t < ntask; t += kWave) {
@@ -120,13 +290,10 @@ __global__ __launch_bounds__(kBlock) void k_select(FitArgs<IdxT, LabT> a) {
       a.scan_tasks[base + t] = tk;
     }
     if (lane == 0) {
-      NAux ax = {};
       ax.scan_base = base;
-      ax.scan_ntask = ntask;
-      ax.nch_s = nch;
-      ax.fpt = fpt;
       ax.hist_base = hbase;
-      ax.best_f = -1;
+      ax.fill_base = fbase;
+      ax.nch_f = v[2];
       a.aux[i] = ax;
     }
   }
@@ -139,6 +306,7 @@ __global__ __launch_bounds__(kBlock) void k_select(FitArgs<IdxT, LabT> a) {
 // ------------------------------------------------------------------------
 template <class IdxT, class LabT>
 __global__ __launch_bounds__(kBlock) void k_scan_hist(FitArgs<IdxT, LabT> a) {
+  if (a.ctl->err) return;  // a list overflowed: the fit fails, nothing more is read
   extern __shared__ uint32_t smem[];
   const uint32_t lane = lane_id();
   const uint32_t K = (uint32_t)a.K;
@@ -178,6 +346,7 @@ __global__ __launch_bounds__(kBlock) void k_scan_hist(FitArgs<IdxT, LabT> a) {
 // Exclusive prefix over the chunks of each multi-chunk (node, feature).
 template <class IdxT, class LabT>
 __global__ __launch_bounds__(kBlock) void k_scan_carry(FitArgs<IdxT, LabT> a) {
+  if (a.ctl->err) return;  // a list overflowed: the fit fails, nothing more is read
   const uint32_t lane = lane_id();
   const uint32_t K = (uint32_t)a.K;
   WAVE_LOOP(a.ctl->n_scan) {
@@ -206,6 +375,7 @@ __global__ __launch_bounds__(kBlock) void k_scan_carry(FitArgs<IdxT, LabT> a) {
 constexpr int kScanU = 4;
 template <class IdxT, class LabT>
 __global__ __launch_bounds__(kBlock) void k_scan(FitArgs<IdxT, LabT> a) {
+  if (a.ctl->err) return;  // a list overflowed: the fit fails, nothing more is read
   extern __shared__ uint32_t smem[];
   const uint32_t lane = lane_id();
   const uint32_t K = (uint32_t)a.K;
@@ -330,12 +500,73 @@ __global__ __launch_bounds__(kBlock) void k_scan(FitArgs<IdxT, LabT> a) {
 }
 
 // ------------------------------------------------------------------------
-// k_decide: best split of each node over its tasks; threshold; child ids.
+// k_decide: best split of each node over its tasks, threshold, class counts
+// of the left child and the node's bitmap-fill tasks.
 // ------------------------------------------------------------------------
+
+// Class counts of positions [0, p] of row f of node i into cntL.  The chunk
+// holding p is recounted from whichever of its ends is nearer: forwards
+// from the chunk's carried counts, or backwards (subtracting) from the next
+// chunk's carried counts (the node's counts for the last chunk).  Carries
+// exist for multi-chunk scans only; a single-chunk node is one chunk
+// starting from zero.
+template <class IdxT, class LabT>
+__device__ void count_left(const FitArgs<IdxT, LabT> &a, uint32_t i, const FNode &nd, const NAux &ax, uint32_t f,
+                           uint32_t p, uint32_t *h) {
+  const uint32_t lane = lane_id();
+  const uint32_t K = (uint32_t)a.K, n = nd.n, nch = ax.nch_s;
+  const IdxT *row = rows_of(a, nd.src, nd.tree, f) + nd.start;
+  const uint32_t c = (nch > 1) ? p / a.Cs : 0u;
+  const uint32_t c0 = c * a.Cs, c1 = (nch > 1) ? min(n, c0 + a.Cs) : n;
+  const uint32_t *before = nullptr;                 // counts before c0 (nullptr: zero)
+  const uint32_t *after = a.cnt + (size_t)i * K;    // counts before c1
+  if (nch > 1) {
+    // slot of f among the node's evaluated features
+    uint32_t j = 0;
+    for (uint32_t j0 = 0; j0 < ax.nf; j0 += kWave) {
+      const uint32_t jj = j0 + lane;
+      const uint64_t mk = ballot(jj < ax.nf && (uint32_t)a.feats[(size_t)i * a.m + jj] == f);
+      if (mk) {
+        j = j0 + (uint32_t)__ffsll((unsigned long long)mk) - 1u;
+        break;
+      }
+    }
+    const uint32_t *hb = a.hist + ax.hist_base + (size_t)j * nch * K;
+    before = hb + (size_t)c * K;
+    if (c + 1 < nch) after = hb + (size_t)(c + 1) * K;
+  }
+  const bool fwd = (p + 1 - c0) <= (c1 - (p + 1));
+  wave_sync();
+  for (uint32_t k = lane; k < K; k += kWave) h[k] = fwd ? (before ? before[k] : 0u) : after[k];
+  wave_sync();
+  const uint32_t r0 = fwd ? c0 : p + 1, r1 = fwd ? p + 1 : c1;
+  const uint32_t delta = fwd ? 1u : 0xFFFFFFFFu;
+  for (uint32_t pb = r0; pb < r1; pb += 4 * kWave) {
+    uint32_t sid[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const uint32_t q = pb + (uint32_t)u * kWave + lane;
+      sid[u] = (q < r1) ? (uint32_t)row[q] : 0xFFFFFFFFu;
+    }
+    uint32_t l[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) l[u] = (sid[u] != 0xFFFFFFFFu) ? (uint32_t)a.y[sid[u]] : 0xFFFFFFFFu;
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+      if (l[u] != 0xFFFFFFFFu) atomicAdd(&h[l[u]], delta);
+  }
+  wave_sync();
+  for (uint32_t k = lane; k < K; k += kWave) a.cntL[(size_t)i * K + k] = h[k];
+  wave_sync();
+}
+
 template <class IdxT, class LabT>
 __global__ __launch_bounds__(kBlock) void k_decide(FitArgs<IdxT, LabT> a) {
+  if (a.ctl->err) return;  // a list overflowed: the fit fails, nothing more is read
+  extern __shared__ uint32_t smem[];
   const uint32_t lane = lane_id();
-  const uint32_t K = (uint32_t)a.K;
+  uint32_t *h = smem + smem_wave_offset(a.K);
+  unsigned long long bytes = 0;
   WAVE_LOOP(a.ctl->q) {
     const uint32_t i = item;
     const FNode nd = a.front[i];
@@ -348,6 +579,7 @@ __global__ __launch_bounds__(kBlock) void k_decide(FitArgs<IdxT, LabT> a) {
     }
     best = wave_best(best);
     const uint32_t id = nd.nid;
+    uint32_t nfill = 0;
     if (!best.ok) {
       // no candidate position: leaf labelled by the first sample of the
       // segment in feature 0's row (random_tree.py:628-634, 50-63)
@@ -358,124 +590,131 @@ __global__ __launch_bounds__(kBlock) void k_decide(FitArgs<IdxT, LabT> a) {
         a.arena.left[id] = kLeaf;
         a.arena.right[id] = kLeaf;
         a.arena.value[id] = a.y[row0[0]];
-        ax.best_f = -1;
-        ax.flags = 0;
-        a.aux[i] = ax;
       }
-      continue;
-    }
-    const uint32_t f = (uint32_t)best.feat, p = best.pos;
-    for (uint32_t k = lane; k < K; k += kWave) a.cntL[(size_t)i * K + k] = 0;
-    if (lane == 0) {
-      const IdxT *row = rows_of(a, nd.src, nd.tree, f) + nd.start;
-      const float *x = a.X_T + (size_t)f * a.N;
-      // threshold midway in float, no contraction (bfs_module.cu:399-400)
-      const float thr = __fmul_rn(__fadd_rn(x[row[p]], x[row[p + 1]]), 0.5f);
-      const uint32_t ids = atomicAdd(&a.ctl->n_nodes, 2u);
-      const uint32_t nch = ceil_div(nd.n, a.Cf);
-      const uint32_t fb = atomicAdd(&a.ctl->n_fill, nch);
-      if ((uint64_t)ids + 2 > a.cap_nodes || (uint64_t)fb + nch > a.cap_fill) {
-        atomicOr(&a.ctl->err, kErrCapacity);
-        ax.flags = 0;
-        ax.best_f = -1;
-      } else {
+      ax.best_f = -1;
+      ax.flags = 0;
+    } else {
+      const uint32_t f = (uint32_t)best.feat, p = best.pos;
+      count_left(a, i, nd, ax, f, p, h);
+      nfill = ceil_div(p + 1, a.Cf);
+      if (lane == 0) {
+        const IdxT *row = rows_of(a, nd.src, nd.tree, f) + nd.start;
+        const float *x = a.X_T + (size_t)f * a.N;
+        // threshold midway in float, no contraction (bfs_module.cu:399-400)
+        const float thr = __fmul_rn(__fadd_rn(x[row[p]], x[row[p + 1]]), 0.5f);
         a.arena.feat[id] = (int32_t)f;
         a.arena.thr[id] = thr;
-        a.arena.left[id] = ids;
-        a.arena.right[id] = ids + 1;
         a.arena.value[id] = 0;
-        ax.best_f = (int32_t)f;
-        ax.best_p = p;
-        ax.nL = p + 1;
-        ax.left_id = ids;
-        ax.right_id = ids + 1;
-        ax.flags = kSplit;
-        ax.dst = (nd.src == 2) ? 1u : 2u;
-        ax.fill_base = fb;
-        ax.nch_f = nch;
-        for (uint32_t c = 0; c < nch; ++c) a.fill_tasks[fb + c] = Task{i, 0u, c, nch};
-        atomicAdd(&a.ctl->bytes_fill, (unsigned long long)nd.n * (sizeof(IdxT) + sizeof(LabT) + 1));
+        bytes += (unsigned long long)(p + 1) * sizeof(IdxT) + (unsigned long long)nd.n / 8;
       }
-      a.aux[i] = ax;
+      ax.best_f = (int32_t)f;
+      ax.best_p = p;
+      ax.nL = p + 1;
+      ax.flags = kSplit;
+      ax.dst = (nd.src == 2) ? 1u : 2u;
     }
+    // the node's fill slots: one task per Cf left positions, the rest empty
+    for (uint32_t c = lane; c < ax.nch_f; c += kWave)
+      a.fill_tasks[ax.fill_base + c] = (c < nfill) ? Task{i, 0u, c, nfill} : Task{kLeaf, 0u, 0u, 0u};
+    if (lane == 0) a.aux[i] = ax;
   }
+  flush_bytes(a, kBytesFill, bytes);
 }
 
 // ------------------------------------------------------------------------
-// k_fill: set the bit of every left-going sample of the split row's segment
-// in the tree's bitmap (the bitmaps are cleared at the start of the level, so
-// right-going samples read 0) and count the left child's classes.
+// Side bitmaps: bit s of tree t is set when sample s goes to the left child
+// of its node (dfs_module.cu:210-221, bfs_module.cu:238-264).
+//
+// k_fill_lds: workgroup (t, j) of J per tree builds the bits of its share of
+// tree t's fill tasks in an LDS copy of the bitmap (no global atomics) and
+// writes the whole copy: directly into the tree's bitmap when J == 1, else
+// into partial bitmap j, which k_merge_bits ORs together.  Every word is
+// written, so the bitmaps need no clearing.
 // ------------------------------------------------------------------------
+constexpr int kFillThreads = 1024;
+constexpr int kFillWaves = kFillThreads / kWave;
+
 template <class IdxT, class LabT>
-__global__ __launch_bounds__(kBlock) void k_fill(FitArgs<IdxT, LabT> a) {
-  extern __shared__ uint32_t smem[];
+__device__ __forceinline__ void fill_task_bits(const FitArgs<IdxT, LabT> &a, const Task &tk, uint32_t *bits) {
   const uint32_t lane = lane_id();
-  const uint32_t K = (uint32_t)a.K;
-  uint32_t *h = smem + smem_wave_offset(a.K);
+  const FNode nd = a.front[tk.node];
+  const NAux ax = a.aux[tk.node];
+  const IdxT *row = rows_of(a, nd.src, nd.tree, (uint32_t)ax.best_f) + nd.start;
+  const uint32_t p0 = tk.chunk * a.Cf, p1 = min(ax.nL, p0 + a.Cf);
+  for (uint32_t pb = p0; pb < p1; pb += 4 * kWave) {
+    uint32_t sid[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const uint32_t p = pb + (uint32_t)u * kWave + lane;
+      sid[u] = (p < p1) ? (uint32_t)row[p] : 0xFFFFFFFFu;
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+      if (sid[u] != 0xFFFFFFFFu) atomicOr(&bits[sid[u] >> 5], 1u << (sid[u] & 31u));
+  }
+}
+
+template <class IdxT, class LabT>
+__global__ __launch_bounds__(kFillThreads) void k_fill_lds(FitArgs<IdxT, LabT> a, uint32_t J, uint32_t *part) {
+  if (a.ctl->err) return;  // a list overflowed: the fit fails, nothing more is read
+  extern __shared__ uint32_t lbits[];
+  const uint32_t t = blockIdx.x / J, j = blockIdx.x % J;
+  const uint32_t lo = a.tlo[t], hi = a.thi[t];
+  if (hi <= lo) return;  // no frontier node in this tree
+  const uint32_t W = a.W, tid = threadIdx.x, wave = tid / kWave;
+  for (uint32_t w = tid; w < W; w += kFillThreads) lbits[w] = 0;
+  __syncthreads();
+  const uint32_t fb = a.aux[lo].fill_base;
+  const NAux last = a.aux[hi - 1];
+  const uint32_t fe = last.fill_base + last.nch_f;
+  for (uint32_t k = fb + j * kFillWaves + wave; k < fe; k += J * kFillWaves) {
+    const Task tk = a.fill_tasks[k];
+    if (tk.node == kLeaf) continue;
+    fill_task_bits(a, tk, lbits);
+  }
+  __syncthreads();
+  uint32_t *out = (J == 1) ? a.bits + (size_t)t * W : part + ((size_t)t * J + j) * W;
+  for (uint32_t w = tid; w < W; w += kFillThreads) out[w] = lbits[w];
+}
+
+// bits of tree blockIdx.y = OR of its J partial bitmaps.
+__global__ void k_merge_bits(const uint32_t *tlo, const uint32_t *thi, uint32_t W, uint32_t J, const uint32_t *part,
+                             uint32_t *bits) {
+  const uint32_t t = blockIdx.y;
+  if (thi[t] <= tlo[t]) return;
+  const uint32_t *src = part + (size_t)t * J * W;
+  for (uint32_t w = blockIdx.x * blockDim.x + threadIdx.x; w < W; w += gridDim.x * blockDim.x) {
+    uint32_t v = 0;
+    for (uint32_t j = 0; j < J; ++j) v |= src[(size_t)j * W + w];
+    bits[(size_t)t * W + w] = v;
+  }
+}
+
+// Bitmaps too large for LDS: global atomics into bitmaps cleared first.
+template <class IdxT, class LabT>
+__global__ __launch_bounds__(kBlock) void k_fill_global(FitArgs<IdxT, LabT> a) {
+  if (a.ctl->err) return;  // a list overflowed: the fit fails, nothing more is read
   WAVE_LOOP(a.ctl->n_fill) {
     const Task tk = a.fill_tasks[item];
-    const FNode nd = a.front[tk.node];
-    const NAux ax = a.aux[tk.node];
-    const IdxT *row = rows_of(a, nd.src, nd.tree, (uint32_t)ax.best_f) + nd.start;
-    uint32_t *bits = a.bits + (size_t)nd.tree * a.W;
-    for (uint32_t k = lane; k < K; k += kWave) h[k] = 0;
-    wave_sync();
-    const uint32_t p0 = tk.chunk * a.Cf, p1 = min(nd.n, p0 + a.Cf);
-    for (uint32_t pb = p0; pb < p1; pb += 4 * kWave) {
-      uint32_t sid[4];
-#pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        const uint32_t p = pb + (uint32_t)u * kWave + lane;
-        sid[u] = (p < p1) ? (uint32_t)row[p] : 0xFFFFFFFFu;
-      }
-      uint32_t l[4];
-#pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        const uint32_t p = pb + (uint32_t)u * kWave + lane;
-        l[u] = (sid[u] != 0xFFFFFFFFu && p <= ax.best_p) ? (uint32_t)a.y[sid[u]] : 0xFFFFFFFFu;
-      }
-#pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        if (l[u] == 0xFFFFFFFFu) continue;  // right side: its bit stays clear
-        const uint32_t s = sid[u];
-        atomicOr(&bits[s >> 5], 1u << (s & 31u));
-        atomicAdd(&h[l[u]], 1u);
-      }
-    }
-    wave_sync();
-    for (uint32_t k = lane; k < K; k += kWave)
-      if (h[k]) atomicAdd(&a.cntL[(size_t)tk.node * K + k], h[k]);
-    wave_sync();
+    if (tk.node == kLeaf) continue;
+    fill_task_bits(a, tk, a.bits + (size_t)a.front[tk.node].tree * a.W);
   }
 }
 
-__device__ __forceinline__ uint32_t bit_of(const uint32_t *bits, uint32_t s) { return (bits[s >> 5] >> (s & 31u)) & 1u; }
-
-// Reshuffle bookkeeping of split node i (n samples) of tree t: rows of a
-// node with at least kBigN samples are moved by whole workgroups (one task
-// per row), smaller nodes by single waves (one task per group of rows).
-// Counts the node's tasks into its tree's lists; k_plan turns the counts
-// into offsets and k_emit writes the task descriptors.
+// ------------------------------------------------------------------------
+// Reshuffle geometry.  Rows of a node with at least kBigN samples are moved
+// by whole 1,024-thread workgroups (one task per row, kReshTile-element
+// tiles); smaller nodes by single waves (one task per group of rows holding
+// about Cr elements).
+// ------------------------------------------------------------------------
 constexpr uint32_t kBigN = 4096;
-
-template <class IdxT, class LabT>
-__device__ void count_resh_tasks(const FitArgs<IdxT, LabT> &a, uint32_t t, uint32_t n, NAux &ax) {
-  const uint32_t lane = lane_id();
-  const uint32_t F = (uint32_t)a.F;
-  const bool big = n >= kBigN;
-  const uint32_t rpt = big ? 1u : max(1u, a.Cr / n);
-  const uint32_t ntask = big ? F : ceil_div(F, rpt);
-  uint32_t off = 0;
-  if (lane == 0) {
-    off = atomicAdd(big ? &a.cnt_big[t] : &a.cnt_small[t], ntask);
-    atomicAdd(&a.elems[t], (unsigned long long)F * n);
-  }
-  off = uniform(__shfl(off, 0));
-  ax.resh_big = big ? 1u : 0u;
-  ax.resh_off = off;
-  ax.resh_ntask = ntask;
-  ax.rpt = rpt;
-}
+constexpr int kReshThreads = 1024;
+constexpr int kReshWaves = kReshThreads / kWave;  // 16
+constexpr int kReshE = 8;                         // elements per thread per tile
+constexpr uint32_t kReshTile = kReshThreads * kReshE;
+constexpr uint32_t kSlots = kReshE * kReshWaves;  // 128 (sub-tile, wave) counts per tile
+static_assert(kSlots == 2 * kWave, "the tile prefix takes two slots per lane");
+constexpr uint64_t kWgElemsMax = 1ull << 19, kWgElemsMin = 1ull << 16;
 
 // Label of the majority class (lowest label on ties).
 __device__ uint32_t wave_majority(const uint32_t *c, uint32_t K) {
@@ -497,212 +736,315 @@ __device__ uint32_t wave_majority(const uint32_t *c, uint32_t K) {
   return bk;
 }
 
+// Reshuffle task shape of a node of n samples with a live child; fills
+// v[2..4] (big tasks, small tasks, big tiles) of its children-scan counts.
+__device__ __forceinline__ void resh_shape(uint32_t n, uint32_t F, uint32_t Cr, NAux &ax, uint32_t *v) {
+  const bool big = n >= kBigN;
+  const uint32_t rpt = big ? 1u : max(1u, Cr / n);
+  const uint32_t ntask = big ? F : ceil_div(F, rpt);
+  ax.resh_big = big ? 1u : 0u;
+  ax.rpt = rpt;
+  ax.resh_ntask = ntask;
+  v[2] = big ? F : 0u;
+  v[3] = big ? 0u : ntask;
+  v[4] = big ? F * ceil_div(n, kReshTile) : 0u;
+}
+
 // ------------------------------------------------------------------------
-// k_children: leaf tests of both children (random_tree.py:123-163),
-// the next frontier and the reshuffle tasks.
+// k_children: leaf tests of both children of every split node
+// (random_tree.py:123-163) and the node's children-scan counts: child ids,
+// next-frontier slots, reshuffle tasks.
 // ------------------------------------------------------------------------
 template <class IdxT, class LabT>
 __global__ __launch_bounds__(kBlock) void k_children(FitArgs<IdxT, LabT> a) {
+  if (a.ctl->err) return;  // a list overflowed: the fit fails, nothing more is read
   extern __shared__ uint32_t smem[];
   const uint32_t lane = lane_id();
   const uint32_t K = (uint32_t)a.K;
   uint32_t *cl = smem + smem_wave_offset(a.K);
   uint32_t *cr = cl + K;
+  unsigned long long bytes = 0;
   WAVE_LOOP(a.ctl->q) {
     const uint32_t i = item;
     NAux ax = a.aux[i];
+    uint32_t v[kChFields] = {0u, 0u, 0u, 0u, 0u};
+    if (ax.flags & kSplit) {
+      const FNode nd = a.front[i];
+      const uint32_t nL = ax.nL, nR = nd.n - ax.nL;
+      uint32_t nzl = 0, nzr = 0, onlyl = 0, onlyr = 0;
+      wave_sync();
+      for (uint32_t k = lane; k < K; k += kWave) {
+        const uint32_t l = a.cntL[(size_t)i * K + k];
+        const uint32_t r = a.cnt[(size_t)i * K + k] - l;
+        cl[k] = l;
+        cr[k] = r;
+        if (l) {
+          ++nzl;
+          onlyl = k;
+        }
+        if (r) {
+          ++nzr;
+          onlyr = k;
+        }
+      }
+      nzl = uniform(wave_sum32(nzl));
+      nzr = uniform(wave_sum32(nzr));
+      // the single present class (max over lanes: absent lanes hold 0)
+      uint32_t ol = onlyl, orr = onlyr;
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) {
+        ol = max(ol, (uint32_t)__shfl_xor(ol, o, kWave));
+        orr = max(orr, (uint32_t)__shfl_xor(orr, o, kWave));
+      }
+      wave_sync();
+      const uint32_t cd = nd.depth + 1;
+      const bool capped = a.max_depth > 0 && (int64_t)cd >= (int64_t)a.max_depth;
+      // 0 live, 1 pure, 2 first-sample, 3 majority
+      auto kind = [&](uint32_t nz, uint32_t nc) -> uint32_t {
+        if (nz <= 1) return 1u;
+        if ((int64_t)nc < (int64_t)a.mss) return 2u;
+        if (capped) return 3u;
+        return 0u;
+      };
+      const uint32_t kl = kind(nzl, nL), kr = kind(nzr, nR);
+      uint32_t labl = ol, labr = orr;
+      if (kl == 3) labl = wave_majority(cl, K);
+      if (kr == 3) labr = wave_majority(cr, K);
+      if (kl == 2 || kr == 2) {
+        // first sample of each child in feature 0's row: the first marked
+        // (left) and first unmarked (right) sample of the parent's segment
+        const IdxT *row0 = rows_of(a, nd.src, nd.tree, 0) + nd.start;
+        const uint32_t *bits = a.bits + (size_t)nd.tree * a.W;
+        bool needl = kl == 2, needr = kr == 2;
+        for (uint32_t pb = 0; pb < nd.n && (needl || needr); pb += kWave) {
+          const uint32_t p = pb + lane;
+          uint32_t s = 0, side = 2;
+          if (p < nd.n) {
+            s = row0[p];
+            side = (bits[s >> 5] >> (s & 31u)) & 1u;
+          }
+          const uint64_t ml = ballot(side == 1), mr = ballot(side == 0);
+          if (needl && ml) {
+            labl = a.y[readlane(s, (uint32_t)__ffsll((unsigned long long)ml) - 1u)];
+            needl = false;
+          }
+          if (needr && mr) {
+            labr = a.y[readlane(s, (uint32_t)__ffsll((unsigned long long)mr) - 1u)];
+            needr = false;
+          }
+        }
+      }
+      ax.lab[0] = kl ? labl : kLeaf;
+      ax.lab[1] = kr ? labr : kLeaf;
+      uint32_t flags = kSplit;
+      if (kl == 0) flags |= kLiveL;
+      if (kr == 0) flags |= kLiveR;
+      ax.flags = flags;
+      v[0] = 2;
+      v[1] = (kl == 0 ? 1u : 0u) + (kr == 0 ? 1u : 0u);
+      ax.resh_ntask = 0;
+      if (v[1]) {
+        resh_shape(nd.n, (uint32_t)a.F, a.Cr, ax, v);
+        bytes += (unsigned long long)a.F * nd.n * (2 * sizeof(IdxT)) + (nd.n + 7) / 8;
+      }
+    }
+    if (lane == 0) {
+      uint32_t *o = a.vals + (size_t)i * kChFields;
+#pragma unroll
+      for (int f = 0; f < kChFields; ++f) o[f] = v[f];
+      a.aux[i] = ax;
+    }
+  }
+  flush_bytes(a, kBytesResh, bytes);
+}
+
+// Reshuffle task descriptors of node i at its scanned offsets.
+template <class IdxT, class LabT>
+__device__ void emit_resh_tasks(const FitArgs<IdxT, LabT> &a, uint32_t i, const NAux &ax, uint32_t bbase,
+                                uint32_t sbase) {
+  const uint32_t lane = lane_id();
+  const uint32_t F = (uint32_t)a.F;
+  if (ax.resh_ntask == 0) return;
+  if (ax.resh_big) {
+    for (uint32_t f = lane; f < F; f += kWave) a.big_tasks[bbase + f] = Task{i, f | (1u << 16), 0u, 1u};
+  } else {
+    for (uint32_t t = lane; t < ax.resh_ntask; t += kWave) {
+      const uint32_t first = t * ax.rpt;
+      a.small_tasks[sbase + t] = Task{i, first | (min(ax.rpt, F - first) << 16), 0u, 1u};
+    }
+  }
+}
+
+// True if node i's scanned ranges fit the lists.
+template <class IdxT, class LabT>
+__device__ __forceinline__ bool resh_fits(const FitArgs<IdxT, LabT> &a, const NAux &ax, const uint32_t *o) {
+  if (ax.resh_ntask == 0) return true;
+  return ax.resh_big ? (uint64_t)o[2] + ax.resh_ntask <= a.cap_big : (uint64_t)o[3] + ax.resh_ntask <= a.cap_small;
+}
+
+// Child ids, arena entries of both children, the next frontier (in parent
+// order, left child first) and the reshuffle tasks of every split node.
+template <class IdxT, class LabT>
+__global__ __launch_bounds__(kBlock) void k_children_emit(FitArgs<IdxT, LabT> a) {
+  if (a.ctl->err) return;  // a list overflowed: the fit fails, nothing more is read
+  const uint32_t lane = lane_id();
+  const uint32_t K = (uint32_t)a.K;
+  const uint32_t id_base = a.ctl->n_nodes;
+  WAVE_LOOP(a.ctl->q) {
+    const uint32_t i = item;
+    const NAux ax = a.aux[i];
     if (!(ax.flags & kSplit)) continue;
     const FNode nd = a.front[i];
-    const uint32_t nL = ax.nL, nR = nd.n - ax.nL;
-    uint32_t nzl = 0, nzr = 0, onlyl = 0, onlyr = 0;
-    wave_sync();
-    for (uint32_t k = lane; k < K; k += kWave) {
-      const uint32_t l = a.cntL[(size_t)i * K + k];
-      const uint32_t r = a.cnt[(size_t)i * K + k] - l;
-      cl[k] = l;
-      cr[k] = r;
-      if (l) {
-        ++nzl;
-        onlyl = k;
-      }
-      if (r) {
-        ++nzr;
-        onlyr = k;
-      }
+    const uint32_t *o = a.offs + (size_t)i * kChFields;
+    const uint32_t id0 = id_base + o[0];
+    uint32_t slot = o[1];
+    const uint32_t nlive = ((ax.flags & kLiveL) ? 1u : 0u) + ((ax.flags & kLiveR) ? 1u : 0u);
+    if ((uint64_t)id0 + 2 > a.cap_nodes || (uint64_t)slot + nlive > a.cap_q || !resh_fits(a, ax, o)) {
+      if (lane == 0) atomicOr(&a.ctl->err, kErrCapacity);
+      continue;
     }
-    nzl = uniform(wave_sum32(nzl));
-    nzr = uniform(wave_sum32(nzr));
-    // the single present class (max over lanes: absent lanes hold 0)
-    uint32_t ol = onlyl, orr = onlyr;
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-      ol = max(ol, (uint32_t)__shfl_xor(ol, o, kWave));
-      orr = max(orr, (uint32_t)__shfl_xor(orr, o, kWave));
+    if (lane == 0) {
+      a.arena.left[nd.nid] = id0;
+      a.arena.right[nd.nid] = id0 + 1;
     }
-    wave_sync();
     const uint32_t cd = nd.depth + 1;
-    const bool capped = a.max_depth > 0 && (int64_t)cd >= (int64_t)a.max_depth;
-    // 0 live, 1 pure, 2 first-sample, 3 majority
-    auto kind = [&](uint32_t nz, uint32_t nc) -> uint32_t {
-      if (nz <= 1) return 1u;
-      if ((int64_t)nc < (int64_t)a.mss) return 2u;
-      if (capped) return 3u;
-      return 0u;
-    };
-    const uint32_t kl = kind(nzl, nL), kr = kind(nzr, nR);
-    uint32_t labl = ol, labr = orr;
-    if (kl == 3) labl = wave_majority(cl, K);
-    if (kr == 3) labr = wave_majority(cr, K);
-    if (kl == 2 || kr == 2) {
-      // first sample of each child in feature 0's row: the first marked
-      // (left) and first unmarked (right) sample of the parent's segment
-      const IdxT *row0 = rows_of(a, nd.src, nd.tree, 0) + nd.start;
-      const uint32_t *bits = a.bits + (size_t)nd.tree * a.W;
-      bool needl = kl == 2, needr = kr == 2;
-      for (uint32_t pb = 0; pb < nd.n && (needl || needr); pb += kWave) {
-        const uint32_t p = pb + lane;
-        uint32_t s = 0, side = 2;
-        if (p < nd.n) {
-          s = row0[p];
-          side = bit_of(bits, s);
-        }
-        const uint64_t ml = ballot(side == 1), mr = ballot(side == 0);
-        if (needl && ml) {
-          labl = a.y[readlane(s, (uint32_t)__ffsll((unsigned long long)ml) - 1u)];
-          needl = false;
-        }
-        if (needr && mr) {
-          labr = a.y[readlane(s, (uint32_t)__ffsll((unsigned long long)mr) - 1u)];
-          needr = false;
-        }
-      }
-    }
-    uint32_t flags = ax.flags;
-    const uint32_t cids[2] = {ax.left_id, ax.right_id};
-    const uint32_t kinds[2] = {kl, kr};
-    const uint32_t labs[2] = {labl, labr};
-    const uint32_t cn[2] = {nL, nR};
-    const uint32_t cs[2] = {nd.start, nd.start + nL};
-    for (int c = 0; c < 2; ++c) {
-      const uint32_t id = cids[c];
+    for (uint32_t c = 0; c < 2; ++c) {
+      const uint32_t id = id0 + c;
+      const uint32_t cn = c ? nd.n - ax.nL : ax.nL;
       if (lane == 0) {
-        a.arena.nsamp[id] = cn[c];
+        a.arena.nsamp[id] = cn;
         a.arena.depth[id] = cd;
       }
-      if (kinds[c] != 0) {
+      if (ax.lab[c] != kLeaf) {
         if (lane == 0) {
           a.arena.feat[id] = -1;
           a.arena.thr[id] = 0.f;
           a.arena.left[id] = kLeaf;
           a.arena.right[id] = kLeaf;
-          a.arena.value[id] = labs[c];
+          a.arena.value[id] = ax.lab[c];
         }
-        continue;
-      }
-      uint32_t slot = 0;
-      if (lane == 0) slot = atomicAdd(&a.ctl->q_next, 1u);
-      slot = uniform(__shfl(slot, 0));
-      if (slot >= a.cap_q) {
-        if (lane == 0) atomicOr(&a.ctl->err, kErrCapacity);
         continue;
       }
       if (lane == 0) {
         FNode ch;
         ch.tree = nd.tree;
-        ch.start = cs[c];
-        ch.n = cn[c];
+        ch.start = c ? nd.start + ax.nL : nd.start;
+        ch.n = cn;
         ch.nid = id;
         ch.depth = cd;
         ch.src = ax.dst;
         ch.pad0 = ch.pad1 = 0;
         a.next_front[slot] = ch;
       }
-      const uint32_t *cc = c ? cr : cl;
-      for (uint32_t k = lane; k < K; k += kWave) a.next_cnt[(size_t)slot * K + k] = cc[k];
-      flags |= c ? kLiveR : kLiveL;
+      for (uint32_t k = lane; k < K; k += kWave) {
+        const uint32_t l = a.cntL[(size_t)i * K + k];
+        a.next_cnt[(size_t)slot * K + k] = c ? a.cnt[(size_t)i * K + k] - l : l;
+      }
+      ++slot;
     }
-    ax.flags = flags;
-    if (flags & (kLiveL | kLiveR)) {
-      count_resh_tasks(a, nd.tree, nd.n, ax);
-      if (lane == 0)
-        atomicAdd(&a.ctl->bytes_resh, (unsigned long long)a.F * nd.n * (2 * sizeof(IdxT) + 1));
-    } else {
-      ax.resh_ntask = 0;
+    emit_resh_tasks(a, i, ax, o[2], o[3]);
+  }
+}
+
+// Reshuffle tasks only (bootstrap compaction, reshuffle hook).
+template <class IdxT, class LabT>
+__global__ __launch_bounds__(kBlock) void k_emit_resh(FitArgs<IdxT, LabT> a) {
+  if (a.ctl->err) return;  // a list overflowed: the fit fails, nothing more is read
+  WAVE_LOOP(a.ctl->q) {
+    const NAux ax = a.aux[item];
+    if (!(ax.flags & kSplit)) continue;
+    const uint32_t *o = a.offs + (size_t)item * kChFields;
+    if (!resh_fits(a, ax, o)) {
+      if (lane_id() == 0) atomicOr(&a.ctl->err, kErrCapacity);
+      continue;
     }
-    if (lane == 0) a.aux[i] = ax;
-    wave_sync();
+    emit_resh_tasks(a, item, ax, o[2], o[3]);
   }
 }
 
 // ------------------------------------------------------------------------
 // Reshuffle: stable partition of every row of the split nodes.
 //
-// k_plan turns the per-tree task counts into offsets and gives each tree
-// ceil(elements / kWgElems) workgroups.  k_emit writes the task lists.
-// k_resh runs 1,024-thread workgroups, each bound to one tree: it stages the
-// tree's side bitmap in LDS (when it fits), then moves its share of the
-// tree's big tasks cooperatively (4,096-element tiles, one barrier per tile,
-// the left count carried across tiles) and its share of the small tasks one
-// wave per task.  Loads and stores are striped (lane-consecutive), and the
-// stable ranks come from one ballot per 64 elements.
+// k_plan reads each tree's task ranges off the children-scan offsets at its
+// frontier range and gives the tree workgroups in proportion to its
+// elements, so that the level's work spreads over about `target`
+// workgroups (each moving between kWgElemsMin and kWgElemsMax elements).
+// It also clears the tree ranges for the next level.  k_resh runs
+// 1,024-thread workgroups, each bound to one tree: it stages the tree's side
+// bitmap in LDS (when it fits), then moves its share of the tree's big tasks
+// cooperatively (kReshTile-element tiles, one barrier per tile, the left
+// count carried across tiles) and its share of the small tasks one wave per
+// task.  Loads and stores are striped (lane-consecutive), and the stable
+// ranks come from one ballot per 64 elements.
 // ------------------------------------------------------------------------
-constexpr int kReshThreads = 1024;
-constexpr int kReshWaves = kReshThreads / kWave;  // 16
-constexpr int kReshE = 8;                         // elements per thread per tile
-constexpr uint32_t kReshTile = kReshThreads * kReshE;
-constexpr uint32_t kSlots = kReshE * kReshWaves;  // 128 (sub-tile, wave) counts per tile
-static_assert(kSlots == 2 * kWave, "the tile prefix takes two slots per lane");
-constexpr uint64_t kWgElems = 1ull << 19;
+constexpr int kPlanThreads = 1024;
 
 template <class IdxT, class LabT>
-__global__ void k_plan(FitArgs<IdxT, LabT> a) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  uint32_t b = 0, sm = 0, w = 0;
-  for (uint32_t t = 0; t < a.T; ++t) {
-    a.big_off[t] = b;
-    a.small_off[t] = sm;
-    a.wg_off[t] = w;
-    const unsigned long long e = a.elems[t];
-    b += a.cnt_big[t];
-    sm += a.cnt_small[t];
-    if (e) w += (uint32_t)min((unsigned long long)4096, (e + kWgElems - 1) / kWgElems);
-  }
-  a.big_off[a.T] = b;
-  a.small_off[a.T] = sm;
-  a.wg_off[a.T] = w;
-  if (b > a.cap_big || sm > a.cap_small) {
-    atomicOr(&a.ctl->err, kErrCapacity);
-    w = 0;
-    a.wg_off[a.T] = 0;
-  }
-  a.ctl->n_plan = w;
-  for (uint32_t t = 0; t < a.T; ++t) {
-    a.cnt_big[t] = 0;
-    a.cnt_small[t] = 0;
-    a.elems[t] = 0;
-  }
-}
-
-// Writes the task descriptors of every split node with a live child.
-template <class IdxT, class LabT>
-__global__ __launch_bounds__(kBlock) void k_emit(FitArgs<IdxT, LabT> a) {
-  const uint32_t lane = lane_id();
-  if (a.ctl->n_plan == 0) return;
-  WAVE_LOOP(a.ctl->q) {
-    const NAux ax = a.aux[item];
-    if (!(ax.flags & kSplit) || !(ax.flags & (kLiveL | kLiveR)) || ax.resh_ntask == 0) continue;
-    const FNode nd = a.front[item];
-    const uint32_t F = (uint32_t)a.F;
-    if (ax.resh_big) {
-      Task *out = a.big_tasks + a.big_off[nd.tree] + ax.resh_off;
-      for (uint32_t f = lane; f < F; f += kWave) out[f] = Task{item, f | (1u << 16), 0u, 1u};
-    } else {
-      Task *out = a.small_tasks + a.small_off[nd.tree] + ax.resh_off;
-      for (uint32_t t = lane; t < ax.resh_ntask; t += kWave) {
-        const uint32_t first = t * ax.rpt;
-        out[t] = Task{item, first | (min(ax.rpt, F - first) << 16), 0u, 1u};
+__global__ __launch_bounds__(kPlanThreads) void k_plan(FitArgs<IdxT, LabT> a, uint32_t target) {
+  constexpr int kW = kPlanThreads / kWave;
+  __shared__ uint32_t ws[kW];
+  __shared__ uint32_t carry;
+  const uint32_t tid = threadIdx.x, lane = tid % kWave, wave = tid / kWave;
+  const uint32_t q = a.ctl->q;
+  const uint32_t tot_big = a.ctl->c_big, tot_small = a.ctl->c_small, tot_tiles = a.ctl->c_tiles;
+  const unsigned long long total =
+      (unsigned long long)tot_tiles * kReshTile + (unsigned long long)tot_small * a.Cr;
+  const unsigned long long per =
+      min(kWgElemsMax, max(kWgElemsMin, total / max(1u, target)));
+  if (tid == 0) carry = 0;
+  __syncthreads();
+  for (uint32_t t0 = 0; t0 < a.T; t0 += kPlanThreads) {
+    const uint32_t t = t0 + tid;
+    uint32_t wgs = 0;
+    if (t < a.T) {
+      const uint32_t lo = a.tlo[t], hi = a.thi[t];
+      uint32_t bb = 0, be = 0, sb = 0, se = 0, tb = 0, te = 0;
+      if (hi > lo) {
+        const uint32_t *ol = a.offs + (size_t)lo * kChFields;
+        bb = ol[2];
+        sb = ol[3];
+        tb = ol[4];
+        if (hi < q) {
+          const uint32_t *oh = a.offs + (size_t)hi * kChFields;
+          be = oh[2];
+          se = oh[3];
+          te = oh[4];
+        } else {
+          be = tot_big;
+          se = tot_small;
+          te = tot_tiles;
+        }
       }
+      uint32_t *pl = a.plan + (size_t)t * 4;
+      pl[0] = bb;
+      pl[1] = be;
+      pl[2] = sb;
+      pl[3] = se;
+      const unsigned long long e =
+          (unsigned long long)(te - tb) * kReshTile + (unsigned long long)(se - sb) * a.Cr;
+      if (e) wgs = (uint32_t)min(4096ull, (e + per - 1) / per);
+      a.tlo[t] = 0;
+      a.thi[t] = 0;
     }
+    const uint32_t inc = wave_incl_scan(wgs);
+    if (lane == kWave - 1) ws[wave] = inc;
+    __syncthreads();
+    uint32_t off = carry + inc - wgs;
+    for (uint32_t w = 0; w < wave; ++w) off += ws[w];
+    if (t < a.T) a.wg_off[t] = off;
+    __syncthreads();
+    if (tid == 0) {
+      uint32_t sum = 0;
+      for (int w = 0; w < kW; ++w) sum += ws[w];
+      carry += sum;
+    }
+    __syncthreads();
+  }
+  if (tid == 0) {
+    uint32_t P = carry;
+    if (tot_big > a.cap_big || tot_small > a.cap_small || a.ctl->err) P = 0;
+    a.wg_off[a.T] = P;
+    a.ctl->n_plan = P;
   }
 }
 
@@ -767,15 +1109,63 @@ __device__ void resh_big(const FitArgs<IdxT, LabT> &a, const Task &tk, const Sid
 }
 
 // One small task: rows [first, first+count) of a node, moved by one wave.
+// A row of n <= 64 * kReshE / 2 samples takes c = ceil(n / 64) pieces, so
+// kReshE / c rows are loaded together (one batch of kReshE striped loads in
+// flight) before their ranks are taken in order.
 template <class IdxT, class LabT, class Side>
 __device__ void resh_small(const FitArgs<IdxT, LabT> &a, const Task &tk, const Side &side) {
   const FNode nd = a.front[tk.node];
   const NAux ax = a.aux[tk.node];
   const bool liveL = ax.flags & kLiveL, liveR = ax.flags & kLiveR;
   const uint32_t first = tk.first_count & 0xFFFFu, count = tk.first_count >> 16;
+  const uint32_t end = first + count;
   const uint32_t n = nd.n, nL = ax.nL;
   const uint32_t lane = lane_id();
-  for (uint32_t f = first; f < first + count; ++f) {
+  const uint32_t c = ceil_div(n, kWave);
+  if (c <= (uint32_t)kReshE / 2) {
+    const uint32_t R = (uint32_t)kReshE / c;  // rows per batch
+    for (uint32_t f0 = first; f0 < end; f0 += R) {
+      IdxT e[kReshE];
+      uint32_t r = 0, pc = 0;
+#pragma unroll
+      for (int k = 0; k < kReshE; ++k) {
+        const uint32_t f = f0 + r, p = pc * kWave + lane;
+        e[k] = 0;
+        if (r < R && f < end && p < n) e[k] = rows_of(a, nd.src, nd.tree, f)[nd.start + p];
+        if (++pc == c) {
+          pc = 0;
+          ++r;
+        }
+      }
+      uint32_t carry = 0;
+      r = 0;
+      pc = 0;
+#pragma unroll
+      for (int k = 0; k < kReshE; ++k) {
+        const uint32_t f = f0 + r, p = pc * kWave + lane;
+        const bool valid = r < R && f < end && p < n;
+        const uint32_t sd = valid ? side(e[k]) : 0u;
+        const uint64_t m = ballot(sd != 0);
+        if (valid) {
+          IdxT *out = rows_out(a, ax.dst, nd.tree, f) + nd.start;
+          const uint32_t lb = carry + mbcnt(m);
+          if (sd) {
+            if (liveL) out[lb] = e[k];
+          } else if (liveR) {
+            out[nL + (p - lb)] = e[k];
+          }
+        }
+        carry += (uint32_t)__popcll(m);
+        if (++pc == c) {
+          pc = 0;
+          ++r;
+          carry = 0;
+        }
+      }
+    }
+    return;
+  }
+  for (uint32_t f = first; f < end; ++f) {
     const IdxT *in = rows_of(a, nd.src, nd.tree, f) + nd.start;
     IdxT *out = rows_out(a, ax.dst, nd.tree, f) + nd.start;
     uint32_t carry = 0;
@@ -839,11 +1229,11 @@ __global__ __launch_bounds__(kReshThreads, 8) void k_resh(FitArgs<IdxT, LabT> a)
       if constexpr (kLds) return (lds[u >> 5] >> (u & 31u)) & 1u;
       else return (gbits[u >> 5] >> (u & 31u)) & 1u;
     };
-    const uint32_t nb = a.big_off[t + 1] - a.big_off[t];
-    for (uint32_t bt = j; bt < nb; bt += wgs) resh_big(a, a.big_tasks[a.big_off[t] + bt], side, wcnt, par);
-    const uint32_t ns = a.small_off[t + 1] - a.small_off[t];
+    const uint32_t *pl = a.plan + (size_t)t * 4;
+    const uint32_t b0 = pl[0], nb = pl[1] - pl[0], s0 = pl[2], ns = pl[3] - pl[2];
+    for (uint32_t bt = j; bt < nb; bt += wgs) resh_big(a, a.big_tasks[b0 + bt], side, wcnt, par);
     for (uint32_t st = j * kReshWaves + wave; st < ns; st += wgs * kReshWaves)
-      resh_small(a, a.small_tasks[a.small_off[t] + st], side);
+      resh_small(a, a.small_tasks[s0 + st], side);
   }
 }
 
@@ -929,11 +1319,13 @@ __global__ void k_root_counts(const LabT *y, const uint32_t *ids, const uint64_t
 }
 
 // Pseudo-frontier of the bootstrap compaction: tree t's presorted rows
-// (src 0, N samples) keep the in-bag samples (mark 1) in buffer 0.
+// (src 0, N samples) keep the in-bag samples (bit set) in buffer 0.  One
+// node per tree, so the frontier range of tree t is [t, t + 1).
 template <class IdxT, class LabT>
 __global__ __launch_bounds__(kBlock) void k_boot_tasks(FitArgs<IdxT, LabT> a, const uint64_t *off, uint32_t T,
-                                                       FNode *front, uint32_t dummy) {
+                                                       FNode *front) {
   const uint32_t lane = lane_id();
+  unsigned long long bytes = 0;
   WAVE_LOOP(T) {
     const uint32_t t = item;
     if (lane == 0) {
@@ -944,68 +1336,75 @@ __global__ __launch_bounds__(kBlock) void k_boot_tasks(FitArgs<IdxT, LabT> a, co
       nd.nid = 0;
       nd.depth = 0;
       nd.src = 0;
-      nd.pad0 = nd.pad1 = dummy;
+      nd.pad0 = nd.pad1 = 0;
       front[t] = nd;
-    }
-    NAux ax = {};
-    ax.best_f = -1;
-    ax.nL = (uint32_t)(off[t + 1] - off[t]);
-    ax.flags = kSplit | kLiveL;
-    ax.dst = 1;
-    count_resh_tasks(a, t, (uint32_t)a.N, ax);
-    if (lane == 0) {
+      NAux ax = {};
+      ax.best_f = -1;
+      ax.nL = (uint32_t)(off[t + 1] - off[t]);
+      ax.flags = kSplit | kLiveL;
+      ax.dst = 1;
+      uint32_t v[kChFields] = {0u, 0u, 0u, 0u, 0u};
+      resh_shape((uint32_t)a.N, (uint32_t)a.F, a.Cr, ax, v);
+      uint32_t *o = a.vals + (size_t)t * kChFields;
+#pragma unroll
+      for (int f = 0; f < kChFields; ++f) o[f] = v[f];
       a.aux[t] = ax;
-      atomicAdd(&a.ctl->bytes_boot, (unsigned long long)a.F * a.N * (sizeof(IdxT) + 1) +
-                                        (unsigned long long)a.F * ax.nL * sizeof(IdxT));
+      a.tlo[t] = t;
+      a.thi[t] = t + 1;
+      bytes += (unsigned long long)a.F * a.N * sizeof(IdxT) + (unsigned long long)a.N / 8 +
+               (unsigned long long)a.F * ax.nL * sizeof(IdxT);
     }
   }
+  flush_bytes(a, kBytesBoot, bytes);
 }
 
-// Roots: tree t's root (arena id t) is a leaf if pure, else enters the frontier.
+// Roots: tree t's root (arena id t) is a leaf if pure, else enters the
+// frontier; one workgroup, so the frontier is in tree order.
 template <class IdxT, class LabT>
-__global__ __launch_bounds__(kBlock) void k_roots(FitArgs<IdxT, LabT> a, const uint32_t *root_cnt,
-                                                  const uint64_t *off, uint32_t T, uint32_t src) {
-  const uint32_t lane = lane_id();
+__global__ __launch_bounds__(kPlanThreads) void k_roots(FitArgs<IdxT, LabT> a, const uint32_t *root_cnt,
+                                                        const uint64_t *off, uint32_t T, uint32_t src) {
+  constexpr int kW = kPlanThreads / kWave;
+  __shared__ uint32_t ws[kW];
+  __shared__ uint32_t carry;
+  const uint32_t tid = threadIdx.x, lane = tid % kWave, wave = tid / kWave;
   const uint32_t K = (uint32_t)a.K;
-  WAVE_LOOP(T) {
-    const uint32_t t = item;
-    const uint32_t n = off ? (uint32_t)(off[t + 1] - off[t]) : (uint32_t)a.N;
-    uint32_t nz = 0, only = 0;
-    for (uint32_t k = lane; k < K; k += kWave)
-      if (root_cnt[(size_t)t * K + k]) {
-        ++nz;
-        only = k;
-      }
-    nz = uniform(wave_sum32(nz));
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) only = max(only, (uint32_t)__shfl_xor(only, o, kWave));
-    if (lane == 0) {
+  if (tid == 0) carry = 0;
+  __syncthreads();
+  for (uint32_t t0 = 0; t0 < T; t0 += kPlanThreads) {
+    const uint32_t t = t0 + tid;
+    uint32_t live = 0, n = 0;
+    if (t < T) {
+      n = off ? (uint32_t)(off[t + 1] - off[t]) : (uint32_t)a.N;
+      uint32_t nz = 0, only = 0;
+      for (uint32_t k = 0; k < K; ++k)
+        if (root_cnt[(size_t)t * K + k]) {
+          ++nz;
+          only = k;
+        }
       a.arena.nsamp[t] = n;
       a.arena.depth[t] = 0;
-    }
-    // The reference never tests the root for purity (random_tree.py:482-486);
-    // a pure root is a leaf here.  min_samples_split applies as at any node;
-    // a positive max_depth never caps the root.
-    uint32_t leaf_label = kLeaf;
-    if (nz <= 1) leaf_label = only;
-    else if ((int64_t)n < (int64_t)a.mss) {
-      const IdxT *row0 = rows_of(a, src, t, 0);
-      leaf_label = a.y[row0[0]];
-    }
-    if (leaf_label != kLeaf) {
-      if (lane == 0) {
+      // The reference never tests the root for purity (random_tree.py:482-486);
+      // a pure root is a leaf here.  min_samples_split applies as at any node;
+      // a positive max_depth never caps the root.
+      uint32_t leaf_label = kLeaf;
+      if (nz <= 1) leaf_label = only;
+      else if ((int64_t)n < (int64_t)a.mss) leaf_label = a.y[rows_of(a, src, t, 0)[0]];
+      if (leaf_label != kLeaf) {
         a.arena.feat[t] = -1;
         a.arena.thr[t] = 0.f;
         a.arena.left[t] = kLeaf;
         a.arena.right[t] = kLeaf;
         a.arena.value[t] = leaf_label;
+      } else {
+        live = 1;
       }
-      continue;
     }
-    uint32_t slot = 0;
-    if (lane == 0) slot = atomicAdd(&a.ctl->q_next, 1u);
-    slot = uniform(__shfl(slot, 0));
-    if (lane == 0) {
+    const uint32_t inc = wave_incl_scan(live);
+    if (lane == kWave - 1) ws[wave] = inc;
+    __syncthreads();
+    uint32_t slot = carry + inc - live;
+    for (uint32_t w = 0; w < wave; ++w) slot += ws[w];
+    if (live) {
       FNode nd;
       nd.tree = t;
       nd.start = 0;
@@ -1015,8 +1414,19 @@ __global__ __launch_bounds__(kBlock) void k_roots(FitArgs<IdxT, LabT> a, const u
       nd.src = src;
       nd.pad0 = nd.pad1 = 0;
       a.next_front[slot] = nd;
+      for (uint32_t k = 0; k < K; ++k) a.next_cnt[(size_t)slot * K + k] = root_cnt[(size_t)t * K + k];
     }
-    for (uint32_t k = lane; k < K; k += kWave) a.next_cnt[(size_t)slot * K + k] = root_cnt[(size_t)t * K + k];
+    __syncthreads();
+    if (tid == 0) {
+      uint32_t sum = 0;
+      for (int w = 0; w < kW; ++w) sum += ws[w];
+      carry += sum;
+    }
+    __syncthreads();
+  }
+  if (tid == 0) {
+    a.ctl->c_live = carry;
+    a.ctl->c_ids = 0;
   }
 }
 
@@ -1113,21 +1523,30 @@ __global__ void k_canon_scatter(uint32_t n, Arena ar, const uint32_t *size, cons
 }
 
 // Moves the next frontier's count into place and clears the level counters.
-__global__ void k_ctl_next(Ctl *ctl) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) {
-    ctl->q = ctl->q_next;
-    ctl->q_next = 0;
-    ctl->n_scan = ctl->n_fill = ctl->n_hist = ctl->n_plan = 0;
+// End of a level: the children-scan totals become the next frontier size
+// and the ids in use; the spread byte counters are folded into Ctl.  One
+// block of 4 waves, one per counter kind.
+__global__ __launch_bounds__(256) void k_ctl_next(Ctl *ctl, unsigned long long *bslot) {
+  const uint32_t tid = threadIdx.x, kind = tid / kWave, slot = tid % kWave;
+  unsigned long long v = 0;
+  for (uint32_t s = slot; s < kByteSlots; s += kWave) {
+    unsigned long long *p = bslot + ((size_t)kind * kByteSlots + s) * kByteStride;
+    v += *p;
+    *p = 0;
+  }
+  v = wave_sum64(v);
+  if (slot == 0 && v) (&ctl->bytes_scan)[kind] += v;
+  if (tid == 0) {
+    ctl->q = ctl->c_live;
+    ctl->n_nodes += ctl->c_ids;
+    ctl->c_live = 0;
+    ctl->c_ids = 0;
   }
 }
 
 // Sets the frontier size for a host-provided frontier.
 __global__ void k_ctl_set(Ctl *ctl, uint32_t q) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) {
-    ctl->q = q;
-    ctl->q_next = 0;
-    ctl->n_scan = ctl->n_fill = ctl->n_hist = ctl->n_plan = 0;
-  }
+  if (threadIdx.x == 0 && blockIdx.x == 0) ctl->q = q;
 }
 
 }  // namespace cta

@@ -27,13 +27,13 @@ struct NAux {
   uint32_t hist_base;                          // class histograms of multi-chunk scans
   int32_t best_f;                              // -1: no candidate (leaf)
   uint32_t best_p, nL;
-  uint32_t left_id, right_id;
   uint32_t flags;                              // kSplit | kLiveL | kLiveR
   uint32_t dst;                                // rows receiving the children
-  uint32_t fill_base, nch_f;
+  uint32_t fill_base, nch_f;                   // bitmap-fill task slots: first, count (ceil(n / Cf))
   uint32_t resh_big;                           // 1: rows reshuffled by whole workgroups
-  uint32_t resh_off;                           // first task in the tree's big or small list
-  uint32_t resh_ntask, rpt;                    // small tasks: count and rows per task
+  uint32_t resh_ntask, rpt;                    // reshuffle tasks of the node, rows per small task
+  uint32_t lab[2];                             // labels of leaf children (kLeaf: live child)
+  uint32_t nf;                                 // evaluated features
   uint32_t pad[2];
 };
 constexpr uint32_t kSplit = 1u, kLiveL = 2u, kLiveR = 4u;
@@ -51,15 +51,24 @@ struct ScanRes {
   uint32_t ok, pad;
 };
 
-// Counters of one level, in device memory.
+// Counters of one level, in device memory.  Nothing is allocated with
+// atomics: the totals are written by the offset scans over the frontier.
 struct Ctl {
-  uint32_t q, q_next;
-  uint32_t n_scan, n_fill, n_hist, n_plan;
-  uint32_t n_nodes;
+  uint32_t q;        // frontier size
+  uint32_t n_nodes;  // arena ids in use
   uint32_t err;
-  uint32_t pad[8];
+  uint32_t n_plan;   // reshuffle plan entries
+  uint32_t n_scan, n_hist, n_fill, pad0;                 // selection-scan totals
+  uint32_t c_ids, c_live, c_big, c_small, c_tiles, pad1[3];  // children-scan totals
   unsigned long long bytes_scan, bytes_fill, bytes_resh, bytes_boot;
 };
+// Fields of the two per-node offset scans.
+constexpr int kSelFields = 3;  // scan tasks, histogram words, fill slots
+constexpr int kChFields = 5;   // child ids, next-frontier slots, big / small reshuffle tasks, big tiles
+// Algorithmic-byte counters: per-wave sums flushed into kByteSlots spread
+// slots (one 128-byte line each) and folded into Ctl at the end of a level.
+constexpr uint32_t kByteSlots = 64, kByteStride = 16;
+enum ByteKind { kBytesScan = 0, kBytesFill = 1, kBytesResh = 2, kBytesBoot = 3 };
 constexpr uint32_t kErrCapacity = 1u, kErrBadNode = 2u;
 
 // Flattened forest node for prediction: pre-order, so a split node's left
@@ -103,12 +112,15 @@ struct FitArgs {
   ScanRes *res;
   uint32_t *hist;  // class histograms / carries of multi-chunk scans
   Task *fill_tasks;
-  // reshuffle plan: per-tree task lists and workgroup ranges
+  // per-node offset scans: counts and exclusive offsets (kSelFields or
+  // kChFields per node)
+  uint32_t *vals, *offs;
+  uint32_t *tlo, *thi;  // [T] frontier range of each tree (frontiers are sorted by tree)
+  // reshuffle plan: per-tree task ranges and workgroup ranges
   Task *big_tasks, *small_tasks;
-  uint32_t *cnt_big, *cnt_small;  // [T] per-level counters
-  unsigned long long *elems;      // [T] elements to move per tree
-  uint32_t *big_off, *small_off;  // [T + 1]
-  uint32_t *wg_off;               // [T + 1] workgroups per tree (prefix)
+  uint32_t *plan;    // [T x 4] big begin/end, small begin/end
+  uint32_t *wg_off;  // [T + 1] workgroups per tree (prefix)
+  unsigned long long *bslot;  // [4 x kByteSlots x kByteStride] byte counters
   Arena arena;
   Ctl *ctl;
   uint32_t cap_q, cap_scan, cap_hist, cap_fill, cap_big, cap_small, cap_nodes;
