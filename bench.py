@@ -125,6 +125,88 @@ def serving_bench(lib, X, y, Xt, args):
     return result
 
 
+TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "pmc_traffic_fit_covtype_581k_x54_50trees.md")
+
+
+def short_kernel(name):
+    """rocprofv3 kernel name -> the name bench.py times it under."""
+    base = name.split("(")[0].replace("void ", "").strip()
+    if base.startswith("cta::"):
+        base = base[5:]
+    ident = base.split("<")[0]
+    if ident == "k_resh":
+        return "k_resh_big" if base.endswith("true>") else "k_resh_small"
+    return ident if base.startswith("k_") else base
+
+
+def pmc_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed counter summary
+    (FETCH_SIZE + WRITE_SIZE, calibrated by a plain 1 GiB copy)."""
+    import re
+
+    if not os.path.exists(TRAFFIC_SUMMARY):
+        return None, None
+    for line in open(TRAFFIC_SUMMARY):
+        m = re.match(r"- (\w+): traffic_bytes_per_launch=(\d+)", line.strip())
+        if m and m.group(1) == kernel:
+            return int(m.group(2)), os.path.relpath(TRAFFIC_SUMMARY, ROOT)
+    return None, None
+
+
+def pmc_summary(fetch_csv, write_csv, l2_csv, out):
+    """Per-kernel FETCH_SIZE / WRITE_SIZE (KiB per dispatch) of two rocprofv3
+    --pmc runs of `bench.py --calibrate-copy`, scaled by what the counters
+    read for the 1 GiB device-to-device copy in the same runs."""
+    import csv
+    from collections import defaultdict
+
+    def load(path, counter):
+        tot, cnt, copies = defaultdict(float), defaultdict(int), []
+        for r in csv.DictReader(open(path)):
+            if r["Counter_Name"] != counter:
+                continue
+            v = float(r["Counter_Value"])
+            k = short_kernel(r["Kernel_Name"])
+            tot[k] += v
+            cnt[k] += 1
+            if "copyBuffer" in r["Kernel_Name"]:
+                copies.append(v)
+        return tot, cnt, copies
+
+    ft, fc, fcopy = load(fetch_csv, "FETCH_SIZE")
+    wt, wc, wcopy = load(write_csv, "WRITE_SIZE")
+    gib_kib = float(1 << 20)
+    wcal = max(wcopy)  # the 1 GiB copy is the largest copy dispatch
+    fcal = max(fcopy)
+    ffac, wfac = gib_kib / fcal, gib_kib / wcal
+    l2 = {}
+    if l2_csv:
+        hit, miss = defaultdict(float), defaultdict(float)
+        for r in csv.DictReader(open(l2_csv)):
+            k = short_kernel(r["Kernel_Name"])
+            if r["Counter_Name"].startswith("TCC_HIT"):
+                hit[k] += float(r["Counter_Value"])
+            elif r["Counter_Name"].startswith("TCC_MISS"):
+                miss[k] += float(r["Counter_Value"])
+        l2 = {k: hit[k] / (hit[k] + miss[k]) for k in hit if hit[k] + miss[k] > 0}
+    lines = ["| kernel | dispatches | FETCH_SIZE KiB/disp | WRITE_SIZE KiB/disp | HBM bytes/launch (calibrated) | L2 hit |",
+             "|---|---:|---:|---:|---:|---:|"]
+    rows = []
+    for k in sorted(ft, key=lambda k: -(ft[k] * ffac + wt.get(k, 0) * wfac)):
+        if k.startswith("__amd") or fc[k] == 0:
+            continue
+        f = ft[k] / fc[k]
+        w = wt.get(k, 0.0) / max(1, wc.get(k, 1))
+        t = (f * ffac + w * wfac) * 1024
+        rows.append((k, fc[k], f, w, t))
+        lines.append("| %s | %d | %.1f | %.1f | %.4g | %s |" % (k, fc[k], f, w, t, "%.3f" % l2[k] if k in l2 else ""))
+    out.write("Calibration: the 1 GiB copy read %.0f KiB by FETCH_SIZE and wrote %.0f KiB by WRITE_SIZE, so\n"
+              "fetch factor %.3f and write factor %.3f convert the counters to bytes.\n\n" % (fcal, wcal, ffac, wfac))
+    out.write("\n".join(lines) + "\n\nMachine-readable (bench.py reads these):\n\n")
+    for k, n, f, w, t in rows:
+        out.write("- %s: traffic_bytes_per_launch=%d fetch_kib=%.1f write_kib=%.1f dispatches=%d\n" % (k, t, f, w, n))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -137,9 +219,15 @@ def main():
     ap.add_argument("--serving-seconds", type=float, default=6.0, help="length of each serving stream (0: skip)")
     ap.add_argument("--serving-trees", type=int, default=500, help="trees of the resident serving forest")
     ap.add_argument("--serving-qps", type=float, default=1000.0, help="predict requests per second")
+    ap.add_argument("--pmc-summary", nargs="+", metavar="CSV",
+                    help="FETCH_SIZE csv, WRITE_SIZE csv [, TCC_HIT/TCC_MISS csv]: print the traffic summary and exit")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run one plain 1 GiB device-to-device copy (calibrates FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()
+    if args.pmc_summary:
+        c = args.pmc_summary
+        pmc_summary(c[0], c[1], c[2] if len(c) > 2 else None, sys.stdout)
+        return
     if args.gpus != 1:
         print("bench.py measures one MI355X: this build has no multi-GPU path (trees would shard across GPUs "
               "with one gather of the node tables); run with --gpus 1", file=sys.stderr)
@@ -187,20 +275,26 @@ def main():
         gbs = v["alg_bytes"] / (v["ms"] * 1e-3) / 1e9 if v["ms"] > 0 and v["alg_bytes"] else 0.0
         log("  %-16s %8d launches %10.3f ms  %8.1f GB/s alg" % (k, v["launches"], v["ms"] / args.steps, gbs))
 
-    # roofline of the dominant HBM kernel (reshuffle), HIP events over the timed steps
+    # roofline of the dominant HBM kernel, HIP events over the timed steps
     roof = None
-    if "k_resh" in stats and stats["k_resh"]["ms"] > 0:
-        s = stats["k_resh"]
-        per_launch_bytes = s["alg_bytes"] / s["launches"]
-        per_launch_s = s["ms"] * 1e-3 / s["launches"]
-        ach = per_launch_bytes / per_launch_s / 1e9
-        roof = {"bound": "hbm", "kernel": "k_resh", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
-                "launches": s["launches"], "avg_launch_us": round(per_launch_s * 1e6, 2),
-                "alg_bytes_per_launch": int(per_launch_bytes)}
-        if "k_scan" in stats and stats["k_scan"]["ms"] > 0:
-            sc = stats["k_scan"]
-            roof["scan_achieved"] = round(sc["alg_bytes"] / (sc["ms"] * 1e-3) / 1e9, 1)
+    hbm_kernels = [k for k in ("k_resh_big", "k_resh_small", "k_scan") if k in stats and stats[k]["ms"] > 0]
+    if hbm_kernels:
+        dom = max(hbm_kernels, key=lambda k: stats[k]["ms"])
+        per_kernel = {}
+        for k in hbm_kernels:
+            s = stats[k]
+            per_launch_bytes = s["alg_bytes"] / s["launches"]
+            per_launch_s = s["ms"] * 1e-3 / s["launches"]
+            per_kernel[k] = {"achieved": round(per_launch_bytes / per_launch_s / 1e9, 1),
+                             "ms_per_step": round(s["ms"] / args.steps, 3), "launches": s["launches"],
+                             "avg_launch_us": round(per_launch_s * 1e6, 2),
+                             "alg_bytes_per_launch": int(per_launch_bytes)}
+        d = per_kernel[dom]
+        traffic, tsrc = pmc_traffic(dom)
+        roof = {"bound": "hbm", "kernel": dom, "achieved": d["achieved"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(d["achieved"] / HBM_PEAK_GBS, 4), "traffic": traffic, "traffic_source": tsrc,
+                "launches": d["launches"], "avg_launch_us": d["avg_launch_us"],
+                "alg_bytes_per_launch": d["alg_bytes_per_launch"], "kernels": per_kernel}
 
     # ---- predict: 100,000 held-out rows resident in HBM
     R = Xt.shape[0]

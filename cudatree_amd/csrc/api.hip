@@ -269,6 +269,38 @@ struct HostEntry {
   std::vector<uint32_t> cnt;
 };
 
+// Both reshuffle launches (big tasks, then small tasks) of a planned level;
+// the side bitmaps are staged in LDS unless `global_bits` or too large.
+template <class IdxT, class LabT>
+void launch_resh(cta_ctx *ctx, const FitArgs<IdxT, LabT> &a, bool global_bits, uint32_t grid,
+                 const char *big_name = "k_resh_big", const char *small_name = "k_resh_small") {
+  const size_t lds = (((size_t)a.W + 3) & ~(size_t)3) * 4 + 2 * kSlots * 4;
+  hipStream_t st = ctx->stream;
+  if (!global_bits && lds <= ctx->lds_per_block) {
+    static bool attr = false;
+    if (!attr) {
+      HIPC(hipFuncSetAttribute((const void *)k_resh<IdxT, LabT, true, true>,
+                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)ctx->lds_per_block));
+      HIPC(hipFuncSetAttribute((const void *)k_resh<IdxT, LabT, true, false>,
+                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)ctx->lds_per_block));
+      attr = true;
+    }
+    {
+      Timed tm(ctx, big_name);
+      k_resh<IdxT, LabT, true, true><<<grid, kReshThreads, lds, st>>>(a);
+    }
+    Timed tm(ctx, small_name);
+    k_resh<IdxT, LabT, true, false><<<grid, kReshThreads, lds, st>>>(a);
+  } else {
+    {
+      Timed tm(ctx, big_name);
+      k_resh<IdxT, LabT, false, true><<<grid, kReshThreads, 2 * kSlots * 4, st>>>(a);
+    }
+    Timed tm(ctx, small_name);
+    k_resh<IdxT, LabT, false, false><<<grid, kReshThreads, 2 * kSlots * 4, st>>>(a);
+  }
+}
+
 template <class IdxT, class LabT>
 struct Fitter {
   cta_ctx *ctx;
@@ -340,7 +372,8 @@ struct Fitter {
     }
     const uint64_t cap_q = n_tot / 2 + Tb + 64;
     const uint64_t cap_nodes = 2 * n_tot + Tb + 64;
-    const uint64_t cap_scan = cap_q + 3ull * m * n_tot / Cs + 64;
+    // split-scan tasks are sized per level
+    const uint64_t cap_scan = units_for((uint64_t)Tb);
     const uint64_t cap_big = (uint64_t)F * (n_tot / kBigN + Tb + 1);
     const uint64_t cap_small = cap_q + 2ull * F * n_tot / Cr + 64;
     const uint64_t cap_fill = cap_q + n_tot / Cf + 64;
@@ -351,12 +384,12 @@ struct Fitter {
 
     IdxT *buf0 = ctx->ws_as<IdxT>("buf0", (size_t)Tb * F * N), *buf1 = ctx->ws_as<IdxT>("buf1", (size_t)Tb * F * N);
     uint32_t *plan = ctx->ws_as<uint32_t>("plan", (size_t)Tb * 4);
-    uint32_t *wg_off = ctx->ws_as<uint32_t>("wg_off", (size_t)Tb + 1);
+    uint32_t *wg_off = ctx->ws_as<uint32_t>("wg_off", 2 * ((size_t)Tb + 1));
     uint32_t *tlo = ctx->ws_as<uint32_t>("tlo", (size_t)Tb * 2);
     uint32_t *vals = ctx->ws_as<uint32_t>("scan_vals", cap_q * kChFields);
     uint32_t *offs = ctx->ws_as<uint32_t>("scan_offs", cap_q * kChFields);
     bsum = ctx->ws_as<uint32_t>("scan_bsum", (cap_q / kOffTile + 2) * kChFields);
-    const size_t nbslot = 4 * (size_t)kByteSlots * kByteStride;
+    const size_t nbslot = kByteKinds * (size_t)kByteSlots * kByteStride;
     unsigned long long *bslot = ctx->ws_as<unsigned long long>("byte_slots", nbslot);
     fill_lds = fill_lds_bytes() <= (size_t)ctx->lds_per_block;
     Jcap = fill_lds ? (uint32_t)std::max<int64_t>(1, std::min<int64_t>(16, (3 * (int64_t)ctx->num_cu) / Tb)) : 1u;
@@ -461,7 +494,7 @@ struct Fitter {
       k_boot_tasks<IdxT, LabT><<<grid_for((size_t)Tb * 64), 256, 0, st>>>(a, d_off, Tb, front[0]);
       offsets<kChFields>(a, (uint32_t)Tb, &a.ctl->c_ids);
       k_emit_resh<IdxT, LabT><<<G, kBlock, 0, st>>>(a);
-      reshuffle(a, "k_boot_compact");
+      reshuffle(a, true);
       src_root = 1;
     } else {
       k_root_counts<LabT><<<dim3(grid_for(N, 256, 256), Tb), 256, K * 4, st>>>(a.y, nullptr, nullptr, Tb, N, K,
@@ -470,7 +503,7 @@ struct Fitter {
     a.next_front = front[1];
     a.next_cnt = cnt[1];
     k_roots<IdxT, LabT><<<1, kPlanThreads, 0, st>>>(a, root_cnt, boot ? d_off : nullptr, Tb, src_root);
-    k_ctl_next<<<1, 256, 0, st>>>(a.ctl, a.bslot);
+    k_ctl_next<<<1, kByteKinds * kWave, 0, st>>>(a.ctl, a.bslot);
     int cur = 1;  // front[cur] holds the frontier
     read_ctl(a, hc);
     if (hc.err) fail(CTA_EINTERNAL, "device capacity check failed during setup (err %u)", hc.err);
@@ -487,8 +520,8 @@ struct Fitter {
       const char *lvl_path = ctx->profiling ? std::getenv("CTA_LEVEL_LOG") : nullptr;
       FILE *lvl = lvl_path ? std::fopen(lvl_path, "a") : nullptr;
       if (lvl)
-        std::fprintf(lvl, "level,q,scan_bytes,resh_bytes,fill_bytes,k_scan_ms,k_resh_ms,k_fill_ms,k_scan_hist_ms,"
-                          "k_select_ms,k_decide_ms,k_children_ms\n");
+        std::fprintf(lvl, "level,q,scan_bytes,resh_big_bytes,resh_small_bytes,fill_bytes,k_scan_ms,k_resh_big_ms,"
+                          "k_resh_small_ms,k_fill_ms,k_scan_hist_ms,k_select_ms,k_decide_ms,k_children_ms\n");
       int depth = 0;
       while (q > 0) {
         a.front = front[cur];
@@ -504,9 +537,10 @@ struct Fitter {
         account(prev, hc);
         if (lvl) {
           auto d = [&](const char *k) { return ctx->stats[k].ms - before[k].ms; };
-          std::fprintf(lvl, "%d,%u,%llu,%llu,%llu,%.4f,%.4f,%.4f,%.4f,%.4f,%.4f,%.4f\n", depth, q,
-                       hc.bytes_scan - prev.bytes_scan, hc.bytes_resh - prev.bytes_resh, hc.bytes_fill - prev.bytes_fill,
-                       d("k_scan"), d("k_resh"), d("k_fill"), d("k_scan_hist"), d("k_select"), d("k_decide"),
+          std::fprintf(lvl, "%d,%u,%llu,%llu,%llu,%llu,%.4f,%.4f,%.4f,%.4f,%.4f,%.4f,%.4f,%.4f\n", depth, q,
+                       hc.bytes_scan - prev.bytes_scan, hc.bytes_resh - prev.bytes_resh,
+                       hc.bytes_resh_small - prev.bytes_resh_small, hc.bytes_fill - prev.bytes_fill, d("k_scan"),
+                       d("k_resh_big"), d("k_resh_small"), d("k_fill"), d("k_scan_hist"), d("k_select"), d("k_decide"),
                        d("k_children"));
         }
         ++depth;
@@ -587,7 +621,8 @@ struct Fitter {
   void account(const Ctl &prev, const Ctl &now) {
     ctx->add_bytes("k_scan", now.bytes_scan - prev.bytes_scan);
     ctx->add_bytes("k_fill", now.bytes_fill - prev.bytes_fill);
-    ctx->add_bytes("k_resh", now.bytes_resh - prev.bytes_resh);
+    ctx->add_bytes("k_resh_big", now.bytes_resh - prev.bytes_resh);
+    ctx->add_bytes("k_resh_small", now.bytes_resh_small - prev.bytes_resh_small);
   }
 
   // Exclusive offsets and totals of the per-node counts in a.vals (NF
@@ -628,10 +663,28 @@ struct Fitter {
     }
   }
 
+  // Split-scan tasks of a level of q nodes: nf * nch per multi-chunk node
+  // (at most m n / Cs + m) and ceil(nf / fpt) <= m n / Cs + 1 per
+  // single-chunk node, so at most q + 3 m n_tot / Cs.
+  uint64_t units_for(uint64_t q) const { return q + 3 * (uint64_t)m * (n_tot / Cs + 1) + 64; }
+
+  // Grows the task and result lists for q nodes.
+  void ensure_units(FitArgs<IdxT, LabT> &a, uint32_t q) {
+    const uint64_t need = units_for(q);
+    if (need <= a.cap_scan) return;
+    const uint64_t cap = std::min<uint64_t>(0xFFFFFFF0ull, std::max(need, 2 * (uint64_t)a.cap_scan));
+    if (need > cap) fail(CTA_EUNSUPPORTED, "too many split-scan units for 32-bit indices");
+    ctx->sync();
+    a.scan_tasks = ctx->ws_as<Task>("scan_tasks", cap);
+    a.res = ctx->ws_as<ScanRes>("res", cap);
+    a.cap_scan = (uint32_t)cap;
+  }
+
   // One level over the frontier in a.front (q nodes, sorted by tree; count
   // also in ctl->q).
-  void level(const FitArgs<IdxT, LabT> &a, uint32_t q) {
+  void level(FitArgs<IdxT, LabT> &a, uint32_t q) {
     const size_t sm = smem();
+    ensure_units(a, q);
     {
       Timed tm(ctx, "k_select");
       k_select<IdxT, LabT><<<G, kBlock, 0, st>>>(a);
@@ -658,28 +711,18 @@ struct Fitter {
       offsets<kChFields>(a, q, &a.ctl->c_ids);
       k_children_emit<IdxT, LabT><<<G, kBlock, 0, st>>>(a);
     }
-    reshuffle(a, "k_resh");
-    k_ctl_next<<<1, 256, 0, st>>>(a.ctl, a.bslot);
+    reshuffle(a);
+    k_ctl_next<<<1, kByteKinds * kWave, 0, st>>>(a.ctl, a.bslot);
     HIPC(hipGetLastError());
   }
 
-  // Plan and partition of every row of the split nodes (task lists written).
-  void reshuffle(const FitArgs<IdxT, LabT> &a, const char *name) {
-    Timed tm(ctx, name);
+  // Plan and partition of every row of the split nodes (task lists written);
+  // `boot`: the bootstrap compaction, timed under its own name.
+  void reshuffle(const FitArgs<IdxT, LabT> &a, bool boot = false) {
     k_plan<IdxT, LabT><<<1, kPlanThreads, 0, st>>>(a, (uint32_t)std::max(8, ctx->num_cu * 4));
-    const size_t lds = (((size_t)a.W + 3) & ~(size_t)3) * 4 + 2 * kSlots * 4;
-    const uint32_t GR = (uint32_t)std::max(8, ctx->num_cu * 2);
-    if (!o.resh_global && lds <= ctx->lds_per_block) {
-      static bool attr = false;
-      if (!attr) {
-        HIPC(hipFuncSetAttribute((const void *)k_resh<IdxT, LabT, true>,
-                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)ctx->lds_per_block));
-        attr = true;
-      }
-      k_resh<IdxT, LabT, true><<<GR, kReshThreads, lds, st>>>(a);
-    } else {
-      k_resh<IdxT, LabT, false><<<GR, kReshThreads, 2 * kSlots * 4, st>>>(a);
-    }
+    const uint32_t grid = (uint32_t)std::max(8, ctx->num_cu * 2);
+    if (boot) launch_resh(ctx, a, o.resh_global != 0, grid, "k_boot_compact", "k_boot_compact");
+    else launch_resh(ctx, a, o.resh_global != 0, grid);
   }
 
   void pull_frontier(const FNode *fr, const uint32_t *c, uint32_t q, std::vector<HostEntry> &out) {
@@ -838,7 +881,11 @@ void presort(cta_ctx *ctx, cta_dataset *ds) {
     }
   }
   ds->si0.alloc(total * sizeof(IdxT));
-  k_sort_finish<IdxT><<<grid_for(total), 256, 0, st>>>(vin, total, ds->si0.as<IdxT>());
+  if (ds->lab_bits == 8)
+    k_sort_finish<IdxT, uint8_t><<<grid_for(total), 256, 0, st>>>(vin, total, ds->y.as<uint8_t>(), ds->si0.as<IdxT>());
+  else
+    k_sort_finish<IdxT, uint16_t><<<grid_for(total), 256, 0, st>>>(vin, total, ds->y.as<uint16_t>(),
+                                                                   ds->si0.as<IdxT>());
   HIPC(hipGetLastError());
   ctx->sync();
 }
@@ -1263,6 +1310,14 @@ int cta_ctx_kernel_stats(cta_ctx *ctx, cta_kernel_stat *out, int32_t max, int32_
 }  // extern "C"
 
 // ---- per-kernel parity hooks --------------------------------------------------------
+// Row entry of sample s as the kernels store it (label in the high bits of
+// 32-bit entries).
+template <class IdxT, class LabT>
+static IdxT row_entry(uint32_t s, const std::vector<LabT> &y) {
+  if constexpr (sizeof(IdxT) == 4) return (IdxT)(s | ((uint32_t)y[s] << kIdBits));
+  else return (IdxT)s;
+}
+
 template <class IdxT, class LabT>
 static void hook_split_scan(cta_ctx *ctx, cta_dataset *ds, const uint32_t *rows, uint32_t n, const int32_t *features,
                             int32_t m_eval, int32_t chunk, int32_t *bf, uint32_t *bp, uint64_t *num, uint64_t *den) {
@@ -1276,7 +1331,7 @@ static void hook_split_scan(cta_ctx *ctx, cta_dataset *ds, const uint32_t *rows,
   HIPC(hipMemcpyAsync(hy.data(), ds->y.p, (size_t)N * sizeof(LabT), hipMemcpyDeviceToHost, st));
   ctx->sync();
   for (int32_t f = 0; f < F; ++f)
-    for (uint32_t i = 0; i < n; ++i) hrows[(size_t)f * N + i] = (IdxT)rows[(size_t)f * n + i];
+    for (uint32_t i = 0; i < n; ++i) hrows[(size_t)f * N + i] = row_entry<IdxT>(rows[(size_t)f * n + i], hy);
   for (uint32_t i = 0; i < n; ++i) counts[hy[rows[i]]]++;
   const uint32_t nch = (n + Cs - 1) / Cs;
   const uint32_t fpt = nch > 1 ? 1u : std::max(1u, Cs / n);
@@ -1306,7 +1361,7 @@ static void hook_split_scan(cta_ctx *ctx, cta_dataset *ds, const uint32_t *rows,
   hc.q = 1;
   hc.n_scan = ntask;
   hc.n_nodes = 1;
-  const size_t nbslot = 4 * (size_t)kByteSlots * kByteStride;
+  const size_t nbslot = kByteKinds * (size_t)kByteSlots * kByteStride;
   DevMem d_rows(hrows.size() * sizeof(IdxT)), d_front(sizeof(FNode)), d_cnt(K * 4), d_aux(sizeof(NAux)),
       d_feats(std::max(nf, 1u) * 4), d_tasks(tasks.size() * sizeof(Task)),
       d_res(std::max(ntask, 1u) * sizeof(ScanRes)), d_hist((size_t)std::max(nf, 1u) * nch * K * 4), d_ctl(sizeof(Ctl)),
@@ -1402,9 +1457,12 @@ static void hook_reshuffle(cta_ctx *ctx, cta_dataset *ds, const uint32_t *rows, 
   hipStream_t st = ctx->stream;
   const uint32_t Cr = chunk > 0 ? (uint32_t)chunk : 2048u;
   const uint32_t W = ((uint32_t)N + 31u) / 32u;
+  std::vector<LabT> hy(N);
+  HIPC(hipMemcpyAsync(hy.data(), ds->y.p, (size_t)N * sizeof(LabT), hipMemcpyDeviceToHost, st));
+  ctx->sync();
   std::vector<IdxT> hrows((size_t)F * N, 0);
   for (int32_t f = 0; f < F; ++f)
-    for (uint32_t i = 0; i < n; ++i) hrows[(size_t)f * N + i] = (IdxT)rows[(size_t)f * n + i];
+    for (uint32_t i = 0; i < n; ++i) hrows[(size_t)f * N + i] = row_entry<IdxT>(rows[(size_t)f * n + i], hy);
   std::vector<uint32_t> hbits(W, 0);
   for (int32_t s = 0; s < N; ++s)
     if (mark[s]) hbits[s >> 5] |= 1u << (s & 31);
@@ -1439,7 +1497,7 @@ static void hook_reshuffle(cta_ctx *ctx, cta_dataset *ds, const uint32_t *rows, 
   DevMem d_in(hrows.size() * sizeof(IdxT)), d_out(hrows.size() * sizeof(IdxT)), d_bits((size_t)W * 4),
       d_front(sizeof(FNode)), d_aux(sizeof(NAux)), d_big((size_t)F * sizeof(Task)),
       d_small((size_t)std::max(v[3], 1u) * sizeof(Task)), d_offs(sizeof zeros), d_range(sizeof range), d_plan(4 * 4),
-      d_wg(2 * 4), d_ctl(sizeof(Ctl));
+      d_wg(4 * 4), d_ctl(sizeof(Ctl));
   HIPC(hipMemcpyAsync(d_in.p, hrows.data(), hrows.size() * sizeof(IdxT), hipMemcpyHostToDevice, st));
   HIPC(hipMemcpyAsync(d_out.p, hrows.data(), hrows.size() * sizeof(IdxT), hipMemcpyHostToDevice, st));
   HIPC(hipMemcpyAsync(d_bits.p, hbits.data(), (size_t)W * 4, hipMemcpyHostToDevice, st));
@@ -1475,14 +1533,7 @@ static void hook_reshuffle(cta_ctx *ctx, cta_dataset *ds, const uint32_t *rows, 
   a.cap_small = std::max(v[3], 1u);
   k_emit_resh<IdxT, LabT><<<8, kBlock, 0, st>>>(a);
   k_plan<IdxT, LabT><<<1, kPlanThreads, 0, st>>>(a, 16u);
-  const size_t lds = (((size_t)W + 3) & ~(size_t)3) * 4 + 2 * kSlots * 4;
-  if (!global_bits && lds <= ctx->lds_per_block) {
-    HIPC(hipFuncSetAttribute((const void *)k_resh<IdxT, LabT, true>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                             (int)ctx->lds_per_block));
-    k_resh<IdxT, LabT, true><<<16, kReshThreads, lds, st>>>(a);
-  } else {
-    k_resh<IdxT, LabT, false><<<16, kReshThreads, 2 * kSlots * 4, st>>>(a);
-  }
+  launch_resh(ctx, a, global_bits != 0, 16u);
   HIPC(hipGetLastError());
   std::vector<IdxT> hout(hrows.size());
   Ctl back = {};
@@ -1491,7 +1542,7 @@ static void hook_reshuffle(cta_ctx *ctx, cta_dataset *ds, const uint32_t *rows, 
   ctx->sync();
   if (back.err) fail(CTA_EINTERNAL, "reshuffle hook: device capacity check failed");
   for (int32_t f = 0; f < F; ++f)
-    for (uint32_t i = 0; i < n; ++i) out[(size_t)f * n + i] = hout[(size_t)f * N + i];
+    for (uint32_t i = 0; i < n; ++i) out[(size_t)f * n + i] = sample_of(hout[(size_t)f * N + i]);
 }
 
 extern "C" int cta_k_reshuffle(cta_ctx *ctx, cta_dataset *ds, const uint32_t *rows, uint32_t n, const uint8_t *mark,

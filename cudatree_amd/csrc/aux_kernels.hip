@@ -157,16 +157,22 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_scatter(const uint32_t *k
   }
 }
 
-template <class IdxT>
-__global__ void k_sort_finish(const uint32_t *vals, size_t total, IdxT *out) {
-  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (size_t)gridDim.x * blockDim.x)
-    out[i] = (IdxT)vals[i];
+// Sorted ids to row entries: 16-bit ids, or 32-bit words with the label in
+// the high bits.
+template <class IdxT, class LabT>
+__global__ void k_sort_finish(const uint32_t *vals, size_t total, const LabT *y, IdxT *out) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (size_t)gridDim.x * blockDim.x) {
+    const uint32_t s = vals[i];
+    if constexpr (sizeof(IdxT) == 4) out[i] = (IdxT)(s | ((uint32_t)y[s] << kIdBits));
+    else out[i] = (IdxT)s;
+  }
 }
 
+// Row entries to plain 32-bit sample ids.
 template <class IdxT>
 __global__ void k_widen(const IdxT *in, size_t total, uint32_t *out) {
   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (size_t)gridDim.x * blockDim.x)
-    out[i] = (uint32_t)in[i];
+    out[i] = sample_of(in[i]);
 }
 
 template <class LabT>

@@ -30,27 +30,64 @@ __device__ __forceinline__ uint64_t uniform64(uint64_t v) {
   return ((uint64_t)hi << 32) | lo;
 }
 
-// Inclusive prefix sum over the 64 lanes.
+// ---- cross-lane moves by DPP (data-parallel primitives of the GFX9 VALU) --
+// Each move is one VALU modifier instead of an LDS-crossbar permute.  A lane
+// whose source lane is out of range, or whose row is masked off, reads 0.
+// Controls: row_shr:n (0x110 + n, within rows of 16 lanes), row_bcast:15
+// (0x142, lane 15 of each row to the next row), row_bcast:31 (0x143, lane 31
+// to rows 2 and 3), wave_shl:1 (0x130, lane i reads lane i + 1).
+constexpr int kDppRowShr = 0x110, kDppBcast15 = 0x142, kDppBcast31 = 0x143, kDppWaveShl1 = 0x130;
+
+template <int kCtrl, int kRowMask = 0xF>
+__device__ __forceinline__ uint32_t dpp(uint32_t v) {
+  return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, kCtrl, kRowMask, 0xF, true);
+}
+
+// Value of lane i + 1 (0 in lane 63).
+__device__ __forceinline__ uint32_t wave_next(uint32_t v) { return dpp<kDppWaveShl1>(v); }
+
+// Inclusive prefix sum over the 64 lanes: shifts by 1, 2, 4, 8 within each
+// row of 16, then the row totals carried across rows.
 __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
-  const uint32_t l = lane_id();
-#pragma unroll
-  for (int o = 1; o < kWave; o <<= 1) {
-    uint32_t t = __shfl_up(v, o, kWave);
-    if (l >= (uint32_t)o) v += t;
-  }
+  v += dpp<kDppRowShr + 1>(v);
+  v += dpp<kDppRowShr + 2>(v);
+  v += dpp<kDppRowShr + 4>(v);
+  v += dpp<kDppRowShr + 8>(v);
+  v += dpp<kDppBcast15, 0xA>(v);
+  v += dpp<kDppBcast31, 0xC>(v);
   return v;
 }
 
-__device__ __forceinline__ uint64_t wave_sum64(uint64_t v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
+// Reduction to lane 63 by the same steps; `comb` is associative and
+// commutative, and lanes reading an out-of-range source see `zero`.
+template <class T, class Move, class Comb>
+__device__ __forceinline__ T wave_reduce_to_63(T v, Move mv, Comb comb) {
+  v = comb(v, mv.template operator()<kDppRowShr + 1, 0xF>(v));
+  v = comb(v, mv.template operator()<kDppRowShr + 2, 0xF>(v));
+  v = comb(v, mv.template operator()<kDppRowShr + 4, 0xF>(v));
+  v = comb(v, mv.template operator()<kDppRowShr + 8, 0xF>(v));
+  v = comb(v, mv.template operator()<kDppBcast15, 0xA>(v));
+  v = comb(v, mv.template operator()<kDppBcast31, 0xC>(v));
   return v;
 }
+
+struct Move64 {
+  template <int kCtrl, int kRowMask>
+  __device__ __forceinline__ uint64_t operator()(uint64_t v) const {
+    return ((uint64_t)dpp<kCtrl, kRowMask>((uint32_t)(v >> 32)) << 32) | dpp<kCtrl, kRowMask>((uint32_t)v);
+  }
+};
 
 __device__ __forceinline__ uint32_t wave_sum32(uint32_t v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
-  return v;
+  return (uint32_t)__builtin_amdgcn_readlane((int)wave_incl_scan(v), kWave - 1);
+}
+
+// Sum over the 64 lanes, returned in every lane.
+__device__ __forceinline__ uint64_t wave_sum64(uint64_t v) {
+  v = wave_reduce_to_63(v, Move64{}, [](uint64_t x, uint64_t y) { return x + y; });
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, kWave - 1);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(v >> 32), kWave - 1);
+  return ((uint64_t)hi << 32) | lo;
 }
 
 // ---- exact score comparison --------------------------------------------
@@ -85,24 +122,33 @@ __device__ __forceinline__ bool cand_better(const Cand &a, const Cand &b) {
   return a.pos < b.pos;
 }
 
-__device__ __forceinline__ Cand cand_shfl_xor(const Cand &c, int o) {
-  Cand r;
-  r.num = __shfl_xor(c.num, o, kWave);
-  r.den = __shfl_xor(c.den, o, kWave);
-  r.pos = __shfl_xor(c.pos, o, kWave);
-  r.feat = __shfl_xor(c.feat, o, kWave);
-  r.ok = __shfl_xor(c.ok, o, kWave);
-  return r;
-}
-
-// Every lane ends with the best candidate of the wave.
-__device__ __forceinline__ Cand wave_best(Cand c) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) {
-    Cand o2 = cand_shfl_xor(c, o);
-    if (cand_better(o2, c)) c = o2;
+struct MoveCand {
+  template <int kCtrl, int kRowMask>
+  __device__ __forceinline__ Cand operator()(const Cand &c) const {
+    const Move64 m;
+    Cand r;
+    r.num = m.template operator()<kCtrl, kRowMask>(c.num);
+    r.den = m.template operator()<kCtrl, kRowMask>(c.den);
+    r.pos = dpp<kCtrl, kRowMask>(c.pos);
+    r.feat = (int32_t)dpp<kCtrl, kRowMask>((uint32_t)c.feat);
+    r.ok = dpp<kCtrl, kRowMask>(c.ok);  // out-of-range sources read ok = 0: never better
+    return r;
   }
-  return c;
+};
+
+// Every lane ends with the best candidate of the wave (the order is total,
+// so the reduction order does not matter).
+__device__ __forceinline__ Cand wave_best(Cand c) {
+  c = wave_reduce_to_63(c, MoveCand{}, [](const Cand &x, const Cand &y) { return cand_better(y, x) ? y : x; });
+  Cand r;
+  r.num = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(c.num >> 32), kWave - 1) << 32) |
+          (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)c.num, kWave - 1);
+  r.den = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(c.den >> 32), kWave - 1) << 32) |
+          (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)c.den, kWave - 1);
+  r.pos = (uint32_t)__builtin_amdgcn_readlane((int)c.pos, kWave - 1);
+  r.feat = __builtin_amdgcn_readlane(c.feat, kWave - 1);
+  r.ok = (uint32_t)__builtin_amdgcn_readlane((int)c.ok, kWave - 1);
+  return r;
 }
 
 // ---- per-node feature permutation ---------------------------------------

@@ -211,6 +211,8 @@ __global__ __launch_bounds__(kBlock) void k_select(FitArgs<IdxT, LabT> a) {
   WAVE_LOOP(q) {
     const uint32_t i = item;
     const FNode nd = a.front[i];
+    const uint32_t n = nd.n;
+    const uint32_t nch = ceil_div(n, a.Cs);
     Perm pm = make_perm(a.seed, a.tree_base + nd.tree, nd.depth, nd.start, F, a.identity != 0);
     uint32_t nf = 0;
     for (uint32_t k0 = 0; k0 < F && nf < m; k0 += kWave) {
@@ -221,15 +223,15 @@ __global__ __launch_bounds__(kBlock) void k_select(FitArgs<IdxT, LabT> a) {
         f = perm_at(pm, k);
         const IdxT *row = rows_of(a, nd.src, nd.tree, f) + nd.start;
         const float *x = a.X_T + (size_t)f * a.N;
-        nonconst = x[row[0]] != x[row[nd.n - 1]];
+        nonconst = x[sample_of(row[0])] != x[sample_of(row[nd.n - 1])];
       }
       const uint64_t mk = ballot(nonconst);
       const uint32_t r = mbcnt(mk);
       if (nonconst && nf + r < m) a.feats[(size_t)i * m + nf + r] = (int32_t)f;
       nf += min((uint32_t)__popcll(mk), m - nf);
     }
-    const uint32_t n = nd.n;
-    const uint32_t nch = ceil_div(n, a.Cs);
+    // tasks: one per (feature, chunk) of a multi-chunk node; a single-chunk
+    // node's features grouped about Cs positions per task
     const uint32_t fpt = (nch > 1) ? 1u : max(1u, a.Cs / n);
     const uint32_t ntask = (nf == 0) ? 0u : (nch > 1 ? nf * nch : ceil_div(nf, fpt));
     if (lane == 0) {
@@ -319,8 +321,14 @@ __global__ __launch_bounds__(kBlock) void k_scan_hist(FitArgs<IdxT, LabT> a) {
     const uint32_t j = tk.first_count & 0xFFFFu;
     const uint32_t f = (uint32_t)a.feats[(size_t)tk.node * a.m + j];
     const IdxT *row = rows_of(a, nd.src, nd.tree, f) + nd.start;
-    for (uint32_t k = lane; k < K; k += kWave) h[k] = 0;
-    wave_sync();
+    // K <= 64: lane k counts class k in a register, one ballot per distinct
+    // label of a tile; else LDS atomics
+    const bool regs = K <= (uint32_t)kWave;
+    uint32_t hv = 0;
+    if (!regs) {
+      for (uint32_t k = lane; k < K; k += kWave) h[k] = 0;
+      wave_sync();
+    }
     const uint32_t p0 = tk.chunk * a.Cs, p1 = min(nd.n, p0 + a.Cs);
     for (uint32_t pb = p0; pb < p1; pb += 4 * kWave) {
       uint32_t sid[4];
@@ -331,15 +339,32 @@ __global__ __launch_bounds__(kBlock) void k_scan_hist(FitArgs<IdxT, LabT> a) {
       }
       uint32_t l[4];
 #pragma unroll
-      for (int u = 0; u < 4; ++u) l[u] = (sid[u] != 0xFFFFFFFFu) ? (uint32_t)a.y[sid[u]] : 0xFFFFFFFFu;
+      for (int u = 0; u < 4; ++u) l[u] = (sid[u] != 0xFFFFFFFFu) ? a.label((IdxT)sid[u]) : 0xFFFFFFFFu;
+      if (regs) {
 #pragma unroll
-      for (int u = 0; u < 4; ++u)
-        if (l[u] != 0xFFFFFFFFu) atomicAdd(&h[l[u]], 1u);
+        for (int u = 0; u < 4; ++u) {
+          uint64_t rem = ballot(l[u] != 0xFFFFFFFFu);
+          while (rem) {
+            const uint32_t L = readlane(l[u], (uint32_t)__ffsll((unsigned long long)rem) - 1u);
+            const uint64_t mm = ballot(l[u] == L);
+            if (lane == L) hv += (uint32_t)__popcll(mm);
+            rem &= ~mm;
+          }
+        }
+      } else {
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+          if (l[u] != 0xFFFFFFFFu) atomicAdd(&h[l[u]], 1u);
+      }
     }
-    wave_sync();
     uint32_t *dst = a.hist + ax.hist_base + ((size_t)j * tk.nch + tk.chunk) * K;
-    for (uint32_t k = lane; k < K; k += kWave) dst[k] = h[k];
-    wave_sync();
+    if (regs) {
+      if (lane < K) dst[lane] = hv;
+    } else {
+      wave_sync();
+      for (uint32_t k = lane; k < K; k += kWave) dst[k] = h[k];
+      wave_sync();
+    }
   }
 }
 
@@ -381,16 +406,20 @@ __global__ __launch_bounds__(kBlock) void k_scan(FitArgs<IdxT, LabT> a) {
   const uint32_t K = (uint32_t)a.K;
   uint32_t *carry = smem + smem_wave_offset(a.K);
   uint32_t *tot = carry + K;
-  WAVE_LOOP(a.ctl->n_scan) {
+  WAVE_LOOP(min(a.ctl->n_scan, a.cap_scan)) {
     const Task tk = a.scan_tasks[item];
     const uint32_t i = tk.node;
     const FNode nd = a.front[i];
     const NAux ax = a.aux[i];
     const uint32_t n = nd.n;
+    // class totals: lane k holds class k when K <= 64 (else in LDS)
+    const bool regs = K <= (uint32_t)kWave;
     uint64_t t2 = 0;
+    uint32_t tv = 0;
     for (uint32_t k = lane; k < K; k += kWave) {
       const uint32_t v = a.cnt[(size_t)i * K + k];
-      tot[k] = v;
+      if (regs) tv = v;
+      else tot[k] = v;
       t2 += (uint64_t)v * v;
     }
     const uint64_t T2 = uniform64(wave_sum64(t2));
@@ -402,22 +431,26 @@ __global__ __launch_bounds__(kBlock) void k_scan(FitArgs<IdxT, LabT> a) {
       const uint32_t f = (uint32_t)a.feats[(size_t)i * a.m + j];
       const IdxT *row = rows_of(a, nd.src, nd.tree, f) + nd.start;
       const float *x = a.X_T + (size_t)f * a.N;
-      // chunk carry and the running sums at the chunk start
+      // chunk carry and the running sums at the chunk start; with K <= 64
+      // lane k carries class k's count in cv
       wave_sync();
       uint64_t sl = 0, dd = 0;
+      uint32_t cv = 0;
       const uint32_t *hc = (tk.nch > 1) ? a.hist + ax.hist_base + ((size_t)j * tk.nch + tk.chunk) * K : nullptr;
       for (uint32_t k = lane; k < K; k += kWave) {
         const uint32_t c = hc ? hc[k] : 0u;
-        carry[k] = c;
+        const uint32_t t = regs ? tv : tot[k];
+        if (regs) cv = c;
+        else carry[k] = c;
         sl += (uint64_t)c * c;
-        dd += (uint64_t)tot[k] * c;
+        dd += (uint64_t)t * c;
       }
       uint64_t SL = uniform64(wave_sum64(sl));
       uint64_t D = uniform64(wave_sum64(dd));
       wave_sync();
       Cand lb = {0, 1, 0, (int32_t)f, 0};
       // value just past the chunk end (the next chunk's first sample)
-      const float chunk_next = (p1 < n) ? x[row[p1]] : 0.f;
+      const float chunk_next = (p1 < n) ? x[sample_of(row[p1])] : 0.f;
       for (uint32_t pb = p0; pb < p1; pb += kWave * kScanU) {
         // kScanU tiles in flight: index loads, then label and value gathers
         uint32_t sid[kScanU], lab[kScanU];
@@ -428,14 +461,14 @@ __global__ __launch_bounds__(kBlock) void k_scan(FitArgs<IdxT, LabT> a) {
           sid[u] = (p < p1) ? (uint32_t)row[p] : 0xFFFFFFFFu;
         }
         const uint32_t pe = pb + kWave * kScanU;  // first position after the batch
-        const float batch_next = (pe < p1 && lane == kWave - 1) ? x[row[pe]] : 0.f;
+        const float batch_next = (pe < p1 && lane == kWave - 1) ? x[sample_of(row[pe])] : 0.f;
 #pragma unroll
         for (int u = 0; u < kScanU; ++u) {
           lab[u] = 0;
           val[u] = 0.f;
           if (sid[u] != 0xFFFFFFFFu) {
-            lab[u] = a.y[sid[u]];
-            val[u] = x[sid[u]];
+            lab[u] = a.label((IdxT)sid[u]);
+            val[u] = x[sample_of((IdxT)sid[u])];
           }
         }
 #pragma unroll
@@ -443,22 +476,38 @@ __global__ __launch_bounds__(kBlock) void k_scan(FitArgs<IdxT, LabT> a) {
           const uint32_t p = pb + (uint32_t)u * kWave + lane;
           const bool valid = p < p1;
           if (!ballot(valid)) break;
-          // rank of this lane among earlier lanes with the same label
-          uint32_t cprev = 0;
+          // rank of this lane among earlier lanes with the same label (one
+          // ballot per distinct label of the tile), and its class total
+          uint32_t cprev = 0, dD = 0;
           uint64_t rem = ballot(valid);
-          while (rem) {
-            const uint32_t leader = (uint32_t)__ffsll((unsigned long long)rem) - 1u;
-            const uint32_t L = readlane(lab[u], leader);
-            const uint64_t mm = ballot(valid && lab[u] == L);
-            const uint32_t b0 = carry[L];
-            if (valid && lab[u] == L) cprev = b0 + mbcnt(mm);
-            wave_sync();
-            if (lane == 0) carry[L] = b0 + (uint32_t)__popcll(mm);
-            wave_sync();
-            rem &= ~mm;
+          if (regs) {
+            while (rem) {
+              const uint32_t leader = (uint32_t)__ffsll((unsigned long long)rem) - 1u;
+              const uint32_t L = readlane(lab[u], leader);
+              const uint64_t mm = ballot(valid && lab[u] == L);
+              const uint32_t b0 = readlane(cv, L), tL = readlane(tv, L);
+              if (valid && lab[u] == L) {
+                cprev = b0 + mbcnt(mm);
+                dD = tL;
+              }
+              if (lane == L) cv += (uint32_t)__popcll(mm);
+              rem &= ~mm;
+            }
+          } else {
+            while (rem) {
+              const uint32_t leader = (uint32_t)__ffsll((unsigned long long)rem) - 1u;
+              const uint32_t L = readlane(lab[u], leader);
+              const uint64_t mm = ballot(valid && lab[u] == L);
+              const uint32_t b0 = carry[L];
+              if (valid && lab[u] == L) cprev = b0 + mbcnt(mm);
+              wave_sync();
+              if (lane == 0) carry[L] = b0 + (uint32_t)__popcll(mm);
+              wave_sync();
+              rem &= ~mm;
+            }
+            if (valid) dD = tot[lab[u]];
           }
           const uint32_t dS = valid ? 2u * cprev + 1u : 0u;
-          const uint32_t dD = valid ? tot[lab[u]] : 0u;
           const uint32_t iS = wave_incl_scan(dS);
           const uint32_t iD = wave_incl_scan(dD);
           const uint64_t SLp = SL + iS, Dp = D + iD;
@@ -466,9 +515,9 @@ __global__ __launch_bounds__(kBlock) void k_scan(FitArgs<IdxT, LabT> a) {
           D += readlane(iD, kWave - 1);
           // value at p + 1: the next lane, the next tile's lane 0, the next
           // batch, or the next chunk
-          float nxt = __shfl_down(val[u], 1, kWave);
+          float nxt = __uint_as_float(wave_next(__float_as_uint(val[u])));
           if (u + 1 < kScanU) {
-            const float first_next = __shfl(val[u + 1 < kScanU ? u + 1 : u], 0, kWave);
+            const float first_next = __uint_as_float(readlane(__float_as_uint(val[u + 1 < kScanU ? u + 1 : u]), 0));
             if (lane == kWave - 1) nxt = first_next;
           } else if (lane == kWave - 1) {
             nxt = batch_next;
@@ -550,7 +599,7 @@ __device__ void count_left(const FitArgs<IdxT, LabT> &a, uint32_t i, const FNode
     }
     uint32_t l[4];
 #pragma unroll
-    for (int u = 0; u < 4; ++u) l[u] = (sid[u] != 0xFFFFFFFFu) ? (uint32_t)a.y[sid[u]] : 0xFFFFFFFFu;
+    for (int u = 0; u < 4; ++u) l[u] = (sid[u] != 0xFFFFFFFFu) ? a.label((IdxT)sid[u]) : 0xFFFFFFFFu;
 #pragma unroll
     for (int u = 0; u < 4; ++u)
       if (l[u] != 0xFFFFFFFFu) atomicAdd(&h[l[u]], delta);
@@ -589,7 +638,7 @@ __global__ __launch_bounds__(kBlock) void k_decide(FitArgs<IdxT, LabT> a) {
         a.arena.thr[id] = 0.f;
         a.arena.left[id] = kLeaf;
         a.arena.right[id] = kLeaf;
-        a.arena.value[id] = a.y[row0[0]];
+        a.arena.value[id] = a.label(row0[0]);
       }
       ax.best_f = -1;
       ax.flags = 0;
@@ -601,7 +650,7 @@ __global__ __launch_bounds__(kBlock) void k_decide(FitArgs<IdxT, LabT> a) {
         const IdxT *row = rows_of(a, nd.src, nd.tree, f) + nd.start;
         const float *x = a.X_T + (size_t)f * a.N;
         // threshold midway in float, no contraction (bfs_module.cu:399-400)
-        const float thr = __fmul_rn(__fadd_rn(x[row[p]], x[row[p + 1]]), 0.5f);
+        const float thr = __fmul_rn(__fadd_rn(x[sample_of(row[p])], x[sample_of(row[p + 1])]), 0.5f);
         a.arena.feat[id] = (int32_t)f;
         a.arena.thr[id] = thr;
         a.arena.value[id] = 0;
@@ -646,7 +695,7 @@ __device__ __forceinline__ void fill_task_bits(const FitArgs<IdxT, LabT> &a, con
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
       const uint32_t p = pb + (uint32_t)u * kWave + lane;
-      sid[u] = (p < p1) ? (uint32_t)row[p] : 0xFFFFFFFFu;
+      sid[u] = (p < p1) ? sample_of(row[p]) : 0xFFFFFFFFu;
     }
 #pragma unroll
     for (int u = 0; u < 4; ++u)
@@ -763,7 +812,7 @@ __global__ __launch_bounds__(kBlock) void k_children(FitArgs<IdxT, LabT> a) {
   const uint32_t K = (uint32_t)a.K;
   uint32_t *cl = smem + smem_wave_offset(a.K);
   uint32_t *cr = cl + K;
-  unsigned long long bytes = 0;
+  unsigned long long bytes = 0, bytes_small = 0;
   WAVE_LOOP(a.ctl->q) {
     const uint32_t i = item;
     NAux ax = a.aux[i];
@@ -818,18 +867,19 @@ __global__ __launch_bounds__(kBlock) void k_children(FitArgs<IdxT, LabT> a) {
         bool needl = kl == 2, needr = kr == 2;
         for (uint32_t pb = 0; pb < nd.n && (needl || needr); pb += kWave) {
           const uint32_t p = pb + lane;
-          uint32_t s = 0, side = 2;
+          uint32_t e = 0, side = 2;
           if (p < nd.n) {
-            s = row0[p];
+            e = (uint32_t)row0[p];
+            const uint32_t s = sample_of((IdxT)e);
             side = (bits[s >> 5] >> (s & 31u)) & 1u;
           }
           const uint64_t ml = ballot(side == 1), mr = ballot(side == 0);
           if (needl && ml) {
-            labl = a.y[readlane(s, (uint32_t)__ffsll((unsigned long long)ml) - 1u)];
+            labl = a.label((IdxT)readlane(e, (uint32_t)__ffsll((unsigned long long)ml) - 1u));
             needl = false;
           }
           if (needr && mr) {
-            labr = a.y[readlane(s, (uint32_t)__ffsll((unsigned long long)mr) - 1u)];
+            labr = a.label((IdxT)readlane(e, (uint32_t)__ffsll((unsigned long long)mr) - 1u));
             needr = false;
           }
         }
@@ -845,7 +895,10 @@ __global__ __launch_bounds__(kBlock) void k_children(FitArgs<IdxT, LabT> a) {
       ax.resh_ntask = 0;
       if (v[1]) {
         resh_shape(nd.n, (uint32_t)a.F, a.Cr, ax, v);
-        bytes += (unsigned long long)a.F * nd.n * (2 * sizeof(IdxT)) + (nd.n + 7) / 8;
+        // every row read and written once, the node's side bits read once
+        const unsigned long long b = (unsigned long long)a.F * nd.n * (2 * sizeof(IdxT)) + (nd.n + 7) / 8;
+        if (ax.resh_big) bytes += b;
+        else bytes_small += b;
       }
     }
     if (lane == 0) {
@@ -856,6 +909,7 @@ __global__ __launch_bounds__(kBlock) void k_children(FitArgs<IdxT, LabT> a) {
     }
   }
   flush_bytes(a, kBytesResh, bytes);
+  flush_bytes(a, kBytesReshSmall, bytes_small);
 }
 
 // Reshuffle task descriptors of node i at its scanned offsets.
@@ -982,21 +1036,21 @@ constexpr int kPlanThreads = 1024;
 template <class IdxT, class LabT>
 __global__ __launch_bounds__(kPlanThreads) void k_plan(FitArgs<IdxT, LabT> a, uint32_t target) {
   constexpr int kW = kPlanThreads / kWave;
-  __shared__ uint32_t ws[kW];
-  __shared__ uint32_t carry;
+  __shared__ uint32_t ws[2][kW];
+  __shared__ uint32_t carry[2];
   const uint32_t tid = threadIdx.x, lane = tid % kWave, wave = tid / kWave;
-  const uint32_t q = a.ctl->q;
+  const uint32_t q = a.ctl->q, T = a.T;
   const uint32_t tot_big = a.ctl->c_big, tot_small = a.ctl->c_small, tot_tiles = a.ctl->c_tiles;
   const unsigned long long total =
       (unsigned long long)tot_tiles * kReshTile + (unsigned long long)tot_small * a.Cr;
   const unsigned long long per =
       min(kWgElemsMax, max(kWgElemsMin, total / max(1u, target)));
-  if (tid == 0) carry = 0;
+  if (tid < 2) carry[tid] = 0;
   __syncthreads();
-  for (uint32_t t0 = 0; t0 < a.T; t0 += kPlanThreads) {
+  for (uint32_t t0 = 0; t0 < T; t0 += kPlanThreads) {
     const uint32_t t = t0 + tid;
-    uint32_t wgs = 0;
-    if (t < a.T) {
+    uint32_t wgs[2] = {0u, 0u};
+    if (t < T) {
       const uint32_t lo = a.tlo[t], hi = a.thi[t];
       uint32_t bb = 0, be = 0, sb = 0, se = 0, tb = 0, te = 0;
       if (hi > lo) {
@@ -1020,65 +1074,98 @@ __global__ __launch_bounds__(kPlanThreads) void k_plan(FitArgs<IdxT, LabT> a, ui
       pl[1] = be;
       pl[2] = sb;
       pl[3] = se;
-      const unsigned long long e =
-          (unsigned long long)(te - tb) * kReshTile + (unsigned long long)(se - sb) * a.Cr;
-      if (e) wgs = (uint32_t)min(4096ull, (e + per - 1) / per);
+      const unsigned long long eb = (unsigned long long)(te - tb) * kReshTile;
+      const unsigned long long es = (unsigned long long)(se - sb) * a.Cr;
+      if (eb) wgs[0] = (uint32_t)min(4096ull, (eb + per - 1) / per);
+      if (es) wgs[1] = (uint32_t)min(4096ull, (es + per - 1) / per);
       a.tlo[t] = 0;
       a.thi[t] = 0;
     }
-    const uint32_t inc = wave_incl_scan(wgs);
-    if (lane == kWave - 1) ws[wave] = inc;
+    uint32_t inc[2];
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+      inc[k] = wave_incl_scan(wgs[k]);
+      if (lane == kWave - 1) ws[k][wave] = inc[k];
+    }
     __syncthreads();
-    uint32_t off = carry + inc - wgs;
-    for (uint32_t w = 0; w < wave; ++w) off += ws[w];
-    if (t < a.T) a.wg_off[t] = off;
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+      uint32_t off = carry[k] + inc[k] - wgs[k];
+      for (uint32_t w = 0; w < wave; ++w) off += ws[k][w];
+      if (t < T) a.wg_off[(size_t)k * (T + 1) + t] = off;
+    }
     __syncthreads();
-    if (tid == 0) {
+    if (tid < 2) {
       uint32_t sum = 0;
-      for (int w = 0; w < kW; ++w) sum += ws[w];
-      carry += sum;
+      for (int w = 0; w < kW; ++w) sum += ws[tid][w];
+      carry[tid] += sum;
     }
     __syncthreads();
   }
-  if (tid == 0) {
-    uint32_t P = carry;
+  if (tid < 2) {
+    uint32_t P = carry[tid];
     if (tot_big > a.cap_big || tot_small > a.cap_small || a.ctl->err) P = 0;
-    a.wg_off[a.T] = P;
-    a.ctl->n_plan = P;
+    a.wg_off[(size_t)tid * (T + 1) + T] = P;
+    a.ctl->n_plan[tid] = P;
   }
+}
+
+// Global-memory pointers held in scalar registers, with row elements
+// addressed by 32-bit byte offsets (rows hold at most 2^22 elements), so
+// loads and stores take the scalar-base global form and keep vector
+// registers free.
+template <class T>
+using gptr = __attribute__((address_space(1))) T *;
+
+template <class T>
+__device__ __forceinline__ gptr<T> sgpr_ptr(T *p) {
+  const uint64_t v = reinterpret_cast<uint64_t>(p);
+  return reinterpret_cast<gptr<T>>(((uint64_t)uniform((uint32_t)(v >> 32)) << 32) | uniform((uint32_t)v));
+}
+template <class T>
+__device__ __forceinline__ T ld32(gptr<const T> base, uint32_t i) {
+  return *reinterpret_cast<gptr<const T>>(reinterpret_cast<gptr<const char>>(base) + i * (uint32_t)sizeof(T));
+}
+template <class T>
+__device__ __forceinline__ void st32(gptr<T> base, uint32_t i, T v) {
+  *reinterpret_cast<gptr<T>>(reinterpret_cast<gptr<char>>(base) + i * (uint32_t)sizeof(T)) = v;
 }
 
 // One big task: all rows' segment [0, n) of one row, moved by the workgroup.
 // `par` selects the half of wcnt a tile writes; it alternates across tiles
 // and across consecutive tasks, so a tile never overwrites counts that a
-// slower wave of the previous tile may still be reading.
+// slower wave of the previous tile may still be reading.  Tiles are
+// software-pipelined over two register sets: the next tile's loads are
+// issued before this tile's stores, so neither waits for the other.
 template <class IdxT, class LabT, class Side>
 __device__ void resh_big(const FitArgs<IdxT, LabT> &a, const Task &tk, const Side &side, uint32_t *wcnt,
                          uint32_t &par) {
-  const FNode nd = a.front[tk.node];
-  const NAux ax = a.aux[tk.node];
-  const uint32_t f = tk.first_count & 0xFFFFu;
-  const IdxT *in = rows_of(a, nd.src, nd.tree, f) + nd.start;
-  IdxT *out = rows_out(a, ax.dst, nd.tree, f) + nd.start;
-  const bool liveL = ax.flags & kLiveL, liveR = ax.flags & kLiveR;
-  const uint32_t n = nd.n, nL = ax.nL;
+  const uint32_t node = uniform(tk.node);
+  const FNode nd = a.front[node];
+  const NAux ax = a.aux[node];
+  const uint32_t f = uniform(tk.first_count & 0xFFFFu);
+  const gptr<const IdxT> in = sgpr_ptr(rows_of(a, nd.src, nd.tree, f) + nd.start);
+  const gptr<IdxT> out = sgpr_ptr(rows_out(a, ax.dst, nd.tree, f) + nd.start);
+  const bool liveL = uniform(ax.flags) & kLiveL, liveR = uniform(ax.flags) & kLiveR;
+  const uint32_t n = uniform(nd.n), nL = uniform(ax.nL);
   const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
   uint32_t carry = 0;
-  for (uint32_t base = 0; base < n; base += kReshTile) {
-    IdxT e[kReshE];
-    uint32_t sd[kReshE], rk[kReshE];
+  // positions past n repeat the last element and are masked
+  auto load = [&](IdxT(&e)[kReshE], uint32_t base) {
+#pragma unroll
+    for (int k = 0; k < kReshE; ++k) e[k] = ld32(in, min(base + (uint32_t)k * kReshThreads + tid, n - 1));
+  };
+  // moves the tile at `base` held in e; loads the next tile into nx first
+  auto step = [&](IdxT(&e)[kReshE], IdxT(&nx)[kReshE], uint32_t base) -> bool {
+    uint32_t sdm = 0;  // bit k: element k goes left
 #pragma unroll
     for (int k = 0; k < kReshE; ++k) {
       const uint32_t p = base + (uint32_t)k * kReshThreads + tid;
-      e[k] = 0;
-      if (p < n) e[k] = in[p];
+      sdm |= ((p < n) ? side(e[k]) : 0u) << k;
     }
 #pragma unroll
     for (int k = 0; k < kReshE; ++k) {
-      const uint32_t p = base + (uint32_t)k * kReshThreads + tid;
-      sd[k] = (p < n) ? side(e[k]) : 0u;
-      const uint64_t m = ballot(sd[k] != 0);
-      rk[k] = mbcnt(m);
+      const uint64_t m = ballot((sdm >> k) & 1u);
       if (lane == 0) wcnt[par * kSlots + k * kReshWaves + wave] = (uint32_t)__popcll(m);
     }
     __syncthreads();
@@ -1089,49 +1176,75 @@ __device__ void resh_big(const FitArgs<IdxT, LabT> &a, const Task &tk, const Sid
     const uint32_t e0 = inc - v0 - v1, e1 = e0 + v0;
     const uint32_t total = readlane(inc, 63);
     const uint32_t esel = (wave & 1u) ? e1 : e0;  // slot parity = wave parity
+    const bool more = base + kReshTile < n;
+    if (more) load(nx, base + kReshTile);
 #pragma unroll
     for (int k = 0; k < kReshE; ++k) {
       const uint32_t slot = (uint32_t)k * kReshWaves + wave;
       const uint32_t before = carry + readlane(esel, slot >> 1);
       const uint32_t p = base + (uint32_t)k * kReshThreads + tid;
+      const uint64_t m = ballot((sdm >> k) & 1u);
       if (p < n) {
-        const uint32_t lb = before + rk[k];
-        if (sd[k]) {
-          if (liveL) out[lb] = e[k];
+        const uint32_t lb = before + mbcnt(m);
+        if ((sdm >> k) & 1u) {
+          if (liveL) st32(out, lb, e[k]);
         } else if (liveR) {
-          out[nL + (p - lb)] = e[k];
+          st32(out, nL + (p - lb), e[k]);
         }
       }
     }
     carry += total;
     par ^= 1u;
+    return more;
+  };
+  IdxT A[kReshE], B[kReshE];
+  load(A, 0);
+  for (uint32_t base = 0;; base += 2 * kReshTile) {
+    if (!step(A, B, base)) break;
+    if (!step(B, A, base + kReshTile)) break;
   }
 }
 
 // One small task: rows [first, first+count) of a node, moved by one wave.
 // A row of n <= 64 * kReshE / 2 samples takes c = ceil(n / 64) pieces, so
 // kReshE / c rows are loaded together (one batch of kReshE striped loads in
-// flight) before their ranks are taken in order.
+// flight) before their ranks are taken in order.  Consecutive rows of a
+// tree are N elements apart.
 template <class IdxT, class LabT, class Side>
 __device__ void resh_small(const FitArgs<IdxT, LabT> &a, const Task &tk, const Side &side) {
-  const FNode nd = a.front[tk.node];
-  const NAux ax = a.aux[tk.node];
-  const bool liveL = ax.flags & kLiveL, liveR = ax.flags & kLiveR;
-  const uint32_t first = tk.first_count & 0xFFFFu, count = tk.first_count >> 16;
+  const uint32_t node = uniform(tk.node);
+  const FNode nd = a.front[node];
+  const NAux ax = a.aux[node];
+  const bool liveL = uniform(ax.flags) & kLiveL, liveR = uniform(ax.flags) & kLiveR;
+  const uint32_t first = uniform(tk.first_count & 0xFFFFu), count = uniform(tk.first_count >> 16);
   const uint32_t end = first + count;
-  const uint32_t n = nd.n, nL = ax.nL;
+  const uint32_t n = uniform(nd.n), nL = uniform(ax.nL), N = (uint32_t)a.N;
   const uint32_t lane = lane_id();
   const uint32_t c = ceil_div(n, kWave);
   if (c <= (uint32_t)kReshE / 2) {
     const uint32_t R = (uint32_t)kReshE / c;  // rows per batch
     for (uint32_t f0 = first; f0 < end; f0 += R) {
+      const gptr<const IdxT> in = sgpr_ptr(rows_of(a, nd.src, nd.tree, f0) + nd.start);
+      const gptr<IdxT> out = sgpr_ptr(rows_out(a, ax.dst, nd.tree, f0) + nd.start);
+      const uint32_t rows = min(R, end - f0);
       IdxT e[kReshE];
       uint32_t r = 0, pc = 0;
 #pragma unroll
       for (int k = 0; k < kReshE; ++k) {
-        const uint32_t f = f0 + r, p = pc * kWave + lane;
-        e[k] = 0;
-        if (r < R && f < end && p < n) e[k] = rows_of(a, nd.src, nd.tree, f)[nd.start + p];
+        const uint32_t p = pc * kWave + lane;
+        e[k] = ld32(in, min(r, rows - 1) * N + min(p, n - 1));  // out-of-range pieces are masked
+        if (++pc == c) {
+          pc = 0;
+          ++r;
+        }
+      }
+      uint32_t sdm = 0;
+      r = 0;
+      pc = 0;
+#pragma unroll
+      for (int k = 0; k < kReshE; ++k) {
+        const uint32_t p = pc * kWave + lane;
+        if (r < rows && p < n && side(e[k])) sdm |= 1u << k;
         if (++pc == c) {
           pc = 0;
           ++r;
@@ -1142,17 +1255,14 @@ __device__ void resh_small(const FitArgs<IdxT, LabT> &a, const Task &tk, const S
       pc = 0;
 #pragma unroll
       for (int k = 0; k < kReshE; ++k) {
-        const uint32_t f = f0 + r, p = pc * kWave + lane;
-        const bool valid = r < R && f < end && p < n;
-        const uint32_t sd = valid ? side(e[k]) : 0u;
-        const uint64_t m = ballot(sd != 0);
-        if (valid) {
-          IdxT *out = rows_out(a, ax.dst, nd.tree, f) + nd.start;
+        const uint32_t p = pc * kWave + lane;
+        const uint64_t m = ballot((sdm >> k) & 1u);
+        if (r < rows && p < n) {
           const uint32_t lb = carry + mbcnt(m);
-          if (sd) {
-            if (liveL) out[lb] = e[k];
+          if ((sdm >> k) & 1u) {
+            if (liveL) st32(out, r * N + lb, e[k]);
           } else if (liveR) {
-            out[nL + (p - lb)] = e[k];
+            st32(out, r * N + nL + (p - lb), e[k]);
           }
         }
         carry += (uint32_t)__popcll(m);
@@ -1166,29 +1276,32 @@ __device__ void resh_small(const FitArgs<IdxT, LabT> &a, const Task &tk, const S
     return;
   }
   for (uint32_t f = first; f < end; ++f) {
-    const IdxT *in = rows_of(a, nd.src, nd.tree, f) + nd.start;
-    IdxT *out = rows_out(a, ax.dst, nd.tree, f) + nd.start;
+    const gptr<const IdxT> in = sgpr_ptr(rows_of(a, nd.src, nd.tree, f) + nd.start);
+    const gptr<IdxT> out = sgpr_ptr(rows_out(a, ax.dst, nd.tree, f) + nd.start);
     uint32_t carry = 0;
     for (uint32_t base = 0; base < n; base += kWave * kReshE) {
       IdxT e[kReshE];
-      uint32_t sd[kReshE];
 #pragma unroll
       for (int k = 0; k < kReshE; ++k) {
         const uint32_t p = base + (uint32_t)k * kWave + lane;
-        e[k] = 0;
-        if (p < n) e[k] = in[p];
+        e[k] = ld32(in, min(p, n - 1));
+      }
+      uint32_t sdm = 0;
+#pragma unroll
+      for (int k = 0; k < kReshE; ++k) {
+        const uint32_t p = base + (uint32_t)k * kWave + lane;
+        if (p < n && side(e[k])) sdm |= 1u << k;
       }
 #pragma unroll
       for (int k = 0; k < kReshE; ++k) {
         const uint32_t p = base + (uint32_t)k * kWave + lane;
-        sd[k] = (p < n) ? side(e[k]) : 0u;
-        const uint64_t m = ballot(sd[k] != 0);
+        const uint64_t m = ballot((sdm >> k) & 1u);
         if (p < n) {
           const uint32_t lb = carry + mbcnt(m);
-          if (sd[k]) {
-            if (liveL) out[lb] = e[k];
+          if ((sdm >> k) & 1u) {
+            if (liveL) st32(out, lb, e[k]);
           } else if (liveR) {
-            out[nL + (p - lb)] = e[k];
+            st32(out, nL + (p - lb), e[k]);
           }
         }
         carry += (uint32_t)__popcll(m);
@@ -1198,13 +1311,15 @@ __device__ void resh_small(const FitArgs<IdxT, LabT> &a, const Task &tk, const S
 }
 
 // kLds: the tree's bitmap is staged in LDS (N/8 bytes); otherwise it is read
-// from global memory.
-template <class IdxT, class LabT, bool kLds>
+// from global memory.  kBig: the launch moves the big tasks of its plan,
+// else the small tasks (separate launches keep each path's registers).
+template <class IdxT, class LabT, bool kLds, bool kBig>
 __global__ __launch_bounds__(kReshThreads, 8) void k_resh(FitArgs<IdxT, LabT> a) {
   extern __shared__ uint32_t lds[];
   uint32_t *wcnt = lds + (kLds ? ((a.W + 3u) & ~3u) : 0u);  // 2 x kSlots counts
-  const uint32_t P = a.ctl->n_plan;
+  const uint32_t P = a.ctl->n_plan[kBig ? 0 : 1];
   const uint32_t T = a.T;
+  const uint32_t *wg_off = a.wg_off + (kBig ? 0u : T + 1);
   const uint32_t wave = threadIdx.x / kWave;
   uint32_t loaded = 0xFFFFFFFFu, par = 0;
   for (uint32_t pe = blockIdx.x; pe < P; pe += gridDim.x) {
@@ -1212,11 +1327,11 @@ __global__ __launch_bounds__(kReshThreads, 8) void k_resh(FitArgs<IdxT, LabT> a)
     uint32_t lo = 0, hi = T;
     while (hi - lo > 1) {
       const uint32_t mid = (lo + hi) / 2;
-      if (a.wg_off[mid] <= pe) lo = mid;
+      if (wg_off[mid] <= pe) lo = mid;
       else hi = mid;
     }
     const uint32_t t = lo;
-    const uint32_t j = pe - a.wg_off[t], wgs = a.wg_off[t + 1] - a.wg_off[t];
+    const uint32_t j = pe - wg_off[t], wgs = wg_off[t + 1] - wg_off[t];
     const uint32_t *gbits = a.bits + (size_t)t * a.W;
     if (kLds && t != loaded) {
       __syncthreads();
@@ -1225,15 +1340,19 @@ __global__ __launch_bounds__(kReshThreads, 8) void k_resh(FitArgs<IdxT, LabT> a)
       loaded = t;
     }
     auto side = [&](IdxT s) -> uint32_t {
-      const uint32_t u = (uint32_t)s;
+      const uint32_t u = sample_of(s);
       if constexpr (kLds) return (lds[u >> 5] >> (u & 31u)) & 1u;
       else return (gbits[u >> 5] >> (u & 31u)) & 1u;
     };
     const uint32_t *pl = a.plan + (size_t)t * 4;
-    const uint32_t b0 = pl[0], nb = pl[1] - pl[0], s0 = pl[2], ns = pl[3] - pl[2];
-    for (uint32_t bt = j; bt < nb; bt += wgs) resh_big(a, a.big_tasks[b0 + bt], side, wcnt, par);
-    for (uint32_t st = j * kReshWaves + wave; st < ns; st += wgs * kReshWaves)
-      resh_small(a, a.small_tasks[s0 + st], side);
+    if constexpr (kBig) {
+      const uint32_t b0 = pl[0], nb = pl[1] - pl[0];
+      for (uint32_t bt = j; bt < nb; bt += wgs) resh_big(a, a.big_tasks[b0 + bt], side, wcnt, par);
+    } else {
+      const uint32_t s0 = pl[2], ns = pl[3] - pl[2];
+      for (uint32_t st = j * kReshWaves + wave; st < ns; st += wgs * kReshWaves)
+        resh_small(a, a.small_tasks[s0 + st], side);
+    }
   }
 }
 
@@ -1388,7 +1507,7 @@ __global__ __launch_bounds__(kPlanThreads) void k_roots(FitArgs<IdxT, LabT> a, c
       // a positive max_depth never caps the root.
       uint32_t leaf_label = kLeaf;
       if (nz <= 1) leaf_label = only;
-      else if ((int64_t)n < (int64_t)a.mss) leaf_label = a.y[rows_of(a, src, t, 0)[0]];
+      else if ((int64_t)n < (int64_t)a.mss) leaf_label = a.label(rows_of(a, src, t, 0)[0]);
       if (leaf_label != kLeaf) {
         a.arena.feat[t] = -1;
         a.arena.thr[t] = 0.f;
@@ -1525,8 +1644,8 @@ __global__ void k_canon_scatter(uint32_t n, Arena ar, const uint32_t *size, cons
 // Moves the next frontier's count into place and clears the level counters.
 // End of a level: the children-scan totals become the next frontier size
 // and the ids in use; the spread byte counters are folded into Ctl.  One
-// block of 4 waves, one per counter kind.
-__global__ __launch_bounds__(256) void k_ctl_next(Ctl *ctl, unsigned long long *bslot) {
+// block of kByteKinds waves, one per counter kind.
+__global__ __launch_bounds__(kByteKinds * kWave) void k_ctl_next(Ctl *ctl, unsigned long long *bslot) {
   const uint32_t tid = threadIdx.x, kind = tid / kWave, slot = tid % kWave;
   unsigned long long v = 0;
   for (uint32_t s = slot; s < kByteSlots; s += kWave) {

@@ -57,10 +57,12 @@ struct Ctl {
   uint32_t q;        // frontier size
   uint32_t n_nodes;  // arena ids in use
   uint32_t err;
-  uint32_t n_plan;   // reshuffle plan entries
-  uint32_t n_scan, n_hist, n_fill, pad0;                 // selection-scan totals
-  uint32_t c_ids, c_live, c_big, c_small, c_tiles, pad1[3];  // children-scan totals
-  unsigned long long bytes_scan, bytes_fill, bytes_resh, bytes_boot;
+  uint32_t pad0;
+  uint32_t n_plan[2];  // reshuffle plan entries: big tasks, small tasks
+  uint32_t pad1[2];
+  uint32_t n_scan, n_hist, n_fill, pad2;                 // selection-scan totals
+  uint32_t c_ids, c_live, c_big, c_small, c_tiles, pad3[3];  // children-scan totals
+  unsigned long long bytes_scan, bytes_fill, bytes_resh, bytes_boot, bytes_resh_small;
 };
 // Fields of the two per-node offset scans.
 constexpr int kSelFields = 3;  // scan tasks, histogram words, fill slots
@@ -68,7 +70,8 @@ constexpr int kChFields = 5;   // child ids, next-frontier slots, big / small re
 // Algorithmic-byte counters: per-wave sums flushed into kByteSlots spread
 // slots (one 128-byte line each) and folded into Ctl at the end of a level.
 constexpr uint32_t kByteSlots = 64, kByteStride = 16;
-enum ByteKind { kBytesScan = 0, kBytesFill = 1, kBytesResh = 2, kBytesBoot = 3 };
+enum ByteKind { kBytesScan = 0, kBytesFill = 1, kBytesResh = 2, kBytesBoot = 3, kBytesReshSmall = 4 };
+constexpr uint32_t kByteKinds = 5;
 constexpr uint32_t kErrCapacity = 1u, kErrBadNode = 2u;
 
 // Flattened forest node for prediction: pre-order, so a split node's left
@@ -86,6 +89,17 @@ struct Arena {
   float *thr;
   uint32_t *left, *right, *value, *nsamp, *depth;
 };
+
+// Row entries: 16-bit sample ids (N <= 65,535), or 32-bit words holding
+// the sample id in the low kIdBits bits and its label in the high bits
+// (N < 2^22, K <= 1,024), so the split scan reads labels with the ids.
+constexpr uint32_t kIdBits = 22, kIdMask = (1u << kIdBits) - 1u;
+
+template <class IdxT>
+__host__ __device__ __forceinline__ uint32_t sample_of(IdxT v) {
+  if constexpr (sizeof(IdxT) == 4) return (uint32_t)v & kIdMask;
+  else return (uint32_t)v;
+}
 
 template <class IdxT, class LabT>
 struct FitArgs {
@@ -119,11 +133,17 @@ struct FitArgs {
   // reshuffle plan: per-tree task ranges and workgroup ranges
   Task *big_tasks, *small_tasks;
   uint32_t *plan;    // [T x 4] big begin/end, small begin/end
-  uint32_t *wg_off;  // [T + 1] workgroups per tree (prefix)
-  unsigned long long *bslot;  // [4 x kByteSlots x kByteStride] byte counters
+  uint32_t *wg_off;  // [2 x (T + 1)] workgroups per tree (prefix): big tasks, then small tasks
+  unsigned long long *bslot;  // [kByteKinds x kByteSlots x kByteStride] byte counters
   Arena arena;
   Ctl *ctl;
   uint32_t cap_q, cap_scan, cap_hist, cap_fill, cap_big, cap_small, cap_nodes;
+
+  // label of a row entry: carried in the word, or looked up by sample id
+  __device__ __forceinline__ uint32_t label(IdxT v) const {
+    if constexpr (sizeof(IdxT) == 4) return (uint32_t)v >> kIdBits;
+    else return (uint32_t)y[v];
+  }
 };
 
 }  // namespace cta
