@@ -212,7 +212,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--cpu-trees", type=int, default=3, help="trees in the bounded CPU-oracle sample")
+    ap.add_argument("--cpu-trees", type=int, default=10, help="trees in the bounded CPU-oracle sample")
     ap.add_argument("--per-node-trees", type=int, default=1, help="trees in the per-node GPU sample (0: skip)")
     ap.add_argument("--sklearn-trees", type=int, default=8, help="trees in the scikit-learn sample (0: skip)")
     ap.add_argument("--no-profile", action="store_true", help="skip per-kernel HIP-event timing")
