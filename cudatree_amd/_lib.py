@@ -55,7 +55,9 @@ class ServerOpts(ctypes.Structure):
         ("max_batch_rows", ctypes.c_int64),
         ("hbm_budget", ctypes.c_uint64),
         ("fit_workers", ctypes.c_int32),
-        ("reserved", ctypes.c_int32 * 7),
+        ("pad0", ctypes.c_int32),
+        ("max_batch_work", ctypes.c_int64),
+        ("reserved", ctypes.c_int32 * 4),
     ]
 
 
@@ -389,7 +391,7 @@ class Server:
     """Serving loop (cta_server): resident forests, coalesced predict
     requests, fit jobs admitted by HBM accounting."""
 
-    def __init__(self, device=0, coalesce=True, max_batch_rows=0, hbm_budget=0, fit_workers=1):
+    def __init__(self, device=0, coalesce=True, max_batch_rows=0, hbm_budget=0, fit_workers=1, max_batch_work=0):
         self.lib = load()
         o = ServerOpts()
         self.lib.cta_server_opts_default(ctypes.byref(o))
@@ -397,6 +399,8 @@ class Server:
         o.coalesce = int(bool(coalesce))
         if max_batch_rows:
             o.max_batch_rows = max_batch_rows
+        if max_batch_work:
+            o.max_batch_work = max_batch_work
         o.hbm_budget = hbm_budget
         o.fit_workers = fit_workers
         h = _vp()

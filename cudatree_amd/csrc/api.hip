@@ -221,6 +221,7 @@ struct cta_dataset {
   cta_ctx *ctx = nullptr;
   int32_t N = 0, F = 0, K = 0, idx_bits = 32, lab_bits = 8;
   DevMem XT, y, si0;
+  DevMem codes, fcode;  // value code columns (see k_code_index)
 };
 
 // ---- forest --------------------------------------------------------------------
@@ -419,6 +420,8 @@ struct Fitter {
     a.X_T = ds->XT.as<float>();
     a.y = ds->y.as<LabT>();
     a.si0 = ds->si0.as<IdxT>();
+    a.codes = ds->codes.as<uint8_t>();
+    a.fcode = ds->fcode.as<uint64_t>();
     a.buf0 = buf0;
     a.buf1 = buf1;
     a.bits = bits;
@@ -890,6 +893,48 @@ void presort(cta_ctx *ctx, cta_dataset *ds) {
   ctx->sync();
 }
 
+// Value code columns: per feature, the width of its distinct-value index
+// (1, 8 or 16 bits) when adjacent distinct values are all further apart than
+// the candidate gap, else none (the split scan reads X_T).
+template <class IdxT>
+void build_codes(cta_ctx *ctx, cta_dataset *ds) {
+  const uint32_t N = ds->N, F = ds->F;
+  hipStream_t st = ctx->stream;
+  uint32_t *d = ctx->ws_as<uint32_t>("code_index", (size_t)N * F);
+  uint32_t *db = ctx->ws_as<uint32_t>("code_meta", 2 * (size_t)F);
+  k_code_index<IdxT><<<F, kCodeThreads, 0, st>>>(ds->XT.as<float>(), ds->si0.as<IdxT>(), N, d, db, db + F);
+  uint32_t *h = static_cast<uint32_t *>(ctx->host("code_meta", 2 * (size_t)F * 4));
+  HIPC(hipMemcpyAsync(h, db, 2 * (size_t)F * 4, hipMemcpyDeviceToHost, st));
+  ctx->sync();
+  std::vector<uint64_t> fc(F);
+  uint64_t off = 0;
+  for (uint32_t f = 0; f < F; ++f) {
+    const uint32_t D = h[f], bad = h[F + f];
+    uint32_t kind = 0;
+    uint64_t bytes = 0;
+    if (!bad && D <= 2) {
+      kind = 1;
+      bytes = ((uint64_t)N + 31) / 32 * 4;
+    } else if (!bad && D <= 256) {
+      kind = 2;
+      bytes = N;
+    } else if (!bad && D <= 65536) {
+      kind = 3;
+      bytes = 2 * (uint64_t)N;
+    }
+    fc[f] = (off << 2) | kind;
+    off += (bytes + 255) & ~(uint64_t)255;
+  }
+  ds->codes.alloc(std::max<uint64_t>(off, 256));
+  ds->fcode.alloc((size_t)F * 8);
+  uint64_t *hf = static_cast<uint64_t *>(ctx->host("code_fcode", (size_t)F * 8));
+  std::memcpy(hf, fc.data(), (size_t)F * 8);
+  HIPC(hipMemcpyAsync(ds->fcode.p, hf, (size_t)F * 8, hipMemcpyHostToDevice, st));
+  k_code_pack<<<dim3(grid_for(N, 256, 256), F), 256, 0, st>>>(d, N, ds->fcode.as<uint64_t>(), ds->codes.as<uint8_t>());
+  HIPC(hipGetLastError());
+  ctx->sync();
+}
+
 void dataset_build(cta_ctx *ctx, cta_dataset *ds, const float *X_dev, const uint32_t *y_dev) {
   const uint32_t N = ds->N, F = ds->F;
   hipStream_t st = ctx->stream;
@@ -912,6 +957,8 @@ void dataset_build(cta_ctx *ctx, cta_dataset *ds, const float *X_dev, const uint
   if (herr & 2u) fail(CTA_EINVAL, "a label is outside [0, K)");
   if (ds->idx_bits == 16) presort<uint16_t>(ctx, ds);
   else presort<uint32_t>(ctx, ds);
+  if (ds->idx_bits == 16) build_codes<uint16_t>(ctx, ds);
+  else build_codes<uint32_t>(ctx, ds);
 }
 
 void check_sizes(int32_t N, int32_t F, int32_t K) {
@@ -1378,6 +1425,8 @@ static void hook_split_scan(cta_ctx *ctx, cta_dataset *ds, const uint32_t *rows,
   a.X_T = ds->XT.as<float>();
   a.y = ds->y.as<LabT>();
   a.si0 = ds->si0.as<IdxT>();
+  a.codes = ds->codes.as<uint8_t>();
+  a.fcode = ds->fcode.as<uint64_t>();
   a.buf0 = d_rows.as<IdxT>();
   a.buf1 = d_rows.as<IdxT>();
   a.N = N;
@@ -1510,6 +1559,8 @@ static void hook_reshuffle(cta_ctx *ctx, cta_dataset *ds, const uint32_t *rows, 
   a.X_T = ds->XT.as<float>();
   a.y = ds->y.as<LabT>();
   a.si0 = ds->si0.as<IdxT>();
+  a.codes = ds->codes.as<uint8_t>();
+  a.fcode = ds->fcode.as<uint64_t>();
   a.buf0 = d_in.as<IdxT>();
   a.buf1 = d_out.as<IdxT>();
   a.bits = d_bits.as<uint32_t>();

@@ -184,6 +184,90 @@ __global__ void k_labels(const uint32_t *y, uint32_t N, uint32_t K, LabT *out, u
   }
 }
 
+// ---- value code columns ------------------------------------------------------
+// For feature f, the presorted row gives the distinct-value index d(s) of
+// every sample (equal floats, +0 and -0 included, share an index).  When
+// every pair of adjacent distinct values is further apart than the
+// candidate gap ((double)a + 1e-5 < (double)b), a split position between
+// two samples is a candidate exactly when their indices differ, and the
+// split scan may read d(s) from a column of 1, 8 or 16 bits per sample
+// instead of the 32-bit value.  One workgroup per feature: flags of value
+// changes along the sorted row, their running sum (the index), scattered to
+// d[f][s]; D[f] = distinct values, bad[f] = some adjacent pair too close.
+constexpr int kCodeThreads = 1024;
+
+template <class IdxT>
+__global__ __launch_bounds__(kCodeThreads) void k_code_index(const float *XT, const IdxT *si0, uint32_t N,
+                                                               uint32_t *d, uint32_t *D, uint32_t *bad) {
+  constexpr int kW = kCodeThreads / 64;
+  __shared__ uint32_t ws[kW];
+  __shared__ uint32_t carry, sbad;
+  const uint32_t f = blockIdx.x, tid = threadIdx.x, lane = tid % 64, wave = tid / 64;
+  const float *x = XT + (size_t)f * N;
+  const IdxT *row = si0 + (size_t)f * N;
+  uint32_t *out = d + (size_t)f * N;
+  if (tid == 0) {
+    carry = 0;
+    sbad = 0;
+  }
+  __syncthreads();
+  uint32_t mybad = 0;
+  for (uint32_t r0 = 0; r0 < N; r0 += kCodeThreads) {
+    const uint32_t r = r0 + tid;
+    uint32_t flag = 0, s = 0;
+    if (r < N) {
+      s = sample_of(row[r]);
+      if (r > 0) {
+        const float v = x[s], u = x[sample_of(row[r - 1])];
+        flag = (u != v) ? 1u : 0u;
+        if (flag && !((double)u + 0.00001 < (double)v)) mybad = 1;
+      }
+    }
+    const uint32_t inc = wave_incl_scan(flag);
+    if (lane == 63) ws[wave] = inc;
+    __syncthreads();
+    uint32_t idx = carry + inc;
+    for (uint32_t w = 0; w < wave; ++w) idx += ws[w];
+    if (r < N) out[s] = idx;
+    __syncthreads();
+    if (tid == 0) {
+      uint32_t sum = 0;
+      for (int w = 0; w < kW; ++w) sum += ws[w];
+      carry += sum;
+    }
+    __syncthreads();
+  }
+  if (mybad) sbad = 1;  // benign race: every writer stores 1
+  __syncthreads();
+  if (tid == 0) {
+    D[f] = carry + 1;
+    bad[f] = sbad;
+  }
+}
+
+// Packs d[f][.] into feature f's code column (kind 1: bits, 2: bytes, 3:
+// 16-bit); fcode[f] = byte offset << 2 | kind (kind 0: no column).
+__global__ void k_code_pack(const uint32_t *d, uint32_t N, const uint64_t *fcode, uint8_t *codes) {
+  const uint32_t f = blockIdx.y;
+  const uint64_t fc = fcode[f];
+  const uint32_t kind = (uint32_t)(fc & 3u);
+  if (kind == 0) return;
+  const uint32_t *src = d + (size_t)f * N;
+  uint8_t *dst = codes + (fc >> 2);
+  const uint32_t n = (kind == 1) ? (N + 31) / 32 : N;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    if (kind == 1) {
+      uint32_t w = 0;
+      for (uint32_t j = 0; j < 32 && 32 * i + j < N; ++j) w |= (src[32 * i + j] & 1u) << j;
+      reinterpret_cast<uint32_t *>(dst)[i] = w;
+    } else if (kind == 2) {
+      dst[i] = (uint8_t)src[i];
+    } else {
+      reinterpret_cast<uint16_t *>(dst)[i] = (uint16_t)src[i];
+    }
+  }
+}
+
 // ---- predict ----------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_predict(const PNode *__restrict__ nodes, const uint32_t *__restrict__ roots,
                                                  uint32_t T, const float *__restrict__ X, uint64_t R, uint32_t F,

@@ -431,6 +431,21 @@ __global__ __launch_bounds__(kBlock) void k_scan(FitArgs<IdxT, LabT> a) {
       const uint32_t f = (uint32_t)a.feats[(size_t)i * a.m + j];
       const IdxT *row = rows_of(a, nd.src, nd.tree, f) + nd.start;
       const float *x = a.X_T + (size_t)f * a.N;
+      // the feature's value code: its distinct-value index from a 1/8/16-bit
+      // column (a candidate exactly where the index changes), or the float
+      // bits (a candidate where the gap test passes)
+      const uint64_t fc = a.fcode[f];
+      const uint32_t kind = uniform((uint32_t)(fc & 3u));
+      const uint8_t *cb = a.codes + (fc >> 2);
+      auto vat = [&](uint32_t s) -> uint32_t {
+        if (kind == 1) return (reinterpret_cast<const uint32_t *>(cb)[s >> 5] >> (s & 31u)) & 1u;
+        if (kind == 2) return cb[s];
+        if (kind == 3) return reinterpret_cast<const uint16_t *>(cb)[s];
+        return __float_as_uint(x[s]);
+      };
+      auto cand = [&](uint32_t v, uint32_t w) -> bool {
+        return kind ? v != w : gap_ok(__uint_as_float(v), __uint_as_float(w));
+      };
       // chunk carry and the running sums at the chunk start; with K <= 64
       // lane k carries class k's count in cv
       wave_sync();
@@ -450,25 +465,24 @@ __global__ __launch_bounds__(kBlock) void k_scan(FitArgs<IdxT, LabT> a) {
       wave_sync();
       Cand lb = {0, 1, 0, (int32_t)f, 0};
       // value just past the chunk end (the next chunk's first sample)
-      const float chunk_next = (p1 < n) ? x[sample_of(row[p1])] : 0.f;
+      const uint32_t chunk_next = (p1 < n) ? vat(sample_of(row[p1])) : 0u;
       for (uint32_t pb = p0; pb < p1; pb += kWave * kScanU) {
         // kScanU tiles in flight: index loads, then label and value gathers
-        uint32_t sid[kScanU], lab[kScanU];
-        float val[kScanU];
+        uint32_t sid[kScanU], lab[kScanU], val[kScanU];
 #pragma unroll
         for (int u = 0; u < kScanU; ++u) {
           const uint32_t p = pb + (uint32_t)u * kWave + lane;
           sid[u] = (p < p1) ? (uint32_t)row[p] : 0xFFFFFFFFu;
         }
         const uint32_t pe = pb + kWave * kScanU;  // first position after the batch
-        const float batch_next = (pe < p1 && lane == kWave - 1) ? x[sample_of(row[pe])] : 0.f;
+        const uint32_t batch_next = (pe < p1 && lane == kWave - 1) ? vat(sample_of(row[pe])) : 0u;
 #pragma unroll
         for (int u = 0; u < kScanU; ++u) {
           lab[u] = 0;
-          val[u] = 0.f;
+          val[u] = 0;
           if (sid[u] != 0xFFFFFFFFu) {
             lab[u] = a.label((IdxT)sid[u]);
-            val[u] = x[sample_of((IdxT)sid[u])];
+            val[u] = vat(sample_of((IdxT)sid[u]));
           }
         }
 #pragma unroll
@@ -515,15 +529,15 @@ __global__ __launch_bounds__(kBlock) void k_scan(FitArgs<IdxT, LabT> a) {
           D += readlane(iD, kWave - 1);
           // value at p + 1: the next lane, the next tile's lane 0, the next
           // batch, or the next chunk
-          float nxt = __uint_as_float(wave_next(__float_as_uint(val[u])));
+          uint32_t nxt = wave_next(val[u]);
           if (u + 1 < kScanU) {
-            const float first_next = __uint_as_float(readlane(__float_as_uint(val[u + 1 < kScanU ? u + 1 : u]), 0));
+            const uint32_t first_next = readlane(val[u + 1 < kScanU ? u + 1 : u], 0);
             if (lane == kWave - 1) nxt = first_next;
           } else if (lane == kWave - 1) {
             nxt = batch_next;
           }
           if (p + 1 == p1) nxt = chunk_next;
-          const bool elig = valid && (p + 1 < n) && gap_ok(val[u], nxt);
+          const bool elig = valid && (p + 1 < n) && cand(val[u], nxt);
           if (elig) {
             const uint64_t nL = (uint64_t)p + 1, nR = (uint64_t)n - nL;
             const uint64_t SR = T2 + SLp - 2 * Dp;

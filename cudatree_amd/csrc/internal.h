@@ -106,6 +106,8 @@ struct FitArgs {
   const float *X_T;   // F x N, feature-major
   const LabT *y;      // N compact labels
   const IdxT *si0;    // F x N presorted rows
+  const uint8_t *codes;    // value code columns of the features that have one
+  const uint64_t *fcode;   // [F] byte offset << 2 | kind (0: read X_T, 1: bits, 2: u8, 3: u16)
   IdxT *buf0, *buf1;  // per tree F x N ping-pong rows
   uint32_t *bits;     // per tree W words: bit s set = sample s goes left
   uint32_t W;         // words per tree bitmap, (N + 31) / 32

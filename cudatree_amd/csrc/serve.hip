@@ -117,7 +117,7 @@ struct TicketState {
 
 struct ForestEntry {
   cta_forest *forest = nullptr;
-  int32_t F = 0, K = 0;
+  int32_t T = 0, F = 0, K = 0;
   uint64_t bytes = 0;
   int refs = 0;
   bool dropped = false;
@@ -155,6 +155,7 @@ struct cta_server {
     cta_ok(cta_forest_info(f, &T, &n, &F, &K), "cta_forest_info");
     ForestEntry e;
     e.forest = f;
+    e.T = T;
     e.F = F;
     e.K = K;
     e.bytes = (uint64_t)n * 16 + (uint64_t)(T + 1) * 4;
@@ -204,6 +205,7 @@ struct cta_server {
     (void)hipSetDevice(o.device);
     (void)hipStreamCreateWithFlags(&stream, hipStreamNonBlocking);
     const int64_t max_rows = o.max_batch_rows > 0 ? o.max_batch_rows : (1 << 20);
+    const int64_t max_work = o.max_batch_work > 0 ? o.max_batch_work : (int64_t)1 << 22;
     for (;;) {
       std::vector<PredReq> batch;
       ForestEntry fe;
@@ -213,9 +215,14 @@ struct cta_server {
         cv_pred.wait(lk, [&] { return stop || !pq.empty(); });
         if (pq.empty()) return;
         fid = pq.front().forest;
+        auto fit = forests.find(fid);
+        const int64_t T = fit != forests.end() ? std::max<int64_t>(1, fit->second.T) : 1;
         int64_t rows = 0;
         for (auto it = pq.begin(); it != pq.end();) {
-          if (it->forest == fid && (batch.empty() || rows + it->R <= max_rows)) {
+          // FIFO within the forest; a batch stays under max_rows rows and
+          // max_work row-tree visits (its first request is always taken)
+          if (it->forest == fid &&
+              (batch.empty() || (rows + it->R <= max_rows && (rows + it->R) * T <= max_work))) {
             rows += it->R;
             batch.push_back(*it);
             it = pq.erase(it);
@@ -350,6 +357,7 @@ void cta_server_opts_default(cta_server_opts *o) {
   o->device = 0;
   o->coalesce = 1;
   o->max_batch_rows = 1 << 20;
+  o->max_batch_work = (int64_t)1 << 22;
   o->hbm_budget = 0;
   o->fit_workers = 1;
 }

@@ -171,7 +171,10 @@ typedef struct {
   int64_t max_batch_rows;  /* rows per launch at most (0: 1,048,576); also the largest request */
   uint64_t hbm_budget;     /* device bytes the server may hold (0: 90% of free memory at creation) */
   int32_t fit_workers;     /* concurrent fit jobs (0: 1) */
-  int32_t reserved[7];
+  int32_t pad0;
+  int64_t max_batch_work;  /* coalesced rows x trees per launch at most (0: 2^22, about 1 ms of
+                              traversal); a single request may exceed it */
+  int32_t reserved[4];
 } cta_server_opts;
 
 typedef struct {

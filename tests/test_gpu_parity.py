@@ -193,6 +193,35 @@ def test_forest_tabular_u32_indices(ctx, orc, idx_bits):
     assert helpers.forest_diff(got, helpers.oracle_forest(case, orc)) is None
 
 
+@pytest.mark.parametrize("idx_bits", [0, 32])
+def test_value_code_columns(ctx, orc, idx_bits):
+    """Features of every value-code kind: binary (bit column), < 256 integer
+    levels (8-bit), thousands of integer levels (16-bit), levels 4e-6 apart
+    (closer than the 1e-5 candidate gap: no column, float gap test),
+    continuous floats, and a feature holding -0.0, +0.0 and 1.0.  Multi-chunk
+    scans read the codes past each chunk and batch boundary."""
+    rng = np.random.default_rng(31)
+    N = 6000
+    signed_zero = np.zeros(N)
+    signed_zero[rng.random(N) < 0.4] = -0.0
+    signed_zero[rng.random(N) < 0.3] = 1.0
+    cols = [rng.integers(0, 2, N).astype(np.float64),
+            rng.integers(0, 200, N).astype(np.float64),
+            rng.integers(0, 60000, N).astype(np.float64),
+            rng.integers(0, 3, N) + np.where(rng.random(N) < 0.5, 0.0, 4e-6),
+            rng.standard_normal(N),
+            signed_zero]
+    X = np.stack(cols, 1).astype(np.float32)
+    assert np.signbit(X[:, 5]).any() and (X[:, 5] == 0).sum() > np.signbit(X[:, 5]).sum()
+    y = ((X[:, 0] > 0).astype(int) + (X[:, 1] > 100) + (X[:, 3] > 1) + (X[:, 5] > 0.5)
+         + (rng.random(N) < 0.1)) % 4
+    case = {"X": X, "y": y, "K": 4, "m": 3, "mss": 1, "max_depth": 0, "identity": False, "seed": 41,
+            "boot": True, "n_trees": 4}
+    print("value codes idx_bits=%d" % idx_bits, flush=True)
+    f, _ = gpu_forest(ctx, case, idx_bits=idx_bits, scan_chunk=512)
+    assert helpers.forest_diff(f.export(), helpers.oracle_forest(case, orc)) is None
+
+
 def test_modes_and_batching_identical(ctx, orc):
     X, y, _, _ = datasets.covtype_like(seed=3, n_train=20000, n_test=1)
     case = {"X": X, "y": y, "K": 7, "m": 5, "mss": 2, "max_depth": 0, "identity": False, "seed": 17,
