@@ -291,10 +291,18 @@ def main():
                              "alg_bytes_per_launch": int(per_launch_bytes)}
         d = per_kernel[dom]
         traffic, tsrc = pmc_traffic(dom)
+        # split scan + reshuffle kernels together (their algorithmic bytes over
+        # their summed time), the figure the >= 50 % of peak target is set on
+        group = [k for k in ("k_scan", "k_scan_hist", "k_resh_big", "k_resh_small", "k_fill") if k in stats]
+        g_bytes = sum(stats[k]["alg_bytes"] for k in group)
+        g_s = sum(stats[k]["ms"] for k in group) * 1e-3
+        combined = {"kernels": group, "achieved": round(g_bytes / g_s / 1e9, 1) if g_s else None,
+                    "frac": round(g_bytes / g_s / 1e9 / HBM_PEAK_GBS, 4) if g_s else None}
         roof = {"bound": "hbm", "kernel": dom, "achieved": d["achieved"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(d["achieved"] / HBM_PEAK_GBS, 4), "traffic": traffic, "traffic_source": tsrc,
                 "launches": d["launches"], "avg_launch_us": d["avg_launch_us"],
-                "alg_bytes_per_launch": d["alg_bytes_per_launch"], "kernels": per_kernel}
+                "alg_bytes_per_launch": d["alg_bytes_per_launch"], "scan_and_reshuffle": combined,
+                "kernels": per_kernel}
 
     # ---- predict: 100,000 held-out rows resident in HBM
     R = Xt.shape[0]

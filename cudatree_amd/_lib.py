@@ -3,9 +3,11 @@
 The library is the only compute path: if it is missing or cannot reach a
 GPU, every call raises.  There is no CPU fallback.
 """
+import atexit
 import ctypes
 import os
 import threading
+import weakref
 
 import numpy as np
 
@@ -153,6 +155,26 @@ class CtaError(RuntimeError):
 _lib = None
 _lock = threading.Lock()
 
+# Open handles, closed at interpreter exit in dependency order (datasets,
+# forests and servers before the contexts they use) while the interpreter
+# and the HIP runtime are still whole.
+_open = {"dataset": weakref.WeakSet(), "forest": weakref.WeakSet(), "server": weakref.WeakSet()}
+
+
+@atexit.register
+def _close_all():
+    for kind in ("server", "forest", "dataset"):
+        for obj in list(_open[kind]):
+            try:
+                obj.close()
+            except Exception:
+                pass
+    for c in list(_ctx.values()):
+        try:
+            c.close()
+        except Exception:
+            pass
+
 
 def load():
     """The loaded library; raises if it has not been built."""
@@ -256,6 +278,7 @@ class Dataset:
         h = _vp()
         check(self.lib.cta_dataset_create(ctx.h, ptr(X, _f32p), ptr(y, _u32p), N, F, K, idx_bits, ctypes.byref(h)))
         self.h = h
+        _open["dataset"].add(self)
         self.N, self.F, self.K = N, F, K
         b = ctypes.c_int32()
         check(self.lib.cta_dataset_info(h, None, None, None, ctypes.byref(b)))
@@ -288,6 +311,7 @@ class Forest:
         self.ctx = ctx
         self.lib = ctx.lib
         self.h = h
+        _open["forest"].add(self)
         T, n, F, K = ctypes.c_int32(), ctypes.c_int64(), ctypes.c_int32(), ctypes.c_int32()
         check(self.lib.cta_forest_info(h, ctypes.byref(T), ctypes.byref(n), ctypes.byref(F), ctypes.byref(K)))
         self.n_trees, self.n_nodes, self.F, self.K = T.value, n.value, F.value, K.value
@@ -406,6 +430,7 @@ class Server:
         h = _vp()
         check_server(self.lib.cta_server_create(ctypes.byref(o), ctypes.byref(h)))
         self.h = h
+        _open["server"].add(self)
         self._keep = {}
         self._lock = threading.Lock()
 

@@ -26,7 +26,7 @@ from . import _lib
 _cache = {}
 
 DEFAULT_GRID = {
-    "scan_chunk": [2048, 8192, 32768],
+    "scan_chunk": [1024, 2048, 8192],
     "resh_chunk": [1024, 2048, 8192],
     "resh_global": [0, 1],
 }

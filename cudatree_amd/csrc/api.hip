@@ -321,7 +321,7 @@ struct Fitter {
     F = d->F;
     K = d->K;
     m = opts.max_features;
-    Cs = opts.scan_chunk > 0 ? (uint32_t)opts.scan_chunk : 8192u;
+    Cs = opts.scan_chunk > 0 ? (uint32_t)opts.scan_chunk : 2048u;
     Cr = opts.resh_chunk > 0 ? (uint32_t)opts.resh_chunk : 2048u;
     Cf = 2048u;
     st = c->stream;
@@ -862,6 +862,8 @@ void fit_impl(cta_ctx *ctx, cta_dataset *ds, int32_t T, const uint32_t *ids, con
   for (int32_t t0 = 0; t0 < T; t0 += per) fz.run_batch(t0, std::min(per, T - t0), T, ids, off, out);
 }
 
+void build_codes(cta_ctx *ctx, cta_dataset *ds, const uint32_t *keys, const uint32_t *ids);
+
 template <class IdxT>
 void presort(cta_ctx *ctx, cta_dataset *ds) {
   const uint32_t N = ds->N, F = ds->F;
@@ -889,20 +891,27 @@ void presort(cta_ctx *ctx, cta_dataset *ds) {
   else
     k_sort_finish<IdxT, uint16_t><<<grid_for(total), 256, 0, st>>>(vin, total, ds->y.as<uint16_t>(),
                                                                    ds->si0.as<IdxT>());
+  build_codes(ctx, ds, kin, vin);
   HIPC(hipGetLastError());
   ctx->sync();
 }
 
 // Value code columns: per feature, the width of its distinct-value index
 // (1, 8 or 16 bits) when adjacent distinct values are all further apart than
-// the candidate gap, else none (the split scan reads X_T).
-template <class IdxT>
-void build_codes(cta_ctx *ctx, cta_dataset *ds) {
+// the candidate gap, else none (the split scan reads X_T).  keys / ids: the
+// presort's sorted keys and sample ids, F x N.
+void build_codes(cta_ctx *ctx, cta_dataset *ds, const uint32_t *keys, const uint32_t *ids) {
   const uint32_t N = ds->N, F = ds->F;
   hipStream_t st = ctx->stream;
+  const uint32_t ntiles = (N + kCodeTile - 1) / kCodeTile;
+  uint32_t *pre = ctx->ws_as<uint32_t>("code_pre", (size_t)N * F);
   uint32_t *d = ctx->ws_as<uint32_t>("code_index", (size_t)N * F);
+  uint32_t *tsum = ctx->ws_as<uint32_t>("code_tsum", (size_t)ntiles * F);
   uint32_t *db = ctx->ws_as<uint32_t>("code_meta", 2 * (size_t)F);
-  k_code_index<IdxT><<<F, kCodeThreads, 0, st>>>(ds->XT.as<float>(), ds->si0.as<IdxT>(), N, d, db, db + F);
+  k_fill_u32<<<grid_for(2 * (size_t)F), 256, 0, st>>>(db, 2 * (size_t)F, 0u);
+  k_code_flags<<<dim3(ntiles, F), kCodeThreads, 0, st>>>(keys, N, ntiles, pre, tsum, db + F);
+  k_code_tscan<<<F, 1024, 0, st>>>(tsum, ntiles, db);
+  k_code_scatter<<<dim3(ntiles, F), kCodeThreads, 0, st>>>(ids, pre, tsum, N, ntiles, d);
   uint32_t *h = static_cast<uint32_t *>(ctx->host("code_meta", 2 * (size_t)F * 4));
   HIPC(hipMemcpyAsync(h, db, 2 * (size_t)F * 4, hipMemcpyDeviceToHost, st));
   ctx->sync();
@@ -936,6 +945,7 @@ void build_codes(cta_ctx *ctx, cta_dataset *ds) {
 }
 
 void dataset_build(cta_ctx *ctx, cta_dataset *ds, const float *X_dev, const uint32_t *y_dev) {
+  const auto t0 = std::chrono::steady_clock::now();
   const uint32_t N = ds->N, F = ds->F;
   hipStream_t st = ctx->stream;
   uint32_t *err = ctx->ws_as<uint32_t>("ds_err", 1);
@@ -957,8 +967,8 @@ void dataset_build(cta_ctx *ctx, cta_dataset *ds, const float *X_dev, const uint
   if (herr & 2u) fail(CTA_EINVAL, "a label is outside [0, K)");
   if (ds->idx_bits == 16) presort<uint16_t>(ctx, ds);
   else presort<uint32_t>(ctx, ds);
-  if (ds->idx_bits == 16) build_codes<uint16_t>(ctx, ds);
-  else build_codes<uint32_t>(ctx, ds);
+  ctx->host_time("host_dataset",
+                 std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count());
 }
 
 void check_sizes(int32_t N, int32_t F, int32_t K) {
@@ -1370,7 +1380,7 @@ static void hook_split_scan(cta_ctx *ctx, cta_dataset *ds, const uint32_t *rows,
                             int32_t m_eval, int32_t chunk, int32_t *bf, uint32_t *bp, uint64_t *num, uint64_t *den) {
   const int32_t N = ds->N, F = ds->F, K = ds->K;
   hipStream_t st = ctx->stream;
-  const uint32_t Cs = chunk > 0 ? (uint32_t)chunk : 8192u;
+  const uint32_t Cs = chunk > 0 ? (uint32_t)chunk : 2048u;
   // node rows into tree 0's buffer 0
   std::vector<IdxT> hrows((size_t)F * N, 0);
   std::vector<uint32_t> counts(K, 0);

@@ -191,57 +191,94 @@ __global__ void k_labels(const uint32_t *y, uint32_t N, uint32_t K, LabT *out, u
 // candidate gap ((double)a + 1e-5 < (double)b), a split position between
 // two samples is a candidate exactly when their indices differ, and the
 // split scan may read d(s) from a column of 1, 8 or 16 bits per sample
-// instead of the 32-bit value.  One workgroup per feature: flags of value
-// changes along the sorted row, their running sum (the index), scattered to
-// d[f][s]; D[f] = distinct values, bad[f] = some adjacent pair too close.
-constexpr int kCodeThreads = 1024;
+// instead of the 32-bit value.  Built from the presort's sorted keys and
+// ids in three passes over tiles of kCodeTile positions: value-change flags
+// and their in-tile prefix, a per-feature scan of the tile totals, and the
+// scatter of d to sample order.  D[f] = distinct values, bad[f] = some
+// adjacent pair closer than the gap.
+constexpr int kCodeThreads = 256, kCodeItems = 4;
+constexpr uint32_t kCodeTile = kCodeThreads * kCodeItems;
 
-template <class IdxT>
-__global__ __launch_bounds__(kCodeThreads) void k_code_index(const float *XT, const IdxT *si0, uint32_t N,
-                                                               uint32_t *d, uint32_t *D, uint32_t *bad) {
-  constexpr int kW = kCodeThreads / 64;
-  __shared__ uint32_t ws[kW];
-  __shared__ uint32_t carry, sbad;
-  const uint32_t f = blockIdx.x, tid = threadIdx.x, lane = tid % 64, wave = tid / 64;
-  const float *x = XT + (size_t)f * N;
-  const IdxT *row = si0 + (size_t)f * N;
-  uint32_t *out = d + (size_t)f * N;
-  if (tid == 0) {
-    carry = 0;
-    sbad = 0;
-  }
-  __syncthreads();
-  uint32_t mybad = 0;
-  for (uint32_t r0 = 0; r0 < N; r0 += kCodeThreads) {
-    const uint32_t r = r0 + tid;
-    uint32_t flag = 0, s = 0;
+__device__ __forceinline__ float key_value(uint32_t k) {
+  return __uint_as_float((k & 0x80000000u) ? (k & 0x7FFFFFFFu) : ~k);
+}
+
+__global__ __launch_bounds__(kCodeThreads) void k_code_flags(const uint32_t *keys, uint32_t N, uint32_t ntiles,
+                                                              uint32_t *pre, uint32_t *tsum, uint32_t *bad) {
+  __shared__ uint32_t ws[kCodeThreads / 64];
+  const uint32_t f = blockIdx.y, tid = threadIdx.x, lane = tid % 64, wave = tid / 64;
+  const uint32_t *k = keys + (size_t)f * N;
+  const uint32_t r0 = blockIdx.x * kCodeTile + tid * kCodeItems;
+  uint32_t fl[kCodeItems], ts = 0, isbad = 0;
+  uint32_t prev = (r0 > 0 && r0 <= N) ? k[r0 - 1] : 0u;
+#pragma unroll
+  for (int j = 0; j < kCodeItems; ++j) {
+    const uint32_t r = r0 + j;
+    fl[j] = 0;
     if (r < N) {
-      s = sample_of(row[r]);
-      if (r > 0) {
-        const float v = x[s], u = x[sample_of(row[r - 1])];
-        flag = (u != v) ? 1u : 0u;
-        if (flag && !((double)u + 0.00001 < (double)v)) mybad = 1;
+      const uint32_t cur = k[r];
+      if (r > 0 && cur != prev) {
+        fl[j] = 1;
+        if (!((double)key_value(prev) + 0.00001 < (double)key_value(cur))) isbad = 1;
       }
+      prev = cur;
     }
-    const uint32_t inc = wave_incl_scan(flag);
+    ts += fl[j];
+  }
+  if (isbad) atomicOr(&bad[f], 1u);
+  const uint32_t inc = wave_incl_scan(ts);
+  if (lane == 63) ws[wave] = inc;
+  __syncthreads();
+  uint32_t run = inc - ts;
+  for (uint32_t w = 0; w < wave; ++w) run += ws[w];
+#pragma unroll
+  for (int j = 0; j < kCodeItems; ++j) {
+    run += fl[j];
+    if (r0 + j < N) pre[(size_t)f * N + r0 + j] = run;  // inclusive, within the tile
+  }
+  if (tid == kCodeThreads - 1) tsum[(size_t)f * ntiles + blockIdx.x] = run;
+}
+
+// Exclusive scan of feature f's tile totals (one workgroup per feature);
+// D[f] = total + 1.
+__global__ __launch_bounds__(1024) void k_code_tscan(uint32_t *tsum, uint32_t ntiles, uint32_t *D) {
+  __shared__ uint32_t ws[16];
+  __shared__ uint32_t carry;
+  const uint32_t f = blockIdx.x, tid = threadIdx.x, lane = tid % 64, wave = tid / 64;
+  uint32_t *t = tsum + (size_t)f * ntiles;
+  if (tid == 0) carry = 0;
+  __syncthreads();
+  for (uint32_t b0 = 0; b0 < ntiles; b0 += 1024) {
+    const uint32_t b = b0 + tid;
+    const uint32_t v = b < ntiles ? t[b] : 0u;
+    const uint32_t inc = wave_incl_scan(v);
     if (lane == 63) ws[wave] = inc;
     __syncthreads();
-    uint32_t idx = carry + inc;
-    for (uint32_t w = 0; w < wave; ++w) idx += ws[w];
-    if (r < N) out[s] = idx;
+    uint32_t off = carry + inc - v;
+    for (uint32_t w = 0; w < wave; ++w) off += ws[w];
+    if (b < ntiles) t[b] = off;
     __syncthreads();
     if (tid == 0) {
       uint32_t sum = 0;
-      for (int w = 0; w < kW; ++w) sum += ws[w];
+      for (int w = 0; w < 16; ++w) sum += ws[w];
       carry += sum;
     }
     __syncthreads();
   }
-  if (mybad) sbad = 1;  // benign race: every writer stores 1
-  __syncthreads();
-  if (tid == 0) {
-    D[f] = carry + 1;
-    bad[f] = sbad;
+  if (tid == 0) D[f] = carry + 1;
+}
+
+// d[f][id of position r] = in-tile prefix + the tile's offset.
+__global__ __launch_bounds__(kCodeThreads) void k_code_scatter(const uint32_t *ids, const uint32_t *pre,
+                                                                const uint32_t *tsum, uint32_t N, uint32_t ntiles,
+                                                                uint32_t *d) {
+  const uint32_t f = blockIdx.y;
+  const uint32_t base = tsum[(size_t)f * ntiles + blockIdx.x];
+  const uint32_t r0 = blockIdx.x * kCodeTile + threadIdx.x * kCodeItems;
+#pragma unroll
+  for (int j = 0; j < kCodeItems; ++j) {
+    const uint32_t r = r0 + j;
+    if (r < N) d[(size_t)f * N + ids[(size_t)f * N + r]] = base + pre[(size_t)f * N + r];
   }
 }
 

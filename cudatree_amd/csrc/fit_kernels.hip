@@ -248,7 +248,10 @@ __global__ __launch_bounds__(kBlock) void k_select(FitArgs<IdxT, LabT> a) {
       a.aux[i] = ax;
       if (i == 0 || a.front[i - 1].tree != nd.tree) a.tlo[nd.tree] = i;
       if (i + 1 == q || a.front[i + 1].tree != nd.tree) a.thi[nd.tree] = i + 1;
-      bytes += (unsigned long long)nf * n * (sizeof(IdxT) + sizeof(LabT) + 4);
+      // class-totals pass n (b + 1) plus per evaluated feature n (b + 1 + 4):
+      // index read, label and value gathers at element size
+      bytes += (unsigned long long)n * (sizeof(IdxT) + sizeof(LabT)) +
+               (unsigned long long)nf * n * (sizeof(IdxT) + sizeof(LabT) + 4);
     }
   }
   flush_bytes(a, kBytesScan, bytes);
@@ -668,7 +671,7 @@ __global__ __launch_bounds__(kBlock) void k_decide(FitArgs<IdxT, LabT> a) {
         a.arena.feat[id] = (int32_t)f;
         a.arena.thr[id] = thr;
         a.arena.value[id] = 0;
-        bytes += (unsigned long long)(p + 1) * sizeof(IdxT) + (unsigned long long)nd.n / 8;
+        bytes += (unsigned long long)nd.n * (sizeof(IdxT) + 1);  // fill: n (b + 1)
       }
       ax.best_f = (int32_t)f;
       ax.best_p = p;
@@ -909,8 +912,8 @@ __global__ __launch_bounds__(kBlock) void k_children(FitArgs<IdxT, LabT> a) {
       ax.resh_ntask = 0;
       if (v[1]) {
         resh_shape(nd.n, (uint32_t)a.F, a.Cr, ax, v);
-        // every row read and written once, the node's side bits read once
-        const unsigned long long b = (unsigned long long)a.F * nd.n * (2 * sizeof(IdxT)) + (nd.n + 7) / 8;
+        // every row read and written, the mark gathered per element: F n (2b + 1)
+        const unsigned long long b = (unsigned long long)a.F * nd.n * (2 * sizeof(IdxT) + 1);
         if (ax.resh_big) bytes += b;
         else bytes_small += b;
       }
@@ -1484,8 +1487,7 @@ __global__ __launch_bounds__(kBlock) void k_boot_tasks(FitArgs<IdxT, LabT> a, co
       a.aux[t] = ax;
       a.tlo[t] = t;
       a.thi[t] = t + 1;
-      bytes += (unsigned long long)a.F * a.N * sizeof(IdxT) + (unsigned long long)a.N / 8 +
-               (unsigned long long)a.F * ax.nL * sizeof(IdxT);
+      bytes += (unsigned long long)a.F * a.N * (sizeof(IdxT) + 1) + (unsigned long long)a.F * ax.nL * sizeof(IdxT);
     }
   }
   flush_bytes(a, kBytesBoot, bytes);
