@@ -302,6 +302,9 @@ void launch_resh(cta_ctx *ctx, const FitArgs<IdxT, LabT> &a, bool global_bits, u
   }
 }
 
+// Batched levels issued between two host reads of the control block.
+constexpr uint32_t kLevelGroup = 4;
+
 template <class IdxT, class LabT>
 struct Fitter {
   cta_ctx *ctx;
@@ -355,9 +358,19 @@ struct Fitter {
     if (boot_gen) {
       // in-bag bitmaps drawn on the device; their counts size the batch
       uint32_t *cnts = ctx->ws_as<uint32_t>("boot_counts", Tb);
-      k_fill_u32<<<grid_for((size_t)Tb * W), 256, 0, st>>>(bits, (size_t)Tb * W, 0u);
       k_fill_u32<<<grid_for(Tb), 256, 0, st>>>(cnts, Tb, 0u);
-      k_boot_gen<<<dim3(grid_for((N + 3) / 4, 256, 256), Tb), 256, 0, st>>>(o.seed, (uint32_t)t0, N, W, bits);
+      if ((size_t)W * 4 <= (size_t)ctx->lds_per_block) {
+        static bool attr = false;
+        if (!attr) {
+          HIPC(hipFuncSetAttribute((const void *)k_boot_gen_lds, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   (int)ctx->lds_per_block));
+          attr = true;
+        }
+        k_boot_gen_lds<<<Tb, 1024, (size_t)W * 4, st>>>(o.seed, (uint32_t)t0, N, W, bits);
+      } else {
+        k_fill_u32<<<grid_for((size_t)Tb * W), 256, 0, st>>>(bits, (size_t)Tb * W, 0u);
+        k_boot_gen<<<dim3(grid_for((N + 3) / 4, 256, 256), Tb), 256, 0, st>>>(o.seed, (uint32_t)t0, N, W, bits);
+      }
       k_bits_count<<<dim3(grid_for(W, 256, 64), Tb), 256, 0, st>>>(bits, W, cnts);
       uint32_t *h = static_cast<uint32_t *>(ctx->host("boot_counts", (size_t)Tb * 4));
       HIPC(hipMemcpyAsync(h, cnts, (size_t)Tb * 4, hipMemcpyDeviceToHost, st));
@@ -526,18 +539,28 @@ struct Fitter {
         std::fprintf(lvl, "level,q,scan_bytes,resh_big_bytes,resh_small_bytes,fill_bytes,k_scan_ms,k_resh_big_ms,"
                           "k_resh_small_ms,k_fill_ms,k_scan_hist_ms,k_select_ms,k_decide_ms,k_children_ms\n");
       int depth = 0;
+      // Levels are issued in groups between host reads of the control block:
+      // a frontier at most doubles per level, so the host sizes launches for
+      // that bound, and levels issued after the frontier empties do nothing.
+      const uint32_t group = lvl ? 1u : kLevelGroup;
       while (q > 0) {
-        a.front = front[cur];
-        a.cnt = cnt[cur];
-        a.next_front = front[cur ^ 1];
-        a.next_cnt = cnt[cur ^ 1];
         std::map<std::string, KStat> before;
         if (lvl) before = ctx->stats;
-        level(a, q);
-        cur ^= 1;
+        const uint32_t d0 = hc.depth;
+        uint32_t qb = q;
+        for (uint32_t g = 0; g < group; ++g) {
+          a.front = front[cur];
+          a.cnt = cnt[cur];
+          a.next_front = front[cur ^ 1];
+          a.next_cnt = cnt[cur ^ 1];
+          level(a, qb);
+          cur ^= 1;
+          qb = (uint32_t)std::min<uint64_t>(2ull * qb, a.cap_q);
+        }
         Ctl prev = hc;
         read_ctl(a, hc);
         account(prev, hc);
+        if (hc.err) fail(CTA_EINTERNAL, "device capacity check failed (err %u)", hc.err);
         if (lvl) {
           auto d = [&](const char *k) { return ctx->stats[k].ms - before[k].ms; };
           std::fprintf(lvl, "%d,%u,%llu,%llu,%llu,%llu,%.4f,%.4f,%.4f,%.4f,%.4f,%.4f,%.4f,%.4f\n", depth, q,
@@ -546,9 +569,13 @@ struct Fitter {
                        d("k_resh_big"), d("k_resh_small"), d("k_fill"), d("k_scan_hist"), d("k_select"), d("k_decide"),
                        d("k_children"));
         }
-        ++depth;
-        if (hc.err) fail(CTA_EINTERNAL, "device capacity check failed (err %u)", hc.err);
-        level_off.push_back(hc.n_nodes);
+        // levels of the group that had a frontier
+        uint32_t qi = q;
+        for (uint32_t d = d0 + 1; d <= hc.depth && qi > 0; ++d) {
+          level_off.push_back(hc.lv_nodes[d % kLevelRing]);
+          qi = hc.lv_q[d % kLevelRing];
+          ++depth;
+        }
         q = hc.q;
       }
       depth_bound = (uint32_t)depth;
@@ -629,15 +656,16 @@ struct Fitter {
   }
 
   // Exclusive offsets and totals of the per-node counts in a.vals (NF
-  // fields, q nodes) into a.offs and tot[0..NF).
+  // fields, ctl->q nodes) into a.offs and tot[0..NF); qb bounds ctl->q.
   template <int NF>
-  void offsets(const FitArgs<IdxT, LabT> &a, uint32_t q, uint32_t *tot) {
-    if (q <= 8 * kOffTile) {
-      k_off_single<NF><<<1, kOffSingleThreads, 0, st>>>(a.vals, q, a.offs, tot);
+  void offsets(const FitArgs<IdxT, LabT> &a, uint32_t qb, uint32_t *tot) {
+    const uint32_t *q = &a.ctl->q;
+    if (qb <= 8 * kOffTile) {
+      k_off_single<NF><<<1, kOffSingleThreads, 0, st>>>(a.vals, q, 1u, a.offs, tot);
     } else {
-      const uint32_t nb = (q + kOffTile - 1) / kOffTile;
+      const uint32_t nb = (qb + kOffTile - 1) / kOffTile;
       k_off_sums<NF><<<nb, kOffThreads, 0, st>>>(a.vals, q, bsum);
-      k_off_single<NF><<<1, kOffSingleThreads, 0, st>>>(bsum, nb, bsum, tot);
+      k_off_single<NF><<<1, kOffSingleThreads, 0, st>>>(bsum, q, kOffTile, bsum, tot);
       k_off_apply<NF><<<nb, kOffThreads, 0, st>>>(a.vals, q, bsum, a.offs);
     }
   }
@@ -683,8 +711,8 @@ struct Fitter {
     a.cap_scan = (uint32_t)cap;
   }
 
-  // One level over the frontier in a.front (q nodes, sorted by tree; count
-  // also in ctl->q).
+  // One level over the frontier in a.front (ctl->q nodes, sorted by tree;
+  // the host knows only a bound q on that count).
   void level(FitArgs<IdxT, LabT> &a, uint32_t q) {
     const size_t sm = smem();
     ensure_units(a, q);

@@ -74,6 +74,14 @@ __device__ __forceinline__ void flush_bytes(const FitArgs<IdxT, LabT> &a, ByteKi
     atomicAdd(&a.bslot[((uint32_t)kind * kByteSlots + blockIdx.x % kByteSlots) * kByteStride], v);
 }
 
+// Distinct-value index of sample s in a value code column of kind 1 (bits),
+// 2 (bytes) or 3 (16-bit).
+__device__ __forceinline__ uint32_t code_at(const uint8_t *cb, uint32_t kind, uint32_t s) {
+  if (kind == 1) return (reinterpret_cast<const uint32_t *>(cb)[s >> 5] >> (s & 31u)) & 1u;
+  if (kind == 2) return cb[s];
+  return reinterpret_cast<const uint16_t *>(cb)[s];
+}
+
 // ------------------------------------------------------------------------
 // Offset scans.  Per-node counts (NF fields, node-major) become exclusive
 // offsets and totals, so every list a level builds (split-scan tasks,
@@ -88,8 +96,10 @@ constexpr int kOffSingleThreads = 1024;
 
 // Per-block sums of each field.
 template <int NF>
-__global__ __launch_bounds__(kOffThreads) void k_off_sums(const uint32_t *v, uint32_t q, uint32_t *bsum) {
+__global__ __launch_bounds__(kOffThreads) void k_off_sums(const uint32_t *v, const uint32_t *qp, uint32_t *bsum) {
   __shared__ uint32_t ws[kOffThreads / kWave][NF];
+  const uint32_t q = *qp;
+  if (blockIdx.x * kOffTile >= q) return;  // grid sized for a bound on q
   const uint32_t tid = threadIdx.x, lane = tid % kWave, wave = tid / kWave;
   const uint32_t i0 = blockIdx.x * kOffTile + tid * kOffItems;
   uint32_t s[NF];
@@ -115,9 +125,10 @@ __global__ __launch_bounds__(kOffThreads) void k_off_sums(const uint32_t *v, uin
 
 // One block: exclusive scan of n items (in place allowed), totals to tot.
 template <int NF>
-__global__ __launch_bounds__(kOffSingleThreads) void k_off_single(const uint32_t *v, uint32_t n, uint32_t *out,
-                                                                 uint32_t *tot) {
+__global__ __launch_bounds__(kOffSingleThreads) void k_off_single(const uint32_t *v, const uint32_t *np,
+                                                                 uint32_t ndiv, uint32_t *out, uint32_t *tot) {
   constexpr int kW = kOffSingleThreads / kWave;
+  const uint32_t n = (*np + ndiv - 1) / ndiv;  // items, or blocks of ndiv items
   __shared__ uint32_t ws[kW][NF];
   __shared__ uint32_t carry[NF];
   const uint32_t tid = threadIdx.x, lane = tid % kWave, wave = tid / kWave;
@@ -164,9 +175,11 @@ __global__ __launch_bounds__(kOffSingleThreads) void k_off_single(const uint32_t
 
 // Exclusive offsets of one block's items given the scanned block sums.
 template <int NF>
-__global__ __launch_bounds__(kOffThreads) void k_off_apply(const uint32_t *v, uint32_t q, const uint32_t *bsum,
-                                                           uint32_t *out) {
+__global__ __launch_bounds__(kOffThreads) void k_off_apply(const uint32_t *v, const uint32_t *qp,
+                                                           const uint32_t *bsum, uint32_t *out) {
   __shared__ uint32_t ws[kOffThreads / kWave][NF];
+  const uint32_t q = *qp;
+  if (blockIdx.x * kOffTile >= q) return;  // grid sized for a bound on q
   const uint32_t tid = threadIdx.x, lane = tid % kWave, wave = tid / kWave;
   const uint32_t i0 = blockIdx.x * kOffTile + tid * kOffItems;
   uint32_t x[kOffItems][NF], ts[NF], inc[NF];
@@ -222,8 +235,17 @@ __global__ __launch_bounds__(kBlock) void k_select(FitArgs<IdxT, LabT> a) {
       if (k < F) {
         f = perm_at(pm, k);
         const IdxT *row = rows_of(a, nd.src, nd.tree, f) + nd.start;
-        const float *x = a.X_T + (size_t)f * a.N;
-        nonconst = x[sample_of(row[0])] != x[sample_of(row[nd.n - 1])];
+        const uint32_t s0 = sample_of(row[0]), s1 = sample_of(row[nd.n - 1]);
+        // constant in the node: first and last sorted values equal (value
+        // codes when the feature has them, else the floats)
+        const uint64_t fc = a.fcode[f];
+        const uint32_t kind = (uint32_t)(fc & 3u);
+        if (kind == 0) {
+          const float *x = a.X_T + (size_t)f * a.N;
+          nonconst = x[s0] != x[s1];
+        } else {
+          nonconst = code_at(a.codes + (fc >> 2), kind, s0) != code_at(a.codes + (fc >> 2), kind, s1);
+        }
       }
       const uint64_t mk = ballot(nonconst);
       const uint32_t r = mbcnt(mk);
@@ -440,12 +462,7 @@ __global__ __launch_bounds__(kBlock) void k_scan(FitArgs<IdxT, LabT> a) {
       const uint64_t fc = a.fcode[f];
       const uint32_t kind = uniform((uint32_t)(fc & 3u));
       const uint8_t *cb = a.codes + (fc >> 2);
-      auto vat = [&](uint32_t s) -> uint32_t {
-        if (kind == 1) return (reinterpret_cast<const uint32_t *>(cb)[s >> 5] >> (s & 31u)) & 1u;
-        if (kind == 2) return cb[s];
-        if (kind == 3) return reinterpret_cast<const uint16_t *>(cb)[s];
-        return __float_as_uint(x[s]);
-      };
+      auto vat = [&](uint32_t s) -> uint32_t { return kind ? code_at(cb, kind, s) : __float_as_uint(x[s]); };
       auto cand = [&](uint32_t v, uint32_t w) -> bool {
         return kind ? v != w : gap_ok(__uint_as_float(v), __uint_as_float(w));
       };
@@ -1412,6 +1429,31 @@ __global__ void k_boot_gen(uint64_t seed, uint32_t tree_base, uint32_t N, uint32
   }
 }
 
+// The same bitmaps built in LDS, one 1,024-thread workgroup per tree: every
+// word is written once, so no clearing and no global atomics (N / 8 bytes
+// must fit the workgroup's LDS).
+__global__ __launch_bounds__(1024) void k_boot_gen_lds(uint64_t seed, uint32_t tree_base, uint32_t N, uint32_t W,
+                                                       uint32_t *bits) {
+  extern __shared__ uint32_t lb[];
+  const uint32_t t = blockIdx.x;
+  for (uint32_t w = threadIdx.x; w < W; w += blockDim.x) lb[w] = 0;
+  __syncthreads();
+  const uint32_t nblk = (N + 3) / 4;
+  for (uint32_t blk = threadIdx.x; blk < nblk; blk += blockDim.x) {
+    uint32_t w[4];
+    boot_block(seed, tree_base + t, blk, w);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      if (blk * 4 + (uint32_t)k >= N) break;
+      const uint32_t s = draw_to_sample(w[k], N);
+      atomicOr(&lb[s >> 5], 1u << (s & 31u));
+    }
+  }
+  __syncthreads();
+  uint32_t *b = bits + (size_t)t * W;
+  for (uint32_t w = threadIdx.x; w < W; w += blockDim.x) b[w] = lb[w];
+}
+
 // Set bits of every tree's bitmap (the in-bag counts).
 __global__ void k_bits_count(const uint32_t *bits, uint32_t W, uint32_t *counts) {
   const uint32_t t = blockIdx.y;
@@ -1659,7 +1701,8 @@ __global__ void k_canon_scatter(uint32_t n, Arena ar, const uint32_t *size, cons
 
 // Moves the next frontier's count into place and clears the level counters.
 // End of a level: the children-scan totals become the next frontier size
-// and the ids in use; the spread byte counters are folded into Ctl.  One
+// and the ids in use (also recorded per level in lv_q / lv_nodes); the spread byte
+// counters are folded into Ctl.  One
 // block of kByteKinds waves, one per counter kind.
 __global__ __launch_bounds__(kByteKinds * kWave) void k_ctl_next(Ctl *ctl, unsigned long long *bslot) {
   const uint32_t tid = threadIdx.x, kind = tid / kWave, slot = tid % kWave;
@@ -1676,6 +1719,11 @@ __global__ __launch_bounds__(kByteKinds * kWave) void k_ctl_next(Ctl *ctl, unsig
     ctl->n_nodes += ctl->c_ids;
     ctl->c_live = 0;
     ctl->c_ids = 0;
+    // per-level record, so the host can read several levels at once
+    const uint32_t d = (ctl->depth + 1) % kLevelRing;
+    ctl->depth += 1;
+    ctl->lv_nodes[d] = ctl->n_nodes;
+    ctl->lv_q[d] = ctl->q;
   }
 }
 

@@ -51,18 +51,23 @@ struct ScanRes {
   uint32_t ok, pad;
 };
 
+// Levels whose frontier size and id count are kept in Ctl.
+constexpr uint32_t kLevelRing = 64;
+
 // Counters of one level, in device memory.  Nothing is allocated with
 // atomics: the totals are written by the offset scans over the frontier.
 struct Ctl {
   uint32_t q;        // frontier size
   uint32_t n_nodes;  // arena ids in use
   uint32_t err;
-  uint32_t pad0;
+  uint32_t depth;      // levels completed (k_ctl_next calls)
   uint32_t n_plan[2];  // reshuffle plan entries: big tasks, small tasks
   uint32_t pad1[2];
   uint32_t n_scan, n_hist, n_fill, pad2;                 // selection-scan totals
   uint32_t c_ids, c_live, c_big, c_small, c_tiles, pad3[3];  // children-scan totals
   unsigned long long bytes_scan, bytes_fill, bytes_resh, bytes_boot, bytes_resh_small;
+  // after level `depth`: frontier size and ids in use, at depth % kLevelRing
+  uint32_t lv_q[kLevelRing], lv_nodes[kLevelRing];
 };
 // Fields of the two per-node offset scans.
 constexpr int kSelFields = 3;  // scan tasks, histogram words, fill slots
