@@ -313,7 +313,8 @@ struct Fitter {
   int32_t N, F, K, m;
   uint32_t Cs, Cf, Cr;
   hipStream_t st;
-  uint32_t G;  // blocks of the wave-loop kernels
+  uint32_t G;      // blocks of the wave-loop kernels
+  uint32_t Gscan;  // blocks of k_scan: all resident at once, a multiple of kXcd
   uint64_t n_tot = 0;      // samples over the batch's trees
   bool fill_lds = false;   // side bitmaps built in LDS (N / 8 bytes fit a workgroup)
   uint32_t Jcap = 1;       // most fill workgroups per tree
@@ -329,6 +330,12 @@ struct Fitter {
     Cf = 2048u;
     st = c->stream;
     G = (uint32_t)std::max(64, ctx->num_cu * 8);
+    // a wave-loop grid larger than what stays resident leaves a tail of
+    // late workgroups; k_scan's registers allow fewer than 8 per CU
+    int nb = 0;
+    HIPC(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_scan<IdxT, LabT>, kBlock, smem()));
+    nb = std::min(std::max(nb, 1), 8);
+    Gscan = std::max<uint32_t>(kXcd, (uint32_t)(ctx->num_cu * nb) / kXcd * kXcd);
   }
 
   size_t fill_lds_bytes() const { return (((size_t)N + 31) / 32) * 4; }
@@ -400,9 +407,10 @@ struct Fitter {
     uint32_t *plan = ctx->ws_as<uint32_t>("plan", (size_t)Tb * 4);
     uint32_t *wg_off = ctx->ws_as<uint32_t>("wg_off", 2 * ((size_t)Tb + 1));
     uint32_t *tlo = ctx->ws_as<uint32_t>("tlo", (size_t)Tb * 2);
-    uint32_t *vals = ctx->ws_as<uint32_t>("scan_vals", cap_q * kChFields);
-    uint32_t *offs = ctx->ws_as<uint32_t>("scan_offs", cap_q * kChFields);
-    bsum = ctx->ws_as<uint32_t>("scan_bsum", (cap_q / kOffTile + 2) * kChFields);
+    constexpr uint64_t kFields = kSelFields > kChFields ? kSelFields : kChFields;
+    uint32_t *vals = ctx->ws_as<uint32_t>("scan_vals", cap_q * kFields);
+    uint32_t *offs = ctx->ws_as<uint32_t>("scan_offs", cap_q * kFields);
+    bsum = ctx->ws_as<uint32_t>("scan_bsum", (cap_q / kOffTile + 2) * kFields);
     const size_t nbslot = kByteKinds * (size_t)kByteSlots * kByteStride;
     unsigned long long *bslot = ctx->ws_as<unsigned long long>("byte_slots", nbslot);
     fill_lds = fill_lds_bytes() <= (size_t)ctx->lds_per_block;
@@ -729,7 +737,7 @@ struct Fitter {
     k_scan_carry<IdxT, LabT><<<G, kBlock, 0, st>>>(a);
     {
       Timed tm(ctx, "k_scan");
-      k_scan<IdxT, LabT><<<G, kBlock, sm, st>>>(a);
+      k_scan<IdxT, LabT><<<Gscan, kBlock, sm, st>>>(a);
     }
     {
       Timed tm(ctx, "k_decide");
@@ -943,7 +951,7 @@ void build_codes(cta_ctx *ctx, cta_dataset *ds, const uint32_t *keys, const uint
   uint32_t *h = static_cast<uint32_t *>(ctx->host("code_meta", 2 * (size_t)F * 4));
   HIPC(hipMemcpyAsync(h, db, 2 * (size_t)F * 4, hipMemcpyDeviceToHost, st));
   ctx->sync();
-  std::vector<uint64_t> fc(F);
+  std::vector<uint64_t> fc(F), col(F);
   uint64_t off = 0;
   for (uint32_t f = 0; f < F; ++f) {
     const uint32_t D = h[f], bad = h[F + f];
@@ -959,8 +967,24 @@ void build_codes(cta_ctx *ctx, cta_dataset *ds, const uint32_t *keys, const uint
       kind = 3;
       bytes = 2 * (uint64_t)N;
     }
-    fc[f] = (off << 2) | kind;
+    fc[f] = off | kind;
     off += (bytes + 255) & ~(uint64_t)255;
+    col[f] = kind ? bytes : 4ull * N;
+  }
+  // Split-scan classes (one per XCD L2): features spread evenly in number,
+  // the largest gathered columns placed first so they land in different L2s.
+  std::vector<uint32_t> order(F);
+  for (uint32_t f = 0; f < F; ++f) order[f] = f;
+  std::stable_sort(order.begin(), order.end(), [&](uint32_t u, uint32_t v) { return col[u] > col[v]; });
+  uint32_t ncls[kXcd] = {};
+  uint64_t foot[kXcd] = {};
+  for (uint32_t f : order) {
+    uint32_t best = 0;
+    for (uint32_t x = 1; x < kXcd; ++x)
+      if (ncls[x] < ncls[best] || (ncls[x] == ncls[best] && foot[x] < foot[best])) best = x;
+    fc[f] |= (uint64_t)best << 2;
+    ++ncls[best];
+    foot[best] += col[f];
   }
   ds->codes.alloc(std::max<uint64_t>(off, 256));
   ds->fcode.alloc((size_t)F * 8);
@@ -1445,6 +1469,7 @@ static void hook_split_scan(cta_ctx *ctx, cta_dataset *ds, const uint32_t *rows,
   Ctl hc = {};
   hc.q = 1;
   hc.n_scan = ntask;
+  hc.n_cls[0] = ntask;  // one task list, class 0
   hc.n_nodes = 1;
   const size_t nbslot = kByteKinds * (size_t)kByteSlots * kByteStride;
   DevMem d_rows(hrows.size() * sizeof(IdxT)), d_front(sizeof(FNode)), d_cnt(K * 4), d_aux(sizeof(NAux)),

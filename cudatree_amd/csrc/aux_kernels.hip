@@ -283,14 +283,14 @@ __global__ __launch_bounds__(kCodeThreads) void k_code_scatter(const uint32_t *i
 }
 
 // Packs d[f][.] into feature f's code column (kind 1: bits, 2: bytes, 3:
-// 16-bit); fcode[f] = byte offset << 2 | kind (kind 0: no column).
+// 16-bit) at the byte offset fcode[f] names (kind 0: no column).
 __global__ void k_code_pack(const uint32_t *d, uint32_t N, const uint64_t *fcode, uint8_t *codes) {
   const uint32_t f = blockIdx.y;
   const uint64_t fc = fcode[f];
-  const uint32_t kind = (uint32_t)(fc & 3u);
+  const uint32_t kind = fc_kind(fc);
   if (kind == 0) return;
   const uint32_t *src = d + (size_t)f * N;
-  uint8_t *dst = codes + (fc >> 2);
+  uint8_t *dst = codes + fc_off(fc);
   const uint32_t n = (kind == 1) ? (N + 31) / 32 : N;
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     if (kind == 1) {

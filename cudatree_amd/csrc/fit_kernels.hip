@@ -227,35 +227,47 @@ __global__ __launch_bounds__(kBlock) void k_select(FitArgs<IdxT, LabT> a) {
     const uint32_t n = nd.n;
     const uint32_t nch = ceil_div(n, a.Cs);
     Perm pm = make_perm(a.seed, a.tree_base + nd.tree, nd.depth, nd.start, F, a.identity != 0);
+    // tasks: one per (feature, chunk) of a multi-chunk node; a single-chunk
+    // node's features grouped about Cs positions per task.  A task's class is
+    // that of its (first) feature; lane c counts the tasks of class c.
+    const uint32_t fpt = (nch > 1) ? 1u : max(1u, a.Cs / n);
+    const uint32_t wt = (nch > 1) ? nch : 1u;
+    uint32_t ncls = 0;
     uint32_t nf = 0;
     for (uint32_t k0 = 0; k0 < F && nf < m; k0 += kWave) {
       const uint32_t k = k0 + lane;
       bool nonconst = false;
       uint32_t f = 0;
+      uint64_t fc = 0;
       if (k < F) {
         f = perm_at(pm, k);
         const IdxT *row = rows_of(a, nd.src, nd.tree, f) + nd.start;
         const uint32_t s0 = sample_of(row[0]), s1 = sample_of(row[nd.n - 1]);
         // constant in the node: first and last sorted values equal (value
         // codes when the feature has them, else the floats)
-        const uint64_t fc = a.fcode[f];
-        const uint32_t kind = (uint32_t)(fc & 3u);
+        fc = a.fcode[f];
+        const uint32_t kind = fc_kind(fc);
         if (kind == 0) {
           const float *x = a.X_T + (size_t)f * a.N;
           nonconst = x[s0] != x[s1];
         } else {
-          nonconst = code_at(a.codes + (fc >> 2), kind, s0) != code_at(a.codes + (fc >> 2), kind, s1);
+          nonconst = code_at(a.codes + fc_off(fc), kind, s0) != code_at(a.codes + fc_off(fc), kind, s1);
         }
       }
       const uint64_t mk = ballot(nonconst);
       const uint32_t r = mbcnt(mk);
-      if (nonconst && nf + r < m) a.feats[(size_t)i * m + nf + r] = (int32_t)f;
+      const bool take = nonconst && nf + r < m;
+      if (take) a.feats[(size_t)i * m + nf + r] = (int32_t)f;
+      const uint32_t c = take && (nf + r) % fpt == 0 ? fc_cls(fc) : kXcd;
+#pragma unroll
+      for (uint32_t x = 0; x < kXcd; ++x) {
+        const uint32_t cnt = (uint32_t)__popcll(ballot(c == x));
+        if (lane == x) ncls += cnt * wt;
+      }
       nf += min((uint32_t)__popcll(mk), m - nf);
     }
-    // tasks: one per (feature, chunk) of a multi-chunk node; a single-chunk
-    // node's features grouped about Cs positions per task
-    const uint32_t fpt = (nch > 1) ? 1u : max(1u, a.Cs / n);
     const uint32_t ntask = (nf == 0) ? 0u : (nch > 1 ? nf * nch : ceil_div(nf, fpt));
+    if (lane < kXcd) a.vals[(size_t)i * kSelFields + 3 + lane] = ncls;
     if (lane == 0) {
       uint32_t *v = a.vals + (size_t)i * kSelFields;
       v[0] = ntask;
@@ -284,6 +296,13 @@ template <class IdxT, class LabT>
 __global__ __launch_bounds__(kBlock) void k_select_emit(FitArgs<IdxT, LabT> a) {
   if (a.ctl->err) return;  // a list overflowed: the fit fails, nothing more is read
   const uint32_t lane = lane_id();
+  // class lists one after another in scan_tasks; lane c < kXcd: start of class c
+  uint32_t cls_base = 0;
+  {
+    const uint32_t cn = lane < kXcd ? a.ctl->n_cls[lane] : 0u;
+    cls_base = wave_incl_scan(cn) - cn;
+  }
+  const uint32_t n_scan = a.ctl->n_scan;
   WAVE_LOOP(a.ctl->q) {
     const uint32_t i = item;
     NAux ax = a.aux[i];
@@ -291,7 +310,7 @@ __global__ __launch_bounds__(kBlock) void k_select_emit(FitArgs<IdxT, LabT> a) {
     const uint32_t *v = a.vals + (size_t)i * kSelFields;
     const uint32_t base = o[0], hbase = o[1], fbase = o[2];
     const uint32_t ntask = ax.scan_ntask, nch = ax.nch_s, fpt = ax.fpt, nf = ax.nf;
-    const bool fits = (uint64_t)base + ntask <= a.cap_scan && (uint64_t)hbase + v[1] <= a.cap_hist &&
+    const bool fits = (uint64_t)n_scan <= a.cap_scan && (uint64_t)hbase + v[1] <= a.cap_hist &&
                       (uint64_t)fbase + v[2] <= a.cap_fill;
     if (!fits) {
       if (lane == 0) {
@@ -302,19 +321,49 @@ __global__ __launch_bounds__(kBlock) void k_select_emit(FitArgs<IdxT, LabT> a) {
       }
       continue;
     }
-    for (uint32_t t = lane; t < ntask; t += kWave) {
-      Task tk;
-      tk.node = i;
-      if (nch > 1) {
-        tk.first_count = (t / nch) | (1u << 16);
-        tk.chunk = t % nch;
-      } else {
-        const uint32_t first = t * fpt;
-        tk.first_count = first | (min(fpt, nf - first) << 16);
-        tk.chunk = 0;
+    // lane c < kXcd: where this node's tasks of class c start
+    uint32_t cpos = lane < kXcd ? cls_base + o[3 + lane] : 0u;
+    // items: features of a multi-chunk node (nch tasks each), else tasks
+    const uint32_t nitem = nch > 1 ? nf : ntask;
+    for (uint32_t c0 = 0; c0 < nitem; c0 += kWave) {
+      const uint32_t it = c0 + lane;
+      const bool valid = it < nitem;
+      const uint32_t c = valid ? fc_cls(a.fcode[a.feats[(size_t)i * a.m + (nch > 1 ? it : it * fpt)]]) : kXcd;
+      uint32_t pos = 0;
+#pragma unroll
+      for (uint32_t x = 0; x < kXcd; ++x) {
+        const uint64_t mk = ballot(c == x);
+        const uint32_t b = readlane(cpos, x);
+        if (c == x) pos = b + mbcnt(mk) * (nch > 1 ? nch : 1u);
+        if (lane == x) cpos += (uint32_t)__popcll(mk) * (nch > 1 ? nch : 1u);
       }
-      tk.nch = nch;
-      a.scan_tasks[base + t] = tk;
+      if (nch == 1) {
+        if (valid) {
+          Task tk;
+          tk.node = i;
+          const uint32_t first = it * fpt;
+          tk.first_count = first | (min(fpt, nf - first) << 16);
+          tk.chunk = 0;
+          tk.nch = 1;
+          a.scan_tasks[pos] = tk;
+        }
+      } else {
+        // the chunk tasks of this group's features, nch in a row per feature
+        const uint32_t cnt = min((uint32_t)kWave, nitem - c0);
+        for (uint32_t k0 = 0; k0 < cnt * nch; k0 += kWave) {
+          const uint32_t k = k0 + lane;
+          const uint32_t jl = min(k / nch, cnt - 1u);
+          const uint32_t p = __shfl(pos, (int)jl);
+          if (k < cnt * nch) {
+            Task tk;
+            tk.node = i;
+            tk.first_count = (c0 + jl) | (1u << 16);
+            tk.chunk = k % nch;
+            tk.nch = nch;
+            a.scan_tasks[p + tk.chunk] = tk;
+          }
+        }
+      }
     }
     if (lane == 0) {
       ax.scan_base = base;
@@ -424,14 +473,21 @@ __global__ __launch_bounds__(kBlock) void k_scan_carry(FitArgs<IdxT, LabT> a) {
 // ------------------------------------------------------------------------
 constexpr int kScanU = 4;
 template <class IdxT, class LabT>
-__global__ __launch_bounds__(kBlock) void k_scan(FitArgs<IdxT, LabT> a) {
+// Six waves per SIMD (a few registers spilled) beat five without spills.
+__global__ __launch_bounds__(kBlock, 6) void k_scan(FitArgs<IdxT, LabT> a) {
   if (a.ctl->err) return;  // a list overflowed: the fit fails, nothing more is read
   extern __shared__ uint32_t smem[];
   const uint32_t lane = lane_id();
   const uint32_t K = (uint32_t)a.K;
   uint32_t *carry = smem + smem_wave_offset(a.K);
   uint32_t *tot = carry + K;
-  WAVE_LOOP(min(a.ctl->n_scan, a.cap_scan)) {
+  // workgroup b takes the task list of class b % kXcd (its XCD's features)
+  const uint32_t cls = blockIdx.x % kXcd;
+  const uint32_t cn = lane < kXcd ? a.ctl->n_cls[lane] : 0u;
+  const uint32_t cbase = wave_incl_scan(cn) - cn;
+  const uint32_t lo = readlane(cbase, cls), hi = min(lo + readlane(cn, cls), a.cap_scan);
+  const uint32_t nwc = (gridDim.x / kXcd) * kWavesPerBlock;
+  for (uint32_t item = lo + (blockIdx.x / kXcd) * kWavesPerBlock + threadIdx.x / kWave; item < hi; item += nwc) {
     const Task tk = a.scan_tasks[item];
     const uint32_t i = tk.node;
     const FNode nd = a.front[i];
@@ -460,12 +516,66 @@ __global__ __launch_bounds__(kBlock) void k_scan(FitArgs<IdxT, LabT> a) {
       // column (a candidate exactly where the index changes), or the float
       // bits (a candidate where the gap test passes)
       const uint64_t fc = a.fcode[f];
-      const uint32_t kind = uniform((uint32_t)(fc & 3u));
-      const uint8_t *cb = a.codes + (fc >> 2);
+      const uint32_t kind = uniform(fc_kind(fc));
+      const uint8_t *cb = a.codes + fc_off(fc);
       auto vat = [&](uint32_t s) -> uint32_t { return kind ? code_at(cb, kind, s) : __float_as_uint(x[s]); };
       auto cand = [&](uint32_t v, uint32_t w) -> bool {
         return kind ? v != w : gap_ok(__uint_as_float(v), __uint_as_float(w));
       };
+      if (kind == 1u && tk.nch == 1 && regs && n > kWave * kScanU) {
+        // Two-valued feature (the node holds both values: the selection
+        // dropped constant features): its one candidate is the position
+        // before the first 1.  Find it by a 64-way search over the row, then
+        // count the labels of the shorter side; the Gini terms are the same
+        // integers the full scan forms at that position.
+        uint32_t lo = 0, hi = n - 1u;  // code 0 at lo, code 1 at hi
+        while (hi - lo > 1u) {
+          const uint32_t span = hi - lo;
+          uint32_t pr = lo + (uint32_t)(((uint64_t)span * (lane + 1u)) / (kWave + 1u));
+          pr = min(max(pr, lo + 1u), hi - 1u);
+          // probes rise with the lane, so the lanes reading 0 come first
+          const uint64_t z0 = ballot(code_at(cb, 1u, sample_of(row[pr])) == 0u);
+          if (z0) lo = readlane(pr, 63u - (uint32_t)__clzll((unsigned long long)z0));
+          if (~z0) hi = readlane(pr, (uint32_t)__ffsll((unsigned long long)~z0) - 1u);
+        }
+        const uint32_t z = hi;  // samples with code 0
+        const bool left = z <= n - z;
+        const uint32_t q0 = left ? 0u : z, q1 = left ? z : n;
+        uint32_t cnt = 0;  // lane k: samples of class k in [q0, q1)
+        for (uint32_t pb = q0; pb < q1; pb += kWave * kScanU) {
+          uint32_t l[kScanU];
+#pragma unroll
+          for (int u = 0; u < kScanU; ++u) {
+            const uint32_t p = pb + (uint32_t)u * kWave + lane;
+            l[u] = (p < q1) ? a.label(row[p]) : 0xFFFFFFFFu;
+          }
+#pragma unroll
+          for (int u = 0; u < kScanU; ++u) {
+            uint64_t rem = ballot(l[u] != 0xFFFFFFFFu);
+            while (rem) {
+              const uint32_t L = readlane(l[u], (uint32_t)__ffsll((unsigned long long)rem) - 1u);
+              const uint64_t mm = ballot(l[u] == L);
+              if (lane == L) cnt += (uint32_t)__popcll(mm);
+              rem &= ~mm;
+            }
+          }
+        }
+        const uint32_t cl = left ? cnt : tv - cnt;  // left-child count of class `lane`
+        const uint64_t SLz = uniform64(wave_sum64((uint64_t)cl * cl));
+        const uint64_t Dz = uniform64(wave_sum64((uint64_t)tv * cl));
+        const uint64_t nL = z, nR = (uint64_t)n - z;
+        const uint64_t SR = T2 + SLz - 2 * Dz;
+        const Cand c = {SLz * nR + SR * nL, nL * nR, z - 1u, (int32_t)f, 1u};
+        if (cand_better(c, best)) best = c;
+        continue;
+      }
+      // Values never decrease along the row, so when the first and the last
+      // value a chunk compares (positions p0 and min(p1, n - 1)) fail the
+      // candidate test, every adjacent pair between them fails it too: such
+      // a chunk (all but one chunk of a two-valued feature) is skipped.
+      const uint32_t pe = min(p1, n - 1u);
+      const uint32_t vpe = (p1 < n || tk.nch > 1) ? vat(sample_of(row[pe])) : 0u;
+      if (tk.nch > 1 && (p0 >= pe || !cand(vat(sample_of(row[p0])), vpe))) continue;
       // chunk carry and the running sums at the chunk start; with K <= 64
       // lane k carries class k's count in cv
       wave_sync();
@@ -485,7 +595,7 @@ __global__ __launch_bounds__(kBlock) void k_scan(FitArgs<IdxT, LabT> a) {
       wave_sync();
       Cand lb = {0, 1, 0, (int32_t)f, 0};
       // value just past the chunk end (the next chunk's first sample)
-      const uint32_t chunk_next = (p1 < n) ? vat(sample_of(row[p1])) : 0u;
+      const uint32_t chunk_next = (p1 < n) ? vpe : 0u;
       for (uint32_t pb = p0; pb < p1; pb += kWave * kScanU) {
         // kScanU tiles in flight: index loads, then label and value gathers
         uint32_t sid[kScanU], lab[kScanU], val[kScanU];
@@ -577,7 +687,9 @@ __global__ __launch_bounds__(kBlock) void k_scan(FitArgs<IdxT, LabT> a) {
       r.feat = best.feat;
       r.ok = best.ok;
       r.pad = 0;
-      a.res[item] = r;
+      // results in node order: the node's task index t as emitted
+      const uint32_t first = tk.first_count & 0xFFFFu;
+      a.res[ax.scan_base + (tk.nch > 1 ? first * tk.nch + tk.chunk : first / ax.fpt)] = r;
     }
   }
 }

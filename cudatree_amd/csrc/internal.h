@@ -53,6 +53,11 @@ struct ScanRes {
 
 // Levels whose frontier size and id count are kept in Ctl.
 constexpr uint32_t kLevelRing = 64;
+// Workgroups are dispatched to the 8 XCDs round-robin (workgroup b on XCD
+// b % 8), each XCD with its own L2.  Split-scan tasks are kept in one list
+// per class, a feature's tasks in the class of that feature, and workgroup b
+// works on class b % 8, so the value gathers of a column hit one L2.
+constexpr uint32_t kXcd = 8;
 
 // Counters of one level, in device memory.  Nothing is allocated with
 // atomics: the totals are written by the offset scans over the frontier.
@@ -63,14 +68,14 @@ struct Ctl {
   uint32_t depth;      // levels completed (k_ctl_next calls)
   uint32_t n_plan[2];  // reshuffle plan entries: big tasks, small tasks
   uint32_t pad1[2];
-  uint32_t n_scan, n_hist, n_fill, pad2;                 // selection-scan totals
+  uint32_t n_scan, n_hist, n_fill, n_cls[kXcd], pad2;    // selection-scan totals
   uint32_t c_ids, c_live, c_big, c_small, c_tiles, pad3[3];  // children-scan totals
   unsigned long long bytes_scan, bytes_fill, bytes_resh, bytes_boot, bytes_resh_small;
   // after level `depth`: frontier size and ids in use, at depth % kLevelRing
   uint32_t lv_q[kLevelRing], lv_nodes[kLevelRing];
 };
 // Fields of the two per-node offset scans.
-constexpr int kSelFields = 3;  // scan tasks, histogram words, fill slots
+constexpr int kSelFields = 3 + kXcd;  // scan tasks, histogram words, fill slots, scan tasks per class
 constexpr int kChFields = 5;   // child ids, next-frontier slots, big / small reshuffle tasks, big tiles
 // Algorithmic-byte counters: per-wave sums flushed into kByteSlots spread
 // slots (one 128-byte line each) and folded into Ctl at the end of a level.
@@ -100,6 +105,13 @@ struct Arena {
 // (N < 2^22, K <= 1,024), so the split scan reads labels with the ids.
 constexpr uint32_t kIdBits = 22, kIdMask = (1u << kIdBits) - 1u;
 
+// fcode[f]: byte offset of feature f's code column (a multiple of 256), its
+// split-scan class (< kXcd) << 2, and the column kind (0: none, read X_T;
+// 1: bits, 2: u8, 3: u16).
+__host__ __device__ __forceinline__ uint32_t fc_kind(uint64_t fc) { return (uint32_t)(fc & 3u); }
+__host__ __device__ __forceinline__ uint32_t fc_cls(uint64_t fc) { return (uint32_t)(fc >> 2) & 7u; }
+__host__ __device__ __forceinline__ uint64_t fc_off(uint64_t fc) { return fc & ~(uint64_t)255; }
+
 template <class IdxT>
 __host__ __device__ __forceinline__ uint32_t sample_of(IdxT v) {
   if constexpr (sizeof(IdxT) == 4) return (uint32_t)v & kIdMask;
@@ -112,7 +124,7 @@ struct FitArgs {
   const LabT *y;      // N compact labels
   const IdxT *si0;    // F x N presorted rows
   const uint8_t *codes;    // value code columns of the features that have one
-  const uint64_t *fcode;   // [F] byte offset << 2 | kind (0: read X_T, 1: bits, 2: u8, 3: u16)
+  const uint64_t *fcode;   // [F] code column byte offset | split-scan class << 2 | kind
   IdxT *buf0, *buf1;  // per tree F x N ping-pong rows
   uint32_t *bits;     // per tree W words: bit s set = sample s goes left
   uint32_t W;         // words per tree bitmap, (N + 31) / 32
