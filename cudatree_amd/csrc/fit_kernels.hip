@@ -448,18 +448,31 @@ __global__ __launch_bounds__(kBlock) void k_scan_carry(FitArgs<IdxT, LabT> a) {
   if (a.ctl->err) return;  // a list overflowed: the fit fails, nothing more is read
   const uint32_t lane = lane_id();
   const uint32_t K = (uint32_t)a.K;
+  constexpr uint32_t kC = 8;  // classes in flight
   WAVE_LOOP(a.ctl->n_scan) {
     const Task tk = a.scan_tasks[item];
     if (tk.nch <= 1 || tk.chunk != 0) continue;
     const NAux ax = a.aux[tk.node];
     const uint32_t j = tk.first_count & 0xFFFFu;
     uint32_t *base = a.hist + ax.hist_base + (size_t)j * tk.nch * K;
-    for (uint32_t k = lane; k < K; k += kWave) {
-      uint32_t run = 0;
-      for (uint32_t c = 0; c < tk.nch; ++c) {
-        uint32_t v = base[(size_t)c * K + k];
-        base[(size_t)c * K + k] = run;
-        run += v;
+    // per class, the chunk histograms become exclusive prefix sums over
+    // the chunks: lanes take 64 chunks at a time, kC classes at once
+    for (uint32_t k0 = 0; k0 < K; k0 += kC) {
+      uint32_t carry[kC];
+#pragma unroll
+      for (uint32_t c = 0; c < kC; ++c) carry[c] = 0;
+      for (uint32_t c0 = 0; c0 < tk.nch; c0 += kWave) {
+        const uint32_t ch = c0 + lane;
+        uint32_t v[kC];
+#pragma unroll
+        for (uint32_t c = 0; c < kC; ++c)
+          v[c] = (ch < tk.nch && k0 + c < K) ? base[(size_t)ch * K + k0 + c] : 0u;
+#pragma unroll
+        for (uint32_t c = 0; c < kC; ++c) {
+          const uint32_t inc = wave_incl_scan(v[c]);
+          if (ch < tk.nch && k0 + c < K) base[(size_t)ch * K + k0 + c] = carry[c] + inc - v[c];
+          carry[c] += readlane(inc, kWave - 1);
+        }
       }
     }
   }
