@@ -1434,38 +1434,60 @@ __device__ void resh_small(const FitArgs<IdxT, LabT> &a, const Task &tk, const S
     }
     return;
   }
-  for (uint32_t f = first; f < end; ++f) {
-    const gptr<const IdxT> in = sgpr_ptr(rows_of(a, nd.src, nd.tree, f) + nd.start);
-    const gptr<IdxT> out = sgpr_ptr(rows_out(a, ax.dst, nd.tree, f) + nd.start);
-    uint32_t carry = 0;
-    for (uint32_t base = 0; base < n; base += kWave * kReshE) {
-      IdxT e[kReshE];
+  // Pieces of kWave * kReshE positions, row by row; the next piece's loads
+  // are issued before this piece's stores (two register sets).
+  constexpr uint32_t kPiece = kWave * kReshE;
+  if (first >= end) return;
+  const size_t rstride = (size_t)a.N;  // consecutive rows of a tree are N entries apart
+  const gptr<const IdxT> in0 = sgpr_ptr(rows_of(a, nd.src, nd.tree, first) + nd.start);
+  const gptr<IdxT> out0 = sgpr_ptr(rows_out(a, ax.dst, nd.tree, first) + nd.start);
+  auto load = [&](IdxT(&e)[kReshE], uint32_t r, uint32_t base) {
+    const gptr<const IdxT> in = in0 + r * rstride;
 #pragma unroll
-      for (int k = 0; k < kReshE; ++k) {
-        const uint32_t p = base + (uint32_t)k * kWave + lane;
-        e[k] = ld32(in, min(p, n - 1));
-      }
-      uint32_t sdm = 0;
-#pragma unroll
-      for (int k = 0; k < kReshE; ++k) {
-        const uint32_t p = base + (uint32_t)k * kWave + lane;
-        if (p < n && side(e[k])) sdm |= 1u << k;
-      }
-#pragma unroll
-      for (int k = 0; k < kReshE; ++k) {
-        const uint32_t p = base + (uint32_t)k * kWave + lane;
-        const uint64_t m = ballot((sdm >> k) & 1u);
-        if (p < n) {
-          const uint32_t lb = carry + mbcnt(m);
-          if ((sdm >> k) & 1u) {
-            if (liveL) st32(out, lb, e[k]);
-          } else if (liveR) {
-            st32(out, nL + (p - lb), e[k]);
-          }
-        }
-        carry += (uint32_t)__popcll(m);
-      }
+    for (int k = 0; k < kReshE; ++k) e[k] = ld32(in, min(base + (uint32_t)k * kWave + lane, n - 1));
+  };
+  uint32_t carry = 0;
+  // moves the piece (row r, base) held in e; loads the next piece into nx first
+  auto step = [&](IdxT(&e)[kReshE], IdxT(&nx)[kReshE], uint32_t &r, uint32_t &base) -> bool {
+    uint32_t r2 = r, b2 = base + kPiece;
+    if (b2 >= n) {
+      b2 = 0;
+      ++r2;
     }
+    const bool more = first + r2 < end;
+    if (more) load(nx, r2, b2);
+    const gptr<IdxT> out = out0 + r * rstride;
+    uint32_t sdm = 0;
+#pragma unroll
+    for (int k = 0; k < kReshE; ++k) {
+      const uint32_t p = base + (uint32_t)k * kWave + lane;
+      if (p < n && side(e[k])) sdm |= 1u << k;
+    }
+#pragma unroll
+    for (int k = 0; k < kReshE; ++k) {
+      const uint32_t p = base + (uint32_t)k * kWave + lane;
+      const uint64_t m = ballot((sdm >> k) & 1u);
+      if (p < n) {
+        const uint32_t lb = carry + mbcnt(m);
+        if ((sdm >> k) & 1u) {
+          if (liveL) st32(out, lb, e[k]);
+        } else if (liveR) {
+          st32(out, nL + (p - lb), e[k]);
+        }
+      }
+      carry += (uint32_t)__popcll(m);
+    }
+    if (r2 != r) carry = 0;
+    r = r2;
+    base = b2;
+    return more;
+  };
+  IdxT A[kReshE], B[kReshE];
+  uint32_t r = 0, base = 0;
+  load(A, 0, 0);
+  for (;;) {
+    if (!step(A, B, r, base)) break;
+    if (!step(B, A, r, base)) break;
   }
 }
 
