@@ -30,6 +30,8 @@ _u64p = ctypes.POINTER(ctypes.c_uint64)
 _f32p = ctypes.POINTER(ctypes.c_float)
 _u8p = ctypes.POINTER(ctypes.c_uint8)
 _vp = ctypes.c_void_p
+# void (*)(void *): the level hook of cta_ctx_set_level_hook
+LEVEL_HOOK = ctypes.CFUNCTYPE(None, ctypes.c_void_p)
 
 
 class FitOpts(ctypes.Structure):
@@ -59,7 +61,8 @@ class ServerOpts(ctypes.Structure):
         ("fit_workers", ctypes.c_int32),
         ("pad0", ctypes.c_int32),
         ("max_batch_work", ctypes.c_int64),
-        ("reserved", ctypes.c_int32 * 4),
+        ("fit_yield_us", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 3),
     ]
 
 
@@ -119,6 +122,7 @@ SIGNATURES = {
     "cta_predict": (ctypes.c_int, [_vp, _vp, _f32p, ctypes.c_int64, ctypes.c_int32, _u32p, _u32p]),
     "cta_predict_device": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int64, ctypes.c_int32, _vp, _vp]),
     "cta_ctx_release_workspace": (ctypes.c_int, [_vp]),
+    "cta_ctx_set_level_hook": (ctypes.c_int, [_vp, LEVEL_HOOK, _vp]),
     "cta_ctx_workspace_bytes": (ctypes.c_int, [_vp, _u64p]),
     "cta_server_opts_default": (None, [ctypes.POINTER(ServerOpts)]),
     "cta_server_last_error": (ctypes.c_char_p, []),
@@ -240,6 +244,12 @@ class Context:
 
     def release_workspace(self):
         check(self.lib.cta_ctx_release_workspace(self.h))
+
+    def set_level_hook(self, fn):
+        """fn() runs on the fitting thread after each group of batched
+        levels (None: no hook)."""
+        self._hook = LEVEL_HOOK(lambda _arg: fn()) if fn is not None else None
+        check(self.lib.cta_ctx_set_level_hook(self.h, self._hook if fn is not None else LEVEL_HOOK(), None))
 
     def kernel_stats(self):
         n = ctypes.c_int32()
@@ -415,7 +425,8 @@ class Server:
     """Serving loop (cta_server): resident forests, coalesced predict
     requests, fit jobs admitted by HBM accounting."""
 
-    def __init__(self, device=0, coalesce=True, max_batch_rows=0, hbm_budget=0, fit_workers=1, max_batch_work=0):
+    def __init__(self, device=0, coalesce=True, max_batch_rows=0, hbm_budget=0, fit_workers=1, max_batch_work=0,
+                 fit_yield_us=0):
         self.lib = load()
         o = ServerOpts()
         self.lib.cta_server_opts_default(ctypes.byref(o))
@@ -427,6 +438,7 @@ class Server:
             o.max_batch_work = max_batch_work
         o.hbm_budget = hbm_budget
         o.fit_workers = fit_workers
+        o.fit_yield_us = fit_yield_us
         h = _vp()
         check_server(self.lib.cta_server_create(ctypes.byref(o), ctypes.byref(h)))
         self.h = h

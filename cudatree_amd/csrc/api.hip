@@ -114,6 +114,8 @@ struct KStat {
 struct cta_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
+  void (*level_hook)(void *) = nullptr;  // called between groups of batched levels
+  void *level_hook_arg = nullptr;
   std::recursive_mutex mu;
   int num_cu = 256;
   size_t lds_per_block = 65536;
@@ -577,6 +579,7 @@ struct Fitter {
                        d("k_resh_big"), d("k_resh_small"), d("k_fill"), d("k_scan_hist"), d("k_select"), d("k_decide"),
                        d("k_children"));
         }
+        if (ctx->level_hook) ctx->level_hook(ctx->level_hook_arg);
         // levels of the group that had a frontier
         uint32_t qi = q;
         for (uint32_t d = d0 + 1; d <= hc.depth && qi > 0; ++d) {
@@ -1366,6 +1369,15 @@ int cta_ctx_release_workspace(cta_ctx *ctx) {
     HIPC(hipSetDevice(ctx->device));
     ctx->sync();
     ctx->ws_release();
+  });
+}
+
+int cta_ctx_set_level_hook(cta_ctx *ctx, void (*fn)(void *), void *arg) {
+  return guarded([&] {
+    if (!ctx) fail(CTA_EINVAL, "NULL context");
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    ctx->level_hook = fn;
+    ctx->level_hook_arg = arg;
   });
 }
 

@@ -130,6 +130,8 @@ struct cta_server {
   std::vector<cta_ctx *> fctx;
   std::mutex mu;
   std::condition_variable cv_pred, cv_fit, cv_done;
+  std::condition_variable cv_idle;  // predict queue empty and no batch on the device
+  bool pred_busy = false;
   std::deque<PredReq> pq;
   std::deque<FitJob> fq;
   std::map<int32_t, ForestEntry> forests;
@@ -238,6 +240,7 @@ struct cta_server {
         }
         f->second.refs++;
         fe = f->second;
+        pred_busy = true;
       }
       int rc = CTA_OK;
       int64_t rows = 0;
@@ -273,11 +276,25 @@ struct cta_server {
       }
       forests[fid].refs--;
       release_forest_locked(fid);
+      pred_busy = false;
+      if (pq.empty()) cv_idle.notify_all();
     }
+  }
+
+  // Runs on a fit worker between groups of levels: queued or running
+  // predict batches get the device first, for at most fit_yield_us.
+  static void yield_to_predict(void *arg) {
+    cta_server *s = static_cast<cta_server *>(arg);
+    const int32_t us = s->o.fit_yield_us;
+    if (us < 0) return;
+    std::unique_lock<std::mutex> lk(s->mu);
+    s->cv_idle.wait_for(lk, std::chrono::microseconds(us ? us : 4000),
+                        [&] { return s->stop || (s->pq.empty() && !s->pred_busy); });
   }
 
   void fit_loop(size_t w) {
     (void)hipSetDevice(o.device);
+    (void)cta_ctx_set_level_hook(fctx[w], &cta_server::yield_to_predict, this);
     for (;;) {
       FitJob job;
       {
@@ -393,6 +410,7 @@ int cta_server_destroy(cta_server *s) {
     }
     s->cv_pred.notify_all();
     s->cv_fit.notify_all();
+    s->cv_idle.notify_all();
     if (s->pthread.joinable()) s->pthread.join();
     for (auto &t : s->fthreads)
       if (t.joinable()) t.join();

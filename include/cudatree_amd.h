@@ -153,6 +153,11 @@ int cta_predict_device(cta_ctx *ctx, cta_forest *forest, const float *X_dev, int
 /* Frees the context's cached device workspaces (they are reused across calls
  * and grow to the largest request); *bytes reports their current size. */
 int cta_ctx_release_workspace(cta_ctx *ctx);
+/* Calls fn(arg) on the fitting thread after each group of levels of a
+ * batched fit on this context (NULL fn: none), e.g. to let other work on the
+ * device go first.  No counterpart in the reference; the serving loop uses it
+ * to give queued predict batches the device between a fit's levels. */
+int cta_ctx_set_level_hook(cta_ctx *ctx, void (*fn)(void *), void *arg);
 int cta_ctx_workspace_bytes(cta_ctx *ctx, uint64_t *bytes);
 
 /* ---- serving loop -------------------------------------------------------
@@ -174,7 +179,9 @@ typedef struct {
   int32_t pad0;
   int64_t max_batch_work;  /* coalesced rows x trees per launch at most (0: 2^22, about 1 ms of
                               traversal); a single request may exceed it */
-  int32_t reserved[4];
+  int32_t fit_yield_us;    /* a fit job waits up to this long after each group of levels while
+                              predict work is queued or running (0: 4,000; negative: never) */
+  int32_t reserved[3];
 } cta_server_opts;
 
 typedef struct {

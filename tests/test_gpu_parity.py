@@ -256,6 +256,25 @@ def test_device_bootstrap_matches_oracle_generator(ctx, orc, tpb):
     assert [int(got["n_samples"][got["tree_off"][t]]) for t in range(5)] == [len(i) for i in ids]
 
 
+def test_level_hook_runs_between_level_groups(ctx, orc):
+    """The context's level hook runs on the fitting thread between groups of
+    levels; the trees do not change."""
+    X, y, _, _ = datasets.covtype_like(seed=21, n_train=6000, n_test=1)
+    case = {"X": X, "y": y, "K": 7, "m": 7, "mss": 1, "max_depth": 0, "identity": False, "seed": 5,
+            "boot": True, "n_trees": 3}
+    calls = []
+    ctx.set_level_hook(lambda: calls.append(1))
+    try:
+        print("level hook", flush=True)
+        f, _ = gpu_forest(ctx, case)
+    finally:
+        ctx.set_level_hook(None)
+    got = f.export()
+    assert helpers.forest_diff(got, helpers.oracle_forest(case, orc)) is None
+    depth = int(got["depth"].max())
+    assert len(calls) >= (depth + 3) // 4  # four levels per group
+
+
 def test_max_depth_and_min_samples_split(ctx, orc):
     X, y, _, _ = datasets.covtype_like(seed=5, n_train=8000, n_test=1)
     for md, mss in [(1, 1), (4, 1), (6, 20), (0, 50)]:
