@@ -1312,6 +1312,11 @@ int cta_forest_import(cta_ctx *ctx, int32_t T, int32_t F, int32_t K, const int64
   });
 }
 
+// Node-table bytes per XCD up to which predict splits the trees over the
+// XCDs (half of a 4 MB L2); larger forests keep the tree-by-tree order, whose
+// waves share each tree's upper nodes.
+constexpr size_t kPredictL2Share = (size_t)2 << 20;
+
 static void predict_dev(cta_ctx *ctx, cta_forest *f, const float *X, int64_t R, uint32_t *labels, uint32_t *votes) {
   upload_forest(f);
   hipStream_t st = ctx->stream;
@@ -1320,9 +1325,17 @@ static void predict_dev(cta_ctx *ctx, cta_forest *f, const float *X, int64_t R, 
   k_fill_u32<<<grid_for(vk), 256, 0, st>>>(v, vk, 0u);
   {
     Timed tm(ctx, "k_predict");
-    k_predict<<<grid_for((size_t)R * f->T, 256, 65535u * 4), 256, 0, st>>>(
-        f->nodes.as<PNode>(), f->roots.as<uint32_t>(), (uint32_t)f->T, X, (uint64_t)R, (uint32_t)f->F,
-        (uint32_t)f->K, v);
+    if (f->T >= (int32_t)kXcd && f->nodes.bytes / kXcd <= kPredictL2Share) {
+      // trees split over the XCDs (each L2 holds its share of the nodes)
+      const uint32_t g = grid_for((size_t)R * f->T, 256, 65535u * 4);
+      k_predict_xcd<<<(g + kXcd - 1) / kXcd * kXcd, 256, 0, st>>>(
+          f->nodes.as<PNode>(), f->roots.as<uint32_t>(), (uint32_t)f->T, X, (uint64_t)R, (uint32_t)f->F,
+          (uint32_t)f->K, v);
+    } else {
+      k_predict<<<grid_for((size_t)R * f->T, 256, 65535u * 4), 256, 0, st>>>(
+          f->nodes.as<PNode>(), f->roots.as<uint32_t>(), (uint32_t)f->T, X, (uint64_t)R, (uint32_t)f->F,
+          (uint32_t)f->K, v);
+    }
   }
   k_vote<<<grid_for(R), 256, 0, st>>>(v, (uint64_t)R, (uint32_t)f->K, labels);
   HIPC(hipGetLastError());

@@ -324,6 +324,33 @@ __global__ __launch_bounds__(256) void k_predict(const PNode *__restrict__ nodes
   }
 }
 
+// The same traversal with the trees split over the XCDs: workgroup b walks
+// the trees t with t % kXcd == b % kXcd, so each XCD's L2 holds its eighth of
+// the node table.  Items (row, tree of the class) run tree-fastest, so the
+// lanes of a wave share a row of X.  Pays when an eighth of the node table
+// fits an L2.  Needs gridDim.x a multiple of kXcd.
+__global__ __launch_bounds__(256) void k_predict_xcd(const PNode *__restrict__ nodes,
+                                                     const uint32_t *__restrict__ roots, uint32_t T,
+                                                     const float *__restrict__ X, uint64_t R, uint32_t F, uint32_t K,
+                                                     uint32_t *__restrict__ votes) {
+  const uint32_t cls = blockIdx.x % kXcd;
+  const uint32_t nt = (T - cls + kXcd - 1) / kXcd;  // trees of this class
+  const uint64_t total = R * nt;
+  const uint64_t step = (uint64_t)(gridDim.x / kXcd) * blockDim.x;
+  for (uint64_t g = (uint64_t)(blockIdx.x / kXcd) * blockDim.x + threadIdx.x; g < total; g += step) {
+    const uint64_t r = g / nt;
+    const uint32_t t = cls + kXcd * (uint32_t)(g % nt);
+    const float *x = X + r * F;
+    uint32_t node = roots[t];
+    PNode nd = nodes[node];
+    while (nd.feat >= 0) {
+      node = (x[nd.feat] < nd.thr) ? node + 1 : nd.right;
+      nd = nodes[node];
+    }
+    atomicAdd(&votes[r * K + nd.value], 1u);
+  }
+}
+
 // Winning label per row: most votes, lowest label on ties (np.argmax of
 // np.bincount, random_forest.py:358).
 __global__ void k_vote(const uint32_t *votes, uint64_t R, uint32_t K, uint32_t *labels) {

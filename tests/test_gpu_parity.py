@@ -286,10 +286,14 @@ def test_max_depth_and_min_samples_split(ctx, orc):
 
 
 # ---- prediction ----------------------------------------------------------------
-def test_predict_matches_oracle_and_import(ctx, orc):
+@pytest.mark.parametrize("n_trees", [7, 13])
+def test_predict_matches_oracle_and_import(ctx, orc, n_trees):
+    """Votes and labels equal the oracle's: tree by tree (7 trees), and with
+    the trees split over the XCDs (13 trees, uneven classes)."""
     X, y, Xt, yt = datasets.covtype_like(seed=11, n_train=10000, n_test=5000)
     case = {"X": X, "y": y, "K": 7, "m": 7, "mss": 1, "max_depth": 0, "identity": False, "seed": 8,
-            "boot": True, "n_trees": 7}
+            "boot": True, "n_trees": n_trees}
+    print("predict n_trees=%d" % n_trees, flush=True)
     f, _ = gpu_forest(ctx, case)
     arrays = f.export()
     want = orc.predict(arrays, Xt, 7, return_votes=True)
