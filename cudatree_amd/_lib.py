@@ -48,7 +48,8 @@ class FitOpts(ctypes.Structure):
         ("trees_per_batch", ctypes.c_int32),
         ("resh_global", ctypes.c_int32),
         ("bootstrap", ctypes.c_int32),
-        ("reserved", ctypes.c_int32 * 3),
+        ("all_rows", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 2),
     ]
 
 

@@ -224,6 +224,11 @@ struct cta_dataset {
   int32_t N = 0, F = 0, K = 0, idx_bits = 32, lab_bits = 8;
   DevMem XT, y, si0;
   DevMem codes, fcode;  // value code columns (see k_code_index)
+  // row-free features (two-valued, other than feature 0): tables of the fits
+  // that keep no rows for them, and identity tables for fits that keep all
+  int32_t Frl = 0, Wsig = 0, Fr = 0;
+  DevMem rl_of, slot_of, row_feat, rl_thr, sig;
+  DevMem ident, rl_none;
 };
 
 // ---- forest --------------------------------------------------------------------
@@ -319,6 +324,10 @@ struct Fitter {
   uint32_t Gscan;  // blocks of k_scan: all resident at once, a multiple of kXcd
   uint64_t n_tot = 0;      // samples over the batch's trees
   bool fill_lds = false;   // side bitmaps built in LDS (N / 8 bytes fit a workgroup)
+  // rows kept: every feature, or all but the row-free ones
+  uint32_t Fr = 0, Frl = 0, Wsig = 0;
+  const int32_t *rl_of_d = nullptr, *slot_of_d = nullptr, *row_feat_d = nullptr;
+  uint64_t rl_cap = 0;  // frontier slots of each row-free count buffer
   uint32_t Jcap = 1;       // most fill workgroups per tree
   uint32_t *part = nullptr, *bsum = nullptr;
 
@@ -332,6 +341,13 @@ struct Fitter {
     Cf = 2048u;
     st = c->stream;
     G = (uint32_t)std::max(64, ctx->num_cu * 8);
+    const bool rowfree = !opts.all_rows && d->Frl > 0;
+    Frl = rowfree ? (uint32_t)d->Frl : 0u;
+    Wsig = rowfree ? (uint32_t)d->Wsig : 0u;
+    Fr = rowfree ? (uint32_t)d->Fr : (uint32_t)F;
+    rl_of_d = (rowfree ? d->rl_of : d->rl_none).as<int32_t>();
+    slot_of_d = (rowfree ? d->slot_of : d->ident).as<int32_t>();
+    row_feat_d = (rowfree ? d->row_feat : d->ident).as<int32_t>();
     // a wave-loop grid larger than what stays resident leaves a tail of
     // late workgroups; k_scan's registers allow fewer than 8 per CU
     int nb = 0;
@@ -346,7 +362,7 @@ struct Fitter {
 
   // Bytes of device memory one resident tree needs (rows, marks, level state).
   size_t bytes_per_tree() const {
-    const size_t rows = 2 * (size_t)F * N * sizeof(IdxT) + N / 8 + 64;
+    const size_t rows = 2 * (size_t)Fr * N * sizeof(IdxT) + N / 8 + 64;
     const size_t per_sample = 2 * sizeof(FNode) + 3 * 4 * (size_t)K + sizeof(NAux) + 4 * (size_t)m +
                               2 * 7 * 4 + (sizeof(Task) + sizeof(ScanRes)) + 2 * sizeof(Task) +
                               2 * kChFields * 4;
@@ -397,15 +413,15 @@ struct Fitter {
     const uint64_t cap_nodes = 2 * n_tot + Tb + 64;
     // split-scan tasks are sized per level
     const uint64_t cap_scan = units_for((uint64_t)Tb);
-    const uint64_t cap_big = (uint64_t)F * (n_tot / kBigN + Tb + 1);
-    const uint64_t cap_small = cap_q + 2ull * F * n_tot / Cr + 64;
+    const uint64_t cap_big = (uint64_t)Fr * (n_tot / kBigN + Tb + 1);
+    const uint64_t cap_small = cap_q + 2ull * Fr * n_tot / Cr + 64;
     const uint64_t cap_fill = cap_q + n_tot / Cf + 64;
     const uint64_t cap_hist = (uint64_t)m * K * (2 * n_tot / Cs + 64);
     if (cap_scan > 0xFFFFFFF0ull || cap_small > 0xFFFFFFF0ull || cap_big > 0xFFFFFFF0ull ||
         cap_nodes > 0xFFFFFFF0ull || cap_hist > 0xFFFFFFF0ull)
       fail(CTA_EUNSUPPORTED, "batch of %d trees too large for 32-bit task indices", Tb);
 
-    IdxT *buf0 = ctx->ws_as<IdxT>("buf0", (size_t)Tb * F * N), *buf1 = ctx->ws_as<IdxT>("buf1", (size_t)Tb * F * N);
+    IdxT *buf0 = ctx->ws_as<IdxT>("buf0", (size_t)Tb * Fr * N), *buf1 = ctx->ws_as<IdxT>("buf1", (size_t)Tb * Fr * N);
     uint32_t *plan = ctx->ws_as<uint32_t>("plan", (size_t)Tb * 4);
     uint32_t *wg_off = ctx->ws_as<uint32_t>("wg_off", 2 * ((size_t)Tb + 1));
     uint32_t *tlo = ctx->ws_as<uint32_t>("tlo", (size_t)Tb * 2);
@@ -445,6 +461,14 @@ struct Fitter {
     a.si0 = ds->si0.as<IdxT>();
     a.codes = ds->codes.as<uint8_t>();
     a.fcode = ds->fcode.as<uint64_t>();
+    a.Frl = Frl;
+    a.Wsig = Wsig;
+    a.Fr = Fr;
+    a.rl_of = rl_of_d;
+    a.slot_of = slot_of_d;
+    a.row_feat = row_feat_d;
+    a.rl_thr = ds->rl_thr.as<float>();
+    a.sig = ds->sig.as<uint32_t>();
     a.buf0 = buf0;
     a.buf1 = buf1;
     a.bits = bits;
@@ -531,6 +555,14 @@ struct Fitter {
     k_roots<IdxT, LabT><<<1, kPlanThreads, 0, st>>>(a, root_cnt, boot ? d_off : nullptr, Tb, src_root);
     k_ctl_next<<<1, kByteKinds * kWave, 0, st>>>(a.ctl, a.bslot);
     int cur = 1;  // front[cur] holds the frontier
+    rl_cap = 0;
+    if (Frl) {
+      uint64_t maxn = boot ? 0 : (uint64_t)N;
+      for (int32_t t = 0; t < Tb && boot; ++t) maxn = std::max<uint64_t>(maxn, off[t + 1] - off[t]);
+      ensure_rl(a, 2ull * Tb, cur);
+      a.front = front[cur];
+      rl_front(a, cur, (uint32_t)Tb, (uint32_t)maxn);
+    }
     read_ctl(a, hc);
     if (hc.err) fail(CTA_EINTERNAL, "device capacity check failed during setup (err %u)", hc.err);
     if (boot) ctx->add_bytes("k_boot_compact", hc.bytes_boot);
@@ -563,6 +595,11 @@ struct Fitter {
           a.cnt = cnt[cur];
           a.next_front = front[cur ^ 1];
           a.next_cnt = cnt[cur ^ 1];
+          if (Frl) {
+            ensure_rl(a, 2ull * qb, cur);
+            a.rlc = rl_buf(cur);
+            a.rlc_next = rl_buf(cur ^ 1);
+          }
           level(a, qb);
           cur ^= 1;
           qb = (uint32_t)std::min<uint64_t>(2ull * qb, a.cap_q);
@@ -610,6 +647,12 @@ struct Fitter {
         a.cnt = cnt[0];
         a.next_front = front[1];
         a.next_cnt = cnt[1];
+        if (Frl) {
+          ensure_rl(a, 2, 0);
+          rl_front(a, 0, 1, e.nd.n);
+          a.rlc = rl_buf(0);
+          a.rlc_next = rl_buf(1);
+        }
         level(a, 1);
         Ctl prev = hc;
         read_ctl(a, hc);
@@ -628,12 +671,24 @@ struct Fitter {
         push_frontier(a, front[0], cnt[0], queue);
         cur = 0;
         uint32_t q = (uint32_t)queue.size();
+        if (Frl) {
+          uint32_t maxn = 0;
+          for (const auto &e : queue) maxn = std::max(maxn, e.nd.n);
+          ensure_rl(a, 2ull * q, cur);
+          a.front = front[cur];
+          rl_front(a, cur, q, maxn);
+        }
         while (q > 0) {
           ++qd;
           a.front = front[cur];
           a.cnt = cnt[cur];
           a.next_front = front[cur ^ 1];
           a.next_cnt = cnt[cur ^ 1];
+          if (Frl) {
+            ensure_rl(a, 2ull * q, cur);
+            a.rlc = rl_buf(cur);
+            a.rlc_next = rl_buf(cur ^ 1);
+          }
           level(a, q);
           cur ^= 1;
           Ctl prev = hc;
@@ -664,6 +719,7 @@ struct Fitter {
     ctx->add_bytes("k_fill", now.bytes_fill - prev.bytes_fill);
     ctx->add_bytes("k_resh_big", now.bytes_resh - prev.bytes_resh);
     ctx->add_bytes("k_resh_small", now.bytes_resh_small - prev.bytes_resh_small);
+    ctx->add_bytes("k_rl_count", now.bytes_rl - prev.bytes_rl);
   }
 
   // Exclusive offsets and totals of the per-node counts in a.vals (NF
@@ -703,6 +759,45 @@ struct Fitter {
       k_fill_u32<<<grid_for((size_t)a.T * a.W), 256, 0, st>>>(a.bits, (size_t)a.T * a.W, 0u);
       k_fill_global<IdxT, LabT><<<G, kBlock, 0, st>>>(a);
     }
+  }
+
+  // Row-free count buffers (frontier slots x Frl x K), one per frontier
+  // parity; grown between levels, the current one (`cur`) keeping its counts.
+  uint32_t *rl_buf(int which) {
+    return static_cast<uint32_t *>(ctx->wspace[which ? "rlc1" : "rlc0"]->p);
+  }
+  void ensure_rl(const FitArgs<IdxT, LabT> &a, uint64_t need, int cur) {
+    if (!Frl) return;
+    need = std::min<uint64_t>(std::max<uint64_t>(need, 1), a.cap_q);
+    if (need <= rl_cap) return;
+    const size_t FK = (size_t)Frl * K;
+    const uint64_t cap = std::min<uint64_t>(a.cap_q, std::max(need, 2 * rl_cap));
+    ctx->sync();
+    for (int w = 0; w < 2; ++w) {
+      auto &slot = ctx->wspace[w ? "rlc1" : "rlc0"];
+      if (!slot) slot.reset(new DevMem());
+      if (slot->p && slot->bytes >= cap * FK * 4) continue;
+      DevMem fresh(cap * FK * 4);
+      if (w == cur && rl_cap && slot->p)
+        HIPC(hipMemcpyAsync(fresh.p, slot->p, rl_cap * FK * 4, hipMemcpyDeviceToDevice, st));
+      ctx->sync();
+      std::swap(slot->p, fresh.p);
+      std::swap(slot->bytes, fresh.bytes);
+    }
+    rl_cap = cap;
+  }
+  // Counts of the frontier in a.front (q <= qb nodes of at most maxn
+  // samples) into buffer `which`, from scratch.
+  void rl_front(FitArgs<IdxT, LabT> a, int which, uint32_t qb, uint32_t maxn) {
+    if (!Frl) return;
+    const size_t FK = (size_t)Frl * K;
+    uint32_t *buf = rl_buf(which);
+    k_fill_u32<<<grid_for((size_t)qb * FK), 256, 0, st>>>(buf, (size_t)qb * FK, 0u);
+    a.rlc = buf;
+    const uint32_t gx = std::min<uint32_t>(1024, std::max<uint32_t>(1, (maxn + Cf * kWavesPerBlock - 1) / (Cf * kWavesPerBlock)));
+    Timed tm(ctx, "k_rl_count");
+    k_rl_front<IdxT, LabT><<<dim3(gx, std::min<uint32_t>(qb, 65535u)), kBlock, (size_t)kWavesPerBlock * kRlCounters * 4,
+                             st>>>(a);
   }
 
   // Split-scan tasks of a level of q nodes: nf * nch per multi-chunk node
@@ -752,6 +847,10 @@ struct Fitter {
       k_children<IdxT, LabT><<<G, kBlock, sm, st>>>(a);
       offsets<kChFields>(a, q, &a.ctl->c_ids);
       k_children_emit<IdxT, LabT><<<G, kBlock, 0, st>>>(a);
+    }
+    if (Frl) {
+      Timed tm(ctx, "k_rl_count");
+      k_rl_children<IdxT, LabT><<<G, kBlock, (size_t)kWavesPerBlock * kRlCounters * 4, st>>>(a);
     }
     reshuffle(a);
     k_ctl_next<<<1, kByteKinds * kWave, 0, st>>>(a.ctl, a.bslot);
@@ -995,6 +1094,49 @@ void build_codes(cta_ctx *ctx, cta_dataset *ds, const uint32_t *keys, const uint
   std::memcpy(hf, fc.data(), (size_t)F * 8);
   HIPC(hipMemcpyAsync(ds->fcode.p, hf, (size_t)F * 8, hipMemcpyHostToDevice, st));
   k_code_pack<<<dim3(grid_for(N, 256, 256), F), 256, 0, st>>>(d, N, ds->fcode.as<uint64_t>(), ds->codes.as<uint8_t>());
+  // Row-free features: the two-valued ones other than feature 0 (which keeps
+  // its row: leaf labels and the children's samples are read from it), as
+  // many as the count kernels' per-wave counters hold (2 x Frl x K).
+  const uint32_t K = ds->K;
+  const uint32_t max_rl = kRlCounters / (2 * K);
+  std::vector<int32_t> tab((size_t)5 * F + 1, -1);  // rl_of, slot_of, ident, none, then row_feat / rl_feat
+  int32_t *rl_of = tab.data(), *slot_of = rl_of + F, *ident = slot_of + F, *none = ident + F;
+  std::vector<int32_t> row_feat, rl_feat;
+  for (uint32_t f = 0; f < F; ++f) {
+    ident[f] = (int32_t)f;
+    if (f != 0 && fc_kind(fc[f]) == 1 && rl_feat.size() < max_rl) {
+      rl_of[f] = (int32_t)rl_feat.size();
+      rl_feat.push_back((int32_t)f);
+    } else {
+      slot_of[f] = (int32_t)row_feat.size();
+      row_feat.push_back((int32_t)f);
+    }
+  }
+  ds->Frl = (int32_t)rl_feat.size();
+  ds->Wsig = (ds->Frl + 31) / 32;
+  ds->Fr = (int32_t)row_feat.size();
+  int32_t *ht = static_cast<int32_t *>(ctx->host("rl_tables", ((size_t)6 * F + 2) * 4));
+  std::memcpy(ht, tab.data(), (size_t)4 * F * 4);
+  std::memcpy(ht + 4 * F, row_feat.data(), row_feat.size() * 4);
+  std::memcpy(ht + 5 * F, rl_feat.data(), rl_feat.size() * 4);
+  auto upload = [&](DevMem &m, const int32_t *h, size_t n) {
+    m.alloc(std::max<size_t>(n, 1) * 4);
+    if (n) HIPC(hipMemcpyAsync(m.p, h, n * 4, hipMemcpyHostToDevice, st));
+  };
+  upload(ds->rl_of, ht, F);
+  upload(ds->slot_of, ht + F, F);
+  upload(ds->ident, ht + 2 * F, F);
+  upload(ds->rl_none, ht + 3 * F, F);
+  upload(ds->row_feat, ht + 4 * F, row_feat.size());
+  DevMem rlf;
+  upload(rlf, ht + 5 * F, rl_feat.size());
+  ds->rl_thr.alloc(std::max<size_t>(ds->Frl, 1) * 4);
+  ds->sig.alloc(std::max<size_t>((size_t)N * ds->Wsig, 1) * 4);
+  if (ds->Frl) {
+    k_rl_thr<<<grid_for(ds->Frl), 256, 0, st>>>(keys, N, rlf.as<int32_t>(), (uint32_t)ds->Frl, ds->rl_thr.as<float>());
+    k_rl_sig<<<grid_for(N), 256, 0, st>>>(ds->codes.as<uint8_t>(), ds->fcode.as<uint64_t>(), rlf.as<int32_t>(),
+                                          (uint32_t)ds->Frl, (uint32_t)ds->Wsig, N, ds->sig.as<uint32_t>());
+  }
   HIPC(hipGetLastError());
   ctx->sync();
 }
@@ -1486,6 +1628,7 @@ static void hook_split_scan(cta_ctx *ctx, cta_dataset *ds, const uint32_t *rows,
   ax.fpt = fpt;
   ax.hist_base = 0;
   ax.best_f = -1;
+  ax.rl_feat = -1;  // no row-free features here
   ax.nf = nf;
   const uint32_t Cf = 2048;
   ax.fill_base = 0;
@@ -1515,6 +1658,10 @@ static void hook_split_scan(cta_ctx *ctx, cta_dataset *ds, const uint32_t *rows,
   a.si0 = ds->si0.as<IdxT>();
   a.codes = ds->codes.as<uint8_t>();
   a.fcode = ds->fcode.as<uint64_t>();
+  a.Fr = (uint32_t)ds->F;  // every feature keeps its row
+  a.slot_of = ds->ident.as<int32_t>();
+  a.row_feat = ds->ident.as<int32_t>();
+  a.rl_of = ds->rl_none.as<int32_t>();
   a.buf0 = d_rows.as<IdxT>();
   a.buf1 = d_rows.as<IdxT>();
   a.N = N;
@@ -1649,6 +1796,10 @@ static void hook_reshuffle(cta_ctx *ctx, cta_dataset *ds, const uint32_t *rows, 
   a.si0 = ds->si0.as<IdxT>();
   a.codes = ds->codes.as<uint8_t>();
   a.fcode = ds->fcode.as<uint64_t>();
+  a.Fr = (uint32_t)ds->F;  // every feature keeps its row
+  a.slot_of = ds->ident.as<int32_t>();
+  a.row_feat = ds->ident.as<int32_t>();
+  a.rl_of = ds->rl_none.as<int32_t>();
   a.buf0 = d_in.as<IdxT>();
   a.buf1 = d_out.as<IdxT>();
   a.bits = d_bits.as<uint32_t>();

@@ -203,6 +203,32 @@ __device__ __forceinline__ float key_value(uint32_t k) {
   return __uint_as_float((k & 0x80000000u) ? (k & 0x7FFFFFFFu) : ~k);
 }
 
+// Row-free features (two-valued, without rows): the threshold between the
+// two values, (lowest + highest) * 0.5f without contraction, the value pair
+// the split scan would meet at the one candidate position.
+__global__ void k_rl_thr(const uint32_t *keys, uint32_t N, const int32_t *rl_feat, uint32_t Frl, float *thr) {
+  const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= Frl) return;
+  const uint32_t *k = keys + (size_t)rl_feat[j] * N;
+  thr[j] = __fmul_rn(__fadd_rn(key_value(k[0]), key_value(k[N - 1])), 0.5f);
+}
+
+// Per-sample signatures: bit j of sig[s] set when sample s holds the higher
+// value of row-free feature j (its value-code bit).
+__global__ void k_rl_sig(const uint8_t *codes, const uint64_t *fcode, const int32_t *rl_feat, uint32_t Frl,
+                         uint32_t Wsig, uint32_t N, uint32_t *sig) {
+  for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < N; s += gridDim.x * blockDim.x) {
+    for (uint32_t w = 0; w < Wsig; ++w) {
+      uint32_t word = 0;
+      for (uint32_t b = 0; b < 32u && w * 32u + b < Frl; ++b) {
+        const uint32_t *col = reinterpret_cast<const uint32_t *>(codes + fc_off(fcode[rl_feat[w * 32u + b]]));
+        word |= ((col[s >> 5] >> (s & 31u)) & 1u) << b;
+      }
+      sig[(size_t)s * Wsig + w] = word;
+    }
+  }
+}
+
 __global__ __launch_bounds__(kCodeThreads) void k_code_flags(const uint32_t *keys, uint32_t N, uint32_t ntiles,
                                                               uint32_t *pre, uint32_t *tsum, uint32_t *bad) {
   __shared__ uint32_t ws[kCodeThreads / 64];

@@ -47,19 +47,29 @@ namespace cta {
 
 __device__ __forceinline__ void wave_sync() { __builtin_amdgcn_wave_barrier(); }
 
+// Rows holding a node's segment: src 0 the presorted rows (one per feature,
+// shared by the trees), 1 / 2 the tree's row slots in buffer 0 / 1.
+template <class IdxT, class LabT>
+__device__ __forceinline__ const IdxT *rows_of_slot(const FitArgs<IdxT, LabT> &a, uint32_t src, uint32_t tree,
+                                                    uint32_t s) {
+  if (src == 0) return a.si0 + (size_t)a.row_feat[s] * a.N;
+  const IdxT *b = (src == 1) ? a.buf0 : a.buf1;
+  return b + ((size_t)tree * a.Fr + s) * a.N;
+}
+// the row of feature f (a feature with rows)
 template <class IdxT, class LabT>
 __device__ __forceinline__ const IdxT *rows_of(const FitArgs<IdxT, LabT> &a, uint32_t src, uint32_t tree,
                                                uint32_t f) {
   if (src == 0) return a.si0 + (size_t)f * a.N;
   const IdxT *b = (src == 1) ? a.buf0 : a.buf1;
-  return b + ((size_t)tree * a.F + f) * a.N;
+  return b + ((size_t)tree * a.Fr + (uint32_t)a.slot_of[f]) * a.N;
 }
 
 template <class IdxT, class LabT>
 __device__ __forceinline__ IdxT *rows_out(const FitArgs<IdxT, LabT> &a, uint32_t dst, uint32_t tree,
-                                          uint32_t f) {
+                                          uint32_t s) {
   IdxT *b = (dst == 1) ? a.buf0 : a.buf1;
-  return b + ((size_t)tree * a.F + f) * a.N;
+  return b + ((size_t)tree * a.Fr + s) * a.N;
 }
 
 __device__ __forceinline__ uint32_t ceil_div(uint32_t a, uint32_t b) { return (a + b - 1) / b; }
@@ -232,40 +242,77 @@ __global__ __launch_bounds__(kBlock) void k_select(FitArgs<IdxT, LabT> a) {
     // that of its (first) feature; lane c counts the tasks of class c.
     const uint32_t fpt = (nch > 1) ? 1u : max(1u, a.Cs / n);
     const uint32_t wt = (nch > 1) ? nch : 1u;
+    const uint32_t K = (uint32_t)a.K;
     uint32_t ncls = 0;
-    uint32_t nf = 0;
-    for (uint32_t k0 = 0; k0 < F && nf < m; k0 += kWave) {
+    uint32_t nf = 0;    // evaluated features with rows (they get scan tasks)
+    uint32_t nsel = 0;  // evaluated features
+    Cand rlb = {0, 1, 0, -1, 0};  // best candidate of the evaluated row-free features
+    for (uint32_t k0 = 0; k0 < F && nsel < m; k0 += kWave) {
       const uint32_t k = k0 + lane;
       bool nonconst = false;
       uint32_t f = 0;
       uint64_t fc = 0;
+      int32_t jr = -1;
+      const uint32_t *oc = nullptr;
       if (k < F) {
         f = perm_at(pm, k);
-        const IdxT *row = rows_of(a, nd.src, nd.tree, f) + nd.start;
-        const uint32_t s0 = sample_of(row[0]), s1 = sample_of(row[nd.n - 1]);
-        // constant in the node: first and last sorted values equal (value
-        // codes when the feature has them, else the floats)
-        fc = a.fcode[f];
-        const uint32_t kind = fc_kind(fc);
-        if (kind == 0) {
-          const float *x = a.X_T + (size_t)f * a.N;
-          nonconst = x[s0] != x[s1];
+        jr = a.Frl ? a.rl_of[f] : -1;
+        if (jr >= 0) {
+          // row-free: constant when no or every sample holds the higher value
+          oc = a.rlc + ((size_t)i * a.Frl + (uint32_t)jr) * K;
+          uint32_t ones = 0;
+          for (uint32_t c = 0; c < K; ++c) ones += oc[c];
+          nonconst = ones != 0u && ones != n;
         } else {
-          nonconst = code_at(a.codes + fc_off(fc), kind, s0) != code_at(a.codes + fc_off(fc), kind, s1);
+          const IdxT *row = rows_of(a, nd.src, nd.tree, f) + nd.start;
+          const uint32_t s0 = sample_of(row[0]), s1 = sample_of(row[nd.n - 1]);
+          // constant in the node: first and last sorted values equal (value
+          // codes when the feature has them, else the floats)
+          fc = a.fcode[f];
+          const uint32_t kind = fc_kind(fc);
+          if (kind == 0) {
+            const float *x = a.X_T + (size_t)f * a.N;
+            nonconst = x[s0] != x[s1];
+          } else {
+            nonconst = code_at(a.codes + fc_off(fc), kind, s0) != code_at(a.codes + fc_off(fc), kind, s1);
+          }
         }
       }
       const uint64_t mk = ballot(nonconst);
       const uint32_t r = mbcnt(mk);
-      const bool take = nonconst && nf + r < m;
-      if (take) a.feats[(size_t)i * m + nf + r] = (int32_t)f;
-      const uint32_t c = take && (nf + r) % fpt == 0 ? fc_cls(fc) : kXcd;
+      const bool take = nonconst && nsel + r < m;
+      // features with rows in visiting order; row-free ones scored here
+      const bool trow = take && jr < 0;
+      const uint64_t mrow = ballot(trow);
+      const uint32_t rr = mbcnt(mrow);
+      if (trow) a.feats[(size_t)i * m + nf + rr] = (int32_t)f;
+      const uint32_t c = trow && (nf + rr) % fpt == 0 ? fc_cls(fc) : kXcd;
 #pragma unroll
       for (uint32_t x = 0; x < kXcd; ++x) {
         const uint32_t cnt = (uint32_t)__popcll(ballot(c == x));
         if (lane == x) ncls += cnt * wt;
       }
-      nf += min((uint32_t)__popcll(mk), m - nf);
+      if (take && jr >= 0) {
+        // its one candidate: the samples holding the lower value go left
+        // (zeros = n - ones), scored with the integers of the full scan
+        uint64_t SL = 0, Dd = 0, T2 = 0;
+        uint32_t ones = 0;
+        for (uint32_t cc = 0; cc < K; ++cc) {
+          const uint64_t t = a.cnt[(size_t)i * K + cc], o = oc[cc], cl = t - o;
+          SL += cl * cl;
+          Dd += t * cl;
+          T2 += t * t;
+          ones += (uint32_t)o;
+        }
+        const uint64_t nL = n - ones, nR = ones;
+        const uint64_t SR = T2 + SL - 2 * Dd;
+        const Cand cd = {SL * nR + SR * nL, nL * nR, (uint32_t)nL - 1u, (int32_t)f, 1u};
+        if (cand_better(cd, rlb)) rlb = cd;
+      }
+      nf += (uint32_t)__popcll(mrow);
+      nsel += min((uint32_t)__popcll(mk), m - nsel);
     }
+    rlb = wave_best(rlb);
     const uint32_t ntask = (nf == 0) ? 0u : (nch > 1 ? nf * nch : ceil_div(nf, fpt));
     if (lane < kXcd) a.vals[(size_t)i * kSelFields + 3 + lane] = ncls;
     if (lane == 0) {
@@ -279,6 +326,10 @@ __global__ __launch_bounds__(kBlock) void k_select(FitArgs<IdxT, LabT> a) {
       ax.fpt = fpt;
       ax.nf = nf;
       ax.best_f = -1;
+      ax.rl_feat = rlb.ok ? rlb.feat : -1;
+      ax.rl_pos = rlb.pos;
+      ax.rl_num = rlb.num;
+      ax.rl_den = rlb.den;
       a.aux[i] = ax;
       if (i == 0 || a.front[i - 1].tree != nd.tree) a.tlo[nd.tree] = i;
       if (i + 1 == q || a.front[i + 1].tree != nd.tree) a.thi[nd.tree] = i + 1;
@@ -786,6 +837,10 @@ __global__ __launch_bounds__(kBlock) void k_decide(FitArgs<IdxT, LabT> a) {
       if (cand_better(c, best)) best = c;
     }
     best = wave_best(best);
+    // the evaluated row-free features' best candidate, under the same order
+    const Cand rlcand = {ax.rl_num, ax.rl_den, ax.rl_pos, ax.rl_feat, (a.Frl && ax.rl_feat >= 0) ? 1u : 0u};
+    const bool rowfree = cand_better(rlcand, best);
+    if (rowfree) best = rlcand;
     const uint32_t id = nd.nid;
     uint32_t nfill = 0;
     if (!best.ok) {
@@ -801,6 +856,26 @@ __global__ __launch_bounds__(kBlock) void k_decide(FitArgs<IdxT, LabT> a) {
       }
       ax.best_f = -1;
       ax.flags = 0;
+    } else if (rowfree) {
+      // lower value left: per-class left counts are totals minus the counts
+      // holding the higher value; the side bits come from the value bits
+      // over the node's whole feature-0 row
+      const uint32_t f = (uint32_t)best.feat, p = best.pos;
+      const uint32_t K = (uint32_t)a.K;
+      const uint32_t *oc = a.rlc + ((size_t)i * a.Frl + (uint32_t)a.rl_of[f]) * K;
+      for (uint32_t k = lane; k < K; k += kWave) a.cntL[(size_t)i * K + k] = a.cnt[(size_t)i * K + k] - oc[k];
+      nfill = ax.nch_f;
+      if (lane == 0) {
+        a.arena.feat[id] = (int32_t)f;
+        a.arena.thr[id] = a.rl_thr[a.rl_of[f]];
+        a.arena.value[id] = 0;
+        bytes += (unsigned long long)nd.n * (sizeof(IdxT) + 1);  // fill: n (b + 1)
+      }
+      ax.best_f = (int32_t)f;
+      ax.best_p = p;
+      ax.nL = p + 1;
+      ax.flags = kSplit;
+      ax.dst = (nd.src == 2) ? 1u : 2u;
     } else {
       const uint32_t f = (uint32_t)best.feat, p = best.pos;
       count_left(a, i, nd, ax, f, p, h);
@@ -821,9 +896,11 @@ __global__ __launch_bounds__(kBlock) void k_decide(FitArgs<IdxT, LabT> a) {
       ax.flags = kSplit;
       ax.dst = (nd.src == 2) ? 1u : 2u;
     }
-    // the node's fill slots: one task per Cf left positions, the rest empty
+    // the node's fill slots, one per Cf positions of its rows: the side
+    // bits come from the first nfill, the children's row-free counts from
+    // all of them; a node that does not split leaves them empty
     for (uint32_t c = lane; c < ax.nch_f; c += kWave)
-      a.fill_tasks[ax.fill_base + c] = (c < nfill) ? Task{i, 0u, c, nfill} : Task{kLeaf, 0u, 0u, 0u};
+      a.fill_tasks[ax.fill_base + c] = (ax.flags & kSplit) ? Task{i, 0u, c, nfill} : Task{kLeaf, 0u, 0u, 0u};
     if (lane == 0) a.aux[i] = ax;
   }
   flush_bytes(a, kBytesFill, bytes);
@@ -847,6 +924,18 @@ __device__ __forceinline__ void fill_task_bits(const FitArgs<IdxT, LabT> &a, con
   const uint32_t lane = lane_id();
   const FNode nd = a.front[tk.node];
   const NAux ax = a.aux[tk.node];
+  const int32_t jr = a.Frl ? a.rl_of[ax.best_f] : -1;
+  if (jr >= 0) {
+    // row-free split: the samples holding the lower value, over feature 0's row
+    const IdxT *row = rows_of(a, nd.src, nd.tree, 0) + nd.start;
+    const uint32_t p0 = tk.chunk * a.Cf, p1 = min(nd.n, p0 + a.Cf);
+    const uint32_t w = (uint32_t)jr >> 5, b = (uint32_t)jr & 31u;
+    for (uint32_t p = p0 + lane; p < p1; p += kWave) {
+      const uint32_t s = sample_of(row[p]);
+      if (!((a.sig[(size_t)s * a.Wsig + w] >> b) & 1u)) atomicOr(&bits[s >> 5], 1u << (s & 31u));
+    }
+    return;
+  }
   const IdxT *row = rows_of(a, nd.src, nd.tree, (uint32_t)ax.best_f) + nd.start;
   const uint32_t p0 = tk.chunk * a.Cf, p1 = min(ax.nL, p0 + a.Cf);
   for (uint32_t pb = p0; pb < p1; pb += 4 * kWave) {
@@ -877,7 +966,7 @@ __global__ __launch_bounds__(kFillThreads) void k_fill_lds(FitArgs<IdxT, LabT> a
   const uint32_t fe = last.fill_base + last.nch_f;
   for (uint32_t k = fb + j * kFillWaves + wave; k < fe; k += J * kFillWaves) {
     const Task tk = a.fill_tasks[k];
-    if (tk.node == kLeaf) continue;
+    if (tk.node == kLeaf || tk.chunk >= tk.nch) continue;
     fill_task_bits(a, tk, lbits);
   }
   __syncthreads();
@@ -904,7 +993,7 @@ __global__ __launch_bounds__(kBlock) void k_fill_global(FitArgs<IdxT, LabT> a) {
   if (a.ctl->err) return;  // a list overflowed: the fit fails, nothing more is read
   WAVE_LOOP(a.ctl->n_fill) {
     const Task tk = a.fill_tasks[item];
-    if (tk.node == kLeaf) continue;
+    if (tk.node == kLeaf || tk.chunk >= tk.nch) continue;
     fill_task_bits(a, tk, a.bits + (size_t)a.front[tk.node].tree * a.W);
   }
 }
@@ -1053,9 +1142,9 @@ __global__ __launch_bounds__(kBlock) void k_children(FitArgs<IdxT, LabT> a) {
       v[1] = (kl == 0 ? 1u : 0u) + (kr == 0 ? 1u : 0u);
       ax.resh_ntask = 0;
       if (v[1]) {
-        resh_shape(nd.n, (uint32_t)a.F, a.Cr, ax, v);
-        // every row read and written, the mark gathered per element: F n (2b + 1)
-        const unsigned long long b = (unsigned long long)a.F * nd.n * (2 * sizeof(IdxT) + 1);
+        resh_shape(nd.n, a.Fr, a.Cr, ax, v);
+        // every row read and written, the mark gathered per element: Fr n (2b + 1)
+        const unsigned long long b = (unsigned long long)a.Fr * nd.n * (2 * sizeof(IdxT) + 1);
         if (ax.resh_big) bytes += b;
         else bytes_small += b;
       }
@@ -1076,7 +1165,7 @@ template <class IdxT, class LabT>
 __device__ void emit_resh_tasks(const FitArgs<IdxT, LabT> &a, uint32_t i, const NAux &ax, uint32_t bbase,
                                 uint32_t sbase) {
   const uint32_t lane = lane_id();
-  const uint32_t F = (uint32_t)a.F;
+  const uint32_t F = a.Fr;  // row slots
   if (ax.resh_ntask == 0) return;
   if (ax.resh_big) {
     for (uint32_t f = lane; f < F; f += kWave) a.big_tasks[bbase + f] = Task{i, f | (1u << 16), 0u, 1u};
@@ -1153,9 +1242,110 @@ __global__ __launch_bounds__(kBlock) void k_children_emit(FitArgs<IdxT, LabT> a)
         const uint32_t l = a.cntL[(size_t)i * K + k];
         a.next_cnt[(size_t)slot * K + k] = c ? a.cnt[(size_t)i * K + k] - l : l;
       }
+      // the child's row-free counts start at zero (k_rl_children adds)
+      for (uint32_t w = lane; w < a.Frl * K; w += kWave) a.rlc_next[(size_t)slot * a.Frl * K + w] = 0u;
       ++slot;
     }
     emit_resh_tasks(a, i, ax, o[2], o[3]);
+  }
+}
+
+// ------------------------------------------------------------------------
+// Row-free counts: per node, per row-free feature j and class k, the
+// samples of class k holding the higher value of j.  A wave counts one
+// chunk of a node's feature-0 row into LDS (one atomic per set signature
+// bit, by side), then adds its non-zero counters to the node's (or
+// children's) counts in global memory.
+// ------------------------------------------------------------------------
+constexpr uint32_t kRlCounters = 1024;  // 2 x Frl x K at most (per-wave LDS counters)
+
+template <class IdxT, class LabT>
+__device__ __forceinline__ void rl_count_chunk(const FitArgs<IdxT, LabT> &a, const FNode &nd, uint32_t p0, uint32_t p1,
+                                               bool by_side, uint32_t *cnt) {
+  const uint32_t lane = lane_id();
+  const uint32_t K = (uint32_t)a.K, FK = a.Frl * K;
+  const IdxT *row = rows_of(a, nd.src, nd.tree, 0) + nd.start;
+  const uint32_t *bits = a.bits + (size_t)nd.tree * a.W;
+  for (uint32_t p = p0 + lane; p < p1; p += kWave) {
+    const IdxT e = row[p];  // (label with the id, or from y)
+    const uint32_t s = sample_of(e), L = a.label(e);
+    // counter half 0: left child (side bit set), 1: right child
+    const uint32_t half = by_side ? 1u - ((bits[s >> 5] >> (s & 31u)) & 1u) : 0u;
+    uint32_t *c = cnt + half * FK + L;
+    for (uint32_t w = 0; w < a.Wsig; ++w) {
+      uint32_t word = a.sig[(size_t)s * a.Wsig + w];
+      while (word) {
+        const uint32_t j = w * 32u + (uint32_t)__builtin_ctz(word);
+        atomicAdd(&c[j * K], 1u);
+        word &= word - 1u;
+      }
+    }
+  }
+}
+
+// Children of every split node: counts over the node's fill slots (Cf
+// positions each), into the children's zeroed entries of rlc_next.
+template <class IdxT, class LabT>
+__global__ __launch_bounds__(kBlock) void k_rl_children(FitArgs<IdxT, LabT> a) {
+  if (a.ctl->err) return;  // a list overflowed: the fit fails, nothing more is read
+  extern __shared__ uint32_t smem[];
+  const uint32_t lane = lane_id();
+  const uint32_t FK = a.Frl * (uint32_t)a.K;
+  uint32_t *cnt = smem + (threadIdx.x / kWave) * kRlCounters;
+  unsigned long long bytes = 0;
+  WAVE_LOOP(min(a.ctl->n_fill, a.cap_fill)) {
+    const Task tk = a.fill_tasks[item];
+    if (tk.node == kLeaf) continue;
+    const NAux ax = a.aux[tk.node];
+    const bool liveL = ax.flags & kLiveL, liveR = ax.flags & kLiveR;
+    if (!liveL && !liveR) continue;
+    const FNode nd = a.front[tk.node];
+    for (uint32_t w = lane; w < 2 * FK; w += kWave) cnt[w] = 0;
+    wave_sync();
+    const uint32_t c0 = tk.chunk * a.Cf, c1 = min(nd.n, (tk.chunk + 1) * a.Cf);
+    rl_count_chunk(a, nd, c0, c1, true, cnt);
+    // row entry, label, side bit and signature words per sample
+    if (lane == 0)
+      bytes += (unsigned long long)(c1 - c0) * (sizeof(IdxT) + (sizeof(IdxT) == 4 ? 0 : sizeof(LabT)) + 1 + 4 * a.Wsig);
+    wave_sync();
+    // child frontier slots: left first when live
+    const uint32_t slot0 = a.offs[(size_t)tk.node * kChFields + 1];
+    for (uint32_t w = lane; w < 2 * FK; w += kWave) {
+      const uint32_t v = cnt[w];
+      const bool right = w >= FK;
+      if (v && (right ? liveR : liveL)) {
+        const uint32_t slot = slot0 + ((right && liveL) ? 1u : 0u);
+        atomicAdd(&a.rlc_next[(size_t)slot * FK + (right ? w - FK : w)], v);
+      }
+    }
+    wave_sync();
+  }
+  flush_bytes(a, kBytesRl, bytes);
+}
+
+// The current frontier's counts from scratch (roots, frontiers set by the
+// host): block (x, i) counts chunks x, x + gridDim.x, ... of node i into
+// rlc, which must be zero.
+template <class IdxT, class LabT>
+__global__ __launch_bounds__(kBlock) void k_rl_front(FitArgs<IdxT, LabT> a) {
+  extern __shared__ uint32_t smem[];
+  const uint32_t lane = lane_id(), wave = threadIdx.x / kWave;
+  const uint32_t FK = a.Frl * (uint32_t)a.K;
+  const uint32_t q = a.ctl->q;
+  uint32_t *cnt = smem + wave * kRlCounters;
+  for (uint32_t i = blockIdx.y; i < q; i += gridDim.y) {
+    const FNode nd = a.front[i];
+    const uint32_t nchunk = ceil_div(nd.n, a.Cf);
+    for (uint32_t c = blockIdx.x * kWavesPerBlock + wave; c < nchunk; c += gridDim.x * kWavesPerBlock) {
+      for (uint32_t w = lane; w < FK; w += kWave) cnt[w] = 0;
+      wave_sync();
+      rl_count_chunk(a, nd, c * a.Cf, min(nd.n, (c + 1) * a.Cf), false, cnt);
+      wave_sync();
+      uint32_t *dst = const_cast<uint32_t *>(a.rlc) + (size_t)i * FK;
+      for (uint32_t w = lane; w < FK; w += kWave)
+        if (cnt[w]) atomicAdd(&dst[w], cnt[w]);
+      wave_sync();
+    }
   }
 }
 
@@ -1303,7 +1493,7 @@ __device__ void resh_big(const FitArgs<IdxT, LabT> &a, const Task &tk, const Sid
   const FNode nd = a.front[node];
   const NAux ax = a.aux[node];
   const uint32_t f = uniform(tk.first_count & 0xFFFFu);
-  const gptr<const IdxT> in = sgpr_ptr(rows_of(a, nd.src, nd.tree, f) + nd.start);
+  const gptr<const IdxT> in = sgpr_ptr(rows_of_slot(a, nd.src, nd.tree, f) + nd.start);
   const gptr<IdxT> out = sgpr_ptr(rows_out(a, ax.dst, nd.tree, f) + nd.start);
   const bool liveL = uniform(ax.flags) & kLiveL, liveR = uniform(ax.flags) & kLiveR;
   const uint32_t n = uniform(nd.n), nL = uniform(ax.nL);
@@ -1383,7 +1573,6 @@ __device__ void resh_small(const FitArgs<IdxT, LabT> &a, const Task &tk, const S
   if (c <= (uint32_t)kReshE / 2) {
     const uint32_t R = (uint32_t)kReshE / c;  // rows per batch
     for (uint32_t f0 = first; f0 < end; f0 += R) {
-      const gptr<const IdxT> in = sgpr_ptr(rows_of(a, nd.src, nd.tree, f0) + nd.start);
       const gptr<IdxT> out = sgpr_ptr(rows_out(a, ax.dst, nd.tree, f0) + nd.start);
       const uint32_t rows = min(R, end - f0);
       IdxT e[kReshE];
@@ -1391,7 +1580,9 @@ __device__ void resh_small(const FitArgs<IdxT, LabT> &a, const Task &tk, const S
 #pragma unroll
       for (int k = 0; k < kReshE; ++k) {
         const uint32_t p = pc * kWave + lane;
-        e[k] = ld32(in, min(r, rows - 1) * N + min(p, n - 1));  // out-of-range pieces are masked
+        // source rows by slot (the presorted rows are one per feature)
+        const gptr<const IdxT> in = sgpr_ptr(rows_of_slot(a, nd.src, nd.tree, f0 + min(r, rows - 1)) + nd.start);
+        e[k] = ld32(in, min(p, n - 1));  // out-of-range pieces are masked
         if (++pc == c) {
           pc = 0;
           ++r;
@@ -1438,11 +1629,10 @@ __device__ void resh_small(const FitArgs<IdxT, LabT> &a, const Task &tk, const S
   // are issued before this piece's stores (two register sets).
   constexpr uint32_t kPiece = kWave * kReshE;
   if (first >= end) return;
-  const size_t rstride = (size_t)a.N;  // consecutive rows of a tree are N entries apart
-  const gptr<const IdxT> in0 = sgpr_ptr(rows_of(a, nd.src, nd.tree, first) + nd.start);
+  const size_t rstride = (size_t)a.N;  // consecutive row slots of a tree are N entries apart
   const gptr<IdxT> out0 = sgpr_ptr(rows_out(a, ax.dst, nd.tree, first) + nd.start);
   auto load = [&](IdxT(&e)[kReshE], uint32_t r, uint32_t base) {
-    const gptr<const IdxT> in = in0 + r * rstride;
+    const gptr<const IdxT> in = sgpr_ptr(rows_of_slot(a, nd.src, nd.tree, first + r) + nd.start);
 #pragma unroll
     for (int k = 0; k < kReshE; ++k) e[k] = ld32(in, min(base + (uint32_t)k * kWave + lane, n - 1));
   };
@@ -1669,14 +1859,14 @@ __global__ __launch_bounds__(kBlock) void k_boot_tasks(FitArgs<IdxT, LabT> a, co
       ax.flags = kSplit | kLiveL;
       ax.dst = 1;
       uint32_t v[kChFields] = {0u, 0u, 0u, 0u, 0u};
-      resh_shape((uint32_t)a.N, (uint32_t)a.F, a.Cr, ax, v);
+      resh_shape((uint32_t)a.N, a.Fr, a.Cr, ax, v);
       uint32_t *o = a.vals + (size_t)t * kChFields;
 #pragma unroll
       for (int f = 0; f < kChFields; ++f) o[f] = v[f];
       a.aux[t] = ax;
       a.tlo[t] = t;
       a.thi[t] = t + 1;
-      bytes += (unsigned long long)a.F * a.N * (sizeof(IdxT) + 1) + (unsigned long long)a.F * ax.nL * sizeof(IdxT);
+      bytes += (unsigned long long)a.Fr * a.N * (sizeof(IdxT) + 1) + (unsigned long long)a.Fr * ax.nL * sizeof(IdxT);
     }
   }
   flush_bytes(a, kBytesBoot, bytes);

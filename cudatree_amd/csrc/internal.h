@@ -33,8 +33,11 @@ struct NAux {
   uint32_t resh_big;                           // 1: rows reshuffled by whole workgroups
   uint32_t resh_ntask, rpt;                    // reshuffle tasks of the node, rows per small task
   uint32_t lab[2];                             // labels of leaf children (kLeaf: live child)
-  uint32_t nf;                                 // evaluated features
-  uint32_t pad[2];
+  uint32_t nf;                                 // evaluated features that have rows (scan tasks)
+  uint32_t rl_pos;                             // best split of the evaluated row-free features:
+  uint64_t rl_num, rl_den;                     //   position (zeros - 1), Q numerator / denominator,
+  int32_t rl_feat;                             //   feature (-1: none)
+  uint32_t pad;
 };
 constexpr uint32_t kSplit = 1u, kLiveL = 2u, kLiveR = 4u;
 
@@ -70,7 +73,7 @@ struct Ctl {
   uint32_t pad1[2];
   uint32_t n_scan, n_hist, n_fill, n_cls[kXcd], pad2;    // selection-scan totals
   uint32_t c_ids, c_live, c_big, c_small, c_tiles, pad3[3];  // children-scan totals
-  unsigned long long bytes_scan, bytes_fill, bytes_resh, bytes_boot, bytes_resh_small;
+  unsigned long long bytes_scan, bytes_fill, bytes_resh, bytes_boot, bytes_resh_small, bytes_rl;
   // after level `depth`: frontier size and ids in use, at depth % kLevelRing
   uint32_t lv_q[kLevelRing], lv_nodes[kLevelRing];
 };
@@ -80,8 +83,8 @@ constexpr int kChFields = 5;   // child ids, next-frontier slots, big / small re
 // Algorithmic-byte counters: per-wave sums flushed into kByteSlots spread
 // slots (one 128-byte line each) and folded into Ctl at the end of a level.
 constexpr uint32_t kByteSlots = 64, kByteStride = 16;
-enum ByteKind { kBytesScan = 0, kBytesFill = 1, kBytesResh = 2, kBytesBoot = 3, kBytesReshSmall = 4 };
-constexpr uint32_t kByteKinds = 5;
+enum ByteKind { kBytesScan = 0, kBytesFill = 1, kBytesResh = 2, kBytesBoot = 3, kBytesReshSmall = 4, kBytesRl = 5 };
+constexpr uint32_t kByteKinds = 6;
 constexpr uint32_t kErrCapacity = 1u, kErrBadNode = 2u;
 
 // Flattened forest node for prediction: pre-order, so a split node's left
@@ -125,7 +128,22 @@ struct FitArgs {
   const IdxT *si0;    // F x N presorted rows
   const uint8_t *codes;    // value code columns of the features that have one
   const uint64_t *fcode;   // [F] code column byte offset | split-scan class << 2 | kind
-  IdxT *buf0, *buf1;  // per tree F x N ping-pong rows
+  // Row-free features: two-valued features other than feature 0 keep no
+  // rows.  Their one candidate split, their constancy and their side bits
+  // follow from per-node, per-class counts of the samples holding the
+  // higher value (rlc), which a pass over each split node's samples
+  // (through its feature-0 row and the per-sample bit signatures) produces
+  // for the children.
+  uint32_t Frl, Wsig;       // row-free features, 32-bit signature words per sample
+  uint32_t Fr;              // features with rows
+  const int32_t *rl_of;     // [F] row-free index of each feature, or -1
+  const int32_t *slot_of;   // [F] row slot of each feature, or -1
+  const int32_t *row_feat;  // [Fr] feature of each row slot
+  const float *rl_thr;      // [Frl] threshold between the two values
+  const uint32_t *sig;      // [N x Wsig] bit j: sample holds the higher value of row-free feature j
+  const uint32_t *rlc;      // frontier: [q x Frl x K] samples of class k holding the higher value
+  uint32_t *rlc_next;       // the same for the next frontier
+  IdxT *buf0, *buf1;  // per tree Fr x N ping-pong rows (row slots)
   uint32_t *bits;     // per tree W words: bit s set = sample s goes left
   uint32_t W;         // words per tree bitmap, (N + 31) / 32
   uint32_t T;         // trees in the batch

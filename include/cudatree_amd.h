@@ -101,7 +101,10 @@ typedef struct {
   int32_t bootstrap;         /* 1 with boot_ids == NULL: draw every tree's in-bag set on the device from
                                 (seed, tree) -- the sorted unique values of N draws
                                 floor(u * N / 2^32), u the words of Philox4x32-10((j/4, tree, "BOOT", 0), seed) */
-  int32_t reserved[3];
+  int32_t all_rows;          /* 1: every feature keeps presorted rows; 0 (default): two-valued features
+                                other than feature 0 keep none (their one split follows from class
+                                counts; the trees are identical) */
+  int32_t reserved[2];
 } cta_fit_opts;
 
 /* Fills opts with the defaults (batched mode, m = ceil(sqrt(F)) left to the caller). */

@@ -24,7 +24,7 @@ def orc():
     return oracle.get()
 
 
-def gpu_forest(ctx, case, mode="batched", scan_chunk=0, resh_chunk=0, idx_bits=0, tpb=0, resh_global=0):
+def gpu_forest(ctx, case, mode="batched", scan_chunk=0, resh_chunk=0, idx_bits=0, tpb=0, resh_global=0, all_rows=0):
     X, y, K = case["X"], case["y"], case["K"]
     ds = _lib.Dataset(ctx, X, y, K, idx_bits=idx_bits)
     o = _lib.default_opts()
@@ -39,6 +39,7 @@ def gpu_forest(ctx, case, mode="batched", scan_chunk=0, resh_chunk=0, idx_bits=0
     o.resh_chunk = resh_chunk
     o.trees_per_batch = tpb
     o.resh_global = resh_global
+    o.all_rows = all_rows
     f = _lib.Forest.fit(ctx, ds, case["n_trees"], o, helpers.case_boot_ids(case))
     return f, ds
 
@@ -143,13 +144,16 @@ def test_reshuffle_hook(ctx, chunk, global_bits):
 TINY = list(range(60))
 
 
-@pytest.mark.parametrize("mode,chunk", [("batched", 0), ("batched", 64), ("per_node", 64), ("hybrid", 64)])
-def test_forest_tiny_cases(ctx, orc, mode, chunk):
+@pytest.mark.parametrize("mode,chunk,all_rows", [("batched", 0, 0), ("batched", 64, 0), ("per_node", 64, 0),
+                                                ("hybrid", 64, 0), ("batched", 0, 1), ("hybrid", 64, 1)])
+def test_forest_tiny_cases(ctx, orc, mode, chunk, all_rows):
+    """Sixty small problems; two-valued features without rows (all_rows=0)
+    and with (all_rows=1) build the same trees."""
     for s in TINY:
         case = datasets.tiny_case(s)
         case["bfs_threshold"] = 8
-        print("tiny forest mode=%s chunk=%d seed=%d" % (mode, chunk, s), flush=True)
-        f, _ = gpu_forest(ctx, case, mode=mode, scan_chunk=chunk, resh_chunk=chunk)
+        print("tiny forest mode=%s chunk=%d all_rows=%d seed=%d" % (mode, chunk, all_rows, s), flush=True)
+        f, _ = gpu_forest(ctx, case, mode=mode, scan_chunk=chunk, resh_chunk=chunk, all_rows=all_rows)
         d = helpers.forest_diff(f.export(), helpers.oracle_forest(case, orc))
         assert d is None, "seed %d: %s" % (s, d)
 
@@ -180,21 +184,22 @@ def test_forest_digits_like_config(ctx, orc):
     assert np.array_equal(f.predict(X), y)   # memorised (bootstrap=False)
 
 
-@pytest.mark.parametrize("idx_bits", [0, 32])
-def test_forest_tabular_u32_indices(ctx, orc, idx_bits):
-    """covtype-shaped data above 65,535 rows (32-bit indices), bootstrap."""
+@pytest.mark.parametrize("idx_bits,all_rows", [(0, 0), (32, 0), (0, 1)])
+def test_forest_tabular_u32_indices(ctx, orc, idx_bits, all_rows):
+    """covtype-shaped data above 65,535 rows (32-bit indices), bootstrap; the
+    44 one-hot features without rows and with."""
     X, y, _, _ = datasets.covtype_like(seed=7, n_train=70000, n_test=1)
     case = {"X": X, "y": y, "K": 7, "m": 7, "mss": 1, "max_depth": 0, "identity": False, "seed": 99,
             "boot": True, "n_trees": 3}
-    print("tabular 70000x54 idx_bits=%d" % idx_bits, flush=True)
-    f, ds = gpu_forest(ctx, case, idx_bits=idx_bits)
+    print("tabular 70000x54 idx_bits=%d all_rows=%d" % (idx_bits, all_rows), flush=True)
+    f, ds = gpu_forest(ctx, case, idx_bits=idx_bits, all_rows=all_rows)
     assert ds.idx_bits == 32
     got = f.export()
     assert helpers.forest_diff(got, helpers.oracle_forest(case, orc)) is None
 
 
-@pytest.mark.parametrize("idx_bits", [0, 32])
-def test_value_code_columns(ctx, orc, idx_bits):
+@pytest.mark.parametrize("idx_bits,all_rows", [(0, 0), (32, 0), (0, 1)])
+def test_value_code_columns(ctx, orc, idx_bits, all_rows):
     """Features of every value-code kind: binary (bit column), < 256 integer
     levels (8-bit), thousands of integer levels (16-bit), levels 4e-6 apart
     (closer than the 1e-5 candidate gap: no column, float gap test),
@@ -217,9 +222,26 @@ def test_value_code_columns(ctx, orc, idx_bits):
          + (rng.random(N) < 0.1)) % 4
     case = {"X": X, "y": y, "K": 4, "m": 3, "mss": 1, "max_depth": 0, "identity": False, "seed": 41,
             "boot": True, "n_trees": 4}
-    print("value codes idx_bits=%d" % idx_bits, flush=True)
-    f, _ = gpu_forest(ctx, case, idx_bits=idx_bits, scan_chunk=512)
+    print("value codes idx_bits=%d all_rows=%d" % (idx_bits, all_rows), flush=True)
+    f, _ = gpu_forest(ctx, case, idx_bits=idx_bits, scan_chunk=512, all_rows=all_rows)
     assert helpers.forest_diff(f.export(), helpers.oracle_forest(case, orc)) is None
+
+
+def test_row_free_budget_split(ctx, orc):
+    """Forty two-valued features at K = 16: the count kernels hold 32 of them
+    without rows, the rest keep rows; feature 0 is two-valued and keeps its
+    row.  Trees equal the oracle's."""
+    rng = np.random.default_rng(77)
+    N, K = 3000, 16
+    X = np.concatenate([rng.integers(0, 2, size=(N, 40)).astype(np.float32) * rng.choice([1.0, 3.5], size=40),
+                        rng.normal(size=(N, 2)).astype(np.float32)], axis=1).astype(np.float32)
+    y = (X[:, :6].sum(1).astype(int) + (X[:, 40] > 0) * 3 + rng.integers(0, 3, N)) % K
+    case = {"X": X, "y": y, "K": K, "m": 9, "mss": 1, "max_depth": 0, "identity": False, "seed": 23,
+            "boot": True, "n_trees": 4}
+    for all_rows in (0, 1):
+        print("row-free budget all_rows=%d" % all_rows, flush=True)
+        f, _ = gpu_forest(ctx, case, all_rows=all_rows)
+        assert helpers.forest_diff(f.export(), helpers.oracle_forest(case, orc)) is None, all_rows
 
 
 def test_modes_and_batching_identical(ctx, orc):
