@@ -599,6 +599,7 @@ struct Fitter {
             ensure_rl(a, 2ull * qb, cur);
             a.rlc = rl_buf(cur);
             a.rlc_next = rl_buf(cur ^ 1);
+            a.rl_small = rl_buf(2);
           }
           level(a, qb);
           cur ^= 1;
@@ -652,6 +653,7 @@ struct Fitter {
           rl_front(a, 0, 1, e.nd.n);
           a.rlc = rl_buf(0);
           a.rlc_next = rl_buf(1);
+          a.rl_small = rl_buf(2);
         }
         level(a, 1);
         Ctl prev = hc;
@@ -688,6 +690,7 @@ struct Fitter {
             ensure_rl(a, 2ull * q, cur);
             a.rlc = rl_buf(cur);
             a.rlc_next = rl_buf(cur ^ 1);
+            a.rl_small = rl_buf(2);
           }
           level(a, q);
           cur ^= 1;
@@ -761,11 +764,11 @@ struct Fitter {
     }
   }
 
-  // Row-free count buffers (frontier slots x Frl x K), one per frontier
-  // parity; grown between levels, the current one (`cur`) keeping its counts.
-  uint32_t *rl_buf(int which) {
-    return static_cast<uint32_t *>(ctx->wspace[which ? "rlc1" : "rlc0"]->p);
-  }
+  // Row-free count buffers (frontier slots x Frl x K): one per frontier
+  // parity and the split nodes' smaller-side counts (2); grown between
+  // levels, the current one (`cur`) keeping its counts.
+  static const char *rl_name(int which) { return which == 0 ? "rlc0" : which == 1 ? "rlc1" : "rlc2"; }
+  uint32_t *rl_buf(int which) { return static_cast<uint32_t *>(ctx->wspace[rl_name(which)]->p); }
   void ensure_rl(const FitArgs<IdxT, LabT> &a, uint64_t need, int cur) {
     if (!Frl) return;
     need = std::min<uint64_t>(std::max<uint64_t>(need, 1), a.cap_q);
@@ -773,8 +776,8 @@ struct Fitter {
     const size_t FK = (size_t)Frl * K;
     const uint64_t cap = std::min<uint64_t>(a.cap_q, std::max(need, 2 * rl_cap));
     ctx->sync();
-    for (int w = 0; w < 2; ++w) {
-      auto &slot = ctx->wspace[w ? "rlc1" : "rlc0"];
+    for (int w = 0; w < 3; ++w) {
+      auto &slot = ctx->wspace[rl_name(w)];
       if (!slot) slot.reset(new DevMem());
       if (slot->p && slot->bytes >= cap * FK * 4) continue;
       DevMem fresh(cap * FK * 4);
@@ -851,6 +854,7 @@ struct Fitter {
     if (Frl) {
       Timed tm(ctx, "k_rl_count");
       k_rl_children<IdxT, LabT><<<G, kBlock, (size_t)kWavesPerBlock * kRlCounters * 4, st>>>(a);
+      k_rl_finish<IdxT, LabT><<<G, kBlock, 0, st>>>(a);
     }
     reshuffle(a);
     k_ctl_next<<<1, kByteKinds * kWave, 0, st>>>(a.ctl, a.bslot);

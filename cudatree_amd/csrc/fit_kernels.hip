@@ -896,6 +896,9 @@ __global__ __launch_bounds__(kBlock) void k_decide(FitArgs<IdxT, LabT> a) {
       ax.flags = kSplit;
       ax.dst = (nd.src == 2) ? 1u : 2u;
     }
+    // the smaller side's row-free counts start at zero (k_rl_children adds)
+    if (ax.flags & kSplit)
+      for (uint32_t w = lane; w < a.Frl * (uint32_t)a.K; w += kWave) a.rl_small[(size_t)i * a.Frl * a.K + w] = 0u;
     // the node's fill slots, one per Cf positions of its rows: the side
     // bits come from the first nfill, the children's row-free counts from
     // all of them; a node that does not split leaves them empty
@@ -930,9 +933,19 @@ __device__ __forceinline__ void fill_task_bits(const FitArgs<IdxT, LabT> &a, con
     const IdxT *row = rows_of(a, nd.src, nd.tree, 0) + nd.start;
     const uint32_t p0 = tk.chunk * a.Cf, p1 = min(nd.n, p0 + a.Cf);
     const uint32_t w = (uint32_t)jr >> 5, b = (uint32_t)jr & 31u;
-    for (uint32_t p = p0 + lane; p < p1; p += kWave) {
-      const uint32_t s = sample_of(row[p]);
-      if (!((a.sig[(size_t)s * a.Wsig + w] >> b) & 1u)) atomicOr(&bits[s >> 5], 1u << (s & 31u));
+    for (uint32_t pb = p0; pb < p1; pb += 4 * kWave) {
+      uint32_t sid[4], low[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const uint32_t p = pb + (uint32_t)u * kWave + lane;
+        sid[u] = (p < p1) ? sample_of(row[p]) : 0xFFFFFFFFu;
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+        low[u] = (sid[u] != 0xFFFFFFFFu) ? !((a.sig[(size_t)sid[u] * a.Wsig + w] >> b) & 1u) : 0u;
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+        if (low[u]) atomicOr(&bits[sid[u] >> 5], 1u << (sid[u] & 31u));
     }
     return;
   }
@@ -1242,8 +1255,6 @@ __global__ __launch_bounds__(kBlock) void k_children_emit(FitArgs<IdxT, LabT> a)
         const uint32_t l = a.cntL[(size_t)i * K + k];
         a.next_cnt[(size_t)slot * K + k] = c ? a.cnt[(size_t)i * K + k] - l : l;
       }
-      // the child's row-free counts start at zero (k_rl_children adds)
-      for (uint32_t w = lane; w < a.Frl * K; w += kWave) a.rlc_next[(size_t)slot * a.Frl * K + w] = 0u;
       ++slot;
     }
     emit_resh_tasks(a, i, ax, o[2], o[3]);
@@ -1259,32 +1270,57 @@ __global__ __launch_bounds__(kBlock) void k_children_emit(FitArgs<IdxT, LabT> a)
 // ------------------------------------------------------------------------
 constexpr uint32_t kRlCounters = 1024;  // 2 x Frl x K at most (per-wave LDS counters)
 
+// Counts the samples of positions [p0, p1) of the node's feature-0 row into
+// cnt[j][k]: all of them (want = 2), or those whose side bit equals want.
 template <class IdxT, class LabT>
 __device__ __forceinline__ void rl_count_chunk(const FitArgs<IdxT, LabT> &a, const FNode &nd, uint32_t p0, uint32_t p1,
-                                               bool by_side, uint32_t *cnt) {
+                                               uint32_t want, uint32_t *cnt) {
   const uint32_t lane = lane_id();
   const uint32_t K = (uint32_t)a.K, FK = a.Frl * K;
   const IdxT *row = rows_of(a, nd.src, nd.tree, 0) + nd.start;
   const uint32_t *bits = a.bits + (size_t)nd.tree * a.W;
-  for (uint32_t p = p0 + lane; p < p1; p += kWave) {
-    const IdxT e = row[p];  // (label with the id, or from y)
-    const uint32_t s = sample_of(e), L = a.label(e);
-    // counter half 0: left child (side bit set), 1: right child
-    const uint32_t half = by_side ? 1u - ((bits[s >> 5] >> (s & 31u)) & 1u) : 0u;
-    uint32_t *c = cnt + half * FK + L;
-    for (uint32_t w = 0; w < a.Wsig; ++w) {
-      uint32_t word = a.sig[(size_t)s * a.Wsig + w];
-      while (word) {
-        const uint32_t j = w * 32u + (uint32_t)__builtin_ctz(word);
-        atomicAdd(&c[j * K], 1u);
-        word &= word - 1u;
+  const uint32_t Ws = a.Wsig;
+  (void)FK;
+  constexpr int U = 4;  // positions per lane in flight: entries, then their gathers
+  for (uint32_t pb = p0; pb < p1; pb += U * kWave) {
+    uint32_t e[U], lab[U], side[U], w0[U], w1[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const uint32_t p = pb + (uint32_t)u * kWave + lane;
+      e[u] = (p < p1) ? (uint32_t)row[p] : 0xFFFFFFFFu;
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      lab[u] = side[u] = w0[u] = w1[u] = 0;
+      if (e[u] != 0xFFFFFFFFu) {
+        const uint32_t s = sample_of((IdxT)e[u]);
+        lab[u] = a.label((IdxT)e[u]);
+        side[u] = want < 2u ? (bits[s >> 5] >> (s & 31u)) & 1u : 2u;
+        if (side[u] == want) {
+          w0[u] = a.sig[(size_t)s * Ws];
+          if (Ws > 1) w1[u] = a.sig[(size_t)s * Ws + 1];
+        }
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      if (e[u] == 0xFFFFFFFFu || side[u] != want) continue;
+      uint32_t *c = cnt + lab[u];
+      for (uint32_t w = 0; w < Ws; ++w) {
+        uint32_t word = w == 0 ? w0[u] : w == 1 ? w1[u] : a.sig[(size_t)sample_of((IdxT)e[u]) * Ws + w];
+        while (word) {
+          const uint32_t j = w * 32u + (uint32_t)__builtin_ctz(word);
+          atomicAdd(&c[j * K], 1u);
+          word &= word - 1u;
+        }
       }
     }
   }
 }
 
-// Children of every split node: counts over the node's fill slots (Cf
-// positions each), into the children's zeroed entries of rlc_next.
+// Smaller side of every split node: counts over the node's fill slots (Cf
+// positions each) of the samples on its smaller side, into rl_small (zeroed
+// by k_decide); k_rl_finish derives both children.
 template <class IdxT, class LabT>
 __global__ __launch_bounds__(kBlock) void k_rl_children(FitArgs<IdxT, LabT> a) {
   if (a.ctl->err) return;  // a list overflowed: the fit fails, nothing more is read
@@ -1297,30 +1333,51 @@ __global__ __launch_bounds__(kBlock) void k_rl_children(FitArgs<IdxT, LabT> a) {
     const Task tk = a.fill_tasks[item];
     if (tk.node == kLeaf) continue;
     const NAux ax = a.aux[tk.node];
-    const bool liveL = ax.flags & kLiveL, liveR = ax.flags & kLiveR;
-    if (!liveL && !liveR) continue;
+    if (!(ax.flags & (kLiveL | kLiveR))) continue;
     const FNode nd = a.front[tk.node];
-    for (uint32_t w = lane; w < 2 * FK; w += kWave) cnt[w] = 0;
+    // side bit 1: left; count the smaller child
+    const uint32_t want = (ax.nL <= nd.n - ax.nL) ? 1u : 0u;
+    for (uint32_t w = lane; w < FK; w += kWave) cnt[w] = 0;
     wave_sync();
     const uint32_t c0 = tk.chunk * a.Cf, c1 = min(nd.n, (tk.chunk + 1) * a.Cf);
-    rl_count_chunk(a, nd, c0, c1, true, cnt);
-    // row entry, label, side bit and signature words per sample
+    rl_count_chunk(a, nd, c0, c1, want, cnt);
+    // row entry, label and side bit per sample, signature words on one side
     if (lane == 0)
-      bytes += (unsigned long long)(c1 - c0) * (sizeof(IdxT) + (sizeof(IdxT) == 4 ? 0 : sizeof(LabT)) + 1 + 4 * a.Wsig);
+      bytes += (unsigned long long)(c1 - c0) * (sizeof(IdxT) + (sizeof(IdxT) == 4 ? 0 : sizeof(LabT)) + 1) +
+               (unsigned long long)min(ax.nL, nd.n - ax.nL) * (c1 - c0) / max(nd.n, 1u) * 4 * a.Wsig;
     wave_sync();
-    // child frontier slots: left first when live
-    const uint32_t slot0 = a.offs[(size_t)tk.node * kChFields + 1];
-    for (uint32_t w = lane; w < 2 * FK; w += kWave) {
-      const uint32_t v = cnt[w];
-      const bool right = w >= FK;
-      if (v && (right ? liveR : liveL)) {
-        const uint32_t slot = slot0 + ((right && liveL) ? 1u : 0u);
-        atomicAdd(&a.rlc_next[(size_t)slot * FK + (right ? w - FK : w)], v);
-      }
-    }
+    uint32_t *dst = a.rl_small + (size_t)tk.node * FK;
+    for (uint32_t w = lane; w < FK; w += kWave)
+      if (cnt[w]) atomicAdd(&dst[w], cnt[w]);
     wave_sync();
   }
   flush_bytes(a, kBytesRl, bytes);
+}
+
+// Children's counts: the smaller child's as counted, the other's as the
+// parent's minus them.
+template <class IdxT, class LabT>
+__global__ __launch_bounds__(kBlock) void k_rl_finish(FitArgs<IdxT, LabT> a) {
+  if (a.ctl->err) return;  // a list overflowed: the fit fails, nothing more is read
+  const uint32_t lane = lane_id();
+  const uint32_t FK = a.Frl * (uint32_t)a.K;
+  WAVE_LOOP(a.ctl->q) {
+    const uint32_t i = item;
+    const NAux ax = a.aux[i];
+    if (!(ax.flags & kSplit)) continue;
+    const bool liveL = ax.flags & kLiveL, liveR = ax.flags & kLiveR;
+    if (!liveL && !liveR) continue;
+    const uint32_t n = a.front[i].n;
+    const bool small_left = ax.nL <= n - ax.nL;
+    const uint32_t slot0 = a.offs[(size_t)i * kChFields + 1];
+    const uint32_t *par = a.rlc + (size_t)i * FK, *sm = a.rl_small + (size_t)i * FK;
+    for (uint32_t c = 0; c < 2; ++c) {
+      if (!(c ? liveR : liveL)) continue;
+      const bool small = (c == 0) == small_left;
+      uint32_t *dst = a.rlc_next + (size_t)(slot0 + ((c && liveL) ? 1u : 0u)) * FK;
+      for (uint32_t w = lane; w < FK; w += kWave) dst[w] = small ? sm[w] : par[w] - sm[w];
+    }
+  }
 }
 
 // The current frontier's counts from scratch (roots, frontiers set by the
@@ -1339,7 +1396,7 @@ __global__ __launch_bounds__(kBlock) void k_rl_front(FitArgs<IdxT, LabT> a) {
     for (uint32_t c = blockIdx.x * kWavesPerBlock + wave; c < nchunk; c += gridDim.x * kWavesPerBlock) {
       for (uint32_t w = lane; w < FK; w += kWave) cnt[w] = 0;
       wave_sync();
-      rl_count_chunk(a, nd, c * a.Cf, min(nd.n, (c + 1) * a.Cf), false, cnt);
+      rl_count_chunk(a, nd, c * a.Cf, min(nd.n, (c + 1) * a.Cf), 2u, cnt);
       wave_sync();
       uint32_t *dst = const_cast<uint32_t *>(a.rlc) + (size_t)i * FK;
       for (uint32_t w = lane; w < FK; w += kWave)

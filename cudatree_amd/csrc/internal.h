@@ -143,6 +143,7 @@ struct FitArgs {
   const uint32_t *sig;      // [N x Wsig] bit j: sample holds the higher value of row-free feature j
   const uint32_t *rlc;      // frontier: [q x Frl x K] samples of class k holding the higher value
   uint32_t *rlc_next;       // the same for the next frontier
+  uint32_t *rl_small;       // frontier: [q x Frl x K] counts of each split node's smaller side
   IdxT *buf0, *buf1;  // per tree Fr x N ping-pong rows (row slots)
   uint32_t *bits;     // per tree W words: bit s set = sample s goes left
   uint32_t W;         // words per tree bitmap, (N + 31) / 32
