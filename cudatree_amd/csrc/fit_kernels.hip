@@ -932,7 +932,8 @@ __device__ __forceinline__ void fill_task_bits(const FitArgs<IdxT, LabT> &a, con
     // row-free split: the samples holding the lower value, over feature 0's row
     const IdxT *row = rows_of(a, nd.src, nd.tree, 0) + nd.start;
     const uint32_t p0 = tk.chunk * a.Cf, p1 = min(nd.n, p0 + a.Cf);
-    const uint32_t w = (uint32_t)jr >> 5, b = (uint32_t)jr & 31u;
+    // the feature's bit column (N / 8 bytes) rather than the signatures
+    const uint32_t *col = reinterpret_cast<const uint32_t *>(a.codes + fc_off(a.fcode[ax.best_f]));
     for (uint32_t pb = p0; pb < p1; pb += 4 * kWave) {
       uint32_t sid[4], low[4];
 #pragma unroll
@@ -942,7 +943,7 @@ __device__ __forceinline__ void fill_task_bits(const FitArgs<IdxT, LabT> &a, con
       }
 #pragma unroll
       for (int u = 0; u < 4; ++u)
-        low[u] = (sid[u] != 0xFFFFFFFFu) ? !((a.sig[(size_t)sid[u] * a.Wsig + w] >> b) & 1u) : 0u;
+        low[u] = (sid[u] != 0xFFFFFFFFu) ? !((col[sid[u] >> 5] >> (sid[u] & 31u)) & 1u) : 0u;
 #pragma unroll
       for (int u = 0; u < 4; ++u)
         if (low[u]) atomicOr(&bits[sid[u] >> 5], 1u << (sid[u] & 31u));
