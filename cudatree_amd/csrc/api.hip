@@ -1057,7 +1057,7 @@ void build_codes(cta_ctx *ctx, cta_dataset *ds, const uint32_t *keys, const uint
   uint32_t *h = static_cast<uint32_t *>(ctx->host("code_meta", 2 * (size_t)F * 4));
   HIPC(hipMemcpyAsync(h, db, 2 * (size_t)F * 4, hipMemcpyDeviceToHost, st));
   ctx->sync();
-  std::vector<uint64_t> fc(F), col(F);
+  std::vector<uint64_t> fc(F);
   uint64_t off = 0;
   for (uint32_t f = 0; f < F; ++f) {
     const uint32_t D = h[f], bad = h[F + f];
@@ -1075,22 +1075,6 @@ void build_codes(cta_ctx *ctx, cta_dataset *ds, const uint32_t *keys, const uint
     }
     fc[f] = off | kind;
     off += (bytes + 255) & ~(uint64_t)255;
-    col[f] = kind ? bytes : 4ull * N;
-  }
-  // Split-scan classes (one per XCD L2): features spread evenly in number,
-  // the largest gathered columns placed first so they land in different L2s.
-  std::vector<uint32_t> order(F);
-  for (uint32_t f = 0; f < F; ++f) order[f] = f;
-  std::stable_sort(order.begin(), order.end(), [&](uint32_t u, uint32_t v) { return col[u] > col[v]; });
-  uint32_t ncls[kXcd] = {};
-  uint64_t foot[kXcd] = {};
-  for (uint32_t f : order) {
-    uint32_t best = 0;
-    for (uint32_t x = 1; x < kXcd; ++x)
-      if (ncls[x] < ncls[best] || (ncls[x] == ncls[best] && foot[x] < foot[best])) best = x;
-    fc[f] |= (uint64_t)best << 2;
-    ++ncls[best];
-    foot[best] += col[f];
   }
   ds->codes.alloc(std::max<uint64_t>(off, 256));
   ds->fcode.alloc((size_t)F * 8);
@@ -1104,7 +1088,7 @@ void build_codes(cta_ctx *ctx, cta_dataset *ds, const uint32_t *keys, const uint
   const uint32_t K = ds->K;
   const uint32_t max_rl = kRlCounters / (2 * K);
   std::vector<int32_t> tab((size_t)5 * F + 1, -1);  // rl_of, slot_of, ident, none, then row_feat / rl_feat
-  int32_t *rl_of = tab.data(), *slot_of = rl_of + F, *ident = slot_of + F, *none = ident + F;
+  int32_t *rl_of = tab.data(), *slot_of = rl_of + F, *ident = slot_of + F;  // then none: all -1
   std::vector<int32_t> row_feat, rl_feat;
   for (uint32_t f = 0; f < F; ++f) {
     ident[f] = (int32_t)f;

@@ -65,6 +65,18 @@ __device__ __forceinline__ const IdxT *rows_of(const FitArgs<IdxT, LabT> &a, uin
   return b + ((size_t)tree * a.Fr + (uint32_t)a.slot_of[f]) * a.N;
 }
 
+// Split-scan class (< kXcd) of the tasks of feature f in frontier node i.
+// Workgroups are dispatched to the XCDs round-robin (workgroup b on XCD
+// b % kXcd, each XCD with its own L2) and k_scan's workgroup b takes class
+// b % kXcd.  Row slot s owns kXcd units, s kXcd .. s kXcd + kXcd - 1, the
+// node picks one of them, and class x holds units x Fr .. x Fr + Fr - 1: the
+// classes get equal shares of the work and a feature's tasks span at most
+// 1 + ceil(kXcd / Fr) classes, so its gathered column sits in few L2s.
+template <class IdxT, class LabT>
+__device__ __forceinline__ uint32_t scan_cls(const FitArgs<IdxT, LabT> &a, uint32_t f, uint32_t i) {
+  return ((uint32_t)a.slot_of[f] * kXcd + i % kXcd) / a.Fr;
+}
+
 template <class IdxT, class LabT>
 __device__ __forceinline__ IdxT *rows_out(const FitArgs<IdxT, LabT> &a, uint32_t dst, uint32_t tree,
                                           uint32_t s) {
@@ -239,7 +251,7 @@ __global__ __launch_bounds__(kBlock) void k_select(FitArgs<IdxT, LabT> a) {
     Perm pm = make_perm(a.seed, a.tree_base + nd.tree, nd.depth, nd.start, F, a.identity != 0);
     // tasks: one per (feature, chunk) of a multi-chunk node; a single-chunk
     // node's features grouped about Cs positions per task.  A task's class is
-    // that of its (first) feature; lane c counts the tasks of class c.
+    // scan_cls of its (first) feature; lane c counts the tasks of class c.
     const uint32_t fpt = (nch > 1) ? 1u : max(1u, a.Cs / n);
     const uint32_t wt = (nch > 1) ? nch : 1u;
     const uint32_t K = (uint32_t)a.K;
@@ -251,7 +263,6 @@ __global__ __launch_bounds__(kBlock) void k_select(FitArgs<IdxT, LabT> a) {
       const uint32_t k = k0 + lane;
       bool nonconst = false;
       uint32_t f = 0;
-      uint64_t fc = 0;
       int32_t jr = -1;
       const uint32_t *oc = nullptr;
       if (k < F) {
@@ -268,7 +279,7 @@ __global__ __launch_bounds__(kBlock) void k_select(FitArgs<IdxT, LabT> a) {
           const uint32_t s0 = sample_of(row[0]), s1 = sample_of(row[nd.n - 1]);
           // constant in the node: first and last sorted values equal (value
           // codes when the feature has them, else the floats)
-          fc = a.fcode[f];
+          const uint64_t fc = a.fcode[f];
           const uint32_t kind = fc_kind(fc);
           if (kind == 0) {
             const float *x = a.X_T + (size_t)f * a.N;
@@ -286,7 +297,7 @@ __global__ __launch_bounds__(kBlock) void k_select(FitArgs<IdxT, LabT> a) {
       const uint64_t mrow = ballot(trow);
       const uint32_t rr = mbcnt(mrow);
       if (trow) a.feats[(size_t)i * m + nf + rr] = (int32_t)f;
-      const uint32_t c = trow && (nf + rr) % fpt == 0 ? fc_cls(fc) : kXcd;
+      const uint32_t c = trow && (nf + rr) % fpt == 0 ? scan_cls(a, f, i) : kXcd;
 #pragma unroll
       for (uint32_t x = 0; x < kXcd; ++x) {
         const uint32_t cnt = (uint32_t)__popcll(ballot(c == x));
@@ -379,7 +390,7 @@ __global__ __launch_bounds__(kBlock) void k_select_emit(FitArgs<IdxT, LabT> a) {
     for (uint32_t c0 = 0; c0 < nitem; c0 += kWave) {
       const uint32_t it = c0 + lane;
       const bool valid = it < nitem;
-      const uint32_t c = valid ? fc_cls(a.fcode[a.feats[(size_t)i * a.m + (nch > 1 ? it : it * fpt)]]) : kXcd;
+      const uint32_t c = valid ? scan_cls(a, (uint32_t)a.feats[(size_t)i * a.m + (nch > 1 ? it : it * fpt)], i) : kXcd;
       uint32_t pos = 0;
 #pragma unroll
       for (uint32_t x = 0; x < kXcd; ++x) {

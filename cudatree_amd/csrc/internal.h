@@ -58,8 +58,8 @@ struct ScanRes {
 constexpr uint32_t kLevelRing = 64;
 // Workgroups are dispatched to the 8 XCDs round-robin (workgroup b on XCD
 // b % 8), each XCD with its own L2.  Split-scan tasks are kept in one list
-// per class, a feature's tasks in the class of that feature, and workgroup b
-// works on class b % 8, so the value gathers of a column hit one L2.
+// per class, a feature's tasks in one or two classes, and workgroup b works
+// on class b % 8, so the value gathers of a column hit few L2s.
 constexpr uint32_t kXcd = 8;
 
 // Counters of one level, in device memory.  Nothing is allocated with
@@ -108,11 +108,9 @@ struct Arena {
 // (N < 2^22, K <= 1,024), so the split scan reads labels with the ids.
 constexpr uint32_t kIdBits = 22, kIdMask = (1u << kIdBits) - 1u;
 
-// fcode[f]: byte offset of feature f's code column (a multiple of 256), its
-// split-scan class (< kXcd) << 2, and the column kind (0: none, read X_T;
-// 1: bits, 2: u8, 3: u16).
+// fcode[f]: byte offset of feature f's code column (a multiple of 256) and
+// the column kind (0: none, read X_T; 1: bits, 2: u8, 3: u16).
 __host__ __device__ __forceinline__ uint32_t fc_kind(uint64_t fc) { return (uint32_t)(fc & 3u); }
-__host__ __device__ __forceinline__ uint32_t fc_cls(uint64_t fc) { return (uint32_t)(fc >> 2) & 7u; }
 __host__ __device__ __forceinline__ uint64_t fc_off(uint64_t fc) { return fc & ~(uint64_t)255; }
 
 template <class IdxT>
@@ -127,7 +125,7 @@ struct FitArgs {
   const LabT *y;      // N compact labels
   const IdxT *si0;    // F x N presorted rows
   const uint8_t *codes;    // value code columns of the features that have one
-  const uint64_t *fcode;   // [F] code column byte offset | split-scan class << 2 | kind
+  const uint64_t *fcode;   // [F] code column byte offset | kind
   // Row-free features: two-valued features other than feature 0 keep no
   // rows.  Their one candidate split, their constancy and their side bits
   // follow from per-node, per-class counts of the samples holding the
