@@ -356,7 +356,8 @@ struct Fitter {
     Gscan = std::max<uint32_t>(kXcd, (uint32_t)(ctx->num_cu * nb) / kXcd * kXcd);
   }
 
-  size_t fill_lds_bytes() const { return (((size_t)N + 31) / 32) * 4; }
+  // side bitmap, then each wave's row-free counters
+  size_t fill_lds_bytes() const { return (((size_t)N + 31) / 32) * 4 + (size_t)kFillWaves * Frl * K * 4; }
 
   size_t smem() const { return (size_t)kWavesPerBlock * 2 * K * sizeof(uint32_t); }
 
@@ -760,7 +761,7 @@ struct Fitter {
                                                                                             part, a.bits);
     } else {
       k_fill_u32<<<grid_for((size_t)a.T * a.W), 256, 0, st>>>(a.bits, (size_t)a.T * a.W, 0u);
-      k_fill_global<IdxT, LabT><<<G, kBlock, 0, st>>>(a);
+      k_fill_global<IdxT, LabT><<<G, kBlock, (size_t)kWavesPerBlock * Frl * K * 4, st>>>(a);
     }
   }
 
@@ -852,8 +853,8 @@ struct Fitter {
       k_children_emit<IdxT, LabT><<<G, kBlock, 0, st>>>(a);
     }
     if (Frl) {
+      // the smaller sides were counted by the fill tasks
       Timed tm(ctx, "k_rl_count");
-      k_rl_children<IdxT, LabT><<<G, kBlock, (size_t)kWavesPerBlock * kRlCounters * 4, st>>>(a);
       k_rl_finish<IdxT, LabT><<<G, kBlock, 0, st>>>(a);
     }
     reshuffle(a);

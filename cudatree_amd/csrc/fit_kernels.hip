@@ -830,6 +830,19 @@ __device__ void count_left(const FitArgs<IdxT, LabT> &a, uint32_t i, const FNode
   wave_sync();
 }
 
+// What a child of a node at `depth` becomes, from its count of present
+// classes nz and its size nc: 0 live (split further), 1 pure leaf, 2 leaf
+// labelled by its first sample (fewer than mss samples), 3 majority leaf
+// (depth cap).
+template <class IdxT, class LabT>
+__device__ __forceinline__ uint32_t child_kind(const FitArgs<IdxT, LabT> &a, uint32_t nz, uint32_t nc,
+                                               uint32_t depth) {
+  if (nz <= 1) return 1u;
+  if ((int64_t)nc < (int64_t)a.mss) return 2u;
+  if (a.max_depth > 0 && (int64_t)depth + 1 >= (int64_t)a.max_depth) return 3u;
+  return 0u;
+}
+
 template <class IdxT, class LabT>
 __global__ __launch_bounds__(kBlock) void k_decide(FitArgs<IdxT, LabT> a) {
   if (a.ctl->err) return;  // a list overflowed: the fit fails, nothing more is read
@@ -875,7 +888,7 @@ __global__ __launch_bounds__(kBlock) void k_decide(FitArgs<IdxT, LabT> a) {
       const uint32_t K = (uint32_t)a.K;
       const uint32_t *oc = a.rlc + ((size_t)i * a.Frl + (uint32_t)a.rl_of[f]) * K;
       for (uint32_t k = lane; k < K; k += kWave) a.cntL[(size_t)i * K + k] = a.cnt[(size_t)i * K + k] - oc[k];
-      nfill = ax.nch_f;
+      nfill = nd.n;
       if (lane == 0) {
         a.arena.feat[id] = (int32_t)f;
         a.arena.thr[id] = a.rl_thr[a.rl_of[f]];
@@ -890,7 +903,7 @@ __global__ __launch_bounds__(kBlock) void k_decide(FitArgs<IdxT, LabT> a) {
     } else {
       const uint32_t f = (uint32_t)best.feat, p = best.pos;
       count_left(a, i, nd, ax, f, p, h);
-      nfill = ceil_div(p + 1, a.Cf);
+      nfill = p + 1;  // positions; chunks below
       if (lane == 0) {
         const IdxT *row = rows_of(a, nd.src, nd.tree, f) + nd.start;
         const float *x = a.X_T + (size_t)f * a.N;
@@ -907,12 +920,35 @@ __global__ __launch_bounds__(kBlock) void k_decide(FitArgs<IdxT, LabT> a) {
       ax.flags = kSplit;
       ax.dst = (nd.src == 2) ? 1u : 2u;
     }
-    // the smaller side's row-free counts start at zero (k_rl_children adds)
-    if (ax.flags & kSplit)
-      for (uint32_t w = lane; w < a.Frl * (uint32_t)a.K; w += kWave) a.rl_small[(size_t)i * a.Frl * a.K + w] = 0u;
-    // the node's fill slots, one per Cf positions of its rows: the side
-    // bits come from the first nfill, the children's row-free counts from
-    // all of them; a node that does not split leaves them empty
+    if ((ax.flags & kSplit) && a.Frl) {
+      // a live child needs row-free counts: the fill tasks count the smaller
+      // side's samples into rl_small (from zero), over the split feature's
+      // row, whose positions nL .. n - 1 are the right child
+      const uint32_t K = (uint32_t)a.K, nL = ax.nL, nR = nd.n - ax.nL;
+      uint32_t nzl = 0, nzr = 0;
+      for (uint32_t k = lane; k < K; k += kWave) {
+        const uint32_t l = a.cntL[(size_t)i * K + k];  // written by this lane above
+        nzl += l ? 1u : 0u;
+        nzr += (a.cnt[(size_t)i * K + k] - l) ? 1u : 0u;
+      }
+      nzl = uniform(wave_sum32(nzl));
+      nzr = uniform(wave_sum32(nzr));
+      if (child_kind(a, nzl, nL, nd.depth) == 0u || child_kind(a, nzr, nR, nd.depth) == 0u) {
+        ax.flags |= kCountRl;
+        const uint32_t small = min(nL, nR);
+        if (nL > nR) nfill = nd.n;
+        for (uint32_t w = lane; w < a.Frl * K; w += kWave) a.rl_small[(size_t)i * a.Frl * K + w] = 0u;
+        // signature words of the smaller side, and its entries past nL of a
+        // split feature's row
+        if (lane == 0)
+          bytes += (unsigned long long)small * 4 * a.Wsig +
+                   (!rowfree && nL > nR ? (unsigned long long)small * sizeof(IdxT) : 0ull);
+      }
+    }
+    nfill = ceil_div(nfill, a.Cf);
+    // the node's fill slots, one per Cf positions of its rows: the side bits
+    // and the smaller side's row-free counts come from the first nfill; a
+    // node that does not split leaves them empty
     for (uint32_t c = lane; c < ax.nch_f; c += kWave)
       a.fill_tasks[ax.fill_base + c] = (ax.flags & kSplit) ? Task{i, 0u, c, nfill} : Task{kLeaf, 0u, 0u, 0u};
     if (lane == 0) a.aux[i] = ax;
@@ -933,12 +969,56 @@ __global__ __launch_bounds__(kBlock) void k_decide(FitArgs<IdxT, LabT> a) {
 constexpr int kFillThreads = 1024;
 constexpr int kFillWaves = kFillThreads / kWave;
 
+// Adds the signatures of the entries e[u] with take[u] set to the wave's
+// row-free counters: cnt[j K + c] counts the samples of class c holding the
+// higher value of row-free feature j.  The gathers of all U entries are
+// issued before the LDS adds.
+template <int U, class IdxT, class LabT>
+__device__ __forceinline__ void rl_add(const FitArgs<IdxT, LabT> &a, const uint32_t (&e)[U], const bool (&take)[U],
+                                       uint32_t *cnt) {
+  const uint32_t K = (uint32_t)a.K, Ws = a.Wsig;
+  uint32_t w0[U], w1[U], lab[U];
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+    w0[u] = w1[u] = lab[u] = 0;
+    if (take[u]) {
+      const uint32_t s = sample_of((IdxT)e[u]);
+      lab[u] = a.label((IdxT)e[u]);
+      w0[u] = a.sig[(size_t)s * Ws];
+      if (Ws > 1) w1[u] = a.sig[(size_t)s * Ws + 1];
+    }
+  }
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+    if (!take[u]) continue;
+    uint32_t *c = cnt + lab[u];
+    for (uint32_t w = 0; w < Ws; ++w) {
+      uint32_t word = w == 0 ? w0[u] : w == 1 ? w1[u] : a.sig[(size_t)sample_of((IdxT)e[u]) * Ws + w];
+      while (word) {
+        atomicAdd(&c[(w * 32u + (uint32_t)__builtin_ctz(word)) * K], 1u);
+        word &= word - 1u;
+      }
+    }
+  }
+}
+
+// One fill task: the side bits of Cf positions of a split node, and, when
+// the node is flagged kCountRl, the row-free counts of its smaller side's
+// samples among them, added to rl_small through the wave's counters cnt.
 template <class IdxT, class LabT>
-__device__ __forceinline__ void fill_task_bits(const FitArgs<IdxT, LabT> &a, const Task &tk, uint32_t *bits) {
+__device__ __forceinline__ void fill_task_bits(const FitArgs<IdxT, LabT> &a, const Task &tk, uint32_t *bits,
+                                               uint32_t *cnt) {
   const uint32_t lane = lane_id();
   const FNode nd = a.front[tk.node];
   const NAux ax = a.aux[tk.node];
   const int32_t jr = a.Frl ? a.rl_of[ax.best_f] : -1;
+  const bool count = (ax.flags & kCountRl) != 0u;
+  const bool small_left = ax.nL <= nd.n - ax.nL;
+  const uint32_t FK = a.Frl * (uint32_t)a.K;
+  if (count) {
+    for (uint32_t w = lane; w < FK; w += kWave) cnt[w] = 0;
+    wave_sync();
+  }
   if (jr >= 0) {
     // row-free split: the samples holding the lower value, over feature 0's row
     const IdxT *row = rows_of(a, nd.src, nd.tree, 0) + nd.start;
@@ -946,33 +1026,64 @@ __device__ __forceinline__ void fill_task_bits(const FitArgs<IdxT, LabT> &a, con
     // the feature's bit column (N / 8 bytes) rather than the signatures
     const uint32_t *col = reinterpret_cast<const uint32_t *>(a.codes + fc_off(a.fcode[ax.best_f]));
     for (uint32_t pb = p0; pb < p1; pb += 4 * kWave) {
-      uint32_t sid[4], low[4];
+      uint32_t e[4], low[4];
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
         const uint32_t p = pb + (uint32_t)u * kWave + lane;
-        sid[u] = (p < p1) ? sample_of(row[p]) : 0xFFFFFFFFu;
+        e[u] = (p < p1) ? (uint32_t)row[p] : 0xFFFFFFFFu;
       }
 #pragma unroll
-      for (int u = 0; u < 4; ++u)
-        low[u] = (sid[u] != 0xFFFFFFFFu) ? !((col[sid[u] >> 5] >> (sid[u] & 31u)) & 1u) : 0u;
+      for (int u = 0; u < 4; ++u) {
+        const uint32_t s = sample_of((IdxT)e[u]);
+        low[u] = (e[u] != 0xFFFFFFFFu) ? !((col[s >> 5] >> (s & 31u)) & 1u) : 0u;
+      }
 #pragma unroll
-      for (int u = 0; u < 4; ++u)
-        if (low[u]) atomicOr(&bits[sid[u] >> 5], 1u << (sid[u] & 31u));
+      for (int u = 0; u < 4; ++u) {
+        const uint32_t s = sample_of((IdxT)e[u]);
+        if (low[u]) atomicOr(&bits[s >> 5], 1u << (s & 31u));
+      }
+      if (count) {
+        bool take[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) take[u] = e[u] != 0xFFFFFFFFu && (low[u] != 0u) == small_left;
+        rl_add(a, e, take, cnt);
+      }
     }
-    return;
+  } else {
+    // split feature's row: positions below nL go left; the right child's
+    // positions are visited only to count a smaller right side
+    const IdxT *row = rows_of(a, nd.src, nd.tree, (uint32_t)ax.best_f) + nd.start;
+    const uint32_t nL = ax.nL;
+    const uint32_t p0 = tk.chunk * a.Cf, p1 = min(count && !small_left ? nd.n : nL, p0 + a.Cf);
+    for (uint32_t pb = p0; pb < p1; pb += 4 * kWave) {
+      uint32_t e[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const uint32_t p = pb + (uint32_t)u * kWave + lane;
+        e[u] = (p < p1) ? (uint32_t)row[p] : 0xFFFFFFFFu;
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const uint32_t p = pb + (uint32_t)u * kWave + lane, s = sample_of((IdxT)e[u]);
+        if (e[u] != 0xFFFFFFFFu && p < nL) atomicOr(&bits[s >> 5], 1u << (s & 31u));
+      }
+      if (count) {
+        bool take[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const uint32_t p = pb + (uint32_t)u * kWave + lane;
+          take[u] = e[u] != 0xFFFFFFFFu && (p < nL) == small_left;
+        }
+        rl_add(a, e, take, cnt);
+      }
+    }
   }
-  const IdxT *row = rows_of(a, nd.src, nd.tree, (uint32_t)ax.best_f) + nd.start;
-  const uint32_t p0 = tk.chunk * a.Cf, p1 = min(ax.nL, p0 + a.Cf);
-  for (uint32_t pb = p0; pb < p1; pb += 4 * kWave) {
-    uint32_t sid[4];
-#pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      const uint32_t p = pb + (uint32_t)u * kWave + lane;
-      sid[u] = (p < p1) ? sample_of(row[p]) : 0xFFFFFFFFu;
-    }
-#pragma unroll
-    for (int u = 0; u < 4; ++u)
-      if (sid[u] != 0xFFFFFFFFu) atomicOr(&bits[sid[u] >> 5], 1u << (sid[u] & 31u));
+  if (count) {
+    wave_sync();
+    uint32_t *dst = a.rl_small + (size_t)tk.node * FK;
+    for (uint32_t w = lane; w < FK; w += kWave)
+      if (cnt[w]) atomicAdd(&dst[w], cnt[w]);
+    wave_sync();
   }
 }
 
@@ -989,10 +1100,12 @@ __global__ __launch_bounds__(kFillThreads) void k_fill_lds(FitArgs<IdxT, LabT> a
   const uint32_t fb = a.aux[lo].fill_base;
   const NAux last = a.aux[hi - 1];
   const uint32_t fe = last.fill_base + last.nch_f;
+  // the waves' row-free counters follow the bitmap
+  uint32_t *cnt = lbits + W + wave * a.Frl * (uint32_t)a.K;
   for (uint32_t k = fb + j * kFillWaves + wave; k < fe; k += J * kFillWaves) {
     const Task tk = a.fill_tasks[k];
     if (tk.node == kLeaf || tk.chunk >= tk.nch) continue;
-    fill_task_bits(a, tk, lbits);
+    fill_task_bits(a, tk, lbits, cnt);
   }
   __syncthreads();
   uint32_t *out = (J == 1) ? a.bits + (size_t)t * W : part + ((size_t)t * J + j) * W;
@@ -1012,14 +1125,17 @@ __global__ void k_merge_bits(const uint32_t *tlo, const uint32_t *thi, uint32_t 
   }
 }
 
-// Bitmaps too large for LDS: global atomics into bitmaps cleared first.
+// Bitmaps too large for LDS: global atomics into bitmaps cleared first
+// (LDS: the waves' row-free counters).
 template <class IdxT, class LabT>
 __global__ __launch_bounds__(kBlock) void k_fill_global(FitArgs<IdxT, LabT> a) {
   if (a.ctl->err) return;  // a list overflowed: the fit fails, nothing more is read
+  extern __shared__ uint32_t smem[];
+  uint32_t *cnt = smem + (threadIdx.x / kWave) * a.Frl * (uint32_t)a.K;
   WAVE_LOOP(a.ctl->n_fill) {
     const Task tk = a.fill_tasks[item];
     if (tk.node == kLeaf || tk.chunk >= tk.nch) continue;
-    fill_task_bits(a, tk, a.bits + (size_t)a.front[tk.node].tree * a.W);
+    fill_task_bits(a, tk, a.bits + (size_t)a.front[tk.node].tree * a.W, cnt);
   }
 }
 
@@ -1119,16 +1235,7 @@ __global__ __launch_bounds__(kBlock) void k_children(FitArgs<IdxT, LabT> a) {
         orr = max(orr, (uint32_t)__shfl_xor(orr, o, kWave));
       }
       wave_sync();
-      const uint32_t cd = nd.depth + 1;
-      const bool capped = a.max_depth > 0 && (int64_t)cd >= (int64_t)a.max_depth;
-      // 0 live, 1 pure, 2 first-sample, 3 majority
-      auto kind = [&](uint32_t nz, uint32_t nc) -> uint32_t {
-        if (nz <= 1) return 1u;
-        if ((int64_t)nc < (int64_t)a.mss) return 2u;
-        if (capped) return 3u;
-        return 0u;
-      };
-      const uint32_t kl = kind(nzl, nL), kr = kind(nzr, nR);
+      const uint32_t kl = child_kind(a, nzl, nL, nd.depth), kr = child_kind(a, nzr, nR, nd.depth);
       uint32_t labl = ol, labr = orr;
       if (kl == 3) labl = wave_majority(cl, K);
       if (kr == 3) labr = wave_majority(cr, K);
@@ -1284,86 +1391,25 @@ constexpr uint32_t kRlCounters = 1024;  // 2 x Frl x K at most (per-wave LDS cou
 
 // Counts the samples of positions [p0, p1) of the node's feature-0 row into
 // cnt[j][k]: all of them (want = 2), or those whose side bit equals want.
+// Row-free counts of every sample at positions [p0, p1) of node nd's
+// feature-0 row, into the wave's counters.
 template <class IdxT, class LabT>
 __device__ __forceinline__ void rl_count_chunk(const FitArgs<IdxT, LabT> &a, const FNode &nd, uint32_t p0, uint32_t p1,
-                                               uint32_t want, uint32_t *cnt) {
+                                               uint32_t *cnt) {
   const uint32_t lane = lane_id();
-  const uint32_t K = (uint32_t)a.K, FK = a.Frl * K;
   const IdxT *row = rows_of(a, nd.src, nd.tree, 0) + nd.start;
-  const uint32_t *bits = a.bits + (size_t)nd.tree * a.W;
-  const uint32_t Ws = a.Wsig;
-  (void)FK;
-  constexpr int U = 4;  // positions per lane in flight: entries, then their gathers
+  constexpr int U = 4;  // positions per lane in flight
   for (uint32_t pb = p0; pb < p1; pb += U * kWave) {
-    uint32_t e[U], lab[U], side[U], w0[U], w1[U];
+    uint32_t e[U];
+    bool take[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const uint32_t p = pb + (uint32_t)u * kWave + lane;
-      e[u] = (p < p1) ? (uint32_t)row[p] : 0xFFFFFFFFu;
+      take[u] = p < p1;
+      e[u] = take[u] ? (uint32_t)row[p] : 0u;
     }
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-      lab[u] = side[u] = w0[u] = w1[u] = 0;
-      if (e[u] != 0xFFFFFFFFu) {
-        const uint32_t s = sample_of((IdxT)e[u]);
-        lab[u] = a.label((IdxT)e[u]);
-        side[u] = want < 2u ? (bits[s >> 5] >> (s & 31u)) & 1u : 2u;
-        if (side[u] == want) {
-          w0[u] = a.sig[(size_t)s * Ws];
-          if (Ws > 1) w1[u] = a.sig[(size_t)s * Ws + 1];
-        }
-      }
-    }
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-      if (e[u] == 0xFFFFFFFFu || side[u] != want) continue;
-      uint32_t *c = cnt + lab[u];
-      for (uint32_t w = 0; w < Ws; ++w) {
-        uint32_t word = w == 0 ? w0[u] : w == 1 ? w1[u] : a.sig[(size_t)sample_of((IdxT)e[u]) * Ws + w];
-        while (word) {
-          const uint32_t j = w * 32u + (uint32_t)__builtin_ctz(word);
-          atomicAdd(&c[j * K], 1u);
-          word &= word - 1u;
-        }
-      }
-    }
+    rl_add(a, e, take, cnt);
   }
-}
-
-// Smaller side of every split node: counts over the node's fill slots (Cf
-// positions each) of the samples on its smaller side, into rl_small (zeroed
-// by k_decide); k_rl_finish derives both children.
-template <class IdxT, class LabT>
-__global__ __launch_bounds__(kBlock) void k_rl_children(FitArgs<IdxT, LabT> a) {
-  if (a.ctl->err) return;  // a list overflowed: the fit fails, nothing more is read
-  extern __shared__ uint32_t smem[];
-  const uint32_t lane = lane_id();
-  const uint32_t FK = a.Frl * (uint32_t)a.K;
-  uint32_t *cnt = smem + (threadIdx.x / kWave) * kRlCounters;
-  unsigned long long bytes = 0;
-  WAVE_LOOP(min(a.ctl->n_fill, a.cap_fill)) {
-    const Task tk = a.fill_tasks[item];
-    if (tk.node == kLeaf) continue;
-    const NAux ax = a.aux[tk.node];
-    if (!(ax.flags & (kLiveL | kLiveR))) continue;
-    const FNode nd = a.front[tk.node];
-    // side bit 1: left; count the smaller child
-    const uint32_t want = (ax.nL <= nd.n - ax.nL) ? 1u : 0u;
-    for (uint32_t w = lane; w < FK; w += kWave) cnt[w] = 0;
-    wave_sync();
-    const uint32_t c0 = tk.chunk * a.Cf, c1 = min(nd.n, (tk.chunk + 1) * a.Cf);
-    rl_count_chunk(a, nd, c0, c1, want, cnt);
-    // row entry, label and side bit per sample, signature words on one side
-    if (lane == 0)
-      bytes += (unsigned long long)(c1 - c0) * (sizeof(IdxT) + (sizeof(IdxT) == 4 ? 0 : sizeof(LabT)) + 1) +
-               (unsigned long long)min(ax.nL, nd.n - ax.nL) * (c1 - c0) / max(nd.n, 1u) * 4 * a.Wsig;
-    wave_sync();
-    uint32_t *dst = a.rl_small + (size_t)tk.node * FK;
-    for (uint32_t w = lane; w < FK; w += kWave)
-      if (cnt[w]) atomicAdd(&dst[w], cnt[w]);
-    wave_sync();
-  }
-  flush_bytes(a, kBytesRl, bytes);
 }
 
 // Children's counts: the smaller child's as counted, the other's as the
@@ -1408,7 +1454,7 @@ __global__ __launch_bounds__(kBlock) void k_rl_front(FitArgs<IdxT, LabT> a) {
     for (uint32_t c = blockIdx.x * kWavesPerBlock + wave; c < nchunk; c += gridDim.x * kWavesPerBlock) {
       for (uint32_t w = lane; w < FK; w += kWave) cnt[w] = 0;
       wave_sync();
-      rl_count_chunk(a, nd, c * a.Cf, min(nd.n, (c + 1) * a.Cf), 2u, cnt);
+      rl_count_chunk(a, nd, c * a.Cf, min(nd.n, (c + 1) * a.Cf), cnt);
       wave_sync();
       uint32_t *dst = const_cast<uint32_t *>(a.rlc) + (size_t)i * FK;
       for (uint32_t w = lane; w < FK; w += kWave)

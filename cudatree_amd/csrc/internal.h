@@ -27,7 +27,7 @@ struct NAux {
   uint32_t hist_base;                          // class histograms of multi-chunk scans
   int32_t best_f;                              // -1: no candidate (leaf)
   uint32_t best_p, nL;
-  uint32_t flags;                              // kSplit | kLiveL | kLiveR
+  uint32_t flags;                              // kSplit | kLiveL | kLiveR, kCountRl
   uint32_t dst;                                // rows receiving the children
   uint32_t fill_base, nch_f;                   // bitmap-fill task slots: first, count (ceil(n / Cf))
   uint32_t resh_big;                           // 1: rows reshuffled by whole workgroups
@@ -40,6 +40,10 @@ struct NAux {
   uint32_t pad;
 };
 constexpr uint32_t kSplit = 1u, kLiveL = 2u, kLiveR = 4u;
+// Set by k_decide on a split node with a live child when there are row-free
+// features: its fill tasks count the smaller side (k_children then resets
+// the flags to kSplit | kLiveL | kLiveR).
+constexpr uint32_t kCountRl = 8u;
 
 // A unit of wave work: node, first row/feature slot and count (low/high
 // 16 bits), chunk index and the node's chunk count.
