@@ -259,27 +259,55 @@ __global__ __launch_bounds__(kBlock) void k_select(FitArgs<IdxT, LabT> a) {
     uint32_t nf = 0;    // evaluated features with rows (they get scan tasks)
     uint32_t nsel = 0;  // evaluated features
     Cand rlb = {0, 1, 0, -1, 0};  // best candidate of the evaluated row-free features
+    const uint32_t *tc = a.cnt + (size_t)i * K;  // the node's class counts
     for (uint32_t k0 = 0; k0 < F && nsel < m; k0 += kWave) {
       const uint32_t k = k0 + lane;
       bool nonconst = false;
       uint32_t f = 0;
       int32_t jr = -1;
-      const uint32_t *oc = nullptr;
+      Cand cd = {0, 1, 0, -1, 0};  // a row-free feature's one candidate
       if (k < F) {
         f = perm_at(pm, k);
+        // the feature's tables read together
         jr = a.Frl ? a.rl_of[f] : -1;
+        const int32_t slot = a.slot_of[f];
+        const uint64_t fc = a.fcode[f];
         if (jr >= 0) {
-          // row-free: constant when no or every sample holds the higher value
-          oc = a.rlc + ((size_t)i * a.Frl + (uint32_t)jr) * K;
+          // row-free: constant when no or every sample holds the higher
+          // value; its one candidate sends the samples holding the lower
+          // value left (zeros = n - ones), scored with the integers of the
+          // full scan.  The class counts are loaded eight at a time.
+          const uint32_t *oc = a.rlc + ((size_t)i * a.Frl + (uint32_t)jr) * K;
+          uint64_t SL = 0, Dd = 0, T2 = 0;
           uint32_t ones = 0;
-          for (uint32_t c = 0; c < K; ++c) ones += oc[c];
+          for (uint32_t c0 = 0; c0 < K; c0 += 8) {
+            uint32_t o[8], t[8];
+#pragma unroll
+            for (uint32_t u = 0; u < 8; ++u) {
+              o[u] = c0 + u < K ? oc[c0 + u] : 0u;
+              t[u] = c0 + u < K ? tc[c0 + u] : 0u;
+            }
+#pragma unroll
+            for (uint32_t u = 0; u < 8; ++u) {
+              const uint64_t cl = t[u] - o[u];
+              SL += cl * cl;
+              Dd += (uint64_t)t[u] * cl;
+              T2 += (uint64_t)t[u] * t[u];
+              ones += o[u];
+            }
+          }
           nonconst = ones != 0u && ones != n;
+          const uint64_t nL = n - ones, nR = ones;
+          const uint64_t SR = T2 + SL - 2 * Dd;
+          cd = {SL * nR + SR * nL, nL * nR, (uint32_t)nL - 1u, (int32_t)f, 1u};
         } else {
-          const IdxT *row = rows_of(a, nd.src, nd.tree, f) + nd.start;
+          const IdxT *row = (nd.src == 0 ? a.si0 + (size_t)f * a.N
+                                         : (nd.src == 1 ? a.buf0 : a.buf1) +
+                                               ((size_t)nd.tree * a.Fr + (uint32_t)slot) * a.N) +
+                            nd.start;
           const uint32_t s0 = sample_of(row[0]), s1 = sample_of(row[nd.n - 1]);
           // constant in the node: first and last sorted values equal (value
           // codes when the feature has them, else the floats)
-          const uint64_t fc = a.fcode[f];
           const uint32_t kind = fc_kind(fc);
           if (kind == 0) {
             const float *x = a.X_T + (size_t)f * a.N;
@@ -303,23 +331,7 @@ __global__ __launch_bounds__(kBlock) void k_select(FitArgs<IdxT, LabT> a) {
         const uint32_t cnt = (uint32_t)__popcll(ballot(c == x));
         if (lane == x) ncls += cnt * wt;
       }
-      if (take && jr >= 0) {
-        // its one candidate: the samples holding the lower value go left
-        // (zeros = n - ones), scored with the integers of the full scan
-        uint64_t SL = 0, Dd = 0, T2 = 0;
-        uint32_t ones = 0;
-        for (uint32_t cc = 0; cc < K; ++cc) {
-          const uint64_t t = a.cnt[(size_t)i * K + cc], o = oc[cc], cl = t - o;
-          SL += cl * cl;
-          Dd += t * cl;
-          T2 += t * t;
-          ones += (uint32_t)o;
-        }
-        const uint64_t nL = n - ones, nR = ones;
-        const uint64_t SR = T2 + SL - 2 * Dd;
-        const Cand cd = {SL * nR + SR * nL, nL * nR, (uint32_t)nL - 1u, (int32_t)f, 1u};
-        if (cand_better(cd, rlb)) rlb = cd;
-      }
+      if (take && jr >= 0 && cand_better(cd, rlb)) rlb = cd;
       nf += (uint32_t)__popcll(mrow);
       nsel += min((uint32_t)__popcll(mk), m - nsel);
     }
