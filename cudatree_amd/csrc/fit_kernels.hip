@@ -1017,6 +1017,9 @@ __device__ __forceinline__ void rl_add(const FitArgs<IdxT, LabT> &a, const uint3
 // One fill task: the side bits of Cf positions of a split node, and, when
 // the node is flagged kCountRl, the row-free counts of its smaller side's
 // samples among them, added to rl_small through the wave's counters cnt.
+// positions per lane in flight in a fill task
+constexpr int kFillU = 8;
+
 template <class IdxT, class LabT>
 __device__ __forceinline__ void fill_task_bits(const FitArgs<IdxT, LabT> &a, const Task &tk, uint32_t *bits,
                                                uint32_t *cnt) {
@@ -1037,27 +1040,27 @@ __device__ __forceinline__ void fill_task_bits(const FitArgs<IdxT, LabT> &a, con
     const uint32_t p0 = tk.chunk * a.Cf, p1 = min(nd.n, p0 + a.Cf);
     // the feature's bit column (N / 8 bytes) rather than the signatures
     const uint32_t *col = reinterpret_cast<const uint32_t *>(a.codes + fc_off(a.fcode[ax.best_f]));
-    for (uint32_t pb = p0; pb < p1; pb += 4 * kWave) {
-      uint32_t e[4], low[4];
+    for (uint32_t pb = p0; pb < p1; pb += kFillU * kWave) {
+      uint32_t e[kFillU], low[kFillU];
 #pragma unroll
-      for (int u = 0; u < 4; ++u) {
+      for (int u = 0; u < kFillU; ++u) {
         const uint32_t p = pb + (uint32_t)u * kWave + lane;
         e[u] = (p < p1) ? (uint32_t)row[p] : 0xFFFFFFFFu;
       }
 #pragma unroll
-      for (int u = 0; u < 4; ++u) {
+      for (int u = 0; u < kFillU; ++u) {
         const uint32_t s = sample_of((IdxT)e[u]);
         low[u] = (e[u] != 0xFFFFFFFFu) ? !((col[s >> 5] >> (s & 31u)) & 1u) : 0u;
       }
 #pragma unroll
-      for (int u = 0; u < 4; ++u) {
+      for (int u = 0; u < kFillU; ++u) {
         const uint32_t s = sample_of((IdxT)e[u]);
         if (low[u]) atomicOr(&bits[s >> 5], 1u << (s & 31u));
       }
       if (count) {
-        bool take[4];
+        bool take[kFillU];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) take[u] = e[u] != 0xFFFFFFFFu && (low[u] != 0u) == small_left;
+        for (int u = 0; u < kFillU; ++u) take[u] = e[u] != 0xFFFFFFFFu && (low[u] != 0u) == small_left;
         rl_add(a, e, take, cnt);
       }
     }
@@ -1067,22 +1070,22 @@ __device__ __forceinline__ void fill_task_bits(const FitArgs<IdxT, LabT> &a, con
     const IdxT *row = rows_of(a, nd.src, nd.tree, (uint32_t)ax.best_f) + nd.start;
     const uint32_t nL = ax.nL;
     const uint32_t p0 = tk.chunk * a.Cf, p1 = min(count && !small_left ? nd.n : nL, p0 + a.Cf);
-    for (uint32_t pb = p0; pb < p1; pb += 4 * kWave) {
-      uint32_t e[4];
+    for (uint32_t pb = p0; pb < p1; pb += kFillU * kWave) {
+      uint32_t e[kFillU];
 #pragma unroll
-      for (int u = 0; u < 4; ++u) {
+      for (int u = 0; u < kFillU; ++u) {
         const uint32_t p = pb + (uint32_t)u * kWave + lane;
         e[u] = (p < p1) ? (uint32_t)row[p] : 0xFFFFFFFFu;
       }
 #pragma unroll
-      for (int u = 0; u < 4; ++u) {
+      for (int u = 0; u < kFillU; ++u) {
         const uint32_t p = pb + (uint32_t)u * kWave + lane, s = sample_of((IdxT)e[u]);
         if (e[u] != 0xFFFFFFFFu && p < nL) atomicOr(&bits[s >> 5], 1u << (s & 31u));
       }
       if (count) {
-        bool take[4];
+        bool take[kFillU];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
+        for (int u = 0; u < kFillU; ++u) {
           const uint32_t p = pb + (uint32_t)u * kWave + lane;
           take[u] = e[u] != 0xFFFFFFFFu && (p < nL) == small_left;
         }
