@@ -1016,18 +1016,48 @@ void presort(cta_ctx *ctx, cta_dataset *ds) {
   uint32_t *v0 = ctx->ws_as<uint32_t>("sort_v0", total), *v1 = ctx->ws_as<uint32_t>("sort_v1", total);
   const uint32_t ntiles = (N + kSortTile - 1) / kSortTile;
   uint32_t *hist = ctx->ws_as<uint32_t>("sort_hist", (size_t)F * kDigits * ntiles);
-  k_sort_init<<<grid_for(total), 256, 0, st>>>(ds->XT.as<float>(), N, F, k0, v0);
-  uint32_t *kin = k0, *vin = v0, *kout = k1, *vout = v1;
+  // the digits that vary over each feature's keys: a pass over a digit that
+  // all keys share keeps their order, so a feature takes only the others
+  uint32_t *mask = ctx->ws_as<uint32_t>("sort_mask", 2 * (size_t)F);
+  k_fill_u32<<<grid_for(F), 256, 0, st>>>(mask, F, 0u);
+  k_fill_u32<<<grid_for(F), 256, 0, st>>>(mask + F, F, ~0u);
+  k_key_mask<<<dim3(std::min<uint32_t>(64, (N + 255) / 256), F), 256, 0, st>>>(ds->XT.as<float>(), N, F, mask);
+  constexpr uint32_t kPasses = 32 / 4;
+  // host table: F masks x 2, then odd[F], then the pass lists (kPasses x F)
+  uint32_t *h = static_cast<uint32_t *>(ctx->host("sort_plan", ((size_t)2 * F + F + kPasses * F) * 4));
+  HIPC(hipMemcpyAsync(h, mask, 2 * (size_t)F * 4, hipMemcpyDeviceToHost, st));
+  ctx->sync();
+  uint32_t *odd = h + 2 * F, *lists = odd + F;
+  uint32_t nlist[kPasses] = {};
+  std::vector<uint32_t> in1(F);
+  for (uint32_t f = 0; f < F; ++f) {
+    const uint32_t vary = h[f] ^ h[F + f];
+    uint32_t p = 0;
+    for (uint32_t s = 0; s < kPasses; ++s) p += ((vary >> (4 * s)) & 15u) ? 1u : 0u;
+    odd[f] = p & 1u;
+    in1[f] = p & 1u;  // where the keys start
+  }
+  for (uint32_t s = 0; s < kPasses; ++s)
+    for (uint32_t f = 0; f < F; ++f)
+      if (((h[f] ^ h[F + f]) >> (4 * s)) & 15u) {
+        lists[(size_t)s * F + nlist[s]++] = f | (in1[f] << 31);
+        in1[f] ^= 1u;
+      }
+  uint32_t *dplan = ctx->ws_as<uint32_t>("sort_plan", (size_t)F + kPasses * F);
+  HIPC(hipMemcpyAsync(dplan, odd, ((size_t)F + kPasses * F) * 4, hipMemcpyHostToDevice, st));
+  k_sort_init<<<grid_for(total), 256, 0, st>>>(ds->XT.as<float>(), N, F, dplan, k0, v0, k1, v1);
   {
     Timed tm(ctx, "k_presort");
-    for (uint32_t shift = 0; shift < 32; shift += 4) {
-      k_sort_hist<<<dim3(ntiles, F), kSortThreads, 0, st>>>(kin, N, ntiles, shift, hist);
-      k_sort_scan<<<F, kSortThreads, 0, st>>>(hist, ntiles);
-      k_sort_scatter<<<dim3(ntiles, F), kSortThreads, 0, st>>>(kin, vin, kout, vout, N, ntiles, shift, hist);
-      std::swap(kin, kout);
-      std::swap(vin, vout);
+    for (uint32_t s = 0; s < kPasses; ++s) {
+      if (!nlist[s]) continue;
+      const SortPass sp = {dplan + F + (size_t)s * F, k0, v0, k1, v1};
+      k_sort_hist<<<dim3(ntiles, nlist[s]), kSortThreads, 0, st>>>(sp, N, ntiles, 4 * s, hist);
+      k_sort_scan<<<nlist[s], kSortThreads, 0, st>>>(sp, hist, ntiles);
+      k_sort_scatter<<<dim3(ntiles, nlist[s]), kSortThreads, 0, st>>>(sp, N, ntiles, 4 * s, hist);
     }
   }
+  // every feature's sorted keys and ids are in buffer 0
+  const uint32_t *kin = k0, *vin = v0;
   ds->si0.alloc(total * sizeof(IdxT));
   if (ds->lab_bits == 8)
     k_sort_finish<IdxT, uint8_t><<<grid_for(total), 256, 0, st>>>(vin, total, ds->y.as<uint8_t>(), ds->si0.as<IdxT>());

@@ -3,8 +3,10 @@
 //   k_transpose        X (N x F, row-major) -> X_T (F x N) through LDS tiles
 //                      (random_forest.py:233)
 //   k_sort_*           stable LSD radix sort of every feature row, 4-bit
-//                      digits, 8 passes; ties keep ascending sample id
-//                      (replaces the host np.argsort of random_forest.py:247)
+//                      digits; a feature takes only the passes of digits
+//                      that vary over its keys; ties keep ascending sample
+//                      id (replaces the host np.argsort of
+//                      random_forest.py:247)
 //   k_predict / k_vote batched traversal of a flattened pre-order forest,
 //                      one thread per (row, tree), votes by integer atomics
 //                      (predict.cu:12-69, random_forest.py:339-377)
@@ -53,22 +55,56 @@ __device__ __forceinline__ uint32_t float_key(float v) {
   return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
 }
 
-__global__ void k_sort_init(const float *XT, uint32_t N, uint32_t F, uint32_t *keys, uint32_t *vals) {
-  const size_t total = (size_t)N * F;
-  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (size_t)gridDim.x * blockDim.x) {
-    keys[i] = float_key(XT[i]);
-    vals[i] = (uint32_t)(i % N);
+// mask[f] |= OR of feature f's keys, mask[F + f] &= their AND (set to 0 and
+// ~0 first): the bits that vary over the feature are their XOR.
+__global__ __launch_bounds__(256) void k_key_mask(const float *XT, uint32_t N, uint32_t F, uint32_t *mask) {
+  const uint32_t f = blockIdx.y;
+  uint32_t o = 0u, n = ~0u;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x) {
+    const uint32_t k = float_key(XT[(size_t)f * N + i]);
+    o |= k;
+    n &= k;
+  }
+#pragma unroll
+  for (int d = 32; d > 0; d >>= 1) {
+    o |= (uint32_t)__shfl_xor((int)o, d, 64);
+    n &= (uint32_t)__shfl_xor((int)n, d, 64);
+  }
+  if (lane_id() == 0) {
+    atomicOr(&mask[f], o);
+    atomicAnd(&mask[F + f], n);
   }
 }
 
-// hist[(f * 16 + d) * ntiles + tile] = count of digit d in the tile.
-__global__ __launch_bounds__(kSortThreads) void k_sort_hist(const uint32_t *keys, uint32_t N, uint32_t ntiles,
-                                                            uint32_t shift, uint32_t *hist) {
+// Keys and ids of every feature, written into buffer 1 for the features
+// with an odd number of passes (odd[f] != 0), else buffer 0, so that every
+// feature ends its passes in buffer 0.
+__global__ void k_sort_init(const float *XT, uint32_t N, uint32_t F, const uint32_t *odd, uint32_t *k0,
+                            uint32_t *v0, uint32_t *k1, uint32_t *v1) {
+  const size_t total = (size_t)N * F;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (size_t)gridDim.x * blockDim.x) {
+    const bool b1 = odd[i / N] != 0u;
+    (b1 ? k1 : k0)[i] = float_key(XT[i]);
+    (b1 ? v1 : v0)[i] = (uint32_t)(i % N);
+  }
+}
+
+// The features of one pass: entry j of `list` is feature f, with bit 31
+// set when its keys are in buffer 1 before the pass.
+struct SortPass {
+  const uint32_t *list;
+  uint32_t *k0, *v0, *k1, *v1;
+};
+
+// hist[(f * 16 + d) * ntiles + tile] = count of digit d in the tile
+// (feature f = entry blockIdx.y of the pass's list).
+__global__ __launch_bounds__(kSortThreads) void k_sort_hist(SortPass sp, uint32_t N, uint32_t ntiles, uint32_t shift,
+                                                            uint32_t *hist) {
   __shared__ uint32_t h[kDigits];
-  const uint32_t tile = blockIdx.x, f = blockIdx.y;
+  const uint32_t tile = blockIdx.x, ent = sp.list[blockIdx.y], f = ent & 0x7FFFFFFFu;
   if (threadIdx.x < kDigits) h[threadIdx.x] = 0;
   __syncthreads();
-  const uint32_t *k = keys + (size_t)f * N;
+  const uint32_t *k = ((ent >> 31) ? sp.k1 : sp.k0) + (size_t)f * N;
   const uint32_t base = tile * kSortTile;
   for (uint32_t j = threadIdx.x; j < (uint32_t)kSortTile; j += kSortThreads) {
     const uint32_t e = base + j;
@@ -78,11 +114,11 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_hist(const uint32_t *keys
   if (threadIdx.x < kDigits) hist[((size_t)f * kDigits + threadIdx.x) * ntiles + tile] = h[threadIdx.x];
 }
 
-// Exclusive scan of each row's 16 x ntiles counts (digit-major): the output
-// offset of digit d's elements of each tile.
-__global__ __launch_bounds__(kSortThreads) void k_sort_scan(uint32_t *hist, uint32_t ntiles) {
+// Exclusive scan of each listed row's 16 x ntiles counts (digit-major): the
+// output offset of digit d's elements of each tile.
+__global__ __launch_bounds__(kSortThreads) void k_sort_scan(SortPass sp, uint32_t *hist, uint32_t ntiles) {
   __shared__ uint32_t part[kSortThreads];
-  uint32_t *h = hist + (size_t)blockIdx.x * kDigits * ntiles;
+  uint32_t *h = hist + (size_t)(sp.list[blockIdx.x] & 0x7FFFFFFFu) * kDigits * ntiles;
   const uint32_t len = kDigits * ntiles;
   const uint32_t per = (len + kSortThreads - 1) / kSortThreads;
   const uint32_t a = threadIdx.x * per, b = min(len, a + per);
@@ -109,14 +145,16 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_scan(uint32_t *hist, uint
 
 // Stable scatter of one tile: rounds of 256 consecutive elements, ranks by
 // one ballot per digit inside each wave, wave and round offsets from LDS.
-__global__ __launch_bounds__(kSortThreads) void k_sort_scatter(const uint32_t *kin, const uint32_t *vin,
-                                                               uint32_t *kout, uint32_t *vout, uint32_t N,
-                                                               uint32_t ntiles, uint32_t shift,
-                                                               const uint32_t *hist) {
+// (from the feature's buffer into the other one)
+__global__ __launch_bounds__(kSortThreads) void k_sort_scatter(SortPass sp, uint32_t N, uint32_t ntiles,
+                                                               uint32_t shift, const uint32_t *hist) {
   __shared__ uint32_t wcnt[kSortThreads / 64][kDigits];
   __shared__ uint32_t run[kDigits];
   __shared__ uint32_t off[kDigits];
-  const uint32_t tile = blockIdx.x, f = blockIdx.y;
+  const uint32_t tile = blockIdx.x, ent = sp.list[blockIdx.y], f = ent & 0x7FFFFFFFu;
+  const bool in1 = ent >> 31;
+  const uint32_t *kin = in1 ? sp.k1 : sp.k0, *vin = in1 ? sp.v1 : sp.v0;
+  uint32_t *kout = in1 ? sp.k0 : sp.k1, *vout = in1 ? sp.v0 : sp.v1;
   const uint32_t lane = lane_id(), w = threadIdx.x / 64;
   if (threadIdx.x < kDigits) {
     run[threadIdx.x] = 0;
