@@ -806,8 +806,12 @@ struct Fitter {
 
   // Split-scan tasks of a level of q nodes: nf * nch per multi-chunk node
   // (at most m n / Cs + m) and ceil(nf / fpt) <= m n / Cs + 1 per
-  // single-chunk node, so at most q + 3 m n_tot / Cs.
-  uint64_t units_for(uint64_t q) const { return q + 3 * (uint64_t)m * (n_tot / Cs + 1) + 64; }
+  // single-chunk node, so at most q + 3 m n_tot / Cs; where fpt is held
+  // down to give a small frontier kScanTaskTarget tasks, ceil(nf / fpt)
+  // sums to at most q + 2 kScanTaskTarget.
+  uint64_t units_for(uint64_t q) const {
+    return q + 3 * (uint64_t)m * (n_tot / Cs + 1) + 2 * (uint64_t)kScanTaskTarget + 64;
+  }
 
   // Grows the task and result lists for q nodes.
   void ensure_units(FitArgs<IdxT, LabT> &a, uint32_t q) {
@@ -941,16 +945,23 @@ struct Fitter {
       ctx->sync();
       doff.assign(hoff, hoff + D + 2);
     }
-    for (int64_t d = D; d >= 0; --d) {
-      const uint32_t b = doff[d], e = doff[d + 1];
-      if (e > b) k_subtree_size<<<grid_for(e - b), 256, 0, st>>>(order, b, e, ar, size);
-    }
-    k_root_sizes<<<grid_for(Tb), 256, 0, st>>>(size, (uint32_t)Tb, toff);
-    k_prefix_serial<<<1, 64, 0, st>>>(toff, (uint32_t)Tb);
-    k_root_pos<<<grid_for(Tb), 256, 0, st>>>((uint32_t)Tb, pos, tr);
-    for (uint32_t d = 0; d <= D; ++d) {
-      const uint32_t b = doff[d], e = doff[d + 1];
-      if (e > b) k_preorder<<<grid_for(e - b), 256, 0, st>>>(order, b, e, ar, size, pos, tr);
+    if (contiguous && D <= kLayoutMaxDepth) {
+      // one workgroup per tree over all depths
+      k_tree_layout<<<Tb, kLayoutThreads, 2 * ((size_t)D + 1) * 4, st>>>(ar, D, size, pos, tr);
+      k_root_sizes<<<grid_for(Tb), 256, 0, st>>>(size, (uint32_t)Tb, toff);
+      k_prefix_serial<<<1, 64, 0, st>>>(toff, (uint32_t)Tb);
+    } else {
+      for (int64_t d = D; d >= 0; --d) {
+        const uint32_t b = doff[d], e = doff[d + 1];
+        if (e > b) k_subtree_size<<<grid_for(e - b), 256, 0, st>>>(order, b, e, ar, size);
+      }
+      k_root_sizes<<<grid_for(Tb), 256, 0, st>>>(size, (uint32_t)Tb, toff);
+      k_prefix_serial<<<1, 64, 0, st>>>(toff, (uint32_t)Tb);
+      k_root_pos<<<grid_for(Tb), 256, 0, st>>>((uint32_t)Tb, pos, tr);
+      for (uint32_t d = 0; d <= D; ++d) {
+        const uint32_t b = doff[d], e = doff[d + 1];
+        if (e > b) k_preorder<<<grid_for(e - b), 256, 0, st>>>(order, b, e, ar, size, pos, tr);
+      }
     }
     k_canon_scatter<<<grid_for(n), 256, 0, st>>>(n, ar, size, pos, tr, toff, co);
     HIPC(hipGetLastError());

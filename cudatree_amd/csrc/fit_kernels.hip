@@ -65,6 +65,9 @@ __device__ __forceinline__ const IdxT *rows_of(const FitArgs<IdxT, LabT> &a, uin
   return b + ((size_t)tree * a.Fr + (uint32_t)a.slot_of[f]) * a.N;
 }
 
+// Split-scan tasks aimed for at a level whose frontier is small.
+constexpr uint32_t kScanTaskTarget = 4096;
+
 // Split-scan class (< kXcd) of the tasks of feature f in frontier node i.
 // Workgroups are dispatched to the XCDs round-robin (workgroup b on XCD
 // b % kXcd, each XCD with its own L2) and k_scan's workgroup b takes class
@@ -252,7 +255,10 @@ __global__ __launch_bounds__(kBlock) void k_select(FitArgs<IdxT, LabT> a) {
     // tasks: one per (feature, chunk) of a multi-chunk node; a single-chunk
     // node's features grouped about Cs positions per task.  A task's class is
     // scan_cls of its (first) feature; lane c counts the tasks of class c.
-    const uint32_t fpt = (nch > 1) ? 1u : max(1u, a.Cs / n);
+    // (a small frontier's nodes grouped less, so that the level still has
+    // about kScanTaskTarget tasks to spread over the waves)
+    const uint32_t fpt =
+        (nch > 1) ? 1u : max(1u, (uint32_t)min<uint64_t>(a.Cs / n, (uint64_t)q * m / kScanTaskTarget));
     const uint32_t wt = (nch > 1) ? nch : 1u;
     const uint32_t K = (uint32_t)a.K;
     uint32_t ncls = 0;
@@ -2133,6 +2139,81 @@ __global__ void k_root_pos(uint32_t T, uint32_t *pos, uint32_t *tr) {
   for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < T; t += gridDim.x * blockDim.x) {
     pos[t] = 0;
     tr[t] = t;
+  }
+}
+
+// Subtree sizes and pre-order positions of the trees of a batch whose node
+// ids were allocated one depth at a time: depth d holds ids
+// [doff[d], doff[d + 1]), grouped by tree in tree order, and a split node's
+// children are allocated as a pair in the order of their parents, so the
+// ids of tree t at depth d + 1 run from the left child of its first split
+// node at depth d to the right child of its last.  Workgroup t handles tree
+// t (root id t): its id range at every depth (kept in LDS, D + 1 of them),
+// then sizes bottom-up and positions top-down, one depth at a time, each
+// depth's writes made visible to the next by the workgroup barrier.
+constexpr int kLayoutThreads = 1024;
+constexpr uint32_t kLayoutMaxDepth = 4095;  // deeper batches take the per-depth kernels
+__global__ __launch_bounds__(kLayoutThreads) void k_tree_layout(Arena ar, uint32_t D, uint32_t *size, uint32_t *pos,
+                                                                uint32_t *tr) {
+  extern __shared__ uint32_t rng[];  // (lo, hi) per depth
+  __shared__ uint32_t red[2][kLayoutThreads / 64];
+  const uint32_t t = blockIdx.x, tid = threadIdx.x, lane = tid & 63u, w = tid >> 6;
+  uint32_t lo = t, hi = t + 1, dmax = 0;
+  for (uint32_t d = 0;; ++d) {
+    if (tid == 0) {
+      rng[2 * d] = lo;
+      rng[2 * d + 1] = hi;
+    }
+    dmax = d;
+    if (d == D) break;
+    uint32_t mn = 0xFFFFFFFFu, mx = 0;
+    for (uint32_t id = lo + tid; id < hi; id += kLayoutThreads)
+      if (ar.feat[id] >= 0) {
+        mn = min(mn, ar.left[id]);
+        mx = max(mx, ar.right[id]);
+      }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      mn = min(mn, (uint32_t)__shfl_xor((int)mn, o, kWave));
+      mx = max(mx, (uint32_t)__shfl_xor((int)mx, o, kWave));
+    }
+    if (lane == 0) {
+      red[0][w] = mn;
+      red[1][w] = mx;
+    }
+    __syncthreads();
+    mn = 0xFFFFFFFFu;
+    mx = 0;
+#pragma unroll
+    for (int k = 0; k < kLayoutThreads / 64; ++k) {
+      mn = min(mn, red[0][k]);
+      mx = max(mx, red[1][k]);
+    }
+    __syncthreads();
+    if (mn == 0xFFFFFFFFu) break;  // no split node: the tree ends at depth d
+    lo = mn;
+    hi = mx + 1;
+  }
+  __syncthreads();
+  for (uint32_t d = dmax + 1; d-- > 0;) {
+    const uint32_t a = rng[2 * d], b = rng[2 * d + 1];
+    for (uint32_t id = a + tid; id < b; id += kLayoutThreads) {
+      size[id] = ar.feat[id] < 0 ? 1u : 1u + size[ar.left[id]] + size[ar.right[id]];
+      tr[id] = t;
+    }
+    __syncthreads();
+  }
+  if (tid == 0) pos[t] = 0;
+  __syncthreads();
+  for (uint32_t d = 0; d < dmax; ++d) {
+    const uint32_t a = rng[2 * d], b = rng[2 * d + 1];
+    for (uint32_t id = a + tid; id < b; id += kLayoutThreads) {
+      if (ar.feat[id] < 0) continue;
+      const uint32_t l = ar.left[id], r = ar.right[id], p = pos[id];
+      pos[l] = p + 1;
+      pos[r] = p + 1 + size[l];
+    }
+    __syncthreads();
   }
 }
 
