@@ -757,8 +757,7 @@ struct Fitter {
       }
       k_fill_lds<IdxT, LabT><<<a.T * J, kFillThreads, fill_lds_bytes(), st>>>(a, J, part);
       if (J > 1)
-        k_merge_bits<<<dim3(std::min<uint32_t>(64, (a.W + 255) / 256), a.T), 256, 0, st>>>(a.tlo, a.thi, a.W, J,
-                                                                                            part, a.bits);
+        k_merge_bits<IdxT, LabT><<<dim3(std::min<uint32_t>(64, (a.W + 255) / 256), a.T), 256, 0, st>>>(a, J, part);
     } else {
       k_fill_u32<<<grid_for((size_t)a.T * a.W), 256, 0, st>>>(a.bits, (size_t)a.T * a.W, 0u);
       k_fill_global<IdxT, LabT><<<G, kBlock, (size_t)kWavesPerBlock * Frl * K * 4, st>>>(a);
@@ -855,11 +854,6 @@ struct Fitter {
       k_children<IdxT, LabT><<<G, kBlock, sm, st>>>(a);
       offsets<kChFields>(a, q, &a.ctl->c_ids);
       k_children_emit<IdxT, LabT><<<G, kBlock, 0, st>>>(a);
-    }
-    if (Frl) {
-      // the smaller sides were counted by the fill tasks
-      Timed tm(ctx, "k_rl_count");
-      k_rl_finish<IdxT, LabT><<<G, kBlock, 0, st>>>(a);
     }
     reshuffle(a);
     k_ctl_next<<<1, kByteKinds * kWave, 0, st>>>(a.ctl, a.bslot);

@@ -1108,6 +1108,18 @@ __device__ __forceinline__ void fill_task_bits(const FitArgs<IdxT, LabT> &a, con
   }
 }
 
+// Fill workgroups of the tree whose frontier is [lo, hi): one per
+// kFillTasksPerGroup of its fill slots [fb, fe), at most J.
+constexpr uint32_t kFillTasksPerGroup = 32;
+template <class IdxT, class LabT>
+__device__ __forceinline__ uint32_t fill_groups(const FitArgs<IdxT, LabT> &a, uint32_t lo, uint32_t hi, uint32_t J,
+                                                uint32_t &fb, uint32_t &fe) {
+  fb = a.aux[lo].fill_base;
+  const NAux last = a.aux[hi - 1];
+  fe = last.fill_base + last.nch_f;
+  return min(J, max(1u, ceil_div(fe - fb, kFillTasksPerGroup)));
+}
+
 template <class IdxT, class LabT>
 __global__ __launch_bounds__(kFillThreads) void k_fill_lds(FitArgs<IdxT, LabT> a, uint32_t J, uint32_t *part) {
   if (a.ctl->err) return;  // a list overflowed: the fit fails, nothing more is read
@@ -1115,34 +1127,40 @@ __global__ __launch_bounds__(kFillThreads) void k_fill_lds(FitArgs<IdxT, LabT> a
   const uint32_t t = blockIdx.x / J, j = blockIdx.x % J;
   const uint32_t lo = a.tlo[t], hi = a.thi[t];
   if (hi <= lo) return;  // no frontier node in this tree
+  uint32_t fb, fe;
+  const uint32_t Je = fill_groups(a, lo, hi, J, fb, fe);
+  if (j >= Je) return;
   const uint32_t W = a.W, tid = threadIdx.x, wave = tid / kWave;
   for (uint32_t w = tid; w < W; w += kFillThreads) lbits[w] = 0;
   __syncthreads();
-  const uint32_t fb = a.aux[lo].fill_base;
-  const NAux last = a.aux[hi - 1];
-  const uint32_t fe = last.fill_base + last.nch_f;
   // the waves' row-free counters follow the bitmap
   uint32_t *cnt = lbits + W + wave * a.Frl * (uint32_t)a.K;
-  for (uint32_t k = fb + j * kFillWaves + wave; k < fe; k += J * kFillWaves) {
+  for (uint32_t k = fb + j * kFillWaves + wave; k < fe; k += Je * kFillWaves) {
     const Task tk = a.fill_tasks[k];
     if (tk.node == kLeaf || tk.chunk >= tk.nch) continue;
     fill_task_bits(a, tk, lbits, cnt);
   }
   __syncthreads();
-  uint32_t *out = (J == 1) ? a.bits + (size_t)t * W : part + ((size_t)t * J + j) * W;
+  uint32_t *out = (Je == 1) ? a.bits + (size_t)t * W : part + ((size_t)t * J + j) * W;
   for (uint32_t w = tid; w < W; w += kFillThreads) out[w] = lbits[w];
 }
 
-// bits of tree blockIdx.y = OR of its J partial bitmaps.
-__global__ void k_merge_bits(const uint32_t *tlo, const uint32_t *thi, uint32_t W, uint32_t J, const uint32_t *part,
-                             uint32_t *bits) {
-  const uint32_t t = blockIdx.y;
-  if (thi[t] <= tlo[t]) return;
+// bits of tree blockIdx.y = OR of its partial bitmaps, when it has more than
+// one.
+template <class IdxT, class LabT>
+__global__ void k_merge_bits(FitArgs<IdxT, LabT> a, uint32_t J, const uint32_t *part) {
+  if (a.ctl->err) return;
+  const uint32_t t = blockIdx.y, lo = a.tlo[t], hi = a.thi[t];
+  if (hi <= lo) return;
+  uint32_t fb, fe;
+  const uint32_t Je = fill_groups(a, lo, hi, J, fb, fe);
+  if (Je == 1) return;  // written whole by its one workgroup
+  const uint32_t W = a.W;
   const uint32_t *src = part + (size_t)t * J * W;
   for (uint32_t w = blockIdx.x * blockDim.x + threadIdx.x; w < W; w += gridDim.x * blockDim.x) {
     uint32_t v = 0;
-    for (uint32_t j = 0; j < J; ++j) v |= src[(size_t)j * W + w];
-    bits[(size_t)t * W + w] = v;
+    for (uint32_t j = 0; j < Je; ++j) v |= src[(size_t)j * W + w];
+    a.bits[(size_t)t * W + w] = v;
   }
 }
 
@@ -1395,6 +1413,15 @@ __global__ __launch_bounds__(kBlock) void k_children_emit(FitArgs<IdxT, LabT> a)
         const uint32_t l = a.cntL[(size_t)i * K + k];
         a.next_cnt[(size_t)slot * K + k] = c ? a.cnt[(size_t)i * K + k] - l : l;
       }
+      if (a.Frl) {
+        // row-free counts: the smaller child's as the fill tasks counted
+        // them, the other's as the parent's minus those
+        const uint32_t FK = a.Frl * K;
+        const bool small = (c == 0) == (ax.nL <= nd.n - ax.nL);
+        const uint32_t *par = a.rlc + (size_t)i * FK, *sm = a.rl_small + (size_t)i * FK;
+        uint32_t *dst = a.rlc_next + (size_t)slot * FK;
+        for (uint32_t w = lane; w < FK; w += kWave) dst[w] = small ? sm[w] : par[w] - sm[w];
+      }
       ++slot;
     }
     emit_resh_tasks(a, i, ax, o[2], o[3]);
@@ -1403,15 +1430,14 @@ __global__ __launch_bounds__(kBlock) void k_children_emit(FitArgs<IdxT, LabT> a)
 
 // ------------------------------------------------------------------------
 // Row-free counts: per node, per row-free feature j and class k, the
-// samples of class k holding the higher value of j.  A wave counts one
-// chunk of a node's feature-0 row into LDS (one atomic per set signature
-// bit, by side), then adds its non-zero counters to the node's (or
-// children's) counts in global memory.
+// samples of class k holding the higher value of j.  The fill tasks count
+// each split node's smaller side and k_children_emit derives the children's
+// counts; the roots (and frontiers set by the host) are counted here, a
+// wave per chunk of a node's feature-0 row into LDS (one atomic per set
+// signature bit), then added to the node's counts in global memory.
 // ------------------------------------------------------------------------
 constexpr uint32_t kRlCounters = 1024;  // 2 x Frl x K at most (per-wave LDS counters)
 
-// Counts the samples of positions [p0, p1) of the node's feature-0 row into
-// cnt[j][k]: all of them (want = 2), or those whose side bit equals want.
 // Row-free counts of every sample at positions [p0, p1) of node nd's
 // feature-0 row, into the wave's counters.
 template <class IdxT, class LabT>
@@ -1430,32 +1456,6 @@ __device__ __forceinline__ void rl_count_chunk(const FitArgs<IdxT, LabT> &a, con
       e[u] = take[u] ? (uint32_t)row[p] : 0u;
     }
     rl_add(a, e, take, cnt);
-  }
-}
-
-// Children's counts: the smaller child's as counted, the other's as the
-// parent's minus them.
-template <class IdxT, class LabT>
-__global__ __launch_bounds__(kBlock) void k_rl_finish(FitArgs<IdxT, LabT> a) {
-  if (a.ctl->err) return;  // a list overflowed: the fit fails, nothing more is read
-  const uint32_t lane = lane_id();
-  const uint32_t FK = a.Frl * (uint32_t)a.K;
-  WAVE_LOOP(a.ctl->q) {
-    const uint32_t i = item;
-    const NAux ax = a.aux[i];
-    if (!(ax.flags & kSplit)) continue;
-    const bool liveL = ax.flags & kLiveL, liveR = ax.flags & kLiveR;
-    if (!liveL && !liveR) continue;
-    const uint32_t n = a.front[i].n;
-    const bool small_left = ax.nL <= n - ax.nL;
-    const uint32_t slot0 = a.offs[(size_t)i * kChFields + 1];
-    const uint32_t *par = a.rlc + (size_t)i * FK, *sm = a.rl_small + (size_t)i * FK;
-    for (uint32_t c = 0; c < 2; ++c) {
-      if (!(c ? liveR : liveL)) continue;
-      const bool small = (c == 0) == small_left;
-      uint32_t *dst = a.rlc_next + (size_t)(slot0 + ((c && liveL) ? 1u : 0u)) * FK;
-      for (uint32_t w = lane; w < FK; w += kWave) dst[w] = small ? sm[w] : par[w] - sm[w];
-    }
   }
 }
 
