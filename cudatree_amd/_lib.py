@@ -49,7 +49,8 @@ class FitOpts(ctypes.Structure):
         ("resh_global", ctypes.c_int32),
         ("bootstrap", ctypes.c_int32),
         ("all_rows", ctypes.c_int32),
-        ("reserved", ctypes.c_int32 * 2),
+        ("lanes", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 1),
     ]
 
 

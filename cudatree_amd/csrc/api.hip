@@ -17,6 +17,7 @@
 #include <mutex>
 #include <stdexcept>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/cudatree_amd.h"
@@ -114,6 +115,9 @@ struct KStat {
 struct cta_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
+  // second fit lane: a context of its own (stream, workspaces, staging) on
+  // the same device, created by the first fit that runs two lanes
+  cta_ctx *lane = nullptr;
   void (*level_hook)(void *) = nullptr;  // called between groups of batched levels
   void *level_hook_arg = nullptr;
   std::recursive_mutex mu;
@@ -245,6 +249,24 @@ struct cta_forest {
 
 namespace {
 
+cta_ctx *ctx_new(int device) {
+  std::unique_ptr<cta_ctx> c(new cta_ctx());
+  c->device = device;
+  HIPC(hipSetDevice(device));
+  HIPC(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  hipDeviceProp_t p;
+  HIPC(hipGetDeviceProperties(&p, device));
+  c->num_cu = p.multiProcessorCount > 0 ? p.multiProcessorCount : 256;
+  c->lds_per_block = p.sharedMemPerBlock > 0 ? p.sharedMemPerBlock : 65536;
+  return c.release();
+}
+
+// The context's second fit lane, created on first use.
+cta_ctx *lane_ctx(cta_ctx *ctx) {
+  if (!ctx->lane) ctx->lane = ctx_new(ctx->device);
+  return ctx->lane;
+}
+
 void upload_forest(cta_forest *f) {
   if (f->device_ready) return;
   const size_t n = f->feature.size();
@@ -285,14 +307,13 @@ void launch_resh(cta_ctx *ctx, const FitArgs<IdxT, LabT> &a, bool global_bits, u
   const size_t lds = (((size_t)a.W + 3) & ~(size_t)3) * 4 + 2 * kSlots * 4;
   hipStream_t st = ctx->stream;
   if (!global_bits && lds <= ctx->lds_per_block) {
-    static bool attr = false;
-    if (!attr) {
+    static std::once_flag attr;
+    std::call_once(attr, [&] {
       HIPC(hipFuncSetAttribute((const void *)k_resh<IdxT, LabT, true, true>,
                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)ctx->lds_per_block));
       HIPC(hipFuncSetAttribute((const void *)k_resh<IdxT, LabT, true, false>,
                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)ctx->lds_per_block));
-      attr = true;
-    }
+    });
     {
       Timed tm(ctx, big_name);
       k_resh<IdxT, LabT, true, true><<<grid, kReshThreads, lds, st>>>(a);
@@ -386,12 +407,11 @@ struct Fitter {
       uint32_t *cnts = ctx->ws_as<uint32_t>("boot_counts", Tb);
       k_fill_u32<<<grid_for(Tb), 256, 0, st>>>(cnts, Tb, 0u);
       if ((size_t)W * 4 <= (size_t)ctx->lds_per_block) {
-        static bool attr = false;
-        if (!attr) {
+        static std::once_flag attr;
+        std::call_once(attr, [&] {
           HIPC(hipFuncSetAttribute((const void *)k_boot_gen_lds, hipFuncAttributeMaxDynamicSharedMemorySize,
                                    (int)ctx->lds_per_block));
-          attr = true;
-        }
+        });
         k_boot_gen_lds<<<Tb, 1024, (size_t)W * 4, st>>>(o.seed, (uint32_t)t0, N, W, bits);
       } else {
         k_fill_u32<<<grid_for((size_t)Tb * W), 256, 0, st>>>(bits, (size_t)Tb * W, 0u);
@@ -749,12 +769,11 @@ struct Fitter {
       // about 32 per workgroup, at most Jcap workgroups per tree
       const uint64_t est = ((uint64_t)q + n_tot / Cf) / std::max<uint32_t>(1, a.T);
       const uint32_t J = (uint32_t)std::min<uint64_t>(Jcap, std::max<uint64_t>(1, (est + 31) / 32));
-      static bool attr = false;
-      if (!attr) {
+      static std::once_flag attr;
+      std::call_once(attr, [&] {
         HIPC(hipFuncSetAttribute((const void *)k_fill_lds<IdxT, LabT>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                  (int)ctx->lds_per_block));
-        attr = true;
-      }
+      });
       k_fill_lds<IdxT, LabT><<<a.T * J, kFillThreads, fill_lds_bytes(), st>>>(a, J, part);
       if (J > 1)
         k_merge_bits<IdxT, LabT><<<dim3(std::min<uint32_t>(64, (a.W + 255) / 256), a.T), 256, 0, st>>>(a, J, part);
@@ -993,21 +1012,113 @@ struct Fitter {
   }
 };
 
+// Trees each of two fit lanes needs at least for the lanes to pay.
+constexpr int32_t kLaneMinTrees = 8;
+
+// Appends trees [t, t + n) of `src` to `dst` (host arrays).
+void append_trees(cta_forest *dst, const cta_forest *src, int32_t t, int32_t n) {
+  const int64_t a = src->tree_off[t], b = src->tree_off[t + n];
+  auto app = [&](auto &d, const auto &s) { d.insert(d.end(), s.begin() + a, s.begin() + b); };
+  app(dst->feature, src->feature);
+  app(dst->threshold, src->threshold);
+  app(dst->left, src->left);
+  app(dst->right, src->right);
+  app(dst->value, src->value);
+  app(dst->nsamp, src->nsamp);
+  app(dst->depth, src->depth);
+  const int64_t base = dst->tree_off.back();
+  for (int32_t k = 1; k <= n; ++k) dst->tree_off.push_back(base + src->tree_off[t + k] - a);
+  dst->T += n;
+}
+
 template <class IdxT, class LabT>
 void fit_impl(cta_ctx *ctx, cta_dataset *ds, int32_t T, const uint32_t *ids, const int64_t *off,
               const cta_fit_opts &o, cta_forest *out) {
   Fitter<IdxT, LabT> fz(ctx, ds, o);
   int32_t per = o.trees_per_batch;
   if (o.mode != CTA_MODE_BATCHED) per = 1;
+  // two lanes: batches alternate between this context's stream and a second
+  // one, so that one lane's short late levels overlap the other's work
+  int32_t lanes = o.mode == CTA_MODE_BATCHED && o.lanes == 2 ? 2 : 1;
   if (per <= 0) {
     size_t fr = 0, tot = 0;
     HIPC(hipMemGetInfo(&fr, &tot));
-    fr += ctx->ws_bytes();  // workspaces are reused by this fit
+    fr += ctx->ws_bytes() + (ctx->lane ? ctx->lane->ws_bytes() : 0);  // workspaces are reused by this fit
     const size_t bpt = fz.bytes_per_tree();
     per = (int32_t)std::max<size_t>(1, (size_t)(0.6 * (double)fr) / bpt);
+    if (lanes == 2) per = std::max<int32_t>(1, per / 2);  // two batches resident at once
   }
-  per = std::min(per, T);
-  for (int32_t t0 = 0; t0 < T; t0 += per) fz.run_batch(t0, std::min(per, T - t0), T, ids, off, out);
+  per = std::min(per, lanes == 2 ? (T + 1) / 2 : T);
+  if (lanes == 2 && (per < kLaneMinTrees || per >= T)) lanes = 1;
+  if (lanes == 1) {
+    for (int32_t t0 = 0; t0 < T; t0 += per) fz.run_batch(t0, std::min(per, T - t0), T, ids, off, out);
+    return;
+  }
+  std::vector<int32_t> b0;  // batch k: trees [b0[k], b0[k] + size), on lane k % 2
+  for (int32_t t0 = 0; t0 < T; t0 += per) b0.push_back(t0);
+  auto bsize = [&](size_t k) { return std::min(per, T - b0[k]); };
+  cta_ctx *lc = lane_ctx(ctx);
+  lc->profiling = ctx->profiling;
+  lc->level_hook = ctx->level_hook;
+  lc->level_hook_arg = ctx->level_hook_arg;
+  cta_forest part[2];
+  for (auto &p : part) {
+    p.ctx = ctx;
+    p.F = out->F;
+    p.K = out->K;
+  }
+  std::exception_ptr lane_err;
+  std::thread th([&] {
+    try {
+      HIPC(hipSetDevice(lc->device));
+      Fitter<IdxT, LabT> fz1(lc, ds, o);
+      for (size_t k = 1; k < b0.size(); k += 2) fz1.run_batch(b0[k], bsize(k), T, ids, off, &part[1]);
+    } catch (...) {
+      lane_err = std::current_exception();
+    }
+  });
+  try {
+    for (size_t k = 0; k < b0.size(); k += 2) fz.run_batch(b0[k], bsize(k), T, ids, off, &part[0]);
+  } catch (...) {
+    th.join();
+    throw;
+  }
+  th.join();
+  if (lane_err) std::rethrow_exception(lane_err);
+  // the lane's kernel times join this context's (its host-side wall times
+  // overlap this lane's and are left out)
+  for (auto &kv : lc->stats) {
+    if (kv.first.rfind("host_", 0) == 0) continue;
+    KStat &d = ctx->stats[kv.first];
+    d.launches += kv.second.launches;
+    d.ms += kv.second.ms;
+    d.bytes += kv.second.bytes;
+  }
+  lc->stats.clear();
+  int32_t pos[2] = {0, 0};
+  for (size_t k = 0; k < b0.size(); ++k) {
+    append_trees(out, &part[k % 2], pos[k % 2], bsize(k));
+    pos[k % 2] += bsize(k);
+  }
+  if (b0.size() == 2) {
+    // one batch per lane: the predict table from the lanes' device tables
+    // (the second's right-child indices shifted past the first's nodes)
+    const int64_t n0 = part[0].tree_off.back(), n1 = part[1].tree_off.back();
+    out->nodes.alloc(sizeof(PNode) * (size_t)std::max<int64_t>(n0 + n1, 1));
+    HIPC(hipMemcpyAsync(out->nodes.p, ctx->ws_as<PNode>("canon_pnode", (size_t)n0), sizeof(PNode) * n0,
+                        hipMemcpyDeviceToDevice, ctx->stream));
+    if (n1)
+      k_pnode_shift<<<grid_for((size_t)n1), 256, 0, ctx->stream>>>(lc->ws_as<PNode>("canon_pnode", (size_t)n1),
+                                                                    (uint32_t)n1, (uint32_t)n0,
+                                                                    out->nodes.as<PNode>() + n0);
+    uint32_t *hr = static_cast<uint32_t *>(ctx->host("lane_roots", ((size_t)T + 1) * 4));
+    for (int32_t t = 0; t <= T; ++t) hr[t] = (uint32_t)out->tree_off[t];
+    out->roots.alloc(((size_t)T + 1) * 4);
+    HIPC(hipMemcpyAsync(out->roots.p, hr, ((size_t)T + 1) * 4, hipMemcpyHostToDevice, ctx->stream));
+    HIPC(hipGetLastError());
+    ctx->sync();
+    out->device_ready = true;
+  }
 }
 
 void build_codes(cta_ctx *ctx, cta_dataset *ds, const uint32_t *keys, const uint32_t *ids);
@@ -1214,21 +1325,17 @@ int cta_ctx_create(int device, cta_ctx **out) {
     int n = 0;
     HIPC(hipGetDeviceCount(&n));
     if (device < 0 || device >= n) fail(CTA_EINVAL, "device %d not present (%d devices)", device, n);
-    std::unique_ptr<cta_ctx> c(new cta_ctx());
-    c->device = device;
-    HIPC(hipSetDevice(device));
-    HIPC(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
-    hipDeviceProp_t p;
-    HIPC(hipGetDeviceProperties(&p, device));
-    c->num_cu = p.multiProcessorCount > 0 ? p.multiProcessorCount : 256;
-    c->lds_per_block = p.sharedMemPerBlock > 0 ? p.sharedMemPerBlock : 65536;
-    *out = c.release();
+    *out = ctx_new(device);
   });
 }
 
 int cta_ctx_destroy(cta_ctx *ctx) {
   return guarded([&] {
     if (!ctx) return;
+    if (ctx->lane) {
+      cta_ctx_destroy(ctx->lane);
+      ctx->lane = nullptr;
+    }
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
     for (auto &p : ctx->pending) {
@@ -1548,6 +1655,7 @@ int cta_ctx_release_workspace(cta_ctx *ctx) {
     HIPC(hipSetDevice(ctx->device));
     ctx->sync();
     ctx->ws_release();
+    if (ctx->lane) ctx->lane->ws_release();  // idle between fits
   });
 }
 

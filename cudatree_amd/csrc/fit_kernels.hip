@@ -916,12 +916,12 @@ __global__ __launch_bounds__(kBlock) void k_decide(FitArgs<IdxT, LabT> a) {
       ax.best_f = (int32_t)f;
       ax.best_p = p;
       ax.nL = p + 1;
-      ax.flags = kSplit;
+      ax.flags = kSplit | (2 * ax.nL > nd.n ? kInvert : 0u);
       ax.dst = (nd.src == 2) ? 1u : 2u;
     } else {
       const uint32_t f = (uint32_t)best.feat, p = best.pos;
       count_left(a, i, nd, ax, f, p, h);
-      nfill = p + 1;  // positions; chunks below
+      nfill = min(p + 1, nd.n - (p + 1));  // the smaller side's positions; chunks below
       if (lane == 0) {
         const IdxT *row = rows_of(a, nd.src, nd.tree, f) + nd.start;
         const float *x = a.X_T + (size_t)f * a.N;
@@ -930,18 +930,17 @@ __global__ __launch_bounds__(kBlock) void k_decide(FitArgs<IdxT, LabT> a) {
         a.arena.feat[id] = (int32_t)f;
         a.arena.thr[id] = thr;
         a.arena.value[id] = 0;
-        bytes += (unsigned long long)nd.n * (sizeof(IdxT) + 1);  // fill: n (b + 1)
+        bytes += (unsigned long long)nfill * (sizeof(IdxT) + 1);  // fill: the smaller side (b + 1)
       }
       ax.best_f = (int32_t)f;
       ax.best_p = p;
       ax.nL = p + 1;
-      ax.flags = kSplit;
+      ax.flags = kSplit | (2 * ax.nL > nd.n ? kInvert : 0u);
       ax.dst = (nd.src == 2) ? 1u : 2u;
     }
     if ((ax.flags & kSplit) && a.Frl) {
       // a live child needs row-free counts: the fill tasks count the smaller
-      // side's samples into rl_small (from zero), over the split feature's
-      // row, whose positions nL .. n - 1 are the right child
+      // side's samples, the ones they mark, into rl_small (from zero)
       const uint32_t K = (uint32_t)a.K, nL = ax.nL, nR = nd.n - ax.nL;
       uint32_t nzl = 0, nzr = 0;
       for (uint32_t k = lane; k < K; k += kWave) {
@@ -953,20 +952,15 @@ __global__ __launch_bounds__(kBlock) void k_decide(FitArgs<IdxT, LabT> a) {
       nzr = uniform(wave_sum32(nzr));
       if (child_kind(a, nzl, nL, nd.depth) == 0u || child_kind(a, nzr, nR, nd.depth) == 0u) {
         ax.flags |= kCountRl;
-        const uint32_t small = min(nL, nR);
-        if (nL > nR) nfill = nd.n;
         for (uint32_t w = lane; w < a.Frl * K; w += kWave) a.rl_small[(size_t)i * a.Frl * K + w] = 0u;
-        // signature words of the smaller side, and its entries past nL of a
-        // split feature's row
-        if (lane == 0)
-          bytes += (unsigned long long)small * 4 * a.Wsig +
-                   (!rowfree && nL > nR ? (unsigned long long)small * sizeof(IdxT) : 0ull);
+        // signature words of the smaller side
+        if (lane == 0) bytes += (unsigned long long)min(nL, nR) * 4 * a.Wsig;
       }
     }
     nfill = ceil_div(nfill, a.Cf);
     // the node's fill slots, one per Cf positions of its rows: the side bits
-    // and the smaller side's row-free counts come from the first nfill; a
-    // node that does not split leaves them empty
+    // of the smaller side and its row-free counts come from the first nfill;
+    // a node that does not split leaves them empty
     for (uint32_t c = lane; c < ax.nch_f; c += kWave)
       a.fill_tasks[ax.fill_base + c] = (ax.flags & kSplit) ? Task{i, 0u, c, nfill} : Task{kLeaf, 0u, 0u, 0u};
     if (lane == 0) a.aux[i] = ax;
@@ -975,8 +969,10 @@ __global__ __launch_bounds__(kBlock) void k_decide(FitArgs<IdxT, LabT> a) {
 }
 
 // ------------------------------------------------------------------------
-// Side bitmaps: bit s of tree t is set when sample s goes to the left child
-// of its node (dfs_module.cu:210-221, bfs_module.cu:238-264).
+// Side bitmaps: bit s of tree t is set when sample s goes to the smaller
+// child of its node: the left one, or the right one when the node is flagged
+// kInvert (dfs_module.cu:210-221, bfs_module.cu:238-264 mark the left
+// child; marking the smaller one halves the marks on average).
 //
 // k_fill_lds: workgroup (t, j) of J per tree builds the bits of its share of
 // tree t's fill tasks in an LDS copy of the bitmap (no global atomics) and
@@ -1020,9 +1016,9 @@ __device__ __forceinline__ void rl_add(const FitArgs<IdxT, LabT> &a, const uint3
   }
 }
 
-// One fill task: the side bits of Cf positions of a split node, and, when
-// the node is flagged kCountRl, the row-free counts of its smaller side's
-// samples among them, added to rl_small through the wave's counters cnt.
+// One fill task: the marks of the smaller side's samples among Cf positions
+// of a split node and, when the node is flagged kCountRl, their row-free
+// counts, added to rl_small through the wave's counters cnt.
 // positions per lane in flight in a fill task
 constexpr int kFillU = 8;
 
@@ -1058,45 +1054,37 @@ __device__ __forceinline__ void fill_task_bits(const FitArgs<IdxT, LabT> &a, con
         const uint32_t s = sample_of((IdxT)e[u]);
         low[u] = (e[u] != 0xFFFFFFFFu) ? !((col[s >> 5] >> (s & 31u)) & 1u) : 0u;
       }
+      // the lower value goes left: the smaller side's samples are marked
+      bool take[kFillU];
 #pragma unroll
       for (int u = 0; u < kFillU; ++u) {
         const uint32_t s = sample_of((IdxT)e[u]);
-        if (low[u]) atomicOr(&bits[s >> 5], 1u << (s & 31u));
+        take[u] = e[u] != 0xFFFFFFFFu && (low[u] != 0u) == small_left;
+        if (take[u]) atomicOr(&bits[s >> 5], 1u << (s & 31u));
       }
-      if (count) {
-        bool take[kFillU];
-#pragma unroll
-        for (int u = 0; u < kFillU; ++u) take[u] = e[u] != 0xFFFFFFFFu && (low[u] != 0u) == small_left;
-        rl_add(a, e, take, cnt);
-      }
+      if (count) rl_add(a, e, take, cnt);
     }
   } else {
-    // split feature's row: positions below nL go left; the right child's
-    // positions are visited only to count a smaller right side
+    // split feature's row: positions below nL go left, so the smaller side
+    // is the range [0, nL) or [nL, n)
     const IdxT *row = rows_of(a, nd.src, nd.tree, (uint32_t)ax.best_f) + nd.start;
-    const uint32_t nL = ax.nL;
-    const uint32_t p0 = tk.chunk * a.Cf, p1 = min(count && !small_left ? nd.n : nL, p0 + a.Cf);
+    const uint32_t r0 = small_left ? 0u : ax.nL, r1 = small_left ? ax.nL : nd.n;
+    const uint32_t p0 = r0 + tk.chunk * a.Cf, p1 = min(r1, p0 + a.Cf);
     for (uint32_t pb = p0; pb < p1; pb += kFillU * kWave) {
       uint32_t e[kFillU];
+      bool take[kFillU];
 #pragma unroll
       for (int u = 0; u < kFillU; ++u) {
         const uint32_t p = pb + (uint32_t)u * kWave + lane;
         e[u] = (p < p1) ? (uint32_t)row[p] : 0xFFFFFFFFu;
+        take[u] = p < p1;
       }
 #pragma unroll
       for (int u = 0; u < kFillU; ++u) {
-        const uint32_t p = pb + (uint32_t)u * kWave + lane, s = sample_of((IdxT)e[u]);
-        if (e[u] != 0xFFFFFFFFu && p < nL) atomicOr(&bits[s >> 5], 1u << (s & 31u));
+        const uint32_t s = sample_of((IdxT)e[u]);
+        if (take[u]) atomicOr(&bits[s >> 5], 1u << (s & 31u));
       }
-      if (count) {
-        bool take[kFillU];
-#pragma unroll
-        for (int u = 0; u < kFillU; ++u) {
-          const uint32_t p = pb + (uint32_t)u * kWave + lane;
-          take[u] = e[u] != 0xFFFFFFFFu && (p < nL) == small_left;
-        }
-        rl_add(a, e, take, cnt);
-      }
+      if (count) rl_add(a, e, take, cnt);
     }
   }
   if (count) {
@@ -1283,6 +1271,7 @@ __global__ __launch_bounds__(kBlock) void k_children(FitArgs<IdxT, LabT> a) {
         // (left) and first unmarked (right) sample of the parent's segment
         const IdxT *row0 = rows_of(a, nd.src, nd.tree, 0) + nd.start;
         const uint32_t *bits = a.bits + (size_t)nd.tree * a.W;
+        const uint32_t inv = (ax.flags & kInvert) ? 1u : 0u;  // the marks are on the right
         bool needl = kl == 2, needr = kr == 2;
         for (uint32_t pb = 0; pb < nd.n && (needl || needr); pb += kWave) {
           const uint32_t p = pb + lane;
@@ -1290,7 +1279,7 @@ __global__ __launch_bounds__(kBlock) void k_children(FitArgs<IdxT, LabT> a) {
           if (p < nd.n) {
             e = (uint32_t)row0[p];
             const uint32_t s = sample_of((IdxT)e);
-            side = (bits[s >> 5] >> (s & 31u)) & 1u;
+            side = ((bits[s >> 5] >> (s & 31u)) & 1u) ^ inv;
           }
           const uint64_t ml = ballot(side == 1), mr = ballot(side == 0);
           if (needl && ml) {
@@ -1305,7 +1294,7 @@ __global__ __launch_bounds__(kBlock) void k_children(FitArgs<IdxT, LabT> a) {
       }
       ax.lab[0] = kl ? labl : kLeaf;
       ax.lab[1] = kr ? labr : kLeaf;
-      uint32_t flags = kSplit;
+      uint32_t flags = kSplit | (ax.flags & kInvert);
       if (kl == 0) flags |= kLiveL;
       if (kr == 0) flags |= kLiveR;
       ax.flags = flags;
@@ -1632,6 +1621,7 @@ __device__ void resh_big(const FitArgs<IdxT, LabT> &a, const Task &tk, const Sid
   const gptr<const IdxT> in = sgpr_ptr(rows_of_slot(a, nd.src, nd.tree, f) + nd.start);
   const gptr<IdxT> out = sgpr_ptr(rows_out(a, ax.dst, nd.tree, f) + nd.start);
   const bool liveL = uniform(ax.flags) & kLiveL, liveR = uniform(ax.flags) & kLiveR;
+  const uint32_t inv = (uniform(ax.flags) & kInvert) ? 1u : 0u;  // marks on the right child
   const uint32_t n = uniform(nd.n), nL = uniform(ax.nL);
   const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
   uint32_t carry = 0;
@@ -1646,7 +1636,7 @@ __device__ void resh_big(const FitArgs<IdxT, LabT> &a, const Task &tk, const Sid
 #pragma unroll
     for (int k = 0; k < kReshE; ++k) {
       const uint32_t p = base + (uint32_t)k * kReshThreads + tid;
-      sdm |= ((p < n) ? side(e[k]) : 0u) << k;
+      sdm |= ((p < n) ? side(e[k]) ^ inv : 0u) << k;
     }
 #pragma unroll
     for (int k = 0; k < kReshE; ++k) {
@@ -1701,6 +1691,7 @@ __device__ void resh_small(const FitArgs<IdxT, LabT> &a, const Task &tk, const S
   const FNode nd = a.front[node];
   const NAux ax = a.aux[node];
   const bool liveL = uniform(ax.flags) & kLiveL, liveR = uniform(ax.flags) & kLiveR;
+  const uint32_t inv = (uniform(ax.flags) & kInvert) ? 1u : 0u;  // marks on the right child
   const uint32_t first = uniform(tk.first_count & 0xFFFFu), count = uniform(tk.first_count >> 16);
   const uint32_t end = first + count;
   const uint32_t n = uniform(nd.n), nL = uniform(ax.nL), N = (uint32_t)a.N;
@@ -1730,7 +1721,7 @@ __device__ void resh_small(const FitArgs<IdxT, LabT> &a, const Task &tk, const S
 #pragma unroll
       for (int k = 0; k < kReshE; ++k) {
         const uint32_t p = pc * kWave + lane;
-        if (r < rows && p < n && side(e[k])) sdm |= 1u << k;
+        if (r < rows && p < n && (side(e[k]) ^ inv)) sdm |= 1u << k;
         if (++pc == c) {
           pc = 0;
           ++r;
@@ -1787,7 +1778,7 @@ __device__ void resh_small(const FitArgs<IdxT, LabT> &a, const Task &tk, const S
 #pragma unroll
     for (int k = 0; k < kReshE; ++k) {
       const uint32_t p = base + (uint32_t)k * kWave + lane;
-      if (p < n && side(e[k])) sdm |= 1u << k;
+      if (p < n && (side(e[k]) ^ inv)) sdm |= 1u << k;
     }
 #pragma unroll
     for (int k = 0; k < kReshE; ++k) {
@@ -2244,6 +2235,16 @@ __global__ void k_canon_scatter(uint32_t n, Arena ar, const uint32_t *size, cons
     pn.right = leaf ? 0u : base + rl;
     pn.value = leaf ? ar.value[id] : 0u;
     o.pnode[g] = pn;
+  }
+}
+
+// dst = src with the right-child index of every split node moved by shift
+// (a part of a predict table placed after `shift` nodes of another).
+__global__ void k_pnode_shift(const PNode *src, uint32_t n, uint32_t shift, PNode *dst) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    PNode p = src[i];
+    if (p.feat >= 0) p.right += shift;
+    dst[i] = p;
   }
 }
 

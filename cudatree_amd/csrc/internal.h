@@ -27,7 +27,7 @@ struct NAux {
   uint32_t hist_base;                          // class histograms of multi-chunk scans
   int32_t best_f;                              // -1: no candidate (leaf)
   uint32_t best_p, nL;
-  uint32_t flags;                              // kSplit | kLiveL | kLiveR, kCountRl
+  uint32_t flags;                              // kSplit | kLiveL | kLiveR, kCountRl, kInvert
   uint32_t dst;                                // rows receiving the children
   uint32_t fill_base, nch_f;                   // bitmap-fill task slots: first, count (ceil(n / Cf))
   uint32_t resh_big;                           // 1: rows reshuffled by whole workgroups
@@ -42,8 +42,11 @@ struct NAux {
 constexpr uint32_t kSplit = 1u, kLiveL = 2u, kLiveR = 4u;
 // Set by k_decide on a split node with a live child when there are row-free
 // features: its fill tasks count the smaller side (k_children then resets
-// the flags to kSplit | kLiveL | kLiveR).
+// the flags to kSplit | kLiveL | kLiveR | kInvert).
 constexpr uint32_t kCountRl = 8u;
+// Set by k_decide on a split node whose right child is the smaller: the side
+// bitmaps mark a split node's smaller child, so its marks are the right one's.
+constexpr uint32_t kInvert = 16u;
 
 // A unit of wave work: node, first row/feature slot and count (low/high
 // 16 bits), chunk index and the node's chunk count.
@@ -147,7 +150,8 @@ struct FitArgs {
   uint32_t *rlc_next;       // the same for the next frontier
   uint32_t *rl_small;       // frontier: [q x Frl x K] counts of each split node's smaller side
   IdxT *buf0, *buf1;  // per tree Fr x N ping-pong rows (row slots)
-  uint32_t *bits;     // per tree W words: bit s set = sample s goes left
+  uint32_t *bits;     // per tree W words: bit s set = sample s goes to its node's smaller child
+                      // (left, or right under kInvert; in-bag marks for the bootstrap compaction)
   uint32_t W;         // words per tree bitmap, (N + 31) / 32
   uint32_t T;         // trees in the batch
   int32_t N, F, K, m, mss, max_depth, identity;

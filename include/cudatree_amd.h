@@ -104,7 +104,10 @@ typedef struct {
   int32_t all_rows;          /* 1: every feature keeps presorted rows; 0 (default): two-valued features
                                 other than feature 0 keep none (their one split follows from class
                                 counts; the trees are identical) */
-  int32_t reserved[2];
+  int32_t lanes;             /* batched mode: 2 runs the batches on two streams of the context at once
+                                (alternate batches, half the trees each when memory allows, each lane holding
+                                at least 8 trees); 0 or 1 (default): one stream.  The trees are identical */
+  int32_t reserved[1];
 } cta_fit_opts;
 
 /* Fills opts with the defaults (batched mode, m = ceil(sqrt(F)) left to the caller). */

@@ -278,6 +278,32 @@ def test_device_bootstrap_matches_oracle_generator(ctx, orc, tpb):
     assert [int(got["n_samples"][got["tree_off"][t]]) for t in range(5)] == [len(i) for i in ids]
 
 
+@pytest.mark.parametrize("T,tpb", [(20, 0), (40, 8)])
+def test_two_lanes_match_one(ctx, orc, T, tpb):
+    """Batches run on two streams at once (lanes=2, alternate batches) give
+    the trees and the predictions of one stream: one batch per lane (predict
+    table joined on the device) and five batches (joined on the host)."""
+    X, y, Xt, _ = datasets.covtype_like(seed=23, n_train=20000, n_test=3000)
+    ds = _lib.Dataset(ctx, X, y, 7)
+    out = {}
+    for lanes in (1, 2):
+        o = _lib.default_opts()
+        o.seed = 77
+        o.max_features = 7
+        o.bootstrap = 1
+        o.trees_per_batch = tpb
+        o.lanes = lanes
+        print("lanes=%d T=%d tpb=%d" % (lanes, T, tpb), flush=True)
+        f = _lib.Forest.fit(ctx, ds, T, o, None)
+        out[lanes] = (f.export(), f.predict(Xt))
+    assert helpers.forest_diff(out[2][0], out[1][0]) is None
+    assert np.array_equal(out[2][1], out[1][1])
+    if tpb == 0:
+        ids = orc.bootstrap_forest_ids(77, T, len(y))
+        want = orc.fit(X, y, 7, T, 7, boot_ids=ids, seed=77)
+        assert helpers.forest_diff(out[2][0], want) is None
+
+
 def test_level_hook_runs_between_level_groups(ctx, orc):
     """The context's level hook runs on the fitting thread between groups of
     levels; the trees do not change."""
