@@ -1,7 +1,7 @@
 """Benchmark: 50-tree random-forest fit at 581,012 x 54 and batched predict on
 one MI355X, with the split-scan / reshuffle roofline and CPU baselines.
 
-    python bench.py [--gpus 1] [--steps 3] [--warmup 1]
+    python bench.py [--gpus 1] [--steps 10] [--warmup 3]
 
 A step is one complete fit of 50 trees (device transpose + presort, bootstrap
 compaction, every level of every tree, export of the canonical trees into
@@ -217,8 +217,8 @@ def pmc_summary(fetch_csv, write_csv, l2_csv, out):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--cpu-trees", type=int, default=10, help="trees in the bounded CPU-oracle sample")
     ap.add_argument("--per-node-trees", type=int, default=1, help="trees in the per-node GPU sample (0: skip)")
     ap.add_argument("--sklearn-trees", type=int, default=8, help="trees in the scikit-learn sample (0: skip)")
