@@ -338,10 +338,11 @@ __global__ __launch_bounds__(kCodeThreads) void k_code_scatter(const uint32_t *i
                                                                 uint32_t *d) {
   const uint32_t f = blockIdx.y;
   const uint32_t base = tsum[(size_t)f * ntiles + blockIdx.x];
-  const uint32_t r0 = blockIdx.x * kCodeTile + threadIdx.x * kCodeItems;
+  // each element stands alone here: consecutive threads take consecutive
+  // positions (coalesced reads; the writes are a scatter by sample id)
 #pragma unroll
   for (int j = 0; j < kCodeItems; ++j) {
-    const uint32_t r = r0 + j;
+    const uint32_t r = blockIdx.x * kCodeTile + (uint32_t)j * kCodeThreads + threadIdx.x;
     if (r < N) d[(size_t)f * N + ids[(size_t)f * N + r]] = base + pre[(size_t)f * N + r];
   }
 }
@@ -355,13 +356,21 @@ __global__ void k_code_pack(const uint32_t *d, uint32_t N, const uint64_t *fcode
   if (kind == 0) return;
   const uint32_t *src = d + (size_t)f * N;
   uint8_t *dst = codes + fc_off(fc);
-  const uint32_t n = (kind == 1) ? (N + 31) / 32 : N;
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    if (kind == 1) {
-      uint32_t w = 0;
-      for (uint32_t j = 0; j < 32 && 32 * i + j < N; ++j) w |= (src[32 * i + j] & 1u) << j;
-      reinterpret_cast<uint32_t *>(dst)[i] = w;
-    } else if (kind == 2) {
+  if (kind == 1) {
+    // bits: one coalesced read per sample, a ballot per 64 samples, the
+    // wave's two words written by lanes 0 and 32
+    const uint32_t lane = threadIdx.x & 63u;
+    uint32_t *w32 = reinterpret_cast<uint32_t *>(dst);
+    for (uint32_t base = blockIdx.x * blockDim.x + (threadIdx.x & ~63u); base < N; base += gridDim.x * blockDim.x) {
+      const uint32_t i = base + lane;
+      const uint64_t m = ballot(i < N && (src[i] & 1u));
+      if (lane == 0) w32[base / 32] = (uint32_t)m;
+      if (lane == 32 && base + 32 < N) w32[base / 32 + 1] = (uint32_t)(m >> 32);
+    }
+    return;
+  }
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x) {
+    if (kind == 2) {
       dst[i] = (uint8_t)src[i];
     } else {
       reinterpret_cast<uint16_t *>(dst)[i] = (uint16_t)src[i];
