@@ -877,15 +877,14 @@ struct Fitter {
       offsets<kChFields>(a, q, &a.ctl->c_ids);
       k_children_emit<IdxT, LabT><<<G, kBlock, 0, st>>>(a);
     }
-    reshuffle(a);
-    k_ctl_next<<<1, kByteKinds * kWave, 0, st>>>(a.ctl, a.bslot);
+    reshuffle(a);  // its plan kernel also ends the level
     HIPC(hipGetLastError());
   }
 
   // Plan and partition of every row of the split nodes (task lists written);
   // `boot`: the bootstrap compaction, timed under its own name.
   void reshuffle(const FitArgs<IdxT, LabT> &a, bool boot = false) {
-    k_plan<IdxT, LabT><<<1, kPlanThreads, 0, st>>>(a, (uint32_t)std::max(8, ctx->num_cu * 4));
+    k_plan<IdxT, LabT><<<1, kPlanThreads, 0, st>>>(a, (uint32_t)std::max(8, ctx->num_cu * 4), boot ? 0u : 1u);
     const uint32_t grid = (uint32_t)std::max(8, ctx->num_cu * 2);
     if (boot) launch_resh(ctx, a, o.resh_global != 0, grid, "k_boot_compact", "k_boot_compact");
     else launch_resh(ctx, a, o.resh_global != 0, grid);
@@ -1957,7 +1956,7 @@ static void hook_reshuffle(cta_ctx *ctx, cta_dataset *ds, const uint32_t *rows, 
   a.cap_big = (uint32_t)F;
   a.cap_small = std::max(v[3], 1u);
   k_emit_resh<IdxT, LabT><<<8, kBlock, 0, st>>>(a);
-  k_plan<IdxT, LabT><<<1, kPlanThreads, 0, st>>>(a, 16u);
+  k_plan<IdxT, LabT><<<1, kPlanThreads, 0, st>>>(a, 16u, 0u);
   launch_resh(ctx, a, global_bits != 0, 16u);
   HIPC(hipGetLastError());
   std::vector<IdxT> hout(hrows.size());

@@ -1507,8 +1507,38 @@ __global__ __launch_bounds__(kBlock) void k_emit_resh(FitArgs<IdxT, LabT> a) {
 // ------------------------------------------------------------------------
 constexpr int kPlanThreads = 1024;
 
+// End of a level: the children-scan totals become the next frontier size
+// and the ids in use (also recorded per level in lv_q / lv_nodes); the
+// spread byte counters are folded into Ctl.  Run by the first kByteKinds
+// waves of one workgroup, one wave per counter kind.
+__device__ __forceinline__ void ctl_next_body(Ctl *ctl, unsigned long long *bslot, uint32_t tid) {
+  if (tid >= kByteKinds * kWave) return;
+  const uint32_t kind = tid / kWave, slot = tid % kWave;
+  unsigned long long v = 0;
+  for (uint32_t s = slot; s < kByteSlots; s += kWave) {
+    unsigned long long *p = bslot + ((size_t)kind * kByteSlots + s) * kByteStride;
+    v += *p;
+    *p = 0;
+  }
+  v = wave_sum64(v);
+  if (slot == 0 && v) (&ctl->bytes_scan)[kind] += v;
+  if (tid == 0) {
+    ctl->q = ctl->c_live;
+    ctl->n_nodes += ctl->c_ids;
+    ctl->c_live = 0;
+    ctl->c_ids = 0;
+    // per-level record, so the host can read several levels at once
+    const uint32_t d = (ctl->depth + 1) % kLevelRing;
+    ctl->depth += 1;
+    ctl->lv_nodes[d] = ctl->n_nodes;
+    ctl->lv_q[d] = ctl->q;
+  }
+}
+
+// With end_level set, the workgroup then ends the level (ctl_next_body):
+// the reshuffle launches that follow read only the plan.
 template <class IdxT, class LabT>
-__global__ __launch_bounds__(kPlanThreads) void k_plan(FitArgs<IdxT, LabT> a, uint32_t target) {
+__global__ __launch_bounds__(kPlanThreads) void k_plan(FitArgs<IdxT, LabT> a, uint32_t target, uint32_t end_level) {
   constexpr int kW = kPlanThreads / kWave;
   __shared__ uint32_t ws[2][kW];
   __shared__ uint32_t carry[2];
@@ -1581,6 +1611,10 @@ __global__ __launch_bounds__(kPlanThreads) void k_plan(FitArgs<IdxT, LabT> a, ui
     if (tot_big > a.cap_big || tot_small > a.cap_small || a.ctl->err) P = 0;
     a.wg_off[(size_t)tid * (T + 1) + T] = P;
     a.ctl->n_plan[tid] = P;
+  }
+  if (end_level) {
+    __syncthreads();  // every thread has read this level's counts
+    ctl_next_body(a.ctl, a.bslot, tid);
   }
 }
 
@@ -2248,32 +2282,9 @@ __global__ void k_pnode_shift(const PNode *src, uint32_t n, uint32_t shift, PNod
   }
 }
 
-// Moves the next frontier's count into place and clears the level counters.
-// End of a level: the children-scan totals become the next frontier size
-// and the ids in use (also recorded per level in lv_q / lv_nodes); the spread byte
-// counters are folded into Ctl.  One
-// block of kByteKinds waves, one per counter kind.
+// End of a level (see ctl_next_body), alone.
 __global__ __launch_bounds__(kByteKinds * kWave) void k_ctl_next(Ctl *ctl, unsigned long long *bslot) {
-  const uint32_t tid = threadIdx.x, kind = tid / kWave, slot = tid % kWave;
-  unsigned long long v = 0;
-  for (uint32_t s = slot; s < kByteSlots; s += kWave) {
-    unsigned long long *p = bslot + ((size_t)kind * kByteSlots + s) * kByteStride;
-    v += *p;
-    *p = 0;
-  }
-  v = wave_sum64(v);
-  if (slot == 0 && v) (&ctl->bytes_scan)[kind] += v;
-  if (tid == 0) {
-    ctl->q = ctl->c_live;
-    ctl->n_nodes += ctl->c_ids;
-    ctl->c_live = 0;
-    ctl->c_ids = 0;
-    // per-level record, so the host can read several levels at once
-    const uint32_t d = (ctl->depth + 1) % kLevelRing;
-    ctl->depth += 1;
-    ctl->lv_nodes[d] = ctl->n_nodes;
-    ctl->lv_q[d] = ctl->q;
-  }
+  ctl_next_body(ctl, bslot, threadIdx.x);
 }
 
 // Sets the frontier size for a host-provided frontier.
