@@ -75,7 +75,7 @@ struct Ctl {
   uint32_t q;        // frontier size
   uint32_t n_nodes;  // arena ids in use
   uint32_t err;
-  uint32_t depth;      // levels completed (k_ctl_next calls)
+  uint32_t depth;      // levels completed (ends of a level: ctl_next_body)
   uint32_t n_plan[2];  // reshuffle plan entries: big tasks, small tasks
   uint32_t pad1[2];
   uint32_t n_scan, n_hist, n_fill, n_cls[kXcd], pad2;    // selection-scan totals
