@@ -151,6 +151,20 @@ struct cta_ctx {
     return b;
   }
   void ws_release() { wspace.clear(); }
+  // Pinned host memory the device writes directly (mapped, coherent): the
+  // host pointer; *dev receives the device's address for it.
+  void *mapped(const std::string &key, size_t bytes, void **dev) {
+    auto &slot = pinned[key];
+    if (slot.second < bytes) {
+      if (slot.first) (void)hipHostFree(slot.first);
+      slot.first = nullptr;
+      slot.second = 0;
+      HIPC(hipHostMalloc(&slot.first, std::max<size_t>(bytes, 64), hipHostMallocMapped | hipHostMallocCoherent));
+      slot.second = bytes;
+    }
+    HIPC(hipHostGetDevicePointer(dev, slot.first, 0));
+    return slot.first;
+  }
   void *host(const std::string &key, size_t bytes) {
     auto &slot = pinned[key];
     if (slot.second < bytes) {
@@ -609,6 +623,16 @@ struct Fitter {
       // a frontier at most doubles per level, so the host sizes launches for
       // that bound, and levels issued after the frontier empties do nothing.
       const uint32_t group = lvl ? 1u : kLevelGroup;
+      // The last level of a group copies the control block to host-mapped
+      // memory and an event marks that copy, so the host reads the counts
+      // and queues the next group while that level's reshuffle still runs.
+      void *mirror_dev = nullptr;
+      Ctl *mirror = lvl ? nullptr : static_cast<Ctl *>(ctx->mapped("ctl_mirror", sizeof(Ctl), &mirror_dev));
+      struct EventOf {
+        cta_ctx *c;
+        hipEvent_t e;
+        ~EventOf() { c->pool.push_back(e); }
+      } ctl_ev{ctx, ctx->ev()};
       while (q > 0) {
         std::map<std::string, KStat> before;
         if (lvl) before = ctx->stats;
@@ -625,12 +649,20 @@ struct Fitter {
             a.rlc_next = rl_buf(cur ^ 1);
             a.rl_small = rl_buf(2);
           }
-          level(a, qb);
+          const bool last = g + 1 == group && mirror;
+          a.ctl_host = last ? static_cast<Ctl *>(mirror_dev) : nullptr;
+          level(a, qb, last ? ctl_ev.e : nullptr);
+          a.ctl_host = nullptr;
           cur ^= 1;
           qb = (uint32_t)std::min<uint64_t>(2ull * qb, a.cap_q);
         }
         Ctl prev = hc;
-        read_ctl(a, hc);
+        if (mirror) {
+          HIPC(hipEventSynchronize(ctl_ev.e));
+          std::memcpy(&hc, mirror, sizeof(Ctl));
+        } else {
+          read_ctl(a, hc);
+        }
         account(prev, hc);
         if (hc.err) fail(CTA_EINTERNAL, "device capacity check failed (err %u)", hc.err);
         if (lvl) {
@@ -848,7 +880,9 @@ struct Fitter {
 
   // One level over the frontier in a.front (ctl->q nodes, sorted by tree;
   // the host knows only a bound q on that count).
-  void level(FitArgs<IdxT, LabT> &a, uint32_t q) {
+  // ev (optional): recorded after the level's plan kernel, which ends the
+  // level (and copies the control block to a.ctl_host when set).
+  void level(FitArgs<IdxT, LabT> &a, uint32_t q, hipEvent_t ev = nullptr) {
     const size_t sm = smem();
     ensure_units(a, q);
     {
@@ -877,14 +911,15 @@ struct Fitter {
       offsets<kChFields>(a, q, &a.ctl->c_ids);
       k_children_emit<IdxT, LabT><<<G, kBlock, 0, st>>>(a);
     }
-    reshuffle(a);  // its plan kernel also ends the level
+    reshuffle(a, false, ev);  // its plan kernel also ends the level
     HIPC(hipGetLastError());
   }
 
   // Plan and partition of every row of the split nodes (task lists written);
   // `boot`: the bootstrap compaction, timed under its own name.
-  void reshuffle(const FitArgs<IdxT, LabT> &a, bool boot = false) {
+  void reshuffle(const FitArgs<IdxT, LabT> &a, bool boot = false, hipEvent_t ev = nullptr) {
     k_plan<IdxT, LabT><<<1, kPlanThreads, 0, st>>>(a, (uint32_t)std::max(8, ctx->num_cu * 4), boot ? 0u : 1u);
+    if (ev) HIPC(hipEventRecord(ev, st));
     const uint32_t grid = (uint32_t)std::max(8, ctx->num_cu * 2);
     if (boot) launch_resh(ctx, a, o.resh_global != 0, grid, "k_boot_compact", "k_boot_compact");
     else launch_resh(ctx, a, o.resh_global != 0, grid);

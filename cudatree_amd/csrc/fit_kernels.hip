@@ -1536,7 +1536,8 @@ __device__ __forceinline__ void ctl_next_body(Ctl *ctl, unsigned long long *bslo
 }
 
 // With end_level set, the workgroup then ends the level (ctl_next_body):
-// the reshuffle launches that follow read only the plan.
+// the reshuffle launches that follow read only the plan.  With a.ctl_host
+// set it then copies the control block there.
 template <class IdxT, class LabT>
 __global__ __launch_bounds__(kPlanThreads) void k_plan(FitArgs<IdxT, LabT> a, uint32_t target, uint32_t end_level) {
   constexpr int kW = kPlanThreads / kWave;
@@ -1615,6 +1616,15 @@ __global__ __launch_bounds__(kPlanThreads) void k_plan(FitArgs<IdxT, LabT> a, ui
   if (end_level) {
     __syncthreads();  // every thread has read this level's counts
     ctl_next_body(a.ctl, a.bslot, tid);
+    if (a.ctl_host) {
+      // the control block for the host, which reads it after an event on
+      // this launch while the level's reshuffle still runs
+      __syncthreads();
+      const uint32_t *src = reinterpret_cast<const uint32_t *>(a.ctl);
+      uint32_t *dst = reinterpret_cast<uint32_t *>(a.ctl_host);
+      for (uint32_t w = tid; w < (uint32_t)(sizeof(Ctl) / 4); w += kPlanThreads) dst[w] = src[w];
+      __threadfence_system();
+    }
   }
 }
 

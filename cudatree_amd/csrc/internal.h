@@ -181,6 +181,7 @@ struct FitArgs {
   unsigned long long *bslot;  // [kByteKinds x kByteSlots x kByteStride] byte counters
   Arena arena;
   Ctl *ctl;
+  Ctl *ctl_host;  // when set, the end of the level also copies Ctl here (host-mapped memory)
   uint32_t cap_q, cap_scan, cap_hist, cap_fill, cap_big, cap_small, cap_nodes;
 
   // label of a row entry: carried in the word, or looked up by sample id
