@@ -347,8 +347,9 @@ void launch_resh(cta_ctx *ctx, const FitArgs<IdxT, LabT> &a, bool global_bits, u
 // Batched levels issued between two host reads of the control block.
 constexpr uint32_t kLevelGroup = 4;
 // Default positions per split-scan task and elements per small-node
-// reshuffle task (measured best at 581,012 x 54, 50 trees).
-constexpr uint32_t kScanChunk = 1024, kReshChunk = 4096;
+// reshuffle task, and positions per fill task (measured best at
+// 581,012 x 54, 50 trees).
+constexpr uint32_t kScanChunk = 1024, kReshChunk = 4096, kFillChunk = 512;
 
 template <class IdxT, class LabT>
 struct Fitter {
@@ -376,7 +377,7 @@ struct Fitter {
     m = opts.max_features;
     Cs = opts.scan_chunk > 0 ? (uint32_t)opts.scan_chunk : kScanChunk;
     Cr = opts.resh_chunk > 0 ? (uint32_t)opts.resh_chunk : kReshChunk;
-    Cf = 2048u;
+    Cf = kFillChunk;
     st = c->stream;
     G = (uint32_t)std::max(64, ctx->num_cu * 8);
     const bool rowfree = !opts.all_rows && d->Frl > 0;
@@ -1799,7 +1800,7 @@ static void hook_split_scan(cta_ctx *ctx, cta_dataset *ds, const uint32_t *rows,
   ax.best_f = -1;
   ax.rl_feat = -1;  // no row-free features here
   ax.nf = nf;
-  const uint32_t Cf = 2048;
+  const uint32_t Cf = kFillChunk;
   ax.fill_base = 0;
   ax.nch_f = (n + Cf - 1) / Cf;
   FNode nd = {0u, 0u, n, 0u, 0u, 1u, 0u, 0u};
