@@ -919,9 +919,11 @@ struct Fitter {
   // Plan and partition of every row of the split nodes (task lists written);
   // `boot`: the bootstrap compaction, timed under its own name.
   void reshuffle(const FitArgs<IdxT, LabT> &a, bool boot = false, hipEvent_t ev = nullptr) {
-    k_plan<IdxT, LabT><<<1, kPlanThreads, 0, st>>>(a, (uint32_t)std::max(8, ctx->num_cu * 4), boot ? 0u : 1u);
+    // the plan splits the work into about 4 units per CU; as many workgroups
+    // (two stay resident per CU) let the CUs that finish early take more
+    const uint32_t grid = (uint32_t)std::max(8, ctx->num_cu * 4);
+    k_plan<IdxT, LabT><<<1, kPlanThreads, 0, st>>>(a, grid, boot ? 0u : 1u);
     if (ev) HIPC(hipEventRecord(ev, st));
-    const uint32_t grid = (uint32_t)std::max(8, ctx->num_cu * 2);
     if (boot) launch_resh(ctx, a, o.resh_global != 0, grid, "k_boot_compact", "k_boot_compact");
     else launch_resh(ctx, a, o.resh_global != 0, grid);
   }
