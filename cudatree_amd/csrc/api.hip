@@ -1018,33 +1018,57 @@ struct Fitter {
     }
     k_canon_scatter<<<grid_for(n), 256, 0, st>>>(n, ar, size, pos, tr, toff, co);
     HIPC(hipGetLastError());
-    // copy out through pinned staging
+    // copy out through pinned staging: each array is appended to the forest
+    // once its own copy has landed (an event per copy), the seven appends on
+    // threads of their own, so they overlap the copies and each other
     uint32_t *stage = static_cast<uint32_t *>(ctx->host("export", (size_t)n * 4 * 7 + ((size_t)Tb + 1) * 4));
-    const void *srcs[7] = {co.feat, co.thr, co.left, co.right, co.value, co.nsamp, co.depth};
-    for (int k = 0; k < 7; ++k)
-      HIPC(hipMemcpyAsync(stage + (size_t)k * n, srcs[k], (size_t)n * 4, hipMemcpyDeviceToHost, st));
     uint32_t *htoff = stage + (size_t)7 * n;
     HIPC(hipMemcpyAsync(htoff, toff, ((size_t)Tb + 1) * 4, hipMemcpyDeviceToHost, st));
+    const void *srcs[7] = {co.feat, co.thr, co.left, co.right, co.value, co.nsamp, co.depth};
+    hipEvent_t evs[7];
+    for (int k = 0; k < 7; ++k) {
+      HIPC(hipMemcpyAsync(stage + (size_t)k * n, srcs[k], (size_t)n * 4, hipMemcpyDeviceToHost, st));
+      evs[k] = ctx->ev();
+      HIPC(hipEventRecord(evs[k], st));
+    }
     if (only_batch) {
       out->nodes.alloc((size_t)n * sizeof(PNode));
       HIPC(hipMemcpyAsync(out->nodes.p, co.pnode, (size_t)n * sizeof(PNode), hipMemcpyDeviceToDevice, st));
       out->roots.alloc(((size_t)Tb + 1) * 4);
       HIPC(hipMemcpyAsync(out->roots.p, toff, (size_t)Tb * 4, hipMemcpyDeviceToDevice, st));
     }
-    ctx->sync();
+    struct Done {  // returns the events to the context's pool
+      cta_ctx *c;
+      hipEvent_t *e;
+      ~Done() {
+        for (int k = 0; k < 7; ++k) c->pool.push_back(e[k]);
+      }
+    } done{ctx, evs};
+    HIPC(hipEventSynchronize(evs[0]));  // the tree offsets landed before the first array
     if (htoff[Tb] != n) fail(CTA_EINTERNAL, "canonical export counted %u of %u nodes", htoff[Tb], n);
+    std::atomic<bool> ok{true};
     auto app = [&](auto &vec, int k) {
       using V = typename std::decay_t<decltype(vec)>::value_type;
+      if (hipEventSynchronize(evs[k]) != hipSuccess) {
+        ok = false;
+        return;
+      }
       const V *p = reinterpret_cast<const V *>(stage + (size_t)k * n);
-      vec.insert(vec.end(), p, p + n);
+      try {
+        vec.insert(vec.end(), p, p + n);
+      } catch (...) {
+        ok = false;
+      }
     };
-    app(out->feature, 0);
-    app(out->threshold, 1);
-    app(out->left, 2);
-    app(out->right, 3);
-    app(out->value, 4);
-    app(out->nsamp, 5);
-    app(out->depth, 6);
+    {
+      std::thread th[6] = {std::thread([&] { app(out->threshold, 1); }), std::thread([&] { app(out->left, 2); }),
+                           std::thread([&] { app(out->right, 3); }),     std::thread([&] { app(out->value, 4); }),
+                           std::thread([&] { app(out->nsamp, 5); }),     std::thread([&] { app(out->depth, 6); })};
+      app(out->feature, 0);
+      for (auto &t : th) t.join();
+    }
+    ctx->sync();
+    if (!ok) fail(CTA_EINTERNAL, "copying the trees to host memory failed");
     const int64_t base = out->tree_off.back();
     for (int32_t t = 1; t <= Tb; ++t) out->tree_off.push_back(base + htoff[t]);
     out->T += Tb;
