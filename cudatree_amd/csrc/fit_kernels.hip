@@ -460,6 +460,9 @@ __global__ __launch_bounds__(kBlock) void k_select_emit(FitArgs<IdxT, LabT> a) {
 // nodes only).  The prefix over chunks gives each chunk its starting counts,
 // the decomposition of scan_gini_large + scan_reduce (dfs_module.cu:24-106).
 // ------------------------------------------------------------------------
+// positions per lane in flight in a chunk-histogram task
+constexpr int kHistU = 8;
+
 template <class IdxT, class LabT>
 __global__ __launch_bounds__(kBlock) void k_scan_hist(FitArgs<IdxT, LabT> a) {
   if (a.ctl->err) return;  // a list overflowed: the fit fails, nothing more is read
@@ -484,19 +487,19 @@ __global__ __launch_bounds__(kBlock) void k_scan_hist(FitArgs<IdxT, LabT> a) {
       wave_sync();
     }
     const uint32_t p0 = tk.chunk * a.Cs, p1 = min(nd.n, p0 + a.Cs);
-    for (uint32_t pb = p0; pb < p1; pb += 4 * kWave) {
-      uint32_t sid[4];
+    for (uint32_t pb = p0; pb < p1; pb += kHistU * kWave) {
+      uint32_t sid[kHistU];
 #pragma unroll
-      for (int u = 0; u < 4; ++u) {
+      for (int u = 0; u < kHistU; ++u) {
         const uint32_t p = pb + (uint32_t)u * kWave + lane;
         sid[u] = (p < p1) ? (uint32_t)row[p] : 0xFFFFFFFFu;
       }
-      uint32_t l[4];
+      uint32_t l[kHistU];
 #pragma unroll
-      for (int u = 0; u < 4; ++u) l[u] = (sid[u] != 0xFFFFFFFFu) ? a.label((IdxT)sid[u]) : 0xFFFFFFFFu;
+      for (int u = 0; u < kHistU; ++u) l[u] = (sid[u] != 0xFFFFFFFFu) ? a.label((IdxT)sid[u]) : 0xFFFFFFFFu;
       if (regs) {
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
+        for (int u = 0; u < kHistU; ++u) {
           uint64_t rem = ballot(l[u] != 0xFFFFFFFFu);
           while (rem) {
             const uint32_t L = readlane(l[u], (uint32_t)__ffsll((unsigned long long)rem) - 1u);
@@ -507,7 +510,7 @@ __global__ __launch_bounds__(kBlock) void k_scan_hist(FitArgs<IdxT, LabT> a) {
         }
       } else {
 #pragma unroll
-        for (int u = 0; u < 4; ++u)
+        for (int u = 0; u < kHistU; ++u)
           if (l[u] != 0xFFFFFFFFu) atomicAdd(&h[l[u]], 1u);
       }
     }
