@@ -1197,29 +1197,41 @@ void presort(cta_ctx *ctx, cta_dataset *ds) {
   const uint32_t ntiles = (N + kSortTile - 1) / kSortTile;
   uint32_t *hist = ctx->ws_as<uint32_t>("sort_hist", (size_t)F * kDigits * ntiles);
   // the digits that vary over each feature's keys: a pass over a digit that
-  // all keys share keeps their order, so a feature takes only the others
-  uint32_t *mask = ctx->ws_as<uint32_t>("sort_mask", 2 * (size_t)F);
+  // all keys share keeps their order, so a feature takes only the others.
+  // A feature with two distinct keys takes one pass, over the digit holding
+  // the highest bit where they differ: that digit orders them (the bits
+  // above it agree), so the pass is the stable partition.
+  uint32_t *mask = ctx->ws_as<uint32_t>("sort_mask", 5 * (size_t)F);
   k_fill_u32<<<grid_for(F), 256, 0, st>>>(mask, F, 0u);
-  k_fill_u32<<<grid_for(F), 256, 0, st>>>(mask + F, F, ~0u);
-  k_key_mask<<<dim3(std::min<uint32_t>(64, (N + 255) / 256), F), 256, 0, st>>>(ds->XT.as<float>(), N, F, mask);
+  k_fill_u32<<<grid_for(2 * (size_t)F), 256, 0, st>>>(mask + F, 2 * (size_t)F, ~0u);
+  k_fill_u32<<<grid_for(2 * (size_t)F), 256, 0, st>>>(mask + 3 * F, 2 * (size_t)F, 0u);
+  const dim3 rgrid(std::min<uint32_t>(64, (N + 255) / 256), F);
+  k_key_mask<<<rgrid, 256, 0, st>>>(ds->XT.as<float>(), N, F, mask);
+  k_key_others<<<rgrid, 256, 0, st>>>(ds->XT.as<float>(), N, F, mask);
   constexpr uint32_t kPasses = 32 / 4;
-  // host table: F masks x 2, then odd[F], then the pass lists (kPasses x F)
-  uint32_t *h = static_cast<uint32_t *>(ctx->host("sort_plan", ((size_t)2 * F + F + kPasses * F) * 4));
-  HIPC(hipMemcpyAsync(h, mask, 2 * (size_t)F * 4, hipMemcpyDeviceToHost, st));
+  // host table: the 5 F reductions, then odd[F], then the pass lists
+  // (kPasses x F)
+  uint32_t *h = static_cast<uint32_t *>(ctx->host("sort_plan", ((size_t)5 * F + F + kPasses * F) * 4));
+  HIPC(hipMemcpyAsync(h, mask, 5 * (size_t)F * 4, hipMemcpyDeviceToHost, st));
   ctx->sync();
-  uint32_t *odd = h + 2 * F, *lists = odd + F;
+  uint32_t *odd = h + 5 * F, *lists = odd + F;
   uint32_t nlist[kPasses] = {};
-  std::vector<uint32_t> in1(F);
+  std::vector<uint32_t> in1(F), passes(F);  // passes: bit s set = pass s taken
   for (uint32_t f = 0; f < F; ++f) {
-    const uint32_t vary = h[f] ^ h[F + f];
+    const uint32_t vary = h[f] ^ h[F + f], lo = h[2 * F + f], hi = h[3 * F + f], others = h[4 * F + f];
     uint32_t p = 0;
-    for (uint32_t s = 0; s < kPasses; ++s) p += ((vary >> (4 * s)) & 15u) ? 1u : 0u;
-    odd[f] = p & 1u;
-    in1[f] = p & 1u;  // where the keys start
+    if (lo != hi && others == 0) {
+      p = 1u << ((31u - (uint32_t)__builtin_clz(lo ^ hi)) / 4u);
+    } else {
+      for (uint32_t s = 0; s < kPasses; ++s) p |= ((vary >> (4 * s)) & 15u) ? 1u << s : 0u;
+    }
+    passes[f] = p;
+    odd[f] = (uint32_t)__builtin_popcount(p) & 1u;
+    in1[f] = odd[f];  // where the keys start
   }
   for (uint32_t s = 0; s < kPasses; ++s)
     for (uint32_t f = 0; f < F; ++f)
-      if (((h[f] ^ h[F + f]) >> (4 * s)) & 15u) {
+      if ((passes[f] >> s) & 1u) {
         lists[(size_t)s * F + nlist[s]++] = f | (in1[f] << 31);
         in1[f] ^= 1u;
       }

@@ -4,8 +4,9 @@
 //                      (random_forest.py:233)
 //   k_sort_*           stable LSD radix sort of every feature row, 4-bit
 //                      digits; a feature takes only the passes of digits
-//                      that vary over its keys; ties keep ascending sample
-//                      id (replaces the host np.argsort of
+//                      that vary over its keys (one pass when it has two
+//                      distinct keys); ties keep ascending sample id
+//                      (replaces the host np.argsort of
 //                      random_forest.py:247)
 //   k_predict / k_vote batched traversal of a flattened pre-order forest,
 //                      one thread per (row, tree), votes by integer atomics
@@ -55,25 +56,46 @@ __device__ __forceinline__ uint32_t float_key(float v) {
   return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
 }
 
-// mask[f] |= OR of feature f's keys, mask[F + f] &= their AND (set to 0 and
-// ~0 first): the bits that vary over the feature are their XOR.
+// Per feature f, over its keys: mask[f] |= OR, mask[F + f] &= AND (the bits
+// that vary over the feature are their XOR), mask[2F + f] = min and
+// mask[3F + f] = max (set to 0, ~0, ~0 and 0 first).
 __global__ __launch_bounds__(256) void k_key_mask(const float *XT, uint32_t N, uint32_t F, uint32_t *mask) {
   const uint32_t f = blockIdx.y;
-  uint32_t o = 0u, n = ~0u;
+  uint32_t o = 0u, n = ~0u, lo = ~0u, hi = 0u;
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x) {
     const uint32_t k = float_key(XT[(size_t)f * N + i]);
     o |= k;
     n &= k;
+    lo = min(lo, k);
+    hi = max(hi, k);
   }
 #pragma unroll
   for (int d = 32; d > 0; d >>= 1) {
     o |= (uint32_t)__shfl_xor((int)o, d, 64);
     n &= (uint32_t)__shfl_xor((int)n, d, 64);
+    lo = min(lo, (uint32_t)__shfl_xor((int)lo, d, 64));
+    hi = max(hi, (uint32_t)__shfl_xor((int)hi, d, 64));
   }
   if (lane_id() == 0) {
     atomicOr(&mask[f], o);
     atomicAnd(&mask[F + f], n);
+    atomicMin(&mask[2 * F + f], lo);
+    atomicMax(&mask[3 * F + f], hi);
   }
+}
+
+// mask[4F + f] += keys of feature f other than its min and max (set to 0
+// first): zero for a feature with at most two distinct keys.
+__global__ __launch_bounds__(256) void k_key_others(const float *XT, uint32_t N, uint32_t F, uint32_t *mask) {
+  const uint32_t f = blockIdx.y;
+  const uint32_t lo = mask[2 * F + f], hi = mask[3 * F + f];
+  uint32_t c = 0;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x) {
+    const uint32_t k = float_key(XT[(size_t)f * N + i]);
+    c += (k != lo && k != hi) ? 1u : 0u;
+  }
+  c = wave_sum32(c);
+  if (lane_id() == 0 && c) atomicAdd(&mask[4 * F + f], c);
 }
 
 // Keys and ids of every feature, written into buffer 1 for the features

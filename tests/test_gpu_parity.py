@@ -49,13 +49,19 @@ def gpu_forest(ctx, case, mode="batched", scan_chunk=0, resh_chunk=0, idx_bits=0
 def test_presort_stable(ctx, N, idx_bits):
     print("presort N=%d idx_bits=%d" % (N, idx_bits), flush=True)
     rng = np.random.default_rng(N)
-    F = 5
+    F = 9
     X = np.empty((N, F), np.float32)
     X[:, 0] = rng.integers(0, 7, N)
     X[:, 1] = rng.normal(size=N)
     X[:, 2] = rng.choice([-0.0, 0.0, 1.0, -1.0, np.inf, -np.inf, 1e-38, -3e38], size=N)
     X[:, 3] = 5.0
     X[:, 4] = -rng.exponential(size=N)
+    # two distinct keys (one partition pass): bits differing high, across the
+    # sign, only in the lowest bit, and with signed zeros tied
+    X[:, 5] = rng.integers(0, 2, N)
+    X[:, 6] = rng.choice([-1.5, 2.25], size=N)
+    X[:, 7] = rng.choice(np.array([1.0, np.nextafter(np.float32(1.0), np.float32(2.0))], np.float32), size=N)
+    X[:, 8] = rng.choice([-0.0, 0.0, 3.0], size=N)
     y = rng.integers(0, 3, N)
     ds = _lib.Dataset(ctx, X, y, 3, idx_bits=idx_bits)
     assert ds.idx_bits == (idx_bits or (16 if N <= 65535 else 32))
