@@ -1099,21 +1099,29 @@ template <class IdxT, class LabT>
 void fit_impl(cta_ctx *ctx, cta_dataset *ds, int32_t T, const uint32_t *ids, const int64_t *off,
               const cta_fit_opts &o, cta_forest *out) {
   Fitter<IdxT, LabT> fz(ctx, ds, o);
+  // trees resident at once on one stream: as asked, else what 60% of the
+  // free memory holds (workspaces are reused by this fit)
   int32_t per = o.trees_per_batch;
   if (o.mode != CTA_MODE_BATCHED) per = 1;
-  // two lanes: batches alternate between this context's stream and a second
-  // one, so that one lane's short late levels overlap the other's work
-  int32_t lanes = o.mode == CTA_MODE_BATCHED && o.lanes == 2 ? 2 : 1;
   if (per <= 0) {
     size_t fr = 0, tot = 0;
     HIPC(hipMemGetInfo(&fr, &tot));
-    fr += ctx->ws_bytes() + (ctx->lane ? ctx->lane->ws_bytes() : 0);  // workspaces are reused by this fit
-    const size_t bpt = fz.bytes_per_tree();
-    per = (int32_t)std::max<size_t>(1, (size_t)(0.6 * (double)fr) / bpt);
-    if (lanes == 2) per = std::max<int32_t>(1, per / 2);  // two batches resident at once
+    fr += ctx->ws_bytes() + (ctx->lane ? ctx->lane->ws_bytes() : 0);
+    per = (int32_t)std::max<size_t>(1, (size_t)(0.6 * (double)fr) / fz.bytes_per_tree());
   }
-  per = std::min(per, lanes == 2 ? (T + 1) / 2 : T);
-  if (lanes == 2 && (per < kLaneMinTrees || per >= T)) lanes = 1;
+  per = std::min(per, T);
+  // two lanes: batches alternate between this context's stream and a second
+  // one, so that one lane's short late levels overlap the other's work; each
+  // holds half the trees (two batches resident at once), at least
+  // kLaneMinTrees of them
+  int32_t lanes = 1;
+  if (o.mode == CTA_MODE_BATCHED && o.lanes == 2) {
+    const int32_t half = std::min(o.trees_per_batch > 0 ? per : std::max<int32_t>(1, per / 2), (T + 1) / 2);
+    if (half >= kLaneMinTrees && half < T) {
+      lanes = 2;
+      per = half;
+    }
+  }
   if (lanes == 1) {
     for (int32_t t0 = 0; t0 < T; t0 += per) fz.run_batch(t0, std::min(per, T - t0), T, ids, off, out);
     return;
