@@ -12,6 +12,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <functional>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -1061,9 +1062,17 @@ struct Fitter {
       }
     };
     {
-      std::thread th[6] = {std::thread([&] { app(out->threshold, 1); }), std::thread([&] { app(out->left, 2); }),
-                           std::thread([&] { app(out->right, 3); }),     std::thread([&] { app(out->value, 4); }),
-                           std::thread([&] { app(out->nsamp, 5); }),     std::thread([&] { app(out->depth, 6); })};
+      const std::function<void()> jobs[6] = {[&] { app(out->threshold, 1); }, [&] { app(out->left, 2); },
+                                             [&] { app(out->right, 3); },     [&] { app(out->value, 4); },
+                                             [&] { app(out->nsamp, 5); },     [&] { app(out->depth, 6); }};
+      std::vector<std::thread> th;
+      for (const auto &job : jobs) {
+        try {
+          th.emplace_back(job);
+        } catch (...) {
+          job();  // no thread to spare: this one here
+        }
+      }
       app(out->feature, 0);
       for (auto &t : th) t.join();
     }
