@@ -179,8 +179,10 @@ __global__ __launch_bounds__(kOffSingleThreads) void k_off_single(const uint32_t
     __syncthreads();
 #pragma unroll
     for (int f = 0; f < NF; ++f) {
-      uint32_t run = carry[f] + inc[f] - ts[f];
-      for (uint32_t w = 0; w < wave; ++w) run += ws[w][f];
+      // the waves before this one: an exclusive scan of the kW wave sums
+      const uint32_t wsum = lane < (uint32_t)kW ? ws[lane][f] : 0u;
+      const uint32_t wincl = wave_incl_scan(wsum);
+      uint32_t run = carry[f] + readlane(wincl - wsum, wave) + inc[f] - ts[f];
 #pragma unroll
       for (int k = 0; k < kOffItems; ++k) {
         if (i0 + k < n) out[(size_t)(i0 + k) * NF + f] = run;
