@@ -436,7 +436,7 @@ struct Fitter {
         k_fill_u32<<<grid_for((size_t)Tb * W), 256, 0, st>>>(bits, (size_t)Tb * W, 0u);
         k_boot_gen<<<dim3(grid_for((N + 3) / 4, 256, 256), Tb), 256, 0, st>>>(o.seed, (uint32_t)t0, N, W, bits);
       }
-      k_bits_count<<<dim3(grid_for(W, 256, 64), Tb), 256, 0, st>>>(bits, W, cnts);
+      k_bits_count<<<dim3(grid_for(W, 256, 8), Tb), 256, 0, st>>>(bits, W, cnts);  // few atomics per counter
       uint32_t *h = static_cast<uint32_t *>(ctx->host("boot_counts", (size_t)Tb * 4));
       HIPC(hipMemcpyAsync(h, cnts, (size_t)Tb * 4, hipMemcpyDeviceToHost, st));
       ctx->sync();
@@ -567,7 +567,7 @@ struct Fitter {
     uint32_t src_root = 0;
     if (boot) {
       if (boot_gen) {
-        k_root_counts_bits<LabT><<<dim3(grid_for(N, 256, 256), Tb), 256, K * 4, st>>>(a.y, bits, N, W, K, root_cnt);
+        k_root_counts_bits<LabT><<<dim3(grid_for(N, 256, 32), Tb), 256, K * 4, st>>>(a.y, bits, N, W, K, root_cnt);
       } else {
         uint32_t *ids = static_cast<uint32_t *>(ctx->host("boot_ids", n_tot * 4));
         for (int32_t t = 0; t < Tb; ++t)
