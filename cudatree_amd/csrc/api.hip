@@ -853,7 +853,9 @@ struct Fitter {
     uint32_t *buf = rl_buf(which);
     k_fill_u32<<<grid_for((size_t)qb * FK), 256, 0, st>>>(buf, (size_t)qb * FK, 0u);
     a.rlc = buf;
-    const uint32_t gx = std::min<uint32_t>(1024, std::max<uint32_t>(1, (maxn + Cf * kWavesPerBlock - 1) / (Cf * kWavesPerBlock)));
+    // about 8 chunks per wave of the largest node
+    const uint32_t per_block = Cf * kWavesPerBlock * 8;
+    const uint32_t gx = std::min<uint32_t>(1024, std::max<uint32_t>(1, (maxn + per_block - 1) / per_block));
     Timed tm(ctx, "k_rl_count");
     k_rl_front<IdxT, LabT><<<dim3(gx, std::min<uint32_t>(qb, 65535u)), kBlock, (size_t)kWavesPerBlock * kRlCounters * 4,
                              st>>>(a);

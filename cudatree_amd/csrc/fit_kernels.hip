@@ -1454,8 +1454,9 @@ __device__ __forceinline__ void rl_count_chunk(const FitArgs<IdxT, LabT> &a, con
 }
 
 // The current frontier's counts from scratch (roots, frontiers set by the
-// host): block (x, i) counts chunks x, x + gridDim.x, ... of node i into
-// rlc, which must be zero.
+// host): wave w of block (x, i) counts chunks x kWavesPerBlock + w, then
+// every gridDim.x kWavesPerBlock-th, of node i and adds them to rlc, which
+// must be zero.
 template <class IdxT, class LabT>
 __global__ __launch_bounds__(kBlock) void k_rl_front(FitArgs<IdxT, LabT> a) {
   extern __shared__ uint32_t smem[];
@@ -1466,16 +1467,19 @@ __global__ __launch_bounds__(kBlock) void k_rl_front(FitArgs<IdxT, LabT> a) {
   for (uint32_t i = blockIdx.y; i < q; i += gridDim.y) {
     const FNode nd = a.front[i];
     const uint32_t nchunk = ceil_div(nd.n, a.Cf);
-    for (uint32_t c = blockIdx.x * kWavesPerBlock + wave; c < nchunk; c += gridDim.x * kWavesPerBlock) {
-      for (uint32_t w = lane; w < FK; w += kWave) cnt[w] = 0;
-      wave_sync();
+    const uint32_t c0 = blockIdx.x * kWavesPerBlock + wave;
+    if (c0 >= nchunk) continue;
+    // every chunk of this wave counted, then one flush (few atomics on each
+    // of the node's counters)
+    for (uint32_t w = lane; w < FK; w += kWave) cnt[w] = 0;
+    wave_sync();
+    for (uint32_t c = c0; c < nchunk; c += gridDim.x * kWavesPerBlock)
       rl_count_chunk(a, nd, c * a.Cf, min(nd.n, (c + 1) * a.Cf), cnt);
-      wave_sync();
-      uint32_t *dst = const_cast<uint32_t *>(a.rlc) + (size_t)i * FK;
-      for (uint32_t w = lane; w < FK; w += kWave)
-        if (cnt[w]) atomicAdd(&dst[w], cnt[w]);
-      wave_sync();
-    }
+    wave_sync();
+    uint32_t *dst = const_cast<uint32_t *>(a.rlc) + (size_t)i * FK;
+    for (uint32_t w = lane; w < FK; w += kWave)
+      if (cnt[w]) atomicAdd(&dst[w], cnt[w]);
+    wave_sync();
   }
 }
 
