@@ -1021,25 +1021,48 @@ __device__ __forceinline__ void rl_add(const FitArgs<IdxT, LabT> &a, const uint3
   }
 }
 
-// One fill task: the marks of the smaller side's samples among Cf positions
-// of a split node and, when the node is flagged kCountRl, their row-free
-// counts, added to rl_small through the wave's counters cnt.
 // positions per lane in flight in a fill task
 constexpr int kFillU = 8;
 
+// Adds the wave's row-free counters (node `node`'s) to rl_small and clears
+// them.
+template <class IdxT, class LabT>
+__device__ __forceinline__ void rl_flush(const FitArgs<IdxT, LabT> &a, uint32_t node, uint32_t *cnt) {
+  const uint32_t lane = lane_id(), FK = a.Frl * (uint32_t)a.K;
+  wave_sync();
+  uint32_t *dst = a.rl_small + (size_t)node * FK;
+  for (uint32_t w = lane; w < FK; w += kWave)
+    if (cnt[w]) {
+      atomicAdd(&dst[w], cnt[w]);
+      cnt[w] = 0;
+    }
+  wave_sync();
+}
+
+// Clears a wave's row-free counters (before its first fill task).
+__device__ __forceinline__ void rl_zero(uint32_t *cnt, uint32_t FK) {
+  for (uint32_t w = lane_id(); w < FK; w += kWave) cnt[w] = 0;
+  wave_sync();
+}
+
+// One fill task: the marks of the smaller side's samples among Cf positions
+// of a split node and, when the node is flagged kCountRl, their row-free
+// counts, added to the wave's counters cnt.  The counters (zero at the
+// wave's start) hold node `cur`'s counts: a task of another counting node
+// flushes them to rl_small first; the wave flushes the last node's at its
+// end.
 template <class IdxT, class LabT>
 __device__ __forceinline__ void fill_task_bits(const FitArgs<IdxT, LabT> &a, const Task &tk, uint32_t *bits,
-                                               uint32_t *cnt) {
+                                               uint32_t *cnt, uint32_t &cur) {
   const uint32_t lane = lane_id();
   const FNode nd = a.front[tk.node];
   const NAux ax = a.aux[tk.node];
   const int32_t jr = a.Frl ? a.rl_of[ax.best_f] : -1;
   const bool count = (ax.flags & kCountRl) != 0u;
   const bool small_left = ax.nL <= nd.n - ax.nL;
-  const uint32_t FK = a.Frl * (uint32_t)a.K;
-  if (count) {
-    for (uint32_t w = lane; w < FK; w += kWave) cnt[w] = 0;
-    wave_sync();
+  if (count && tk.node != cur) {
+    if (cur != kLeaf) rl_flush(a, cur, cnt);
+    cur = tk.node;
   }
   if (jr >= 0) {
     // row-free split: the samples holding the lower value, over feature 0's row
@@ -1092,13 +1115,6 @@ __device__ __forceinline__ void fill_task_bits(const FitArgs<IdxT, LabT> &a, con
       if (count) rl_add(a, e, take, cnt);
     }
   }
-  if (count) {
-    wave_sync();
-    uint32_t *dst = a.rl_small + (size_t)tk.node * FK;
-    for (uint32_t w = lane; w < FK; w += kWave)
-      if (cnt[w]) atomicAdd(&dst[w], cnt[w]);
-    wave_sync();
-  }
 }
 
 // Fill workgroups of the tree whose frontier is [lo, hi): one per
@@ -1128,11 +1144,14 @@ __global__ __launch_bounds__(kFillThreads) void k_fill_lds(FitArgs<IdxT, LabT> a
   __syncthreads();
   // the waves' row-free counters follow the bitmap
   uint32_t *cnt = lbits + W + wave * a.Frl * (uint32_t)a.K;
+  rl_zero(cnt, a.Frl * (uint32_t)a.K);
+  uint32_t cur = kLeaf;
   for (uint32_t k = fb + j * kFillWaves + wave; k < fe; k += Je * kFillWaves) {
     const Task tk = a.fill_tasks[k];
     if (tk.node == kLeaf || tk.chunk >= tk.nch) continue;
-    fill_task_bits(a, tk, lbits, cnt);
+    fill_task_bits(a, tk, lbits, cnt, cur);
   }
+  if (cur != kLeaf) rl_flush(a, cur, cnt);
   __syncthreads();
   uint32_t *out = (Je == 1) ? a.bits + (size_t)t * W : part + ((size_t)t * J + j) * W;
   for (uint32_t w = tid; w < W; w += kFillThreads) out[w] = lbits[w];
@@ -1164,11 +1183,14 @@ __global__ __launch_bounds__(kBlock) void k_fill_global(FitArgs<IdxT, LabT> a) {
   if (a.ctl->err) return;  // a list overflowed: the fit fails, nothing more is read
   extern __shared__ uint32_t smem[];
   uint32_t *cnt = smem + (threadIdx.x / kWave) * a.Frl * (uint32_t)a.K;
+  rl_zero(cnt, a.Frl * (uint32_t)a.K);
+  uint32_t cur = kLeaf;
   WAVE_LOOP(a.ctl->n_fill) {
     const Task tk = a.fill_tasks[item];
     if (tk.node == kLeaf || tk.chunk >= tk.nch) continue;
-    fill_task_bits(a, tk, a.bits + (size_t)a.front[tk.node].tree * a.W, cnt);
+    fill_task_bits(a, tk, a.bits + (size_t)a.front[tk.node].tree * a.W, cnt, cur);
   }
+  if (cur != kLeaf) rl_flush(a, cur, cnt);
 }
 
 // ------------------------------------------------------------------------
