@@ -50,7 +50,9 @@ class FitOpts(ctypes.Structure):
         ("bootstrap", ctypes.c_int32),
         ("all_rows", ctypes.c_int32),
         ("lanes", ctypes.c_int32),
-        ("reserved", ctypes.c_int32 * 1),
+        ("fill_global", ctypes.c_int32),
+        ("hist_layout", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 2),
     ]
 
 
@@ -121,6 +123,8 @@ SIGNATURES = {
     "cta_forest_export": (ctypes.c_int, [_vp, _i64p, _i32p, _f32p, _u32p, _u32p, _u32p, _u32p, _u32p]),
     "cta_forest_import": (ctypes.c_int, [_vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, _i64p, _i32p, _f32p,
                                          _u32p, _u32p, _u32p, ctypes.POINTER(_vp)]),
+    "cta_forest_save": (ctypes.c_int, [_vp, _vp, ctypes.c_uint64, _u64p]),
+    "cta_forest_load": (ctypes.c_int, [_vp, _vp, ctypes.c_uint64, ctypes.POINTER(_vp)]),
     "cta_predict": (ctypes.c_int, [_vp, _vp, _f32p, ctypes.c_int64, ctypes.c_int32, _u32p, _u32p]),
     "cta_predict_device": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int64, ctypes.c_int32, _vp, _vp]),
     "cta_ctx_release_workspace": (ctypes.c_int, [_vp]),
@@ -354,6 +358,22 @@ class Forest:
                                         ptr(a["value"], _u32p), ctypes.byref(h)))
         return cls(ctx, h)
 
+    @classmethod
+    def load(cls, ctx, image):
+        """Resident forest from a serialised image (cta_forest_load)."""
+        buf = np.frombuffer(bytes(image), np.uint8)
+        h = _vp()
+        check(ctx.lib.cta_forest_load(ctx.h, buf.ctypes.data_as(_vp), buf.size, ctypes.byref(h)))
+        return cls(ctx, h)
+
+    def save(self):
+        """Serialised image of the forest (cta_forest_save), as bytes."""
+        size = ctypes.c_uint64()
+        check(self.lib.cta_forest_save(self.h, None, 0, ctypes.byref(size)))
+        buf = np.zeros(size.value, np.uint8)
+        check(self.lib.cta_forest_save(self.h, buf.ctypes.data_as(_vp), size.value, ctypes.byref(size)))
+        return buf.tobytes()
+
     def export(self):
         off = np.zeros(self.n_trees + 1, np.int64)
         n = self.n_nodes
@@ -516,3 +536,56 @@ class Server:
             self.close()
         except Exception:
             pass
+
+
+# ---- serialised forests ---------------------------------------------------------
+# The byte image of cta_forest_save / cta_forest_load (include/cudatree_amd.h):
+# a 48-byte header, tree offsets, then the seven node arrays.
+FOREST_MAGIC = b"CTAFORST"
+FOREST_FORMAT = 1
+_HEADER = np.dtype([("magic", "S8"), ("version", "<u4"), ("header_bytes", "<u4"), ("T", "<i4"), ("F", "<i4"),
+                    ("K", "<i4"), ("flags", "<u4"), ("nodes", "<i8"), ("reserved", "<u8")])
+_IMAGE_ARRAYS = (("feature", "<i4"), ("threshold", "<f4"), ("left", "<u4"), ("right", "<u4"), ("value", "<u4"),
+                 ("n_samples", "<u4"), ("depth", "<u4"))
+
+
+def forest_to_bytes(arrays, F, K):
+    """Byte image of exported forest arrays, as cta_forest_save writes it
+    (host only; no device needed)."""
+    off = np.ascontiguousarray(arrays["tree_off"], dtype="<i8")
+    T, n = len(off) - 1, int(off[-1])
+    h = np.zeros(1, _HEADER)
+    h["magic"], h["version"], h["header_bytes"] = FOREST_MAGIC, FOREST_FORMAT, _HEADER.itemsize
+    h["T"], h["F"], h["K"], h["nodes"] = T, F, K, n
+    parts = [h.tobytes(), off.tobytes()]
+    for k, dt in _IMAGE_ARRAYS:
+        a = arrays.get(k)
+        parts.append((np.zeros(n, dt) if a is None else np.ascontiguousarray(a, dtype=dt)).tobytes())
+    return b"".join(parts)
+
+
+def forest_from_bytes(image):
+    """(arrays, F, K) of a byte image; ValueError for another format version
+    or a malformed image (host only)."""
+    image = bytes(image)
+    if len(image) < _HEADER.itemsize:
+        raise ValueError("forest image shorter than its header")
+    h = np.frombuffer(image[:_HEADER.itemsize], _HEADER)[0]
+    if h["magic"] != FOREST_MAGIC:
+        raise ValueError("not a serialised forest (bad magic)")
+    if int(h["version"]) != FOREST_FORMAT:
+        raise ValueError("forest format version %d is not supported (this build reads version %d)"
+                         % (int(h["version"]), FOREST_FORMAT))
+    T, n = int(h["T"]), int(h["nodes"])
+    if int(h["header_bytes"]) != _HEADER.itemsize or T <= 0 or n <= 0 or \
+            len(image) != _HEADER.itemsize + 8 * (T + 1) + 28 * n:
+        raise ValueError("corrupt forest image")
+    p = _HEADER.itemsize
+    out = {"tree_off": np.frombuffer(image, "<i8", T + 1, p).astype(np.int64)}
+    p += 8 * (T + 1)
+    for k, dt in _IMAGE_ARRAYS:
+        out[k] = np.frombuffer(image, dt, n, p).copy()
+        p += 4 * n
+    if out["tree_off"][0] != 0 or out["tree_off"][-1] != n:
+        raise ValueError("tree offsets do not match the node count")
+    return out, int(h["F"]), int(h["K"])

@@ -471,7 +471,7 @@ struct Fitter {
     bsum = ctx->ws_as<uint32_t>("scan_bsum", (cap_q / kOffTile + 2) * kFields);
     const size_t nbslot = kByteKinds * (size_t)kByteSlots * kByteStride;
     unsigned long long *bslot = ctx->ws_as<unsigned long long>("byte_slots", nbslot);
-    fill_lds = fill_lds_bytes() <= (size_t)ctx->lds_per_block;
+    fill_lds = !o.fill_global && fill_lds_bytes() <= (size_t)ctx->lds_per_block;
     Jcap = fill_lds ? (uint32_t)std::max<int64_t>(1, std::min<int64_t>(16, (3 * (int64_t)ctx->num_cu) / Tb)) : 1u;
     part = Jcap > 1 ? ctx->ws_as<uint32_t>("fill_part", (size_t)Tb * Jcap * W) : nullptr;
     FNode *front[2] = {ctx->ws_as<FNode>("front0", cap_q), ctx->ws_as<FNode>("front1", cap_q)};
@@ -526,6 +526,7 @@ struct Fitter {
     a.Cs = Cs;
     a.Cf = Cf;
     a.Cr = Cr;
+    a.hist_lds = o.hist_layout == CTA_HIST_LDS ? 1u : 0u;
     a.cntL = cntL;
     a.aux = aux;
     a.feats = feats;
@@ -1412,7 +1413,7 @@ extern "C" {
 
 const char *cta_last_error(void) { return g_err.c_str(); }
 
-int cta_version(void) { return 100; }
+int cta_version(void) { return 20000; }
 
 int cta_ctx_create(int device, cta_ctx **out) {
   return guarded([&] {
@@ -1567,6 +1568,10 @@ int cta_fit(cta_ctx *ctx, cta_dataset *ds, int32_t T, const uint32_t *boot_ids, 
     if (o.scan_chunk < 0 || o.resh_chunk < 0) fail(CTA_EINVAL, "chunk sizes must be >= 0");
     if ((o.scan_chunk && o.scan_chunk < 64) || (o.resh_chunk && o.resh_chunk < 64))
       fail(CTA_EINVAL, "chunk sizes must be 0 or >= 64");
+    if (o.hist_layout < CTA_HIST_AUTO || o.hist_layout > CTA_HIST_LDS)
+      fail(CTA_EINVAL, "unknown hist_layout %d", o.hist_layout);
+    if (o.hist_layout == CTA_HIST_REGS && ds->K > 64)
+      fail(CTA_EINVAL, "hist_layout CTA_HIST_REGS needs K <= 64 (K = %d)", ds->K);
     if (boot_ids) {
       if (!boot_off) fail(CTA_EINVAL, "boot_off is NULL");
       for (int32_t t = 0; t < T; ++t) {
@@ -1635,48 +1640,149 @@ int cta_forest_export(const cta_forest *f, int64_t *tree_off, int32_t *feature, 
   });
 }
 
+// Checks exported arrays (pre-order layout, ids and labels in range) and
+// makes a resident forest of them; n_samples / depth may be NULL.
+static cta_forest *import_arrays(cta_ctx *ctx, int32_t T, int32_t F, int32_t K, const int64_t *tree_off,
+                                 const int32_t *feature, const float *threshold, const uint32_t *left,
+                                 const uint32_t *right, const uint32_t *value, const uint32_t *nsamp,
+                                 const uint32_t *depth) {
+  if (T <= 0 || F <= 0 || K <= 0 || F > CTA_MAX_FEATURES || K > CTA_MAX_CLASSES) fail(CTA_EINVAL, "bad sizes");
+  if (tree_off[0] != 0) fail(CTA_EINVAL, "tree_off[0] must be 0");
+  for (int32_t t = 0; t < T; ++t) {
+    const int64_t a = tree_off[t], b = tree_off[t + 1];
+    if (b <= a) fail(CTA_EINVAL, "tree %d is empty", t);
+    const int64_t n = b - a;
+    for (int64_t j = a; j < b; ++j) {
+      const bool leaf = left[j] == 0 || right[j] == 0;
+      if (leaf) {
+        if (value[j] >= (uint32_t)K) fail(CTA_EINVAL, "leaf label %u out of range", value[j]);
+      } else {
+        if (feature[j] < 0 || feature[j] >= F) fail(CTA_EINVAL, "feature %d out of range", feature[j]);
+        // pre-order: left child next, right child later, both inside the tree
+        if ((int64_t)left[j] != j - a + 1 || (int64_t)right[j] <= j - a + 1 || (int64_t)right[j] >= n)
+          fail(CTA_EINVAL, "tree %d node %lld is not in pre-order layout", t, (long long)(j - a));
+      }
+    }
+  }
+  std::unique_ptr<cta_forest> f(new cta_forest());
+  f->ctx = ctx;
+  f->T = T;
+  f->F = F;
+  f->K = K;
+  const int64_t n = tree_off[T];
+  f->tree_off.assign(tree_off, tree_off + T + 1);
+  f->feature.assign(feature, feature + n);
+  f->threshold.assign(threshold, threshold + n);
+  f->left.assign(left, left + n);
+  f->right.assign(right, right + n);
+  f->value.assign(value, value + n);
+  if (nsamp) f->nsamp.assign(nsamp, nsamp + n);
+  if (depth) f->depth.assign(depth, depth + n);
+  for (int64_t j = 0; j < n; ++j)
+    if (f->left[j] == 0 || f->right[j] == 0) f->left[j] = f->right[j] = 0;
+  upload_forest(f.get());
+  return f.release();
+}
+
 int cta_forest_import(cta_ctx *ctx, int32_t T, int32_t F, int32_t K, const int64_t *tree_off, const int32_t *feature,
                       const float *threshold, const uint32_t *left, const uint32_t *right, const uint32_t *value,
                       cta_forest **out) {
   return guarded([&] {
     if (!ctx || !tree_off || !feature || !threshold || !left || !right || !value || !out)
       fail(CTA_EINVAL, "NULL argument");
-    if (T <= 0 || F <= 0 || K <= 0 || F > CTA_MAX_FEATURES || K > CTA_MAX_CLASSES) fail(CTA_EINVAL, "bad sizes");
-    if (tree_off[0] != 0) fail(CTA_EINVAL, "tree_off[0] must be 0");
-    for (int32_t t = 0; t < T; ++t) {
-      const int64_t a = tree_off[t], b = tree_off[t + 1];
-      if (b <= a) fail(CTA_EINVAL, "tree %d is empty", t);
-      const int64_t n = b - a;
-      for (int64_t j = a; j < b; ++j) {
-        const bool leaf = left[j] == 0 || right[j] == 0;
-        if (leaf) {
-          if (value[j] >= (uint32_t)K) fail(CTA_EINVAL, "leaf label %u out of range", value[j]);
-        } else {
-          if (feature[j] < 0 || feature[j] >= F) fail(CTA_EINVAL, "feature %d out of range", feature[j]);
-          // pre-order: left child next, right child later, both inside the tree
-          if ((int64_t)left[j] != j - a + 1 || (int64_t)right[j] <= j - a + 1 || (int64_t)right[j] >= n)
-            fail(CTA_EINVAL, "tree %d node %lld is not in pre-order layout", t, (long long)(j - a));
-        }
-      }
-    }
     std::lock_guard<std::recursive_mutex> lk(ctx->mu);
     HIPC(hipSetDevice(ctx->device));
-    std::unique_ptr<cta_forest> f(new cta_forest());
-    f->ctx = ctx;
-    f->T = T;
-    f->F = F;
-    f->K = K;
-    const int64_t n = tree_off[T];
-    f->tree_off.assign(tree_off, tree_off + T + 1);
-    f->feature.assign(feature, feature + n);
-    f->threshold.assign(threshold, threshold + n);
-    f->left.assign(left, left + n);
-    f->right.assign(right, right + n);
-    f->value.assign(value, value + n);
-    for (int64_t j = 0; j < n; ++j)
-      if (f->left[j] == 0 || f->right[j] == 0) f->left[j] = f->right[j] = 0;
-    upload_forest(f.get());
-    *out = f.release();
+    *out = import_arrays(ctx, T, F, K, tree_off, feature, threshold, left, right, value, nullptr, nullptr);
+  });
+}
+
+// ---- serialised forests ------------------------------------------------------------
+// Byte image (little-endian): a 48-byte header {magic "CTAFORST", u32
+// version, u32 header bytes, i32 T, F, K, u32 flags, i64 nodes, u64 0}, then
+// tree_off (T + 1 x i64) and the node arrays feature (i32), threshold (f32),
+// left, right, value, n_samples, depth (u32), n entries each.
+namespace {
+constexpr char kForestMagic[8] = {'C', 'T', 'A', 'F', 'O', 'R', 'S', 'T'};
+constexpr uint32_t kForestHeader = 48;
+struct ForestHeader {
+  char magic[8];
+  uint32_t version, header_bytes;
+  int32_t T, F, K;
+  uint32_t flags;
+  int64_t nodes;
+  uint64_t reserved;
+};
+static_assert(sizeof(ForestHeader) == kForestHeader, "forest header layout");
+uint64_t forest_image_bytes(int64_t T, int64_t n) { return kForestHeader + 8 * (uint64_t)(T + 1) + 28 * (uint64_t)n; }
+}  // namespace
+
+int cta_forest_save(const cta_forest *f, void *buf, uint64_t cap, uint64_t *size) {
+  return guarded([&] {
+    if (!f || !size) fail(CTA_EINVAL, "NULL argument");
+    const int64_t n = f->tree_off.back();
+    const uint64_t need = forest_image_bytes(f->T, n);
+    *size = need;
+    if (!buf) return;
+    if (cap < need) fail(CTA_EINVAL, "buffer of %llu bytes holds less than the %llu-byte image",
+                         (unsigned long long)cap, (unsigned long long)need);
+    ForestHeader h = {};
+    std::memcpy(h.magic, kForestMagic, 8);
+    h.version = CTA_FOREST_FORMAT;
+    h.header_bytes = kForestHeader;
+    h.T = f->T;
+    h.F = f->F;
+    h.K = f->K;
+    h.nodes = n;
+    char *p = static_cast<char *>(buf);
+    std::memcpy(p, &h, sizeof h);
+    p += sizeof h;
+    std::memcpy(p, f->tree_off.data(), 8 * (size_t)(f->T + 1));
+    p += 8 * (size_t)(f->T + 1);
+    auto put = [&](const void *src, bool have) {
+      if (have) std::memcpy(p, src, 4 * (size_t)n);
+      else std::memset(p, 0, 4 * (size_t)n);
+      p += 4 * (size_t)n;
+    };
+    put(f->feature.data(), true);
+    put(f->threshold.data(), true);
+    put(f->left.data(), true);
+    put(f->right.data(), true);
+    put(f->value.data(), true);
+    put(f->nsamp.data(), f->nsamp.size() == (size_t)n);
+    put(f->depth.data(), f->depth.size() == (size_t)n);
+  });
+}
+
+int cta_forest_load(cta_ctx *ctx, const void *buf, uint64_t size, cta_forest **out) {
+  return guarded([&] {
+    if (!ctx || !buf || !out) fail(CTA_EINVAL, "NULL argument");
+    ForestHeader h;
+    if (size < sizeof h) fail(CTA_EINVAL, "image of %llu bytes is shorter than its header", (unsigned long long)size);
+    std::memcpy(&h, buf, sizeof h);
+    if (std::memcmp(h.magic, kForestMagic, 8) != 0) fail(CTA_EINVAL, "not a serialised forest (bad magic)");
+    if (h.version != CTA_FOREST_FORMAT)
+      fail(CTA_EUNSUPPORTED, "forest format version %u is not supported (this library reads version %u)",
+           h.version, (uint32_t)CTA_FOREST_FORMAT);
+    if (h.header_bytes != kForestHeader || h.T <= 0 || h.nodes <= 0 || h.nodes > (int64_t)UINT32_MAX)
+      fail(CTA_EINVAL, "corrupt forest header");
+    if (forest_image_bytes(h.T, h.nodes) != size)
+      fail(CTA_EINVAL, "image holds %llu bytes, its header describes %llu", (unsigned long long)size,
+           (unsigned long long)forest_image_bytes(h.T, h.nodes));
+    const char *p = static_cast<const char *>(buf) + sizeof h;
+    // copies: the arrays of the image need not be aligned
+    std::vector<int64_t> off((size_t)h.T + 1);
+    std::memcpy(off.data(), p, 8 * off.size());
+    p += 8 * off.size();
+    if (off[0] != 0 || off[h.T] != h.nodes) fail(CTA_EINVAL, "tree offsets do not match the node count");
+    const size_t n = (size_t)h.nodes;
+    std::vector<uint32_t> arr(7 * n);
+    std::memcpy(arr.data(), p, 28 * n);
+    const uint32_t *a = arr.data();
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    HIPC(hipSetDevice(ctx->device));
+    *out = import_arrays(ctx, h.T, h.F, h.K, off.data(), reinterpret_cast<const int32_t *>(a),
+                         reinterpret_cast<const float *>(a + n), a + 2 * n, a + 3 * n, a + 4 * n, a + 5 * n,
+                         a + 6 * n);
   });
 }
 

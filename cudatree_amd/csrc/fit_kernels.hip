@@ -482,7 +482,7 @@ __global__ __launch_bounds__(kBlock) void k_scan_hist(FitArgs<IdxT, LabT> a) {
     const IdxT *row = rows_of(a, nd.src, nd.tree, f) + nd.start;
     // K <= 64: lane k counts class k in a register, one ballot per distinct
     // label of a tile; else LDS atomics
-    const bool regs = K <= (uint32_t)kWave;
+    const bool regs = K <= (uint32_t)kWave && !a.hist_lds;
     uint32_t hv = 0;
     if (!regs) {
       for (uint32_t k = lane; k < K; k += kWave) h[k] = 0;
@@ -592,7 +592,7 @@ __global__ __launch_bounds__(kBlock, 6) void k_scan(FitArgs<IdxT, LabT> a) {
     const NAux ax = a.aux[i];
     const uint32_t n = nd.n;
     // class totals: lane k holds class k when K <= 64 (else in LDS)
-    const bool regs = K <= (uint32_t)kWave;
+    const bool regs = K <= (uint32_t)kWave && !a.hist_lds;
     uint64_t t2 = 0;
     uint32_t tv = 0;
     for (uint32_t k = lane; k < K; k += kWave) {

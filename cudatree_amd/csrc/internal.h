@@ -158,6 +158,7 @@ struct FitArgs {
   uint32_t tree_base;  // global index of batch tree 0 (keys the permutation)
   uint64_t seed;
   uint32_t Cs, Cf, Cr;  // chunk sizes: scan, fill, reshuffle
+  uint32_t hist_lds;    // 1: split-scan class counts in per-wave LDS counters even when K <= 64
   // level state
   const FNode *front;
   const uint32_t *cnt;  // q x K class counts of the frontier

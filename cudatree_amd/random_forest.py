@@ -191,13 +191,33 @@ class RandomForestClassifier(object):
         return np.mean(self.predict(X) == Y)
 
     # -- pickling -------------------------------------------------------------
+    # Pickled state: the constructor and fit parameters, the label table
+    # (compt_table, the original class values) and the trees as the versioned
+    # byte image of cta_forest_save.  The reference pickled bare tree arrays
+    # with no version and restored a forest that could not predict
+    # (random_forest.py:17-23, 42-43).
+    STATE_VERSION = 1
+
     def __getstate__(self):
         state = {k: v for k, v in self.__dict__.items() if k not in ("_forest", "_arrays")}
+        state["state_version"] = self.STATE_VERSION
         if self._forest is not None or self._arrays is not None:
-            state["_arrays"] = {k: np.array(v) for k, v in self.export_trees().items()}
+            state["forest_image"] = _lib.forest_to_bytes(self.export_trees(), self.n_features, self.n_labels)
         return state
 
     def __setstate__(self, state):
+        version = state.get("state_version")
+        if version != self.STATE_VERSION:
+            raise ValueError("pickled RandomForestClassifier has state version %r; this build reads version %d"
+                             % (version, self.STATE_VERSION))
+        state = dict(state)
+        image = state.pop("forest_image", None)
+        state.pop("state_version")
         self.__dict__.update(state)
         self._forest = None
-        self._arrays = state.get("_arrays")
+        self._arrays = None
+        if image is not None:
+            arrays, F, K = _lib.forest_from_bytes(image)
+            if F != self.n_features or K != self.n_labels or len(arrays["tree_off"]) - 1 != self.n_estimators:
+                raise ValueError("pickled forest image does not match the classifier's shape")
+            self._arrays = arrays

@@ -107,8 +107,17 @@ typedef struct {
   int32_t lanes;             /* batched mode: 2 runs the batches on two streams of the context at once
                                 (alternate batches, half the trees each when memory allows, each lane holding
                                 at least 8 trees); 0 or 1 (default): one stream.  The trees are identical */
-  int32_t reserved[1];
+  int32_t fill_global;       /* 1: the side bitmaps are built with global atomics into cleared bitmaps instead
+                                of in LDS (the path taken anyway when N / 8 bytes exceed a workgroup's LDS) */
+  int32_t hist_layout;       /* class-count layout of the split scan: CTA_HIST_AUTO (0), CTA_HIST_REGS (1: lane k
+                                holds class k, K <= 64 only) or CTA_HIST_LDS (2: per-wave LDS counters).  The
+                                trees are identical */
+  int32_t reserved[2];
 } cta_fit_opts;
+
+#define CTA_HIST_AUTO 0
+#define CTA_HIST_REGS 1
+#define CTA_HIST_LDS 2
 
 /* Fills opts with the defaults (batched mode, m = ceil(sqrt(F)) left to the caller). */
 void cta_fit_opts_default(cta_fit_opts *opts);
@@ -143,6 +152,23 @@ int cta_forest_export(const cta_forest *forest, int64_t *tree_off, int32_t *feat
 int cta_forest_import(cta_ctx *ctx, int32_t T, int32_t F, int32_t K, const int64_t *tree_off,
                       const int32_t *feature, const float *threshold, const uint32_t *left,
                       const uint32_t *right, const uint32_t *value, cta_forest **out);
+
+/* Serialised forest: a versioned little-endian byte image of the flattened
+ * node table, 48-byte header {char magic[8] = "CTAFORST", uint32 version
+ * (CTA_FOREST_FORMAT), uint32 header bytes (48), int32 T, F, K, uint32 flags
+ * (0), int64 nodes, uint64 0}, then tree_off (T + 1 x int64) and the node
+ * arrays in the layout of cta_forest_export: feature (int32), threshold
+ * (float32), left, right, value, n_samples, depth (uint32), one entry per
+ * node each.  The label table (original class values) is the caller's.
+ * cta_forest_save with buf == NULL reports the image size in *size.
+ * cta_forest_load rejects an image of another version with
+ * CTA_EUNSUPPORTED and a malformed one with CTA_EINVAL.
+ * Replaces: the pickled tree arrays of random_tree.py:203-216 and the
+ * restore path of random_forest.py:17-23 (which carried no format version
+ * and left the restored forest unable to predict). */
+#define CTA_FOREST_FORMAT 1
+int cta_forest_save(const cta_forest *forest, void *buf, uint64_t cap, uint64_t *size);
+int cta_forest_load(cta_ctx *ctx, const void *buf, uint64_t size, cta_forest **out);
 
 /* Majority vote of every tree for R row-major rows (host X, R x F).  Writes
  * the winning label per row (lowest label on ties) and, if votes != NULL,

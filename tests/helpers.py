@@ -58,3 +58,15 @@ def forest_diff(a, b, fields=FIELDS):
             idx = np.nonzero(x != y)[0] if x.shape == y.shape else []
             return "%s differs (first at %s)" % (k, idx[:5])
     return None
+
+
+def forest_slice(a, t0, t1):
+    """Trees [t0, t1) of an exported forest (node arrays and tree offsets
+    rebased to 0)."""
+    off = np.asarray(a["tree_off"])
+    b, e = int(off[t0]), int(off[t1])
+    out = {"tree_off": off[t0:t1 + 1] - b}
+    for k in FIELDS:
+        if k in a:
+            out[k] = np.asarray(a[k])[b:e]
+    return out

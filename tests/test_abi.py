@@ -50,3 +50,28 @@ def test_error_paths_without_gpu():
     assert lib.cta_version() >= 100
     rc = lib.cta_ctx_synchronize(None)
     assert rc == 1 and b"NULL" in lib.cta_last_error()
+
+
+def test_struct_layouts_match_header(tmp_path):
+    """The ctypes mirrors of the ABI structs have the header's size and
+    field offsets (compiled with the host C compiler, no GPU)."""
+    from cudatree_amd import _lib
+
+    structs = {"cta_fit_opts": _lib.FitOpts, "cta_server_opts": _lib.ServerOpts,
+               "cta_server_stats": _lib.ServerStats, "cta_kernel_stat": _lib.KernelStat}
+    lines = ["#include <stddef.h>", "#include <stdio.h>", '#include "cudatree_amd.h"', "int main(void) {"]
+    for cname, py in structs.items():
+        lines.append('  printf("%s %%zu\\n", sizeof(%s));' % (cname, cname))
+        for field, _ in py._fields_:
+            lines.append('  printf("%s.%s %%zu\\n", offsetof(%s, %s));' % (cname, field, cname, field))
+    lines.append("  return 0;\n}")
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), "-o", str(exe), str(src)], check=True)
+    got = dict(line.split() for line in subprocess.run([str(exe)], check=True, capture_output=True,
+                                                         text=True).stdout.splitlines())
+    for cname, py in structs.items():
+        assert int(got[cname]) == ctypes.sizeof(py), cname
+        for field, _ in py._fields_:
+            assert int(got["%s.%s" % (cname, field)]) == getattr(py, field).offset, (cname, field)

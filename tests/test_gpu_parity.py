@@ -24,7 +24,8 @@ def orc():
     return oracle.get()
 
 
-def gpu_forest(ctx, case, mode="batched", scan_chunk=0, resh_chunk=0, idx_bits=0, tpb=0, resh_global=0, all_rows=0):
+def gpu_forest(ctx, case, mode="batched", scan_chunk=0, resh_chunk=0, idx_bits=0, tpb=0, resh_global=0, all_rows=0,
+               fill_global=0, hist_layout=0):
     X, y, K = case["X"], case["y"], case["K"]
     ds = _lib.Dataset(ctx, X, y, K, idx_bits=idx_bits)
     o = _lib.default_opts()
@@ -40,6 +41,8 @@ def gpu_forest(ctx, case, mode="batched", scan_chunk=0, resh_chunk=0, idx_bits=0
     o.trees_per_batch = tpb
     o.resh_global = resh_global
     o.all_rows = all_rows
+    o.fill_global = fill_global
+    o.hist_layout = hist_layout
     f = _lib.Forest.fit(ctx, ds, case["n_trees"], o, helpers.case_boot_ids(case))
     return f, ds
 
@@ -164,16 +167,97 @@ def test_forest_tiny_cases(ctx, orc, mode, chunk, all_rows):
         assert d is None, "seed %d: %s" % (s, d)
 
 
-def test_forest_many_classes_u16_labels(ctx, orc):
+@pytest.mark.parametrize("idx_bits", [0, 32])
+def test_forest_many_classes_u16_labels(ctx, orc, idx_bits):
+    """K = 300: 16-bit labels, with 16-bit sample ids (labels read from y)
+    and with 32-bit row entries (labels in bits 22-31 of each entry, class
+    counts in LDS)."""
     rng = np.random.default_rng(9)
     N, F, K = 3000, 6, 300
     X = rng.integers(0, 40, size=(N, F)).astype(np.float32)
     y = rng.integers(0, K, N)
     case = {"X": X, "y": y, "K": K, "m": 3, "mss": 1, "max_depth": 0, "identity": False, "seed": 5,
             "boot": True, "n_trees": 3}
-    print("forest K=300", flush=True)
-    f, _ = gpu_forest(ctx, case, scan_chunk=256, resh_chunk=256)
+    print("forest K=300 idx_bits=%d" % idx_bits, flush=True)
+    f, ds = gpu_forest(ctx, case, scan_chunk=256, resh_chunk=256, idx_bits=idx_bits)
+    assert ds.idx_bits == (idx_bits or 16)
     assert helpers.forest_diff(f.export(), helpers.oracle_forest(case, orc)) is None
+
+
+@pytest.mark.parametrize("mode", ["batched", "per_node"])
+def test_fill_global_bitmaps(ctx, orc, mode):
+    """Side bitmaps built with global atomics (fill_global=1: the path of
+    N / 8 bytes beyond LDS), with row-free two-valued features counted by
+    the fill tasks, bootstrap, several chunks per node."""
+    X, y, _, _ = datasets.covtype_like(seed=29, n_train=20000, n_test=1)
+    case = {"X": X, "y": y, "K": 7, "m": 7, "mss": 1, "max_depth": 0, "identity": False, "seed": 31,
+            "boot": True, "n_trees": 3 if mode == "batched" else 1}
+    want = helpers.oracle_forest(case, orc)
+    for chunk in (0, 512):
+        print("fill_global mode=%s chunk=%d" % (mode, chunk), flush=True)
+        f, _ = gpu_forest(ctx, case, mode=mode, scan_chunk=chunk, fill_global=1)
+        assert helpers.forest_diff(f.export(), want) is None, chunk
+    for s in TINY[:20]:
+        tc = datasets.tiny_case(s)
+        print("fill_global tiny seed=%d" % s, flush=True)
+        f, _ = gpu_forest(ctx, tc, mode=mode, fill_global=1, scan_chunk=64, resh_chunk=64)
+        assert helpers.forest_diff(f.export(), helpers.oracle_forest(tc, orc)) is None, s
+
+
+def test_hist_layout_lds(ctx, orc):
+    """Split-scan class counts in per-wave LDS counters for K <= 64 (the
+    layout K > 64 always takes) build the same trees; CTA_HIST_REGS is
+    refused for K > 64."""
+    X, y, _, _ = datasets.covtype_like(seed=37, n_train=15000, n_test=1)
+    case = {"X": X, "y": y, "K": 7, "m": 7, "mss": 1, "max_depth": 0, "identity": False, "seed": 3,
+            "boot": True, "n_trees": 3}
+    want = helpers.oracle_forest(case, orc)
+    for layout, chunk in ((1, 0), (2, 0), (2, 256)):
+        print("hist_layout=%d chunk=%d" % (layout, chunk), flush=True)
+        f, _ = gpu_forest(ctx, case, hist_layout=layout, scan_chunk=chunk)
+        assert helpers.forest_diff(f.export(), want) is None, (layout, chunk)
+    big = {"X": X[:500], "y": np.arange(500) % 70, "K": 70, "m": 7, "mss": 1, "max_depth": 0, "identity": False,
+           "seed": 3, "boot": False, "n_trees": 1}
+    with pytest.raises(_lib.CtaError):
+        gpu_forest(ctx, big, hist_layout=1)
+
+
+# (1.0, 1.0000100135803223) is a candidate pair in double precision but not in
+# float (1 + 1e-5f rounds up to the second value); (1.0, 1.000009) is no
+# candidate in either.
+GAP_A = np.float32(1.0)
+GAP_B = np.float32(1.0000100135803223)
+GAP_C = np.float32(1.000009)
+
+
+@pytest.mark.parametrize("all_rows", [0, 1])
+def test_candidate_gap_double_on_device(ctx, orc, all_rows):
+    """The candidate gap test is done in double on the device as in the
+    oracle: a two-valued feature {GAP_A, GAP_B} (bit code column; row-free
+    unless all_rows), a feature whose levels include a pair closer than the
+    gap (no code column: the float gap test over X_T) and a feature {GAP_A,
+    GAP_C} that is never a candidate.  The labels follow GAP_B, so the trees
+    split on the pair."""
+    rng = np.random.default_rng(43)
+    N = 5000
+    X = np.empty((N, 4), np.float32)
+    X[:, 0] = rng.normal(size=N)
+    hi1 = rng.random(N) < 0.4
+    X[:, 1] = np.where(hi1, GAP_B, GAP_A)
+    lv = rng.integers(0, 4, N)
+    X[:, 2] = np.array([0.5, 0.500004, GAP_A, GAP_B], np.float32)[lv]
+    X[:, 3] = np.where(rng.random(N) < 0.5, GAP_A, GAP_C)
+    y = (hi1.astype(int) + 2 * (lv == 3) + (rng.random(N) < 0.05)) % 4
+    case = {"X": X, "y": y, "K": 4, "m": 2, "mss": 1, "max_depth": 0, "identity": False, "seed": 47,
+            "boot": True, "n_trees": 4}
+    print("gap pair all_rows=%d" % all_rows, flush=True)
+    f, _ = gpu_forest(ctx, case, all_rows=all_rows)
+    got = f.export()
+    assert helpers.forest_diff(got, helpers.oracle_forest(case, orc)) is None
+    split = got["left"] != 0
+    mid = np.float32((GAP_A + GAP_B) * np.float32(0.5))
+    assert np.any(split & (got["feature"] == 1)) and np.all(got["threshold"][split & (got["feature"] == 1)] == mid)
+    assert not np.any(split & (got["feature"] == 3))
 
 
 def test_forest_digits_like_config(ctx, orc):
