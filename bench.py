@@ -35,7 +35,7 @@ def log(msg):
     print("[bench] " + msg, file=sys.stderr, flush=True)
 
 
-def fit_once(ctx, lib, Xd, yd, N, F, K, mode="batched", n_trees=N_TREES, export=True):
+def fit_once(ctx, lib, Xd, yd, N, F, K, mode="batched", n_trees=N_TREES, export=True, m=MAX_FEATURES, seed=SEED):
     """One complete fit from the resident inputs to a forest whose canonical
     trees are in host memory; export=True also copies them into numpy arrays."""
     ds_h = lib.load()
@@ -47,14 +47,90 @@ def fit_once(ctx, lib, Xd, yd, N, F, K, mode="batched", n_trees=N_TREES, export=
     ds = lib.Dataset.__new__(lib.Dataset)
     ds.ctx, ds.lib, ds.h, ds.N, ds.F, ds.K = ctx, ds_h, h, N, F, K
     o = lib.default_opts()
-    o.seed = SEED
-    o.max_features = MAX_FEATURES
+    o.seed = seed
+    o.max_features = m
     o.mode = lib.MODES[mode]
     o.bootstrap = 1  # in-bag sets drawn on the device from (seed, tree)
     forest = lib.Forest.fit(ctx, ds, n_trees, o, None)
     arrays = forest.export() if export else None
     ds.close()
     return forest, arrays
+
+
+def kernel_table(stats, steps):
+    """Per-kernel ms per fit, achieved algorithmic GB/s and fraction of peak
+    from the context's HIP-event statistics."""
+    out = {}
+    for k, v in sorted(stats.items(), key=lambda kv: -kv[1]["ms"]):
+        if k.startswith("host_") or v["ms"] <= 0:
+            continue
+        e = {"ms_per_fit": round(v["ms"] / steps, 3), "launches_per_fit": round(v["launches"] / steps, 1)}
+        if v["alg_bytes"]:
+            gbs = v["alg_bytes"] / (v["ms"] * 1e-3) / 1e9
+            e.update({"achieved_gbs": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4)})
+        out[k] = e
+    return out
+
+
+# The other fit workloads: full-size synthetic sets of the generators in
+# cudatree_amd.datasets, timed as fits from resident inputs.
+WORKLOADS = [
+    ("gaussian_clusters_200000x256", "200,000 x 256 float32, 20 classes of 3 Gaussian clusters (centroids N(0, 2^2), "
+     "sigma 1); 200 trees, bootstrap, max_features=16, full depth", "gaussian_clusters", {}, 200, 16),
+    ("zipf_quantized_1000000x64", "1,000,000 x 64 float32 quantised to 256 levels, 100 classes with Zipf(1.1) "
+     "frequencies; 100 trees, bootstrap, max_features=8, full depth", "zipf_quantized", {}, 100, 8),
+    ("wide_sparse_20000x4096", "20,000 x 4,096 float32, 5 classes, 64 informative features (0.5 sigma shift); "
+     "100 trees, bootstrap, max_features=64, full depth", "wide_sparse_signal", {}, 100, 64),
+]
+
+
+def workload_bench(lib, ctx, args, torch, dev):
+    """s per fit and per-kernel fractions of the other workloads; the
+    autotuner's sweep on the 200,000 x 256 set."""
+    from cudatree_amd import autotune, datasets
+
+    out = {}
+    for name, desc, gen, kw, T, m in WORKLOADS:
+        X, y = getattr(datasets, gen)(**kw)
+        N, F = X.shape
+        K = int(y.max()) + 1
+        Xd = torch.from_numpy(X).to(dev)
+        yd = torch.from_numpy(y.astype(np.uint32).view(np.int32)).to(dev)
+        torch.cuda.synchronize()
+        forest, arrays = fit_once(ctx, lib, Xd, yd, N, F, K, n_trees=T, m=m, seed=SEED + 7)
+        nodes = len(arrays["feature"])
+        del forest
+        steps = args.workload_steps
+        ctx.synchronize()
+        t = time.perf_counter()
+        for _ in range(steps):
+            forest, _ = fit_once(ctx, lib, Xd, yd, N, F, K, n_trees=T, m=m, seed=SEED + 7, export=False)
+            del forest
+        ctx.synchronize()
+        dt = (time.perf_counter() - t) / steps
+        ctx.reset_stats()
+        ctx.set_profiling(True)
+        forest, _ = fit_once(ctx, lib, Xd, yd, N, F, K, n_trees=T, m=m, seed=SEED + 7, export=False)
+        ctx.set_profiling(False)
+        del forest
+        e = {"workload": desc, "s_per_fit": round(dt, 4), "steps": steps, "n_trees": T, "nodes": nodes,
+             "kernels": kernel_table(ctx.kernel_stats(), 1)}
+        ctx.reset_stats()
+        log("workload %s: %.4f s per fit of %d trees (%d nodes)" % (name, dt, T, nodes))
+        if name.startswith("gaussian") and args.tune_probe_trees > 0:
+            t = time.perf_counter()
+            best, table = autotune.tune(X, y, K, m, probe_trees=args.tune_probe_trees, repeats=1, seed=SEED + 7,
+                                        use_cache=False)
+            e["autotune"] = {"probe": "%d of %d trees on the full set, one timing per setting" %
+                                      (args.tune_probe_trees, T),
+                             "best": best, "sweep_s": round(time.perf_counter() - t, 1),
+                             "table": [{"setting": st, "s": round(sec, 4)} for st, sec in table]}
+            log("autotune %s: best %s (%.4f s) of %d settings" % (name, best, table[0][1], len(table)))
+        out[name] = e
+        del Xd, yd
+        ctx.release_workspace()
+        torch.cuda.empty_cache()
+    return out
 
 
 def serving_bench(lib, X, y, Xt, args):
@@ -226,6 +302,10 @@ def main():
     ap.add_argument("--serving-seconds", type=float, default=6.0, help="length of each serving stream (0: skip)")
     ap.add_argument("--serving-trees", type=int, default=500, help="trees of the resident serving forest")
     ap.add_argument("--serving-qps", type=float, default=1000.0, help="predict requests per second")
+    ap.add_argument("--workload-steps", type=int, default=3, help="timed fits per other workload (0: skip them)")
+    ap.add_argument("--tune-probe-trees", type=int, default=8,
+                    help="trees per autotuner probe on the 200,000 x 256 workload (0: no sweep)")
+    ap.add_argument("--facade-steps", type=int, default=3, help="timed facade fits from host arrays (0: skip)")
     ap.add_argument("--pmc-summary", nargs="+", metavar="CSV",
                     help="FETCH_SIZE csv, WRITE_SIZE csv [, TCC_HIT/TCC_MISS csv]: print the traffic summary and exit")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -362,7 +442,10 @@ def main():
         t = time.perf_counter()
         f2, a2 = fit_once(ctx, lib, Xd, yd, N, F, K, mode="per_node", n_trees=nt)
         dt = time.perf_counter() - t
-        same = all(np.array_equal(a2[k], arrays[k][: len(a2[k])]) for k in ("feature", "left", "right", "value"))
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import helpers  # tests/helpers.py: exact forest comparison
+
+        same = helpers.forest_diff(helpers.forest_slice(arrays, 0, nt), a2) is None
         per_node = {"sample": "%d of %d trees, per-node launches, scaled x%g" % (nt, N_TREES, N_TREES / nt),
                     "s_per_50_trees": round(dt * N_TREES / nt, 3), "speedup_batched": round(dt * N_TREES / nt / t_fit, 2),
                     "identical_trees": bool(same)}
@@ -381,7 +464,12 @@ def main():
         t = time.perf_counter()
         fo = o.fit(X, y, K, nt, MAX_FEATURES, boot_ids=ids, seed=SEED)
         dt = time.perf_counter() - t
-        same = all(np.array_equal(fo[k], arrays[k][: len(fo[k])]) for k in ("feature", "left", "right", "value"))
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import helpers  # tests/helpers.py: exact forest comparison
+
+        # every field of the first nt trees: thresholds bit for bit, sample
+        # counts, depths and the tree boundaries
+        same = helpers.forest_diff(helpers.forest_slice(arrays, 0, nt), fo) is None
         cpu = {"value": round(dt * N_TREES / nt, 2), "unit": "s/50 trees", "cores": 1, "kind": "port",
                "sample": "%d of %d trees of the same workload, scaled x%.3g" % (nt, N_TREES, N_TREES / nt),
                "identical_trees": bool(same)}
@@ -412,6 +500,31 @@ def main():
         log("calibration copy: 1 GiB read + 1 GiB written")
         del src, dst
 
+    # ---- the facade as an API caller runs it: host X and labels to a forest
+    facade = None
+    if args.facade_steps > 0:
+        import cudatree_amd
+
+        times = []
+        for k in range(args.facade_steps + 1):
+            t = time.perf_counter()
+            rf = cudatree_amd.RandomForestClassifier(n_estimators=N_TREES, max_features=MAX_FEATURES,
+                                                     random_state=SEED).fit(X, y)
+            dt = time.perf_counter() - t
+            if k:
+                times.append(dt)
+            del rf
+        facade = {"s_per_fit": round(float(np.median(times)), 4), "steps": args.facade_steps,
+                  "what": "RandomForestClassifier(n_estimators=50, max_features=7).fit(X, y) from host numpy "
+                          "arrays: label compaction, upload, presort, fit, forest resident for predict"}
+        log("facade fit from host arrays: %.4f s" % facade["s_per_fit"])
+
+    workloads = None
+    if args.workload_steps > 0:
+        del forest
+        ctx.release_workspace()
+        workloads = workload_bench(lib, ctx, args, torch, dev)
+
     serving = None
     if args.serving_seconds > 0:
         serving = serving_bench(lib, X, y, Xt, args)
@@ -424,7 +537,8 @@ def main():
                                   "max_features=7, full depth; predict 100,000 held-out rows",
                       "n_samples": N, "n_features": F, "n_classes": K, "n_trees": N_TREES,
                       "nodes": int(len(arrays["feature"])), "mode": "batched"},
-           "roofline": roof, "cpu_baseline": cpu, "predict": predict, "per_node_gpu": per_node, "sklearn": sk,
+           "roofline": roof, "cpu_baseline": cpu, "facade_fit_s": facade["s_per_fit"] if facade else None,
+           "facade": facade, "predict": predict, "per_node_gpu": per_node, "sklearn": sk, "workloads": workloads,
            "serving": serving}
     print(json.dumps(out))
 

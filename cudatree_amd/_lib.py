@@ -18,6 +18,7 @@ CTA_MODE_BATCHED = 0
 CTA_MODE_HYBRID = 1
 CTA_MODE_PER_NODE = 2
 MODES = {"batched": CTA_MODE_BATCHED, "hybrid": CTA_MODE_HYBRID, "per_node": CTA_MODE_PER_NODE}
+CTA_HIST_AUTO, CTA_HIST_REGS, CTA_HIST_LDS = 0, 1, 2
 
 MAX_SAMPLES = 4194303
 MAX_FEATURES = 65535

@@ -15,6 +15,8 @@ Tuned settings:
   scan_chunk            samples per split-scan task (tile width of the scan)
   resh_chunk            samples per small-node reshuffle task
   resh_global           side bitmap staged in LDS or read from global memory
+  hist_layout           the split scan's class counts: lane k holding class
+                        k in a register (K <= 64) or per-wave LDS counters
 """
 import itertools
 import time
@@ -26,9 +28,10 @@ from . import _lib
 _cache = {}
 
 DEFAULT_GRID = {
-    "scan_chunk": [1024, 2048, 8192],
-    "resh_chunk": [1024, 2048, 8192],
+    "scan_chunk": [512, 1024, 2048, 8192],
+    "resh_chunk": [2048, 4096, 8192],
     "resh_global": [0, 1],
+    "hist_layout": [_lib.CTA_HIST_REGS, _lib.CTA_HIST_LDS],
 }
 HYBRID_THRESHOLDS = [256, 1024, 4096, 16384, 65536]
 
@@ -37,10 +40,13 @@ def _signature(N, F, K, m, bootstrap, max_depth):
     return (N, F, K, m, bool(bootstrap), int(max_depth or 0))
 
 
-def candidate_settings(grid=None, hybrid=True):
+def candidate_settings(grid=None, hybrid=True, K=None):
     """Settings to try: the product of the grid in batched mode plus the
-    hybrid schedule at each bfs_threshold (default chunks)."""
-    grid = grid or DEFAULT_GRID
+    hybrid schedule at each bfs_threshold (default chunks).  With K > 64
+    classes the register layout does not exist and the layout knob drops."""
+    grid = dict(grid or DEFAULT_GRID)
+    if K is not None and K > 64 and "hist_layout" in grid:
+        grid["hist_layout"] = [_lib.CTA_HIST_LDS]
     keys = sorted(grid)
     out = []
     for vals in itertools.product(*(grid[k] for k in keys)):
@@ -74,7 +80,7 @@ def tune(X, y, K, max_features, bootstrap=True, max_depth=0, seed=0, probe_trees
     ctx = _lib.default_context(device)
     ds = _lib.Dataset(ctx, X, y, K)
     table = []
-    for setting in candidate_settings(grid, hybrid):
+    for setting in candidate_settings(grid, hybrid, K):
         best = None
         for _ in range(repeats):
             o = _lib.default_opts()
