@@ -135,9 +135,13 @@ def workload_bench(lib, ctx, args, torch, dev):
 
 def serving_bench(lib, X, y, Xt, args):
     """Resident forest of the tabular type (serving_trees trees, depth <= 24)
-    behind the serving loop; a Poisson stream of predict requests
-    (log-uniform 1..65,536 rows) mixed with small fit jobs (1,000-200,000 rows,
-    10-50 trees), replayed with coalesced and with per-request launches."""
+    behind the serving loop.  A sweep of Poisson streams of predict requests
+    (log-uniform 1..65,536 rows) mixed with small fit jobs (1,000-200,000
+    rows, 10-50 trees), each replayed with coalesced launches (short and long
+    request lanes) and with one launch per request, up to rates the
+    per-request launches cannot keep up with; then a stream of small requests
+    (log-uniform 1..64 rows), where launches rather than traversal set the
+    rate."""
     import threading
 
     ctx = lib.default_context(0)
@@ -153,20 +157,20 @@ def serving_bench(lib, X, y, Xt, args):
     ds.close()
     ctx.release_workspace()
     log("serving forest: %d trees, %d nodes, fit %.2fs" % (forest.n_trees, forest.n_nodes, t_fit))
-    rng = np.random.default_rng(11)
     dur = args.serving_seconds
-    n_req = int(args.serving_qps * dur)
-    arrivals = np.cumsum(rng.exponential(1.0 / args.serving_qps, size=n_req))
-    sizes = np.minimum(np.exp(rng.uniform(0, np.log(65536), size=n_req)).astype(int), 65536)
-    starts = rng.integers(0, len(Xt) - 65536, size=n_req)
-    fits = []
-    for k in range(int(dur // 2)):
-        n = int(np.exp(rng.uniform(np.log(1000), np.log(200000))))
-        fits.append((2.0 * k + 1.0, n, int(rng.integers(10, 51)), int(rng.integers(0, len(X) - n))))
-    result = {"forest_trees": forest.n_trees, "forest_nodes": forest.n_nodes, "forest_fit_s": round(t_fit, 2),
-              "stream": "Poisson %.0f req/s for %.0fs, predict rows log-uniform 1..65536, %d fit jobs "
-                        "(1k-200k rows, 10-50 trees)" % (args.serving_qps, dur, len(fits))}
-    for coalesce in (True, False):
+    K = int(y.max()) + 1
+
+    def stream(qps, largest, with_fits, coalesce, seed):
+        rng = np.random.default_rng(seed)
+        n_req = int(qps * dur)
+        arrivals = np.cumsum(rng.exponential(1.0 / qps, size=n_req))
+        sizes = np.minimum(np.exp(rng.uniform(0, np.log(largest), size=n_req)).astype(int), largest)
+        starts = rng.integers(0, len(Xt) - largest, size=n_req)
+        fits = []
+        if with_fits:
+            for k in range(int(dur // 2)):
+                n = int(np.exp(rng.uniform(np.log(1000), np.log(200000))))
+                fits.append((2.0 * k + 1.0, n, int(rng.integers(10, 51)), int(rng.integers(0, len(X) - n))))
         srv = lib.Server(coalesce=coalesce, fit_workers=1)
         fid = srv.add_forest(forest)
         tickets, fit_tickets = [], []
@@ -181,7 +185,7 @@ def serving_bench(lib, X, y, Xt, args):
                     fo.seed = 5 + fi
                     fo.max_features = MAX_FEATURES
                     fo.bootstrap = 1
-                    fit_tickets.append(srv.submit_fit(X[a:a + n], y[a:a + n], int(y.max()) + 1, T, fo))
+                    fit_tickets.append(srv.submit_fit(X[a:a + n], y[a:a + n], K, T, fo))
                     fi += 1
                 delay = arrivals[i] - (time.perf_counter() - t0)
                 if delay > 0:
@@ -191,6 +195,7 @@ def serving_bench(lib, X, y, Xt, args):
         th = threading.Thread(target=producer)
         th.start()
         th.join()
+        offered = n_req / max(time.perf_counter() - t0, 1e-9)
         for tk in tickets:
             st, _ = srv.wait(tk)
             assert st == 0
@@ -198,13 +203,37 @@ def serving_bench(lib, X, y, Xt, args):
             srv.wait(tk)
         s = srv.stats()
         srv.close()
-        key = "coalesced" if coalesce else "per_request"
-        result[key] = {"p50_ms": round(s["p50_ms"], 3), "p99_ms": round(s["p99_ms"], 3),
-                       "rows_per_s": round(s["rows_per_s"], 1), "requests": s["predict_requests"],
-                       "launches": s["predict_launches"], "fit_jobs": s["fit_jobs"],
-                       "hbm_peak_gb": round(s["hbm_peak"] / 1e9, 2)}
-        log("serving %s: p50 %.2f ms p99 %.2f ms, %.0f rows/s, %d launches for %d requests" %
-            (key, s["p50_ms"], s["p99_ms"], s["rows_per_s"], s["predict_launches"], s["predict_requests"]))
+        return {"p50_ms": round(s["p50_ms"], 3), "p99_ms": round(s["p99_ms"], 3),
+                "rows_per_s": round(s["rows_per_s"], 1), "requests": s["predict_requests"],
+                "requests_per_s_submitted": round(offered, 1), "launches": s["predict_launches"],
+                "fit_jobs": s["fit_jobs"], "hbm_peak_gb": round(s["hbm_peak"] / 1e9, 2),
+                "fit_mem_measured_over_admitted": round(s["fit_mem_ratio_max"], 3)}
+
+    result = {"forest_trees": forest.n_trees, "forest_nodes": forest.n_nodes, "forest_fit_s": round(t_fit, 2),
+              "stream": "Poisson arrivals for %.0fs per rate, predict rows log-uniform 1..65536, one fit job "
+                        "(1k-200k rows, 10-50 trees) every 2 s; coalesced = short (<= 4,096 rows) and long "
+                        "request lanes, per_request = one launch per request on one lane" % dur,
+              "sweep": []}
+    for qps in args.serving_qps:
+        e = {"qps": qps}
+        for coalesce in (True, False):
+            key = "coalesced" if coalesce else "per_request"
+            e[key] = stream(qps, 65536, True, coalesce, 11)
+            log("serving %5.0f req/s %-11s: p50 %.2f ms p99 %.2f ms, %.0f rows/s, %d launches for %d requests" %
+                (qps, key, e[key]["p50_ms"], e[key]["p99_ms"], e[key]["rows_per_s"], e[key]["launches"],
+                 e[key]["requests"]))
+        e["rows_per_s_ratio"] = round(e["coalesced"]["rows_per_s"] / max(e["per_request"]["rows_per_s"], 1e-9), 2)
+        result["sweep"].append(e)
+    if args.serving_small_qps > 0:
+        e = {"qps": args.serving_small_qps, "request_rows": "log-uniform 1..64, no fit jobs"}
+        for coalesce in (True, False):
+            key = "coalesced" if coalesce else "per_request"
+            e[key] = stream(args.serving_small_qps, 64, False, coalesce, 12)
+            log("serving small %5.0f req/s %-11s: p50 %.2f ms p99 %.2f ms, %.0f rows/s, %d launches" %
+                (args.serving_small_qps, key, e[key]["p50_ms"], e[key]["p99_ms"], e[key]["rows_per_s"],
+                 e[key]["launches"]))
+        e["rows_per_s_ratio"] = round(e["coalesced"]["rows_per_s"] / max(e["per_request"]["rows_per_s"], 1e-9), 2)
+        result["small_requests"] = e
     return result
 
 
@@ -299,9 +328,12 @@ def main():
     ap.add_argument("--per-node-trees", type=int, default=1, help="trees in the per-node GPU sample (0: skip)")
     ap.add_argument("--sklearn-trees", type=int, default=8, help="trees in the scikit-learn sample (0: skip)")
     ap.add_argument("--no-profile", action="store_true", help="skip per-kernel HIP-event timing")
-    ap.add_argument("--serving-seconds", type=float, default=6.0, help="length of each serving stream (0: skip)")
+    ap.add_argument("--serving-seconds", type=float, default=3.0, help="length of each serving stream (0: skip)")
     ap.add_argument("--serving-trees", type=int, default=500, help="trees of the resident serving forest")
-    ap.add_argument("--serving-qps", type=float, default=1000.0, help="predict requests per second")
+    ap.add_argument("--serving-qps", type=float, nargs="+", default=[500.0, 1000.0, 2000.0, 4000.0],
+                    help="predict request rates of the serving sweep")
+    ap.add_argument("--serving-small-qps", type=float, default=20000.0,
+                    help="request rate of the small-request stream (0: skip)")
     ap.add_argument("--workload-steps", type=int, default=3, help="timed fits per other workload (0: skip them)")
     ap.add_argument("--tune-probe-trees", type=int, default=8,
                     help="trees per autotuner probe on the 200,000 x 256 workload (0: no sweep)")

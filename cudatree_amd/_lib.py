@@ -67,7 +67,8 @@ class ServerOpts(ctypes.Structure):
         ("pad0", ctypes.c_int32),
         ("max_batch_work", ctypes.c_int64),
         ("fit_yield_us", ctypes.c_int32),
-        ("reserved", ctypes.c_int32 * 3),
+        ("split_rows", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 2),
     ]
 
 
@@ -89,6 +90,8 @@ class ServerStats(ctypes.Structure):
         ("queued_predict", ctypes.c_int32),
         ("queued_fit", ctypes.c_int32),
         ("reserved", ctypes.c_int32),
+        ("fit_mem_measured_max", ctypes.c_uint64),
+        ("fit_mem_ratio_max", ctypes.c_double),
     ]
 
 
@@ -113,6 +116,8 @@ SIGNATURES = {
                                           ctypes.c_int32, ctypes.POINTER(_vp)]),
     "cta_dataset_create_device": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
                                                  ctypes.c_int32, ctypes.POINTER(_vp)]),
+    "cta_dataset_create_keys": (ctypes.c_int, [_vp, _f32p, _vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+                                               ctypes.c_int32, _i32p, _u64p, ctypes.POINTER(_vp)]),
     "cta_dataset_destroy": (ctypes.c_int, [_vp]),
     "cta_dataset_info": (ctypes.c_int, [_vp, _i32p, _i32p, _i32p, _i32p]),
     "cta_dataset_sorted": (ctypes.c_int, [_vp, _u32p]),
@@ -131,6 +136,9 @@ SIGNATURES = {
     "cta_ctx_release_workspace": (ctypes.c_int, [_vp]),
     "cta_ctx_set_level_hook": (ctypes.c_int, [_vp, LEVEL_HOOK, _vp]),
     "cta_ctx_workspace_bytes": (ctypes.c_int, [_vp, _u64p]),
+    "cta_ctx_set_workspace_limit": (ctypes.c_int, [_vp, ctypes.c_uint64]),
+    "cta_fit_estimate": (ctypes.c_int, [ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+                                        ctypes.POINTER(FitOpts), _u64p, _u64p]),
     "cta_server_opts_default": (None, [ctypes.POINTER(ServerOpts)]),
     "cta_server_last_error": (ctypes.c_char_p, []),
     "cta_server_create": (ctypes.c_int, [ctypes.POINTER(ServerOpts), ctypes.POINTER(_vp)]),
@@ -301,6 +309,34 @@ class Dataset:
         check(self.lib.cta_dataset_info(h, None, None, None, ctypes.byref(b)))
         self.idx_bits = b.value
 
+    @classmethod
+    def from_labels(cls, ctx, X, target, idx_bits=0):
+        """Training set with the class values of target compacted on the
+        device (cta_dataset_create_keys): returns (dataset, classes), classes
+        the distinct values of target in ascending order, a sample's label
+        the index of its value.  None for a dtype class_keys cannot key."""
+        ck = class_keys(target)
+        if ck is None:
+            return None
+        keys, bits, decode = ck
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        if X.ndim != 2 or keys.ndim != 1 or len(keys) != X.shape[0]:
+            raise ValueError("X must be (N, F) and target (N,)")
+        N, F = X.shape
+        self = cls.__new__(cls)
+        self.ctx, self.lib = ctx, ctx.lib
+        h, K = _vp(), ctypes.c_int32()
+        classes = np.zeros(MAX_CLASSES, np.uint64)
+        check(self.lib.cta_dataset_create_keys(ctx.h, ptr(X, _f32p), keys.ctypes.data_as(_vp), bits, N, F, idx_bits,
+                                               ctypes.byref(K), ptr(classes, _u64p), ctypes.byref(h)))
+        self.h = h
+        _open["dataset"].add(self)
+        self.N, self.F, self.K = N, F, K.value
+        b = ctypes.c_int32()
+        check(self.lib.cta_dataset_info(h, None, None, None, ctypes.byref(b)))
+        self.idx_bits = b.value
+        return self, decode(classes[:K.value])
+
     def sorted_rows(self):
         out = np.zeros((self.F, self.N), np.uint32)
         check(self.lib.cta_dataset_sorted(self.h, ptr(out, _u32p)))
@@ -316,6 +352,41 @@ class Dataset:
             self.close()
         except Exception:
             pass
+
+
+def class_keys(target):
+    """Order-preserving unsigned keys of numeric class values for
+    cta_dataset_create_keys: (keys, key_bits, decode) with decode(keys)
+    giving the class values back in target's dtype, or None for a dtype
+    without such keys (strings, objects).  Equal values get equal keys
+    (-0.0 and +0.0 included); NaN is refused, as np.unique would give every
+    NaN a class of its own."""
+    y = np.ascontiguousarray(target)
+    dt = y.dtype
+    if dt == np.bool_ or (dt.kind == "u" and dt.itemsize <= 4):
+        return y.astype(np.uint32), 32, lambda k: k.astype(np.uint32).astype(dt)
+    if dt.kind == "i" and dt.itemsize <= 4:
+        return (y.astype(np.int32).view(np.uint32) ^ np.uint32(0x80000000)), 32, \
+            lambda k: (k.astype(np.uint32) ^ np.uint32(0x80000000)).view(np.int32).astype(dt)
+    if dt.kind == "u":
+        return y.astype(np.uint64), 64, lambda k: k.astype(np.uint64).astype(dt)
+    if dt.kind == "i":
+        return (y.astype(np.int64).view(np.uint64) ^ np.uint64(1 << 63)), 64, \
+            lambda k: (k.astype(np.uint64) ^ np.uint64(1 << 63)).view(np.int64).astype(dt)
+    if dt.kind == "f" and dt.itemsize <= 8:
+        v = y.astype(np.float64)
+        if np.isnan(v).any():
+            raise ValueError("class values contain NaN")
+        u = (v + 0.0).view(np.uint64)  # -0.0 + 0.0 is +0.0
+        sign = np.uint64(1 << 63)
+        keys = np.where(u & sign, ~u, u | sign)
+
+        def decode(k):
+            k = k.astype(np.uint64)
+            return np.where(k & sign, k ^ sign, ~k).view(np.float64).astype(dt)
+
+        return keys, 64, decode
+    return None
 
 
 FOREST_FIELDS = ("feature", "threshold", "left", "right", "value", "n_samples", "depth")
@@ -409,6 +480,15 @@ class Forest:
             pass
 
 
+def fit_estimate(N, F, K, T, opts=None):
+    """(dataset bytes, workspace bytes) the library estimates for a fit of T
+    trees on an N x F set with K classes (host computation, no device)."""
+    o = opts if opts is not None else default_opts()
+    ds, ws = ctypes.c_uint64(), ctypes.c_uint64()
+    check(load().cta_fit_estimate(N, F, K, T, ctypes.byref(o), ctypes.byref(ds), ctypes.byref(ws)))
+    return ds.value, ws.value
+
+
 def default_opts():
     o = FitOpts()
     load().cta_fit_opts_default(ctypes.byref(o))
@@ -449,7 +529,7 @@ class Server:
     requests, fit jobs admitted by HBM accounting."""
 
     def __init__(self, device=0, coalesce=True, max_batch_rows=0, hbm_budget=0, fit_workers=1, max_batch_work=0,
-                 fit_yield_us=0):
+                 fit_yield_us=0, split_rows=0):
         self.lib = load()
         o = ServerOpts()
         self.lib.cta_server_opts_default(ctypes.byref(o))
@@ -462,6 +542,7 @@ class Server:
         o.hbm_budget = hbm_budget
         o.fit_workers = fit_workers
         o.fit_yield_us = fit_yield_us
+        o.split_rows = split_rows
         h = _vp()
         check_server(self.lib.cta_server_create(ctypes.byref(o), ctypes.byref(h)))
         self.h = h

@@ -16,6 +16,7 @@
 #include <map>
 #include <memory>
 #include <mutex>
+#include <optional>
 #include <stdexcept>
 #include <string>
 #include <thread>
@@ -136,10 +137,21 @@ struct cta_ctx {
   std::map<std::string, std::unique_ptr<DevMem>> wspace;
   std::map<std::string, std::pair<void *, size_t>> pinned;
 
+  // Workspace bytes the context may hold (0: no limit); a growth beyond it
+  // fails with CTA_ENOMEM instead of allocating.
+  uint64_t ws_limit = 0;
+  void ws_admit(size_t held, size_t want) const {
+    if (ws_limit && ws_bytes() - held + want > ws_limit)
+      fail(CTA_ENOMEM, "workspace of %zu more bytes would exceed the context's limit of %llu bytes", want - held,
+           (unsigned long long)ws_limit);
+  }
   void *ws(const std::string &key, size_t bytes) {
     auto &slot = wspace[key];
     if (!slot) slot.reset(new DevMem());
-    if (slot->bytes < bytes || !slot->p) slot->alloc(bytes);
+    if (slot->bytes < bytes || !slot->p) {
+      ws_admit(slot->bytes, bytes);
+      slot->alloc(bytes);
+    }
     return slot->p;
   }
   template <class T>
@@ -345,6 +357,16 @@ void launch_resh(cta_ctx *ctx, const FitArgs<IdxT, LabT> &a, bool global_bits, u
   }
 }
 
+// Device bytes one resident tree of a fit needs: ping-pong rows of the Fr
+// features that keep rows, the side bitmap and fill partials, and per sample
+// pair the frontier, node arena and level-list entries.
+size_t tree_bytes(int64_t N, int64_t Fr, int64_t K, int64_t m, size_t idx_bytes) {
+  const size_t rows = 2 * (size_t)Fr * N * idx_bytes + N / 8 + 64;
+  const size_t per_sample = 2 * sizeof(FNode) + 3 * 4 * (size_t)K + sizeof(NAux) + 4 * (size_t)m + 2 * 7 * 4 +
+                            (sizeof(Task) + sizeof(ScanRes)) + 2 * sizeof(Task) + 2 * kChFields * 4;
+  return rows + per_sample * (size_t)N / 2 + 16 * ((size_t)N / 8) + 4096;
+}
+
 // Batched levels issued between two host reads of the control block.
 constexpr uint32_t kLevelGroup = 4;
 // Default positions per split-scan task and elements per small-node
@@ -402,13 +424,7 @@ struct Fitter {
   size_t smem() const { return (size_t)kWavesPerBlock * 2 * K * sizeof(uint32_t); }
 
   // Bytes of device memory one resident tree needs (rows, marks, level state).
-  size_t bytes_per_tree() const {
-    const size_t rows = 2 * (size_t)Fr * N * sizeof(IdxT) + N / 8 + 64;
-    const size_t per_sample = 2 * sizeof(FNode) + 3 * 4 * (size_t)K + sizeof(NAux) + 4 * (size_t)m +
-                              2 * 7 * 4 + (sizeof(Task) + sizeof(ScanRes)) + 2 * sizeof(Task) +
-                              2 * kChFields * 4;
-    return rows + per_sample * (size_t)N / 2 + 16 * ((size_t)N / 8) + 4096;
-  }
+  size_t bytes_per_tree() const { return tree_bytes(N, Fr, K, m, sizeof(IdxT)); }
 
   // Grows trees [t0, t0 + Tb) and appends them to `out` in pre-order.
   void run_batch(int32_t t0, int32_t Tb, int32_t total_trees, const uint32_t *boot_ids, const int64_t *boot_off,
@@ -837,6 +853,7 @@ struct Fitter {
       auto &slot = ctx->wspace[rl_name(w)];
       if (!slot) slot.reset(new DevMem());
       if (slot->p && slot->bytes >= cap * FK * 4) continue;
+      ctx->ws_admit(slot->bytes, cap * FK * 4);
       DevMem fresh(cap * FK * 4);
       if (w == cur && rl_cap && slot->p)
         HIPC(hipMemcpyAsync(fresh.p, slot->p, rl_cap * FK * 4, hipMemcpyDeviceToDevice, st));
@@ -1115,7 +1132,12 @@ void fit_impl(cta_ctx *ctx, cta_dataset *ds, int32_t T, const uint32_t *ids, con
   // free memory holds (workspaces are reused by this fit)
   int32_t per = o.trees_per_batch;
   if (o.mode != CTA_MODE_BATCHED) per = 1;
-  if (per <= 0) {
+  if (per <= 0 && ctx->ws_limit) {
+    // a context with a workspace limit: the trees the rest of it holds
+    const size_t held = ctx->ws_bytes();
+    const size_t avail = ctx->ws_limit > held ? (size_t)(ctx->ws_limit - held) : 0;
+    per = (int32_t)std::max<size_t>(1, avail / fz.bytes_per_tree());
+  } else if (per <= 0) {
     size_t fr = 0, tot = 0;
     HIPC(hipMemGetInfo(&fr, &tot));
     fr += ctx->ws_bytes() + (ctx->lane ? ctx->lane->ws_bytes() : 0);
@@ -1496,6 +1518,83 @@ static int dataset_create_common(cta_ctx *ctx, const float *X, const uint32_t *y
   });
 }
 
+// Distinct keys of the device keys k[0, N) in ascending order into table
+// (device, kKeyTile entries); returns their count.  Rounds of
+// k_key_distinct shrink the list until one tile holds it: a tile of at most
+// CTA_MAX_CLASSES distinct keys leaves at most half of its kKeyTile, so a
+// round that leaves more than n / 2 + kKeyTile keys has met more classes
+// than the limit.
+uint32_t key_classes(cta_ctx *ctx, const unsigned long long *k, uint32_t N, unsigned long long *table) {
+  static_assert(kKeyTile >= 2 * CTA_MAX_CLASSES, "a key tile holds twice the class limit");
+  hipStream_t st = ctx->stream;
+  uint32_t *cnt = ctx->ws_as<uint32_t>("key_count", 2);
+  uint32_t *hc = static_cast<uint32_t *>(ctx->host("key_count", 8));
+  const unsigned long long *cur = k;
+  unsigned long long *buf[2] = {nullptr, nullptr};
+  uint32_t n = N;
+  int w = 0;
+  while (n > kKeyTile) {
+    // a round writes at most one key per input key
+    if (!buf[w]) buf[w] = ctx->ws_as<unsigned long long>(w ? "key_b" : "key_a", n);
+    k_fill_u32<<<1, 64, 0, st>>>(cnt, 1, 0u);
+    k_key_distinct<<<(n + kKeyTile - 1) / kKeyTile, kKeyThreads, 0, st>>>(cur, n, buf[w], cnt);
+    HIPC(hipMemcpyAsync(hc, cnt, 4, hipMemcpyDeviceToHost, st));
+    ctx->sync();
+    const uint32_t n2 = hc[0];
+    if (n2 > n / 2 + kKeyTile) fail(CTA_EUNSUPPORTED, "more than %d distinct class values", CTA_MAX_CLASSES);
+    cur = buf[w];
+    n = n2;
+    w ^= 1;
+  }
+  k_key_table<<<1, kKeyThreads, 0, st>>>(cur, n, table, cnt + 1);
+  HIPC(hipMemcpyAsync(hc + 1, cnt + 1, 4, hipMemcpyDeviceToHost, st));
+  ctx->sync();
+  if (hc[1] > (uint32_t)CTA_MAX_CLASSES) fail(CTA_EUNSUPPORTED, "more than %d distinct class values", CTA_MAX_CLASSES);
+  return hc[1];
+}
+
+int cta_dataset_create_keys(cta_ctx *ctx, const float *X, const void *keys, int32_t key_bits, int32_t N, int32_t F,
+                            int32_t idx_bits, int32_t *K_out, uint64_t *classes, cta_dataset **out) {
+  return guarded([&] {
+    if (!ctx || !X || !keys || !K_out || !classes || !out) fail(CTA_EINVAL, "NULL argument");
+    if (key_bits != 32 && key_bits != 64) fail(CTA_EINVAL, "key_bits must be 32 or 64");
+    check_sizes(N, F, 1);
+    if (idx_bits == 0) idx_bits = N <= 65535 ? 16 : 32;
+    if (idx_bits != 16 && idx_bits != 32) fail(CTA_EINVAL, "idx_bits must be 0, 16 or 32");
+    if (idx_bits == 16 && N > 65535) fail(CTA_EINVAL, "16-bit indices need N <= 65535");
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    HIPC(hipSetDevice(ctx->device));
+    hipStream_t st = ctx->stream;
+    DevMem dX((size_t)N * F * 4), dk((size_t)N * 8), dy((size_t)N * 4);
+    HIPC(hipMemcpyAsync(dX.p, X, (size_t)N * F * 4, hipMemcpyHostToDevice, st));
+    if (key_bits == 64) {
+      HIPC(hipMemcpyAsync(dk.p, keys, (size_t)N * 8, hipMemcpyHostToDevice, st));
+    } else {
+      HIPC(hipMemcpyAsync(dy.p, keys, (size_t)N * 4, hipMemcpyHostToDevice, st));
+      k_key_widen<<<grid_for(N), 256, 0, st>>>(dy.as<uint32_t>(), (uint32_t)N, dk.as<unsigned long long>());
+    }
+    unsigned long long *table = ctx->ws_as<unsigned long long>("key_table", kKeyTile);
+    const uint32_t K = key_classes(ctx, dk.as<unsigned long long>(), (uint32_t)N, table);
+    k_key_map<<<grid_for(N, kKeyThreads, 1024), kKeyThreads, 0, st>>>(dk.as<unsigned long long>(), (uint32_t)N,
+                                                                       table, ctx->ws_as<uint32_t>("key_count", 2) + 1,
+                                                                       dy.as<uint32_t>());
+    HIPC(hipMemcpyAsync(classes, table, (size_t)K * 8, hipMemcpyDeviceToHost, st));
+    HIPC(hipGetLastError());
+    ctx->sync();
+    check_sizes(N, F, (int32_t)K);
+    std::unique_ptr<cta_dataset> ds(new cta_dataset());
+    ds->ctx = ctx;
+    ds->N = N;
+    ds->F = F;
+    ds->K = (int32_t)K;
+    ds->idx_bits = idx_bits;
+    ds->lab_bits = K <= 255 ? 8 : 16;
+    dataset_build(ctx, ds.get(), dX.as<float>(), dy.as<uint32_t>());
+    *K_out = (int32_t)K;
+    *out = ds.release();
+  });
+}
+
 int cta_dataset_create(cta_ctx *ctx, const float *X, const uint32_t *y, int32_t N, int32_t F, int32_t K,
                        int32_t idx_bits, cta_dataset **out) {
   return dataset_create_common(ctx, X, y, N, F, K, idx_bits, false, out);
@@ -1791,14 +1890,16 @@ int cta_forest_load(cta_ctx *ctx, const void *buf, uint64_t size, cta_forest **o
 // waves share each tree's upper nodes.
 constexpr size_t kPredictL2Share = (size_t)2 << 20;
 
-static void predict_dev(cta_ctx *ctx, cta_forest *f, const float *X, int64_t R, uint32_t *labels, uint32_t *votes) {
-  upload_forest(f);
+// Clears the vote matrix v (R x K, device memory), traverses every tree for
+// every row and writes the majority label per row; allocates nothing.
+static void predict_launch(cta_ctx *ctx, const cta_forest *f, const float *X, int64_t R, uint32_t *labels,
+                           uint32_t *v, bool timed) {
   hipStream_t st = ctx->stream;
   const size_t vk = (size_t)R * f->K;
-  uint32_t *v = votes ? votes : ctx->ws_as<uint32_t>("pred_votes", vk);
   k_fill_u32<<<grid_for(vk), 256, 0, st>>>(v, vk, 0u);
   {
-    Timed tm(ctx, "k_predict");
+    std::optional<Timed> tm;
+    if (timed) tm.emplace(ctx, "k_predict");
     if (f->T >= (int32_t)kXcd && f->nodes.bytes / kXcd <= kPredictL2Share) {
       // trees split over the XCDs (each L2 holds its share of the nodes)
       const uint32_t g = grid_for((size_t)R * f->T, 256, 65535u * 4);
@@ -1814,6 +1915,13 @@ static void predict_dev(cta_ctx *ctx, cta_forest *f, const float *X, int64_t R, 
   k_vote<<<grid_for(R), 256, 0, st>>>(v, (uint64_t)R, (uint32_t)f->K, labels);
   HIPC(hipGetLastError());
 }
+
+static void predict_dev(cta_ctx *ctx, cta_forest *f, const float *X, int64_t R, uint32_t *labels, uint32_t *votes) {
+  upload_forest(f);
+  uint32_t *v = votes ? votes : ctx->ws_as<uint32_t>("pred_votes", (size_t)R * f->K);
+  predict_launch(ctx, f, X, R, labels, v, true);
+}
+
 
 int cta_predict(cta_ctx *ctx, cta_forest *f, const float *X, int64_t R, int32_t F, uint32_t *labels,
                 uint32_t *votes) {
@@ -1874,6 +1982,43 @@ int cta_ctx_workspace_bytes(cta_ctx *ctx, uint64_t *bytes) {
     if (!ctx || !bytes) fail(CTA_EINVAL, "NULL argument");
     std::lock_guard<std::recursive_mutex> lk(ctx->mu);
     *bytes = ctx->ws_bytes();
+  });
+}
+
+int cta_ctx_set_workspace_limit(cta_ctx *ctx, uint64_t bytes) {
+  return guarded([&] {
+    if (!ctx) fail(CTA_EINVAL, "ctx is NULL");
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    ctx->ws_limit = bytes;
+  });
+}
+
+int cta_fit_estimate(int32_t N, int32_t F, int32_t K, int32_t T, const cta_fit_opts *opts, uint64_t *dataset_bytes,
+                     uint64_t *workspace_bytes) {
+  return guarded([&] {
+    if (!opts || !dataset_bytes || !workspace_bytes) fail(CTA_EINVAL, "NULL argument");
+    check_sizes(N, F, K);
+    if (T <= 0) fail(CTA_EINVAL, "T must be positive");
+    const uint64_t n = (uint64_t)N, f = (uint64_t)F, nf = n * f;
+    const size_t b = N <= 65535 ? 2 : 4;
+    const uint64_t lab = K <= 255 ? 1 : 2;
+    // the dataset's own buffers: X_T, labels, presorted rows, code columns
+    // (at most 16 bits per value), row-free signatures and feature tables,
+    // plus the copies of host X and y while it is built
+    *dataset_bytes = nf * 4 + n * lab + nf * b + 2 * nf + 256 * f + 4 * n * ((f + 31) / 32) + 64 * f + 4096 +
+                     nf * 4 + n * 4;
+    // the context's workspaces: radix-sort and code-column temporaries of the
+    // build, then per resident tree (every feature keeping rows), the node
+    // arena and canonical export of at most two nodes per in-bag sample, and
+    // the device predict table
+    const uint64_t ntiles = (n + kSortTile - 1) / kSortTile;
+    const uint64_t build = 6 * nf * 4 + f * kDigits * ntiles * 4 + 64 * f + ((n + kCodeTile - 1) / kCodeTile) * f * 4;
+    const int32_t m = opts->max_features > 0 ? opts->max_features : (int32_t)std::ceil(std::sqrt((double)F));
+    const int64_t Tb = opts->mode != CTA_MODE_BATCHED ? 1
+                       : opts->trees_per_batch > 0 ? std::min(opts->trees_per_batch, T)
+                                                   : T;
+    const uint64_t per_tree = tree_bytes(N, F, K, m, b) + 2 * n * (13 * 4 + sizeof(PNode));
+    *workspace_bytes = build + (uint64_t)Tb * per_tree + (uint64_t)T * 2 * n * sizeof(PNode) + (64ull << 20);
   });
 }
 

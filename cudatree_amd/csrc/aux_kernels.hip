@@ -244,6 +244,131 @@ __global__ void k_labels(const uint32_t *y, uint32_t N, uint32_t K, LabT *out, u
   }
 }
 
+// ---- label compaction ---------------------------------------------------------
+// Class values arrive as order-preserving 64-bit keys; their distinct keys in
+// ascending order are the classes, and a label is its key's index among them
+// (np.unique(..., return_inverse=True), random_forest.py:85-99).
+// k_key_distinct reduces a list of keys to the distinct keys of each tile of
+// kKeyTile (bitonic sort in LDS, then the first of every run), appended to an
+// output list in any order; repeated until one tile holds the list, whose
+// sorted distinct keys k_key_table writes.  k_key_map: each label by binary
+// search in the table (held in LDS).
+constexpr int kKeyThreads = 256;
+constexpr uint32_t kKeyTile = 2048;
+
+// Sorts the kKeyTile keys in LDS ascending (bitonic network; all threads).
+__device__ __forceinline__ void lds_bitonic(unsigned long long *k) {
+  for (uint32_t size = 2; size <= kKeyTile; size <<= 1)
+    for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
+      __syncthreads();
+      for (uint32_t t = threadIdx.x; t < kKeyTile / 2; t += kKeyThreads) {
+        const uint32_t lo = 2 * t - (t & (stride - 1)), hi = lo + stride;
+        const bool up = (lo & size) == 0;
+        const unsigned long long a = k[lo], b = k[hi];
+        if ((a > b) == up) {
+          k[lo] = b;
+          k[hi] = a;
+        }
+      }
+    }
+  __syncthreads();
+}
+
+// Tile b of keys[0, n): its keys sorted, padded with copies of its first key.
+__device__ __forceinline__ void load_key_tile(const unsigned long long *keys, uint32_t n, uint32_t b,
+                                              unsigned long long *k) {
+  const uint32_t base = b * kKeyTile;
+  const unsigned long long pad = keys[base];
+  for (uint32_t j = threadIdx.x; j < kKeyTile; j += kKeyThreads) k[j] = base + j < n ? keys[base + j] : pad;
+  lds_bitonic(k);
+}
+
+__global__ __launch_bounds__(kKeyThreads) void k_key_distinct(const unsigned long long *keys, uint32_t n,
+                                                              unsigned long long *out, uint32_t *count) {
+  __shared__ unsigned long long k[kKeyTile];
+  __shared__ uint32_t base;
+  if (blockIdx.x * kKeyTile >= n) return;
+  load_key_tile(keys, n, blockIdx.x, k);
+  // thread t owns the contiguous positions [t E, t E + E), so the distinct
+  // keys leave in sorted order
+  constexpr uint32_t E = kKeyTile / kKeyThreads;
+  const uint32_t j0 = threadIdx.x * E;
+  uint32_t nd = 0;
+  for (uint32_t j = j0; j < j0 + E; ++j) nd += (j == 0 || k[j] != k[j - 1]) ? 1u : 0u;
+  // distinct keys of the tile, appended in tile order at one reserved range
+  __shared__ uint32_t part[kKeyThreads];
+  part[threadIdx.x] = nd;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t t = 0;
+    for (int i = 0; i < kKeyThreads; ++i) {
+      const uint32_t v = part[i];
+      part[i] = t;
+      t += v;
+    }
+    base = atomicAdd(count, t);
+  }
+  __syncthreads();
+  uint32_t o = base + part[threadIdx.x];
+  for (uint32_t j = j0; j < j0 + E; ++j)
+    if (j == 0 || k[j] != k[j - 1]) out[o++] = k[j];
+}
+
+// One workgroup: the sorted distinct keys of keys[0, n) (n <= kKeyTile) into
+// table, their count into *K.
+__global__ __launch_bounds__(kKeyThreads) void k_key_table(const unsigned long long *keys, uint32_t n,
+                                                           unsigned long long *table, uint32_t *K) {
+  __shared__ unsigned long long k[kKeyTile];
+  __shared__ uint32_t part[kKeyThreads];
+  load_key_tile(keys, n, 0, k);
+  // thread t owns the contiguous positions [t E, t E + E), so the distinct
+  // keys leave in sorted order
+  constexpr uint32_t E = kKeyTile / kKeyThreads;
+  const uint32_t j0 = threadIdx.x * E;
+  uint32_t nd = 0;
+  for (uint32_t j = j0; j < j0 + E; ++j) nd += (j == 0 || k[j] != k[j - 1]) ? 1u : 0u;
+  part[threadIdx.x] = nd;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t t = 0;
+    for (int i = 0; i < kKeyThreads; ++i) {
+      const uint32_t v = part[i];
+      part[i] = t;
+      t += v;
+    }
+    *K = t;
+  }
+  __syncthreads();
+  uint32_t o = part[threadIdx.x];
+  for (uint32_t j = j0; j < j0 + E; ++j)
+    if (j == 0 || k[j] != k[j - 1]) table[o++] = k[j];
+}
+
+// label[i] = index of keys[i] in the sorted table of K (<= kKeyTile) keys.
+__global__ __launch_bounds__(kKeyThreads) void k_key_map(const unsigned long long *keys, uint32_t n,
+                                                         const unsigned long long *table, const uint32_t *Kp,
+                                                         uint32_t *label) {
+  __shared__ unsigned long long t[kKeyTile];
+  const uint32_t K = *Kp;
+  for (uint32_t j = threadIdx.x; j < K; j += kKeyThreads) t[j] = table[j];
+  __syncthreads();
+  for (uint32_t i = blockIdx.x * kKeyThreads + threadIdx.x; i < n; i += gridDim.x * kKeyThreads) {
+    const unsigned long long v = keys[i];
+    uint32_t lo = 0, hi = K;  // first index with t[idx] >= v
+    while (lo < hi) {
+      const uint32_t mid = (lo + hi) / 2;
+      if (t[mid] < v) lo = mid + 1;
+      else hi = mid;
+    }
+    label[i] = lo;
+  }
+}
+
+// 32-bit keys widened to 64 bits.
+__global__ void k_key_widen(const uint32_t *in, uint32_t n, unsigned long long *out) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) out[i] = in[i];
+}
+
 // ---- value code columns ------------------------------------------------------
 // For feature f, the presorted row gives the distinct-value index d(s) of
 // every sample (equal floats, +0 and -0 included, share an index).  When

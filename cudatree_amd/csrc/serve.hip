@@ -79,15 +79,11 @@ double now_ms() {
   return duration<double, std::milli>(steady_clock::now().time_since_epoch()).count();
 }
 
-// Device bytes a fit job may hold at its peak: the transposed and presorted
-// training set with its radix-sort temporaries, and per tree the ping-pong
-// sorted rows, the side bitmap and the level state.
-uint64_t fit_estimate(int32_t N, int32_t F, int32_t K, int32_t T) {
-  const uint64_t b = N <= 65535 ? 2 : 4;
-  const uint64_t nf = (uint64_t)N * F;
-  const uint64_t data = nf * (4 + b) + (uint64_t)N * 2 + nf * 16;
-  const uint64_t per_tree = 2 * nf * b + (uint64_t)N / 8 + (uint64_t)N * (220 + 6 * (uint64_t)K);
-  return data + (uint64_t)T * per_tree + (64ull << 20);
+// Device bytes a fit job is admitted with: the library's estimate of its
+// dataset and of the workspaces it may hold (the fit context is capped at
+// the latter, so the job cannot grow past it).
+void fit_estimate(int32_t N, int32_t F, int32_t K, int32_t T, const cta_fit_opts &o, uint64_t *ds, uint64_t *ws) {
+  cta_ok(cta_fit_estimate(N, F, K, T, &o, ds, ws), "cta_fit_estimate");
 }
 
 }  // namespace
@@ -106,7 +102,8 @@ struct FitJob {
   std::vector<uint32_t> y;
   int32_t N, F, K, T;
   cta_fit_opts opts;
-  uint64_t est;
+  uint64_t est;     // admitted device bytes (dataset + workspace cap)
+  uint64_t ws_cap;  // workspace cap of the fit context
 };
 
 struct TicketState {
@@ -123,15 +120,31 @@ struct ForestEntry {
   bool dropped = false;
 };
 
+// One predict lane: its context (stream), staging buffers and worker thread.
+// With coalescing, lane 0 serves the requests of at most split_rows rows and
+// lane 1 the larger ones, so short requests never queue behind the traversal
+// of a long one (the two lanes' launches run side by side on the device).
+struct Lane {
+  int cls = 0;  // 0: short requests (or every request with one lane), 1: long
+  cta_ctx *ctx = nullptr;
+  hipStream_t stream = nullptr;
+  float *hX = nullptr, *dX = nullptr;
+  uint32_t *hL = nullptr, *dL = nullptr;
+  size_t cap_rows = 0, cap_F = 0;
+  bool busy = false;
+  std::thread th;
+};
+
 struct cta_server {
   cta_server_opts o{};
   uint64_t budget = 0, in_use = 0, peak = 0;
-  cta_ctx *pctx = nullptr;
+  std::vector<std::unique_ptr<Lane>> lanes;
+  int64_t split_rows = 0;  // largest short request (0: one lane for every request)
   std::vector<cta_ctx *> fctx;
   std::mutex mu;
   std::condition_variable cv_pred, cv_fit, cv_done;
   std::condition_variable cv_idle;  // predict queue empty and no batch on the device
-  bool pred_busy = false;
+  int busy_lanes = 0;
   std::deque<PredReq> pq;
   std::deque<FitJob> fq;
   std::map<int32_t, ForestEntry> forests;
@@ -139,17 +152,17 @@ struct cta_server {
   uint64_t next_ticket = 1;
   int32_t next_forest = 0;
   bool stop = false;
-  std::thread pthread;
   std::vector<std::thread> fthreads;
-  // predict staging (owned by the predict worker)
-  float *hX = nullptr, *dX = nullptr;
-  uint32_t *hL = nullptr, *dL = nullptr;
-  size_t cap_rows = 0, cap_F = 0;
-  hipStream_t stream = nullptr;
   // statistics
   std::vector<double> lat_ms;
   uint64_t pred_requests = 0, pred_rows = 0, pred_launches = 0, fit_done = 0, fit_rejected = 0;
   double t_first = 0, t_last = 0;
+  // device free-memory drop over each fit job (dataset build to the end of
+  // its fit), against the bytes it was admitted with
+  uint64_t fit_mem_max = 0;
+  double fit_mem_ratio = 0;
+
+  cta_ctx *home_ctx() const { return lanes[0]->ctx; }
 
   void register_forest_locked(cta_forest *f, int32_t *id) {
     int32_t T = 0, F = 0, K = 0;
@@ -175,37 +188,50 @@ struct cta_server {
     forests.erase(it);
   }
 
-  void ensure_staging(int64_t rows, int32_t F) {
+  void ensure_staging(Lane &L, int64_t rows, int32_t F) {
     const size_t need = (size_t)rows * F;
-    if (cap_rows * cap_F >= need && cap_rows >= (size_t)rows) return;
-    const size_t r = std::max<size_t>((size_t)rows, cap_rows * 2 > (size_t)rows ? cap_rows * 2 : (size_t)rows);
-    const size_t f = std::max<size_t>((size_t)F, cap_F);
-    if (hX) (void)hipHostFree(hX);
-    if (hL) (void)hipHostFree(hL);
-    if (dX) (void)hipFree(dX);
-    if (dL) (void)hipFree(dL);
-    hX = nullptr;
-    hL = nullptr;
-    dX = nullptr;
-    dL = nullptr;
+    if (L.cap_rows * L.cap_F >= need && L.cap_rows >= (size_t)rows) return;
+    const size_t r =
+        std::max<size_t>((size_t)rows, L.cap_rows * 2 > (size_t)rows ? L.cap_rows * 2 : (size_t)rows);
+    const size_t f = std::max<size_t>((size_t)F, L.cap_F);
+    if (L.hX) (void)hipHostFree(L.hX);
+    if (L.hL) (void)hipHostFree(L.hL);
+    if (L.dX) (void)hipFree(L.dX);
+    if (L.dL) (void)hipFree(L.dL);
+    L.hX = nullptr;
+    L.hL = nullptr;
+    L.dX = nullptr;
+    L.dL = nullptr;
     {
       std::lock_guard<std::mutex> lk(mu);
-      in_use -= (uint64_t)cap_rows * cap_F * 4 + (uint64_t)cap_rows * 4;
+      in_use -= (uint64_t)L.cap_rows * L.cap_F * 4 + (uint64_t)L.cap_rows * 4;
+      L.cap_rows = 0;
+      L.cap_F = 0;
       in_use += (uint64_t)r * f * 4 + (uint64_t)r * 4;
       peak = std::max(peak, in_use);
     }
-    hip_ok(hipHostMalloc((void **)&hX, r * f * 4, hipHostMallocDefault), "hipHostMalloc");
-    hip_ok(hipHostMalloc((void **)&hL, r * 4, hipHostMallocDefault), "hipHostMalloc");
-    hip_ok(hipMalloc((void **)&dX, r * f * 4), "hipMalloc");
-    hip_ok(hipMalloc((void **)&dL, r * 4), "hipMalloc");
-    cap_rows = r;
-    cap_F = f;
+    hip_ok(hipHostMalloc((void **)&L.hX, r * f * 4, hipHostMallocDefault), "hipHostMalloc");
+    hip_ok(hipHostMalloc((void **)&L.hL, r * 4, hipHostMallocDefault), "hipHostMalloc");
+    hip_ok(hipMalloc((void **)&L.dX, r * f * 4), "hipMalloc");
+    hip_ok(hipMalloc((void **)&L.dL, r * 4), "hipMalloc");
+    L.cap_rows = r;
+    L.cap_F = f;
   }
 
-  // One coalesced (or single) launch over the queued requests of a forest.
-  void predict_loop() {
+  // Lane of a request of R rows.
+  int lane_of(int64_t R) const { return (lanes.size() > 1 && R > split_rows) ? 1 : 0; }
+
+  // First queued request of lane class `cls`, or pq.end().
+  std::deque<PredReq>::iterator first_of(int cls) {
+    for (auto it = pq.begin(); it != pq.end(); ++it)
+      if (lane_of(it->R) == cls) return it;
+    return pq.end();
+  }
+
+  // One coalesced (or single) launch over the queued requests of a forest,
+  // in arrival order within the lane's class.
+  void predict_loop(Lane &L) {
     (void)hipSetDevice(o.device);
-    (void)hipStreamCreateWithFlags(&stream, hipStreamNonBlocking);
     const int64_t max_rows = o.max_batch_rows > 0 ? o.max_batch_rows : (1 << 20);
     const int64_t max_work = o.max_batch_work > 0 ? o.max_batch_work : (int64_t)1 << 22;
     for (;;) {
@@ -214,16 +240,17 @@ struct cta_server {
       int32_t fid = -1;
       {
         std::unique_lock<std::mutex> lk(mu);
-        cv_pred.wait(lk, [&] { return stop || !pq.empty(); });
-        if (pq.empty()) return;
-        fid = pq.front().forest;
+        cv_pred.wait(lk, [&] { return stop || first_of(L.cls) != pq.end(); });
+        auto head = first_of(L.cls);
+        if (head == pq.end()) return;  // stopping, nothing left for this lane
+        fid = head->forest;
         auto fit = forests.find(fid);
         const int64_t T = fit != forests.end() ? std::max<int64_t>(1, fit->second.T) : 1;
         int64_t rows = 0;
-        for (auto it = pq.begin(); it != pq.end();) {
-          // FIFO within the forest; a batch stays under max_rows rows and
-          // max_work row-tree visits (its first request is always taken)
-          if (it->forest == fid &&
+        for (auto it = head; it != pq.end();) {
+          // a batch stays under max_rows rows and max_work row-tree visits
+          // (its first request is always taken)
+          if (it->forest == fid && lane_of(it->R) == L.cls &&
               (batch.empty() || (rows + it->R <= max_rows && (rows + it->R) * T <= max_work))) {
             rows += it->R;
             batch.push_back(*it);
@@ -240,27 +267,28 @@ struct cta_server {
         }
         f->second.refs++;
         fe = f->second;
-        pred_busy = true;
+        L.busy = true;
+        ++busy_lanes;
       }
       int rc = CTA_OK;
       int64_t rows = 0;
       for (auto &r : batch) rows += r.R;
       try {
-        ensure_staging(rows, fe.F);
+        ensure_staging(L, rows, fe.F);
         int64_t off = 0;
         for (auto &r : batch) {
-          std::memcpy(hX + (size_t)off * fe.F, r.X, (size_t)r.R * fe.F * 4);
+          std::memcpy(L.hX + (size_t)off * fe.F, r.X, (size_t)r.R * fe.F * 4);
           off += r.R;
         }
-        hip_ok(hipMemcpyAsync(dX, hX, (size_t)rows * fe.F * 4, hipMemcpyHostToDevice, stream), "copy in");
-        hip_ok(hipStreamSynchronize(stream), "sync");
-        rc = cta_predict_device(pctx, fe.forest, dX, rows, fe.F, dL, nullptr);
+        hip_ok(hipMemcpyAsync(L.dX, L.hX, (size_t)rows * fe.F * 4, hipMemcpyHostToDevice, L.stream), "copy in");
+        hip_ok(hipStreamSynchronize(L.stream), "sync");
+        rc = cta_predict_device(L.ctx, fe.forest, L.dX, rows, fe.F, L.dL, nullptr);
         if (rc == CTA_OK) {
-          hip_ok(hipMemcpyAsync(hL, dL, (size_t)rows * 4, hipMemcpyDeviceToHost, stream), "copy out");
-          hip_ok(hipStreamSynchronize(stream), "sync");
+          hip_ok(hipMemcpyAsync(L.hL, L.dL, (size_t)rows * 4, hipMemcpyDeviceToHost, L.stream), "copy out");
+          hip_ok(hipStreamSynchronize(L.stream), "sync");
           off = 0;
           for (auto &r : batch) {
-            std::memcpy(r.out, hL + off, (size_t)r.R * 4);
+            std::memcpy(r.out, L.hL + off, (size_t)r.R * 4);
             off += r.R;
           }
         }
@@ -276,8 +304,9 @@ struct cta_server {
       }
       forests[fid].refs--;
       release_forest_locked(fid);
-      pred_busy = false;
-      if (pq.empty()) cv_idle.notify_all();
+      L.busy = false;
+      --busy_lanes;
+      if (pq.empty() && busy_lanes == 0) cv_idle.notify_all();
     }
   }
 
@@ -289,7 +318,7 @@ struct cta_server {
     if (us < 0) return;
     std::unique_lock<std::mutex> lk(s->mu);
     s->cv_idle.wait_for(lk, std::chrono::microseconds(us ? us : 4000),
-                        [&] { return s->stop || (s->pq.empty() && !s->pred_busy); });
+                        [&] { return s->stop || (s->pq.empty() && s->busy_lanes == 0); });
   }
 
   void fit_loop(size_t w) {
@@ -322,11 +351,19 @@ struct cta_server {
       int rc = CTA_OK;
       cta_forest *forest = nullptr;
       cta_dataset *ds = nullptr;
-      rc = cta_dataset_create(fctx[w], job.X.data(), job.y.data(), job.N, job.F, job.K, 0, &ds);
+      uint64_t free0 = 0, free1 = 0, tot = 0;
+      (void)cta_ctx_mem_info(fctx[w], &free0, &tot);
+      rc = cta_ctx_set_workspace_limit(fctx[w], job.ws_cap);
+      if (rc == CTA_OK) rc = cta_dataset_create(fctx[w], job.X.data(), job.y.data(), job.N, job.F, job.K, 0, &ds);
       if (rc == CTA_OK) rc = cta_fit(fctx[w], ds, job.T, nullptr, nullptr, &job.opts, &forest);
+      (void)cta_ctx_mem_info(fctx[w], &free1, &tot);
       if (ds) cta_dataset_destroy(ds);
       cta_ctx_release_workspace(fctx[w]);
       std::lock_guard<std::mutex> lk(mu);
+      if (rc == CTA_OK && free0 > free1) {
+        fit_mem_max = std::max(fit_mem_max, free0 - free1);
+        fit_mem_ratio = std::max(fit_mem_ratio, (double)(free0 - free1) / (double)job.est);
+      }
       in_use -= job.est;
       int32_t id = -1;
       if (rc == CTA_OK) {
@@ -385,17 +422,30 @@ int cta_server_create(const cta_server_opts *opts, cta_server **out) {
     std::unique_ptr<cta_server> s(new cta_server());
     s->o = *opts;
     if (s->o.fit_workers <= 0) s->o.fit_workers = 1;
-    cta_ok(cta_ctx_create(s->o.device, &s->pctx), "cta_ctx_create");
+    // a second (long-request) lane only when coalescing; split_rows < 0: one lane
+    s->split_rows = (s->o.coalesce && s->o.split_rows >= 0) ? (s->o.split_rows ? s->o.split_rows : 4096) : 0;
+    const int nl = s->split_rows > 0 ? 2 : 1;
+    for (int l = 0; l < nl; ++l) {
+      std::unique_ptr<Lane> L(new Lane());
+      L->cls = l;
+      cta_ok(cta_ctx_create(s->o.device, &L->ctx), "cta_ctx_create");
+      hip_ok(hipSetDevice(s->o.device), "hipSetDevice");
+      hip_ok(hipStreamCreateWithFlags(&L->stream, hipStreamNonBlocking), "hipStreamCreateWithFlags");
+      s->lanes.push_back(std::move(L));
+    }
     for (int i = 0; i < s->o.fit_workers; ++i) {
       cta_ctx *c = nullptr;
       cta_ok(cta_ctx_create(s->o.device, &c), "cta_ctx_create");
       s->fctx.push_back(c);
     }
     uint64_t fr = 0, tot = 0;
-    cta_ok(cta_ctx_mem_info(s->pctx, &fr, &tot), "cta_ctx_mem_info");
+    cta_ok(cta_ctx_mem_info(s->home_ctx(), &fr, &tot), "cta_ctx_mem_info");
     s->budget = s->o.hbm_budget ? s->o.hbm_budget : (uint64_t)(0.9 * (double)fr);
     cta_server *raw = s.get();
-    raw->pthread = std::thread([raw] { raw->predict_loop(); });
+    for (auto &L : raw->lanes) {
+      Lane *lp = L.get();
+      lp->th = std::thread([raw, lp] { raw->predict_loop(*lp); });
+    }
     for (size_t w = 0; w < raw->fctx.size(); ++w) raw->fthreads.emplace_back([raw, w] { raw->fit_loop(w); });
     *out = s.release();
   });
@@ -411,17 +461,20 @@ int cta_server_destroy(cta_server *s) {
     s->cv_pred.notify_all();
     s->cv_fit.notify_all();
     s->cv_idle.notify_all();
-    if (s->pthread.joinable()) s->pthread.join();
+    for (auto &L : s->lanes)
+      if (L->th.joinable()) L->th.join();
     for (auto &t : s->fthreads)
       if (t.joinable()) t.join();
     for (auto &kv : s->forests) cta_forest_destroy(kv.second.forest);
     (void)hipSetDevice(s->o.device);
-    if (s->hX) (void)hipHostFree(s->hX);
-    if (s->hL) (void)hipHostFree(s->hL);
-    if (s->dX) (void)hipFree(s->dX);
-    if (s->dL) (void)hipFree(s->dL);
-    if (s->stream) (void)hipStreamDestroy(s->stream);
-    cta_ctx_destroy(s->pctx);
+    for (auto &L : s->lanes) {
+      if (L->hX) (void)hipHostFree(L->hX);
+      if (L->hL) (void)hipHostFree(L->hL);
+      if (L->dX) (void)hipFree(L->dX);
+      if (L->dL) (void)hipFree(L->dL);
+      if (L->stream) (void)hipStreamDestroy(L->stream);
+      cta_ctx_destroy(L->ctx);
+    }
     for (auto c : s->fctx) cta_ctx_destroy(c);
     delete s;
   });
@@ -440,7 +493,7 @@ int cta_server_add_forest(cta_server *s, const cta_forest *f, int32_t *forest_id
     cta_ok(cta_forest_export(f, off.data(), feat.data(), thr.data(), l.data(), r.data(), v.data(), nullptr, nullptr),
            "cta_forest_export");
     cta_forest *copy = nullptr;
-    cta_ok(cta_forest_import(s->pctx, T, F, K, off.data(), feat.data(), thr.data(), l.data(), r.data(), v.data(),
+    cta_ok(cta_forest_import(s->home_ctx(), T, F, K, off.data(), feat.data(), thr.data(), l.data(), r.data(), v.data(),
                              &copy),
            "cta_forest_import");
     std::lock_guard<std::mutex> lk(s->mu);
@@ -483,7 +536,7 @@ int cta_server_submit_predict(cta_server *s, int32_t forest_id, const float *X, 
     const uint64_t t = s->new_ticket_locked();
     s->pq.push_back(PredReq{t, forest_id, X, R, labels});
     *ticket = t;
-    s->cv_pred.notify_one();
+    s->cv_pred.notify_all();
   });
 }
 
@@ -500,7 +553,9 @@ int cta_server_submit_fit(cta_server *s, const float *X, const uint32_t *y, int3
     j.K = K;
     j.T = T;
     j.opts = *opts;
-    j.est = fit_estimate(N, F, K, T);
+    uint64_t dsb = 0;
+    fit_estimate(N, F, K, T, j.opts, &dsb, &j.ws_cap);
+    j.est = dsb + j.ws_cap;
     std::lock_guard<std::mutex> lk(s->mu);
     j.ticket = s->new_ticket_locked();
     *ticket = j.ticket;
@@ -551,6 +606,8 @@ int cta_server_get_stats(cta_server *s, cta_server_stats *out) {
       for (double x : v) sum += x;
       out->mean_ms = sum / v.size();
     }
+    out->fit_mem_measured_max = s->fit_mem_max;
+    out->fit_mem_ratio_max = s->fit_mem_ratio;
     const double span = s->t_last - s->t_first;
     out->rows_per_s = span > 0 ? (double)s->pred_rows / (span * 1e-3) : 0.0;
   });
@@ -562,6 +619,8 @@ int cta_server_reset_stats(cta_server *s) {
     std::lock_guard<std::mutex> lk(s->mu);
     s->lat_ms.clear();
     s->pred_requests = s->pred_rows = s->pred_launches = s->fit_done = s->fit_rejected = 0;
+    s->fit_mem_max = 0;
+    s->fit_mem_ratio = 0;
     s->t_first = s->t_last = 0;
     s->peak = s->in_use;
   });

@@ -90,24 +90,34 @@ class RandomForestClassifier(object):
         target = np.asarray(target)
         if samples.ndim != 2 or target.ndim != 1 or samples.shape[0] != target.shape[0]:
             raise ValueError("samples must be (n_samples, n_features) and target (n_samples,)")
-        # label compaction (random_forest.py:85-99)
-        self.compt_table, y = np.unique(target, return_inverse=True)
-        self.n_labels = len(self.compt_table)
         N, F = samples.shape
-        self.n_features = F
-        self.dtype_labels = _best_dtype(self.n_labels)
-        self.dtype_indices = max(_best_dtype(N), np.dtype(np.uint16))
         m = self.max_features
         if m is None:
             m = int(math.ceil(np.sqrt(F)))
         if not 1 <= m <= F:
             raise ValueError("max_features must be in [1, %d]" % F)
+        ctx = _lib.default_context(self.device)
+        # label compaction (random_forest.py:85-99): on the device for
+        # numeric class values (the dataset is built in the same call), with
+        # np.unique on the host for other dtypes and for the autotuner's probe
+        ds = None
+        if not self.autotune:
+            made = _lib.Dataset.from_labels(ctx, samples, target)
+            if made is not None:
+                ds, self.compt_table = made
+                y = None
+        if ds is None:
+            self.compt_table, y = np.unique(target, return_inverse=True)
+        self.n_labels = len(self.compt_table)
+        self.n_features = F
+        self.dtype_labels = _best_dtype(self.n_labels)
+        self.dtype_indices = max(_best_dtype(N), np.dtype(np.uint16))
         self.max_features_ = m
         self.bfs_threshold = _best_bfs_threshold(self.n_labels, N, m) if bfs_threshold is None else bfs_threshold
         seed = self._seed()
         self.seed_ = seed
-        ctx = _lib.default_context(self.device)
-        ds = _lib.Dataset(ctx, samples.astype(np.float32), y.astype(np.uint32), self.n_labels)
+        if ds is None:
+            ds = _lib.Dataset(ctx, samples.astype(np.float32), y.astype(np.uint32), self.n_labels)
         opts = _lib.default_opts()
         # in-bag sets drawn on the device from (seed, tree): the sorted unique
         # ids of N uniform draws, as np.unique(randint(0, N, N)) in the

@@ -80,6 +80,19 @@ int cta_dataset_create(cta_ctx *ctx, const float *X, const uint32_t *y, int32_t 
 /* As cta_dataset_create with X (N x F, float32) and y (N, uint32) in device memory. */
 int cta_dataset_create_device(cta_ctx *ctx, const float *X_dev, const uint32_t *y_dev, int32_t N, int32_t F,
                               int32_t K, int32_t idx_bits, cta_dataset **out);
+/* As cta_dataset_create with the class values given raw and compacted on the
+ * device: keys holds N order-preserving keys of the class values (uint32
+ * for key_bits 32, uint64 for 64; key order is class order, equal classes
+ * have equal keys).  The distinct keys in ascending order are the classes:
+ * they are written to classes (room for CTA_MAX_CLASSES uint64), their count
+ * to *K, and sample i gets the index of its key as its label.  More than
+ * CTA_MAX_CLASSES distinct keys fail with CTA_EUNSUPPORTED.  The label width
+ * follows K (8 bits up to 255 classes, else 16) and the index width N, as in
+ * cta_dataset_create.
+ * Replaces: __compact_labels (random_forest.py:85-99: np.unique with
+ * return_inverse on the host) and fit_init's label dtype choice (:229-231). */
+int cta_dataset_create_keys(cta_ctx *ctx, const float *X, const void *keys, int32_t key_bits, int32_t N, int32_t F,
+                            int32_t idx_bits, int32_t *K, uint64_t *classes, cta_dataset **out);
 int cta_dataset_destroy(cta_dataset *ds);
 int cta_dataset_info(const cta_dataset *ds, int32_t *N, int32_t *F, int32_t *K, int32_t *idx_bits);
 /* Copies the presorted index rows (F x N, widened to uint32) to host memory.
@@ -191,6 +204,22 @@ int cta_ctx_release_workspace(cta_ctx *ctx);
  * to give queued predict batches the device between a fit's levels. */
 int cta_ctx_set_level_hook(cta_ctx *ctx, void (*fn)(void *), void *arg);
 int cta_ctx_workspace_bytes(cta_ctx *ctx, uint64_t *bytes);
+/* Caps the context's workspaces (0: no cap).  A fit on the context then
+ * sizes its batches of resident trees from what the cap leaves, and any
+ * workspace growth beyond it fails with CTA_ENOMEM instead of allocating.
+ * No counterpart in the reference; the serving loop holds each fit job to
+ * the bytes it was admitted with. */
+int cta_ctx_set_workspace_limit(cta_ctx *ctx, uint64_t bytes);
+/* Device bytes of a dataset of N x F with K classes (its own buffers and the
+ * transient copies of host X and y) and a bound on the context workspaces a
+ * fit of T trees with these options holds (the build's sort temporaries,
+ * every resident tree's rows and level state, the node arena, the canonical
+ * export and the predict table).  Computed on the host; no device is needed.
+ * No counterpart in the reference (which allocates per tree without
+ * accounting, random_forest.py:146-174, random_tree.py:255-282); the
+ * serving loop admits fit jobs with it. */
+int cta_fit_estimate(int32_t N, int32_t F, int32_t K, int32_t T, const cta_fit_opts *opts, uint64_t *dataset_bytes,
+                     uint64_t *workspace_bytes);
 
 /* ---- serving loop -------------------------------------------------------
  * No reference counterpart: the reference re-uploads rows and trees on every
@@ -213,7 +242,11 @@ typedef struct {
                               traversal); a single request may exceed it */
   int32_t fit_yield_us;    /* a fit job waits up to this long after each group of levels while
                               predict work is queued or running (0: 4,000; negative: never) */
-  int32_t reserved[3];
+  int32_t split_rows;      /* with coalescing, requests of at most this many rows go to a short-request
+                              lane and larger ones to a long-request lane of their own (two predict
+                              workers, contexts and streams), so short requests never wait behind a long
+                              traversal (0: 4,096; negative: one lane) */
+  int32_t reserved[2];
 } cta_server_opts;
 
 typedef struct {
@@ -223,6 +256,9 @@ typedef struct {
   double rows_per_s;               /* predict rows over first submission .. last completion */
   uint64_t hbm_budget, hbm_in_use, hbm_peak;
   int32_t resident_forests, queued_predict, queued_fit, reserved;
+  uint64_t fit_mem_measured_max;   /* largest drop of the device's free memory over a fit job (dataset build
+                                      to the end of its fit) */
+  double fit_mem_ratio_max;        /* largest such drop over the bytes the job was admitted with */
 } cta_server_stats;
 
 void cta_server_opts_default(cta_server_opts *opts);

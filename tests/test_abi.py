@@ -75,3 +75,25 @@ def test_struct_layouts_match_header(tmp_path):
         assert int(got[cname]) == ctypes.sizeof(py), cname
         for field, _ in py._fields_:
             assert int(got["%s.%s" % (cname, field)]) == getattr(py, field).offset, (cname, field)
+
+
+def test_fit_estimate_host_only():
+    """cta_fit_estimate is a host computation: it grows with rows, trees and
+    features, a batch cap bounds the resident trees, bad sizes are refused."""
+    from cudatree_amd import _lib
+
+    o = _lib.default_opts()
+    o.max_features = 7
+    ds1, ws1 = _lib.fit_estimate(20000, 54, 7, 10, o)
+    ds2, ws2 = _lib.fit_estimate(40000, 54, 7, 10, o)
+    _, ws3 = _lib.fit_estimate(20000, 54, 7, 20, o)
+    assert 0 < ds1 < ds2 and 0 < ws1 < ws2 and ws1 < ws3
+    # the dataset alone holds X_T, the presorted rows and the copies of X and y
+    assert ds1 >= 20000 * 54 * (4 + 4 + 4)
+    o.trees_per_batch = 2
+    _, ws4 = _lib.fit_estimate(20000, 54, 7, 20, o)
+    assert ws4 < ws3
+    with pytest.raises(_lib.CtaError):
+        _lib.fit_estimate(0, 54, 7, 10, o)
+    with pytest.raises(_lib.CtaError):
+        _lib.fit_estimate(100, 54, 7, 0, o)

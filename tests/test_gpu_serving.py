@@ -6,6 +6,7 @@ import numpy as np
 import pytest
 
 import helpers
+import oracle
 from cudatree_amd import _lib, datasets
 
 pytestmark = pytest.mark.gpu
@@ -34,28 +35,35 @@ def base_forest(data):
     return _lib.Forest.fit(ctx, ds, 20, fit_opts(3), None)
 
 
-def _requests(Xt, n, seed):
+def _requests(Xt, n, seed, largest=3000):
     rng = np.random.default_rng(seed)
     out = []
     for _ in range(n):
-        r = int(np.exp(rng.uniform(0, np.log(3000))))
+        r = int(np.exp(rng.uniform(0, np.log(largest))))
         a = int(rng.integers(0, len(Xt) - r))
         out.append(Xt[a:a + r])
     return out
 
 
-@pytest.mark.parametrize("coalesce", [True, False])
-def test_predict_alone_vs_in_mix(data, base_forest, coalesce):
+# (coalesce, split_rows): one lane per request; coalesced with the short and
+# long lanes split at 512 rows; coalesced on one lane
+@pytest.mark.parametrize("coalesce,split", [(True, 512), (False, 0), (True, -1)])
+def test_predict_alone_vs_in_mix(data, base_forest, coalesce, split):
     X, y, Xt = data
-    reqs = _requests(Xt, 60, 5)
+    reqs = _requests(Xt, 60, 5, largest=12000)
     alone = [base_forest.predict(r) for r in reqs]
+    # the served labels are the oracle's traversal and vote of the same trees
+    trees = base_forest.export()
+    orc = oracle.get()
+    for r, a in zip(reqs, alone):
+        assert np.array_equal(a, orc.predict(trees, r, 7))
     fit_jobs = [(X[:5000 * (k + 1)], y[:5000 * (k + 1)], 10 + 5 * k, 100 + k) for k in range(3)]
     fit_alone = []
     ctx = _lib.default_context(0)
     for Xf, yf, T, seed in fit_jobs:
         ds = _lib.Dataset(ctx, Xf, yf, 7)
         fit_alone.append(_lib.Forest.fit(ctx, ds, T, fit_opts(seed), None).export())
-    srv = _lib.Server(coalesce=coalesce, fit_workers=1)
+    srv = _lib.Server(coalesce=coalesce, fit_workers=1, split_rows=split)
     try:
         fid = srv.add_forest(base_forest)
         tickets = [None] * len(reqs)
@@ -65,7 +73,7 @@ def test_predict_alone_vs_in_mix(data, base_forest, coalesce):
             for i in range(part, len(reqs), 4):
                 tickets[i] = srv.submit_predict(fid, reqs[i])
 
-        print("serving mix coalesce=%d" % coalesce, flush=True)
+        print("serving mix coalesce=%d split_rows=%d" % (coalesce, split), flush=True)
         for Xf, yf, T, seed in fit_jobs:
             fit_tickets.append(srv.submit_fit(Xf, yf, 7, T, fit_opts(seed)))
         threads = [threading.Thread(target=client, args=(p,)) for p in range(4)]
@@ -104,6 +112,8 @@ def test_fit_admission_by_hbm_budget(data):
         assert st == 0 and fid >= 0
         s = srv.stats()
         assert s["fit_rejected"] == 1 and s["fit_jobs"] == 1 and s["hbm_peak"] <= 256 << 20
+        # the device's free memory fell by no more than the job was admitted with
+        assert 0 < s["fit_mem_measured_max"] and s["fit_mem_ratio_max"] <= 1.0, s
     finally:
         srv.close()
 
@@ -112,8 +122,8 @@ def test_jobs_queue_until_memory_frees(data):
     X, y, _ = data
     import ctypes
 
-    # budget for about one of these jobs at a time
-    one = 20000 * 54 * (4 + 2 + 16) + 8 * (2 * 20000 * 54 * 2 + 20000 // 8 + 20000 * (220 + 42)) + (64 << 20)
+    # budget for about one of these jobs at a time (the library's estimate)
+    one = sum(_lib.fit_estimate(len(y), X.shape[1], 7, 8, fit_opts(1)))
     srv = _lib.Server(hbm_budget=int(one * 1.5), fit_workers=2)
     try:
         ts = [srv.submit_fit(X, y, 7, 8, fit_opts(s)) for s in (1, 2, 3)]
@@ -122,6 +132,7 @@ def test_jobs_queue_until_memory_frees(data):
             assert st == 0
         s = srv.stats()
         assert s["fit_jobs"] == 3 and s["hbm_peak"] <= int(one * 1.5)
+        assert s["fit_mem_ratio_max"] <= 1.0, s
     finally:
         srv.close()
 
