@@ -133,6 +133,7 @@ SIGNATURES = {
     "cta_forest_load": (ctypes.c_int, [_vp, _vp, ctypes.c_uint64, ctypes.POINTER(_vp)]),
     "cta_predict": (ctypes.c_int, [_vp, _vp, _f32p, ctypes.c_int64, ctypes.c_int32, _u32p, _u32p]),
     "cta_predict_device": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int64, ctypes.c_int32, _vp, _vp]),
+    "cta_predict_device_noalloc": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int64, ctypes.c_int32, _vp]),
     "cta_ctx_release_workspace": (ctypes.c_int, [_vp]),
     "cta_ctx_set_level_hook": (ctypes.c_int, [_vp, LEVEL_HOOK, _vp]),
     "cta_ctx_workspace_bytes": (ctypes.c_int, [_vp, _u64p]),

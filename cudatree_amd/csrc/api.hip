@@ -274,6 +274,37 @@ struct cta_forest {
   bool device_ready = false;
 };
 
+// ---- allocation-free predict ---------------------------------------------------
+// The vote matrix of cta_predict_device_noalloc lives in this file-scope
+// device array, one copy per GPU, held until the process ends.  Host code
+// never passes the array's name to a kernel (on the host it only names the
+// symbol): each device's address comes from hipGetSymbolAddress, taken once
+// per device when a context is created on it, and only that address is used.
+static __device__ __align__(256) uint32_t scratch[CTA_NOALLOC_VOTE_WORDS];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+uint32_t *g_scratch[kMaxDevices] = {};
+
+// Records this device's copy of `scratch` (device `device` is current).
+void scratch_setup(int device) {
+  if (device < 0 || device >= kMaxDevices) return;  // the no-alloc entry point rejects such a device
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch[device]) return;
+  void *p = nullptr;
+  HIPC(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+  if (!p) fail(CTA_EHIP, "hipGetSymbolAddress(scratch) returned no address on device %d", device);
+  g_scratch[device] = static_cast<uint32_t *>(p);
+}
+
+uint32_t *scratch_of(int device) {
+  if (device < 0 || device >= kMaxDevices) return nullptr;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  return g_scratch[device];
+}
+}  // namespace
+
 namespace {
 
 cta_ctx *ctx_new(int device) {
@@ -285,6 +316,7 @@ cta_ctx *ctx_new(int device) {
   HIPC(hipGetDeviceProperties(&p, device));
   c->num_cu = p.multiProcessorCount > 0 ? p.multiProcessorCount : 256;
   c->lds_per_block = p.sharedMemPerBlock > 0 ? p.sharedMemPerBlock : 65536;
+  scratch_setup(device);
   return c.release();
 }
 
@@ -1954,6 +1986,30 @@ int cta_predict_device(cta_ctx *ctx, cta_forest *f, const float *X_dev, int64_t 
     HIPC(hipSetDevice(ctx->device));
     predict_dev(ctx, f, X_dev, R, labels_dev, votes_dev);
     ctx->sync();
+  });
+}
+
+int cta_predict_device_noalloc(cta_ctx *ctx, cta_forest *f, const float *X_dev, int64_t R, int32_t F,
+                               uint32_t *labels_dev) {
+  return guarded([&] {
+    if (!ctx || !f || !X_dev || !labels_dev) fail(CTA_EINVAL, "NULL argument");
+    if (F != f->F) fail(CTA_EINVAL, "X has %d features, the forest %d", F, f->F);
+    if (R < 0) fail(CTA_EINVAL, "R must be >= 0");
+    if (R == 0) return;
+    if ((uint64_t)R * (uint64_t)f->K > (uint64_t)CTA_NOALLOC_VOTE_WORDS)
+      fail(CTA_EUNSUPPORTED, "R x K = %lld x %d exceeds the %d vote words of the allocation-free predict", (long long)R,
+           f->K, (int)CTA_NOALLOC_VOTE_WORDS);
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    if (f->ctx && f->ctx->device != ctx->device)
+      fail(CTA_EINVAL, "the forest lives on device %d, the context on device %d", f->ctx->device, ctx->device);
+    if (!f->device_ready)
+      fail(CTA_EINVAL, "the forest's node table is not on the device (a fit, cta_predict or cta_predict_device "
+                       "puts it there)");
+    uint32_t *v = scratch_of(ctx->device);
+    if (!v) fail(CTA_EINVAL, "device %d has no allocation-free vote buffer", ctx->device);
+    HIPC(hipSetDevice(ctx->device));
+    predict_launch(ctx, f, X_dev, R, labels_dev, v, false);
+    HIPC(hipStreamSynchronize(ctx->stream));
   });
 }
 

@@ -195,6 +195,21 @@ int cta_predict(cta_ctx *ctx, cta_forest *forest, const float *X, int64_t R, int
 int cta_predict_device(cta_ctx *ctx, cta_forest *forest, const float *X_dev, int64_t R, int32_t F,
                        uint32_t *labels_dev, uint32_t *votes_dev);
 
+/* Allocation-free predict.  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  As
+ * cta_predict_device without the votes output: X_dev (R x F float32) and
+ * labels_dev (R uint32) in device memory; the R x K vote matrix lives in a
+ * fixed per-GPU device buffer of CTA_NOALLOC_VOTE_WORDS uint32 (4 MiB, held
+ * until the process ends), so R x K above that fails with CTA_EUNSUPPORTED.
+ * The call allocates nothing: the forest's node table must already be on
+ * the device (a fit, cta_predict or cta_predict_device puts it there, else
+ * CTA_EINVAL).  The buffer keeps the last call's votes and is cleared at the
+ * start of each call; calls that overlap on one GPU (any contexts) race. */
+#define CTA_NOALLOC_VOTE_WORDS (1 << 20)
+int cta_predict_device_noalloc(cta_ctx *ctx, cta_forest *forest, const float *X_dev, int64_t R, int32_t F,
+                               uint32_t *labels_dev);
+
 /* Frees the context's cached device workspaces (they are reused across calls
  * and grow to the largest request); *bytes reports their current size. */
 int cta_ctx_release_workspace(cta_ctx *ctx);

@@ -70,3 +70,59 @@ def forest_slice(a, t0, t1):
         if k in a:
             out[k] = np.asarray(a[k])[b:e]
     return out
+
+
+class DeviceArray:
+    """A device buffer from the HIP runtime the library itself links (no
+    torch, whose bundled runtime is a second copy in the process), with
+    host copies in and out.  For tests of the *_device entry points."""
+
+    _hip = None
+
+    @classmethod
+    def hip(cls):
+        if cls._hip is None:
+            import ctypes
+
+            from cudatree_amd import _lib
+
+            _lib.load()  # the library pulls in its HIP runtime
+            cls._hip = ctypes.CDLL("libamdhip64.so.7")
+        return cls._hip
+
+    def __init__(self, shape, dtype):
+        import ctypes
+
+        self.shape, self.dtype = tuple(np.atleast_1d(shape)), np.dtype(dtype)
+        self.nbytes = int(np.prod(self.shape)) * self.dtype.itemsize
+        self.ptr = ctypes.c_void_p()
+        rc = self.hip().hipMalloc(ctypes.byref(self.ptr), ctypes.c_size_t(max(self.nbytes, 1)))
+        assert rc == 0, "hipMalloc failed (%d)" % rc
+
+    @classmethod
+    def from_host(cls, a):
+        a = np.ascontiguousarray(a)
+        d = cls(a.shape, a.dtype)
+        d.upload(a)
+        return d
+
+    def upload(self, a):
+        import ctypes
+
+        a = np.ascontiguousarray(a, dtype=self.dtype)
+        assert a.nbytes == self.nbytes
+        rc = self.hip().hipMemcpy(self.ptr, a.ctypes.data_as(ctypes.c_void_p), ctypes.c_size_t(self.nbytes), 1)
+        assert rc == 0, "hipMemcpy to the device failed (%d)" % rc
+
+    def host(self):
+        import ctypes
+
+        out = np.empty(self.shape, self.dtype)
+        rc = self.hip().hipMemcpy(out.ctypes.data_as(ctypes.c_void_p), self.ptr, ctypes.c_size_t(self.nbytes), 2)
+        assert rc == 0, "hipMemcpy to the host failed (%d)" % rc
+        return out
+
+    def free(self):
+        if self.ptr:
+            self.hip().hipFree(self.ptr)
+            self.ptr = None
