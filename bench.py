@@ -407,7 +407,8 @@ def main():
 
     # roofline of the dominant HBM kernel, HIP events over the timed steps
     roof = None
-    hbm_kernels = [k for k in ("k_resh_big", "k_resh_small", "k_scan", "k_rl_count") if k in stats and stats[k]["ms"] > 0]
+    hbm_kernels = [k for k in ("k_resh_big", "k_resh_small", "k_scan", "k_subtree", "k_rl_count")
+                   if k in stats and stats[k]["ms"] > 0]
     if hbm_kernels:
         dom = max(hbm_kernels, key=lambda k: stats[k]["ms"])
         per_kernel = {}
@@ -423,8 +424,8 @@ def main():
         traffic, tsrc = pmc_traffic(dom)
         # split scan + reshuffle kernels together (their algorithmic bytes over
         # their summed time), the figure the >= 50 % of peak target is set on
-        group = [k for k in ("k_scan", "k_scan_hist", "k_rl_count", "k_resh_big", "k_resh_small", "k_fill")
-                 if k in stats]
+        group = [k for k in ("k_scan", "k_scan_hist", "k_rl_count", "k_resh_big", "k_resh_small", "k_fill",
+                             "k_subtree") if k in stats]
         g_bytes = sum(stats[k]["alg_bytes"] for k in group)
         g_s = sum(stats[k]["ms"] for k in group) * 1e-3
         combined = {"kernels": group, "achieved": round(g_bytes / g_s / 1e9, 1) if g_s else None,

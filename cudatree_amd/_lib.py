@@ -53,7 +53,8 @@ class FitOpts(ctypes.Structure):
         ("lanes", ctypes.c_int32),
         ("fill_global", ctypes.c_int32),
         ("hist_layout", ctypes.c_int32),
-        ("reserved", ctypes.c_int32 * 2),
+        ("subtree_max", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 1),
     ]
 
 

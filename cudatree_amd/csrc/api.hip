@@ -25,6 +25,7 @@
 #include "../../include/cudatree_amd.h"
 #include "aux_kernels.hip"
 #include "fit_kernels.hip"
+#include "subtree_kernels.hip"
 
 using namespace cta;
 
@@ -406,6 +407,7 @@ constexpr uint32_t kLevelGroup = 4;
 // 581,012 x 54, 50 trees).
 constexpr uint32_t kScanChunk = 1024, kReshChunk = 4096, kFillChunk = 512;
 
+
 template <class IdxT, class LabT>
 struct Fitter {
   cta_ctx *ctx;
@@ -413,6 +415,7 @@ struct Fitter {
   const cta_fit_opts &o;
   int32_t N, F, K, m;
   uint32_t Cs, Cf, Cr;
+  uint32_t sub_max;  // small-subtree threshold of batched fits (0: none)
   hipStream_t st;
   uint32_t G;      // blocks of the wave-loop kernels
   uint32_t Gscan;  // blocks of k_scan: all resident at once, a multiple of kXcd
@@ -433,6 +436,7 @@ struct Fitter {
     Cs = opts.scan_chunk > 0 ? (uint32_t)opts.scan_chunk : kScanChunk;
     Cr = opts.resh_chunk > 0 ? (uint32_t)opts.resh_chunk : kReshChunk;
     Cf = kFillChunk;
+    sub_max = (opts.mode == CTA_MODE_BATCHED && opts.subtree_max > 0) ? (uint32_t)opts.subtree_max : 0u;
     st = c->stream;
     G = (uint32_t)std::max(64, ctx->num_cu * 8);
     const bool rowfree = !opts.all_rows && d->Frl > 0;
@@ -600,6 +604,10 @@ struct Fitter {
     a.cap_big = (uint32_t)cap_big;
     a.cap_small = (uint32_t)cap_small;
     a.cap_nodes = (uint32_t)cap_nodes;
+    // live children of at most sub_max samples: queued for k_subtree
+    a.sub_max = sub_max;
+    a.sub_queue = sub_max ? ctx->ws_as<FNode>("sub_queue", cap_q) : nullptr;
+    a.cap_sub = sub_max ? (uint32_t)cap_q : 0u;
 
     // ---- setup: bootstrap compaction and roots
     Ctl hc = {};
@@ -736,6 +744,11 @@ struct Fitter {
       }
       depth_bound = (uint32_t)depth;
       if (lvl) std::fclose(lvl);
+      if (hc.n_sub) {
+        subtrees(a, hc, Tb);
+        depth_bound = std::max(depth_bound, hc.sub_depth);
+        level_off.clear();  // subtree node ids are not grouped by depth
+      }
     } else {
       // per-node (and hybrid) schedule: host stack of frontier entries
       std::vector<HostEntry> stack, queue;
@@ -817,6 +830,25 @@ struct Fitter {
                    std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_export).count());
   }
 
+  // Every queued small node grown to its leaves by one k_subtree launch
+  // (the side bitmaps start cleared; the waves keep them so).
+  void subtrees(const FitArgs<IdxT, LabT> &a, Ctl &hc, int32_t Tb) {
+    k_fill_u32<<<grid_for((size_t)Tb * a.W), 256, 0, st>>>(a.bits, (size_t)Tb * a.W, 0u);
+    const size_t lds = (size_t)kSubWaves * sub_wave_words((uint32_t)K, Frl) * 4;
+    int nb = 0;
+    HIPC(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_subtree<IdxT, LabT>, kSubThreads, lds));
+    const uint32_t grid = (uint32_t)(ctx->num_cu * std::min(std::max(nb, 1), 8));
+    {
+      Timed tm(ctx, "k_subtree");
+      k_subtree<IdxT, LabT><<<grid, kSubThreads, lds, st>>>(a);
+    }
+    HIPC(hipGetLastError());
+    Ctl prev = hc;
+    read_ctl(a, hc);
+    account(prev, hc);
+    if (hc.err) fail(CTA_EINTERNAL, "device capacity check failed in the small subtrees (err %u)", hc.err);
+  }
+
   void read_ctl(const FitArgs<IdxT, LabT> &a, Ctl &hc) {
     Ctl *h = static_cast<Ctl *>(ctx->host("ctl", sizeof(Ctl)));
     HIPC(hipMemcpyAsync(h, a.ctl, sizeof(Ctl), hipMemcpyDeviceToHost, st));
@@ -830,6 +862,7 @@ struct Fitter {
     ctx->add_bytes("k_resh_big", now.bytes_resh - prev.bytes_resh);
     ctx->add_bytes("k_resh_small", now.bytes_resh_small - prev.bytes_resh_small);
     ctx->add_bytes("k_rl_count", now.bytes_rl - prev.bytes_rl);
+    ctx->add_bytes("k_subtree", now.bytes_sub - prev.bytes_sub);
   }
 
   // Exclusive offsets and totals of the per-node counts in a.vals (NF

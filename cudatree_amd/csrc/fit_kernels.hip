@@ -570,6 +570,106 @@ __global__ __launch_bounds__(kBlock) void k_scan_carry(FitArgs<IdxT, LabT> a) {
 // in order.
 // ------------------------------------------------------------------------
 constexpr int kScanU = 4;
+
+// Gini scan of positions [p0, p1) of a node's sorted row (n samples, feature
+// f): the best candidate position of the range, in every lane.  SL and D are
+// the running sums S_L and sum_k tot_k c_k at p0; cv (K <= 64 with regs: lane
+// k holds class k) or carry[] (LDS) the class counts before p0; tv / tot the
+// node's class totals, T2 their sum of squares; chunk_next the value at p1
+// when p1 < n.  vat(s) is sample s's value (code or float bits) and
+// cand(v, w) the candidate test between adjacent values.  kScanU tiles of 64
+// positions are in flight: their index loads, then their label and value
+// gathers, are issued together before the tiles are scanned in order.
+template <class IdxT, class LabT, class VAt, class CandF>
+__device__ __forceinline__ Cand scan_positions(const FitArgs<IdxT, LabT> &a, const IdxT *row, uint32_t p0,
+                                               uint32_t p1, uint32_t n, uint32_t f, bool regs, uint32_t cv,
+                                               uint32_t tv, uint32_t *carry, const uint32_t *tot, uint64_t SL,
+                                               uint64_t D, uint64_t T2, uint32_t chunk_next, VAt vat, CandF cand) {
+  const uint32_t lane = lane_id();
+  Cand lb = {0, 1, 0, (int32_t)f, 0};
+  for (uint32_t pb = p0; pb < p1; pb += kWave * kScanU) {
+    // kScanU tiles in flight: index loads, then label and value gathers
+    uint32_t sid[kScanU], lab[kScanU], val[kScanU];
+#pragma unroll
+    for (int u = 0; u < kScanU; ++u) {
+      const uint32_t p = pb + (uint32_t)u * kWave + lane;
+      sid[u] = (p < p1) ? (uint32_t)row[p] : 0xFFFFFFFFu;
+    }
+    const uint32_t pe = pb + kWave * kScanU;  // first position after the batch
+    const uint32_t batch_next = (pe < p1 && lane == kWave - 1) ? vat(sample_of(row[pe])) : 0u;
+#pragma unroll
+    for (int u = 0; u < kScanU; ++u) {
+      lab[u] = 0;
+      val[u] = 0;
+      if (sid[u] != 0xFFFFFFFFu) {
+        lab[u] = a.label((IdxT)sid[u]);
+        val[u] = vat(sample_of((IdxT)sid[u]));
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < kScanU; ++u) {
+      const uint32_t p = pb + (uint32_t)u * kWave + lane;
+      const bool valid = p < p1;
+      if (!ballot(valid)) break;
+      // rank of this lane among earlier lanes with the same label (one
+      // ballot per distinct label of the tile), and its class total
+      uint32_t cprev = 0, dD = 0;
+      uint64_t rem = ballot(valid);
+      if (regs) {
+        while (rem) {
+          const uint32_t leader = (uint32_t)__ffsll((unsigned long long)rem) - 1u;
+          const uint32_t L = readlane(lab[u], leader);
+          const uint64_t mm = ballot(valid && lab[u] == L);
+          const uint32_t b0 = readlane(cv, L), tL = readlane(tv, L);
+          if (valid && lab[u] == L) {
+            cprev = b0 + mbcnt(mm);
+            dD = tL;
+          }
+          if (lane == L) cv += (uint32_t)__popcll(mm);
+          rem &= ~mm;
+        }
+      } else {
+        while (rem) {
+          const uint32_t leader = (uint32_t)__ffsll((unsigned long long)rem) - 1u;
+          const uint32_t L = readlane(lab[u], leader);
+          const uint64_t mm = ballot(valid && lab[u] == L);
+          const uint32_t b0 = carry[L];
+          if (valid && lab[u] == L) cprev = b0 + mbcnt(mm);
+          wave_sync();
+          if (lane == 0) carry[L] = b0 + (uint32_t)__popcll(mm);
+          wave_sync();
+          rem &= ~mm;
+        }
+        if (valid) dD = tot[lab[u]];
+      }
+      const uint32_t dS = valid ? 2u * cprev + 1u : 0u;
+      const uint32_t iS = wave_incl_scan(dS);
+      const uint32_t iD = wave_incl_scan(dD);
+      const uint64_t SLp = SL + iS, Dp = D + iD;
+      SL += readlane(iS, kWave - 1);
+      D += readlane(iD, kWave - 1);
+      // value at p + 1: the next lane, the next tile's lane 0, the next
+      // batch, or the next chunk
+      uint32_t nxt = wave_next(val[u]);
+      if (u + 1 < kScanU) {
+        const uint32_t first_next = readlane(val[u + 1 < kScanU ? u + 1 : u], 0);
+        if (lane == kWave - 1) nxt = first_next;
+      } else if (lane == kWave - 1) {
+        nxt = batch_next;
+      }
+      if (p + 1 == p1) nxt = chunk_next;
+      const bool elig = valid && (p + 1 < n) && cand(val[u], nxt);
+      if (elig) {
+        const uint64_t nL = (uint64_t)p + 1, nR = (uint64_t)n - nL;
+        const uint64_t SR = T2 + SLp - 2 * Dp;
+        Cand c = {SLp * nR + SR * nL, nL * nR, p, (int32_t)f, 1u};
+        if (!lb.ok || q_cmp(c.num, c.den, lb.num, lb.den) > 0) lb = c;
+      }
+    }
+  }
+  return wave_best(lb);
+}
+
 template <class IdxT, class LabT>
 // Six waves per SIMD (a few registers spilled) beat five without spills.
 __global__ __launch_bounds__(kBlock, 6) void k_scan(FitArgs<IdxT, LabT> a) {
@@ -691,90 +791,9 @@ __global__ __launch_bounds__(kBlock, 6) void k_scan(FitArgs<IdxT, LabT> a) {
       uint64_t SL = uniform64(wave_sum64(sl));
       uint64_t D = uniform64(wave_sum64(dd));
       wave_sync();
-      Cand lb = {0, 1, 0, (int32_t)f, 0};
       // value just past the chunk end (the next chunk's first sample)
       const uint32_t chunk_next = (p1 < n) ? vpe : 0u;
-      for (uint32_t pb = p0; pb < p1; pb += kWave * kScanU) {
-        // kScanU tiles in flight: index loads, then label and value gathers
-        uint32_t sid[kScanU], lab[kScanU], val[kScanU];
-#pragma unroll
-        for (int u = 0; u < kScanU; ++u) {
-          const uint32_t p = pb + (uint32_t)u * kWave + lane;
-          sid[u] = (p < p1) ? (uint32_t)row[p] : 0xFFFFFFFFu;
-        }
-        const uint32_t pe = pb + kWave * kScanU;  // first position after the batch
-        const uint32_t batch_next = (pe < p1 && lane == kWave - 1) ? vat(sample_of(row[pe])) : 0u;
-#pragma unroll
-        for (int u = 0; u < kScanU; ++u) {
-          lab[u] = 0;
-          val[u] = 0;
-          if (sid[u] != 0xFFFFFFFFu) {
-            lab[u] = a.label((IdxT)sid[u]);
-            val[u] = vat(sample_of((IdxT)sid[u]));
-          }
-        }
-#pragma unroll
-        for (int u = 0; u < kScanU; ++u) {
-          const uint32_t p = pb + (uint32_t)u * kWave + lane;
-          const bool valid = p < p1;
-          if (!ballot(valid)) break;
-          // rank of this lane among earlier lanes with the same label (one
-          // ballot per distinct label of the tile), and its class total
-          uint32_t cprev = 0, dD = 0;
-          uint64_t rem = ballot(valid);
-          if (regs) {
-            while (rem) {
-              const uint32_t leader = (uint32_t)__ffsll((unsigned long long)rem) - 1u;
-              const uint32_t L = readlane(lab[u], leader);
-              const uint64_t mm = ballot(valid && lab[u] == L);
-              const uint32_t b0 = readlane(cv, L), tL = readlane(tv, L);
-              if (valid && lab[u] == L) {
-                cprev = b0 + mbcnt(mm);
-                dD = tL;
-              }
-              if (lane == L) cv += (uint32_t)__popcll(mm);
-              rem &= ~mm;
-            }
-          } else {
-            while (rem) {
-              const uint32_t leader = (uint32_t)__ffsll((unsigned long long)rem) - 1u;
-              const uint32_t L = readlane(lab[u], leader);
-              const uint64_t mm = ballot(valid && lab[u] == L);
-              const uint32_t b0 = carry[L];
-              if (valid && lab[u] == L) cprev = b0 + mbcnt(mm);
-              wave_sync();
-              if (lane == 0) carry[L] = b0 + (uint32_t)__popcll(mm);
-              wave_sync();
-              rem &= ~mm;
-            }
-            if (valid) dD = tot[lab[u]];
-          }
-          const uint32_t dS = valid ? 2u * cprev + 1u : 0u;
-          const uint32_t iS = wave_incl_scan(dS);
-          const uint32_t iD = wave_incl_scan(dD);
-          const uint64_t SLp = SL + iS, Dp = D + iD;
-          SL += readlane(iS, kWave - 1);
-          D += readlane(iD, kWave - 1);
-          // value at p + 1: the next lane, the next tile's lane 0, the next
-          // batch, or the next chunk
-          uint32_t nxt = wave_next(val[u]);
-          if (u + 1 < kScanU) {
-            const uint32_t first_next = readlane(val[u + 1 < kScanU ? u + 1 : u], 0);
-            if (lane == kWave - 1) nxt = first_next;
-          } else if (lane == kWave - 1) {
-            nxt = batch_next;
-          }
-          if (p + 1 == p1) nxt = chunk_next;
-          const bool elig = valid && (p + 1 < n) && cand(val[u], nxt);
-          if (elig) {
-            const uint64_t nL = (uint64_t)p + 1, nR = (uint64_t)n - nL;
-            const uint64_t SR = T2 + SLp - 2 * Dp;
-            Cand c = {SLp * nR + SR * nL, nL * nR, p, (int32_t)f, 1u};
-            if (!lb.ok || q_cmp(c.num, c.den, lb.num, lb.den) > 0) lb = c;
-          }
-        }
-      }
-      lb = wave_best(lb);
+      Cand lb = scan_positions(a, row, p0, p1, n, f, regs, cv, tv, carry, tot, SL, D, T2, chunk_next, vat, cand);
       if (cand_better(lb, best)) best = lb;
     }
     if (lane == 0) {
@@ -955,7 +974,9 @@ __global__ __launch_bounds__(kBlock) void k_decide(FitArgs<IdxT, LabT> a) {
       }
       nzl = uniform(wave_sum32(nzl));
       nzr = uniform(wave_sum32(nzr));
-      if (child_kind(a, nzl, nL, nd.depth) == 0u || child_kind(a, nzr, nR, nd.depth) == 0u) {
+      // (children queued for the small subtrees need none)
+      if ((child_kind(a, nzl, nL, nd.depth) == 0u && nL > a.sub_max) ||
+          (child_kind(a, nzr, nR, nd.depth) == 0u && nR > a.sub_max)) {
         ax.flags |= kCountRl;
         for (uint32_t w = lane; w < a.Frl * K; w += kWave) a.rl_small[(size_t)i * a.Frl * K + w] = 0u;
         // signature words of the smaller side
@@ -1326,9 +1347,13 @@ __global__ __launch_bounds__(kBlock) void k_children(FitArgs<IdxT, LabT> a) {
       if (kr == 0) flags |= kLiveR;
       ax.flags = flags;
       v[0] = 2;
-      v[1] = (kl == 0 ? 1u : 0u) + (kr == 0 ? 1u : 0u);
+      // a live child of at most sub_max samples goes to the small-subtree
+      // queue, a larger one to the next frontier
+      const uint32_t subl = (kl == 0 && nL <= a.sub_max) ? 1u : 0u, subr = (kr == 0 && nR <= a.sub_max) ? 1u : 0u;
+      v[1] = (kl == 0 ? 1u : 0u) + (kr == 0 ? 1u : 0u) - subl - subr;
+      v[5] = subl + subr;
       ax.resh_ntask = 0;
-      if (v[1]) {
+      if (kl == 0 || kr == 0) {
         resh_shape(nd.n, a.Fr, a.Cr, ax, v);
         // every row read and written, the mark gathered per element: Fr n (2b + 1)
         const unsigned long long b = (unsigned long long)a.Fr * nd.n * (2 * sizeof(IdxT) + 1);
@@ -1378,7 +1403,7 @@ __global__ __launch_bounds__(kBlock) void k_children_emit(FitArgs<IdxT, LabT> a)
   if (a.ctl->err) return;  // a list overflowed: the fit fails, nothing more is read
   const uint32_t lane = lane_id();
   const uint32_t K = (uint32_t)a.K;
-  const uint32_t id_base = a.ctl->n_nodes;
+  const uint32_t id_base = a.ctl->n_nodes, sub_base = a.ctl->n_sub;
   WAVE_LOOP(a.ctl->q) {
     const uint32_t i = item;
     const NAux ax = a.aux[i];
@@ -1386,9 +1411,12 @@ __global__ __launch_bounds__(kBlock) void k_children_emit(FitArgs<IdxT, LabT> a)
     const FNode nd = a.front[i];
     const uint32_t *o = a.offs + (size_t)i * kChFields;
     const uint32_t id0 = id_base + o[0];
-    uint32_t slot = o[1];
-    const uint32_t nlive = ((ax.flags & kLiveL) ? 1u : 0u) + ((ax.flags & kLiveR) ? 1u : 0u);
-    if ((uint64_t)id0 + 2 > a.cap_nodes || (uint64_t)slot + nlive > a.cap_q || !resh_fits(a, ax, o)) {
+    uint32_t slot = o[1], qslot = sub_base + o[5];
+    const uint32_t nsub = ((ax.flags & kLiveL) && ax.nL <= a.sub_max ? 1u : 0u) +
+                          ((ax.flags & kLiveR) && nd.n - ax.nL <= a.sub_max ? 1u : 0u);
+    const uint32_t nlive = ((ax.flags & kLiveL) ? 1u : 0u) + ((ax.flags & kLiveR) ? 1u : 0u) - nsub;
+    if ((uint64_t)id0 + 2 > a.cap_nodes || (uint64_t)slot + nlive > a.cap_q ||
+        (uint64_t)qslot + nsub > a.cap_sub || !resh_fits(a, ax, o)) {
       if (lane == 0) atomicOr(&a.ctl->err, kErrCapacity);
       continue;
     }
@@ -1414,17 +1442,21 @@ __global__ __launch_bounds__(kBlock) void k_children_emit(FitArgs<IdxT, LabT> a)
         }
         continue;
       }
-      if (lane == 0) {
-        FNode ch;
-        ch.tree = nd.tree;
-        ch.start = c ? nd.start + ax.nL : nd.start;
-        ch.n = cn;
-        ch.nid = id;
-        ch.depth = cd;
-        ch.src = ax.dst;
-        ch.pad0 = ch.pad1 = 0;
-        a.next_front[slot] = ch;
+      FNode ch;
+      ch.tree = nd.tree;
+      ch.start = c ? nd.start + ax.nL : nd.start;
+      ch.n = cn;
+      ch.nid = id;
+      ch.depth = cd;
+      ch.src = ax.dst;
+      ch.pad0 = ch.pad1 = 0;
+      if (cn <= a.sub_max) {
+        // grown to its leaves by a subtree wave after the levels
+        if (lane == 0) a.sub_queue[qslot] = ch;
+        ++qslot;
+        continue;
       }
+      if (lane == 0) a.next_front[slot] = ch;
       for (uint32_t k = lane; k < K; k += kWave) {
         const uint32_t l = a.cntL[(size_t)i * K + k];
         a.next_cnt[(size_t)slot * K + k] = c ? a.cnt[(size_t)i * K + k] - l : l;
@@ -1556,8 +1588,10 @@ __device__ __forceinline__ void ctl_next_body(Ctl *ctl, unsigned long long *bslo
   if (tid == 0) {
     ctl->q = ctl->c_live;
     ctl->n_nodes += ctl->c_ids;
+    ctl->n_sub += ctl->c_sub;
     ctl->c_live = 0;
     ctl->c_ids = 0;
+    ctl->c_sub = 0;
     // per-level record, so the host can read several levels at once
     const uint32_t d = (ctl->depth + 1) % kLevelRing;
     ctl->depth += 1;
@@ -1755,27 +1789,26 @@ __device__ void resh_big(const FitArgs<IdxT, LabT> &a, const Task &tk, const Sid
   }
 }
 
-// One small task: rows [first, first+count) of a node, moved by one wave.
-// A row of n <= 64 * kReshE / 2 samples takes c = ceil(n / 64) pieces, so
-// kReshE / c rows are loaded together (one batch of kReshE striped loads in
-// flight) before their ranks are taken in order.  Consecutive rows of a
-// tree are N elements apart.
+// Row slots [first, end) of the node of tree `tree` whose segment
+// [start, start + n) sits in rows `src`, partitioned by one wave into rows
+// `dst`: samples whose side(s) ^ inv is 1 to [start, start + nL), the others
+// after them (flags: kLiveL / kLiveR, the children whose rows are written,
+// and kInvert).  A row of n <= 64 * kReshE / 2 samples takes c = ceil(n / 64)
+// pieces, so kReshE / c rows are loaded together (one batch of kReshE
+// striped loads in flight) before their ranks are taken in order.
+// Consecutive rows of a tree are N elements apart.
 template <class IdxT, class LabT, class Side>
-__device__ void resh_small(const FitArgs<IdxT, LabT> &a, const Task &tk, const Side &side) {
-  const uint32_t node = uniform(tk.node);
-  const FNode nd = a.front[node];
-  const NAux ax = a.aux[node];
-  const bool liveL = uniform(ax.flags) & kLiveL, liveR = uniform(ax.flags) & kLiveR;
-  const uint32_t inv = (uniform(ax.flags) & kInvert) ? 1u : 0u;  // marks on the right child
-  const uint32_t first = uniform(tk.first_count & 0xFFFFu), count = uniform(tk.first_count >> 16);
-  const uint32_t end = first + count;
-  const uint32_t n = uniform(nd.n), nL = uniform(ax.nL), N = (uint32_t)a.N;
+__device__ void resh_rows(const FitArgs<IdxT, LabT> &a, uint32_t tree, uint32_t start, uint32_t src, uint32_t dst,
+                          uint32_t n, uint32_t nL, uint32_t flags, uint32_t first, uint32_t end, const Side &side) {
+  const bool liveL = flags & kLiveL, liveR = flags & kLiveR;
+  const uint32_t inv = (flags & kInvert) ? 1u : 0u;  // marks on the right child
+  const uint32_t N = (uint32_t)a.N;
   const uint32_t lane = lane_id();
   const uint32_t c = ceil_div(n, kWave);
   if (c <= (uint32_t)kReshE / 2) {
     const uint32_t R = (uint32_t)kReshE / c;  // rows per batch
     for (uint32_t f0 = first; f0 < end; f0 += R) {
-      const gptr<IdxT> out = sgpr_ptr(rows_out(a, ax.dst, nd.tree, f0) + nd.start);
+      const gptr<IdxT> out = sgpr_ptr(rows_out(a, dst, tree, f0) + start);
       const uint32_t rows = min(R, end - f0);
       IdxT e[kReshE];
       uint32_t r = 0, pc = 0;
@@ -1783,7 +1816,7 @@ __device__ void resh_small(const FitArgs<IdxT, LabT> &a, const Task &tk, const S
       for (int k = 0; k < kReshE; ++k) {
         const uint32_t p = pc * kWave + lane;
         // source rows by slot (the presorted rows are one per feature)
-        const gptr<const IdxT> in = sgpr_ptr(rows_of_slot(a, nd.src, nd.tree, f0 + min(r, rows - 1)) + nd.start);
+        const gptr<const IdxT> in = sgpr_ptr(rows_of_slot(a, src, tree, f0 + min(r, rows - 1)) + start);
         e[k] = ld32(in, min(p, n - 1));  // out-of-range pieces are masked
         if (++pc == c) {
           pc = 0;
@@ -1832,9 +1865,9 @@ __device__ void resh_small(const FitArgs<IdxT, LabT> &a, const Task &tk, const S
   constexpr uint32_t kPiece = kWave * kReshE;
   if (first >= end) return;
   const size_t rstride = (size_t)a.N;  // consecutive row slots of a tree are N entries apart
-  const gptr<IdxT> out0 = sgpr_ptr(rows_out(a, ax.dst, nd.tree, first) + nd.start);
+  const gptr<IdxT> out0 = sgpr_ptr(rows_out(a, dst, tree, first) + start);
   auto load = [&](IdxT(&e)[kReshE], uint32_t r, uint32_t base) {
-    const gptr<const IdxT> in = sgpr_ptr(rows_of_slot(a, nd.src, nd.tree, first + r) + nd.start);
+    const gptr<const IdxT> in = sgpr_ptr(rows_of_slot(a, src, tree, first + r) + start);
 #pragma unroll
     for (int k = 0; k < kReshE; ++k) e[k] = ld32(in, min(base + (uint32_t)k * kWave + lane, n - 1));
   };
@@ -1881,6 +1914,17 @@ __device__ void resh_small(const FitArgs<IdxT, LabT> &a, const Task &tk, const S
     if (!step(A, B, r, base)) break;
     if (!step(B, A, r, base)) break;
   }
+}
+
+// One small task: rows [first, first+count) of a node, moved by one wave.
+template <class IdxT, class LabT, class Side>
+__device__ void resh_small(const FitArgs<IdxT, LabT> &a, const Task &tk, const Side &side) {
+  const uint32_t node = uniform(tk.node);
+  const FNode nd = a.front[node];
+  const NAux ax = a.aux[node];
+  const uint32_t first = uniform(tk.first_count & 0xFFFFu), count = uniform(tk.first_count >> 16);
+  resh_rows(a, uniform(nd.tree), uniform(nd.start), uniform(nd.src), uniform(ax.dst), uniform(nd.n), uniform(ax.nL),
+            uniform(ax.flags), first, first + count, side);
 }
 
 // kLds: the tree's bitmap is staged in LDS (N/8 bytes); otherwise it is read

@@ -79,19 +79,32 @@ struct Ctl {
   uint32_t n_plan[2];  // reshuffle plan entries: big tasks, small tasks
   uint32_t pad1[2];
   uint32_t n_scan, n_hist, n_fill, n_cls[kXcd], pad2;    // selection-scan totals
-  uint32_t c_ids, c_live, c_big, c_small, c_tiles, pad3[3];  // children-scan totals
-  unsigned long long bytes_scan, bytes_fill, bytes_resh, bytes_boot, bytes_resh_small, bytes_rl;
+  uint32_t c_ids, c_live, c_big, c_small, c_tiles, c_sub, pad3[2];  // children-scan totals
+  unsigned long long bytes_scan, bytes_fill, bytes_resh, bytes_boot, bytes_resh_small, bytes_rl, bytes_sub;
+  // small subtrees: nodes queued so far (whole levels), the next one a
+  // subtree wave takes, and the deepest node depth the subtree waves made
+  uint32_t n_sub, sub_next, sub_depth, pad4;
   // after level `depth`: frontier size and ids in use, at depth % kLevelRing
   uint32_t lv_q[kLevelRing], lv_nodes[kLevelRing];
 };
 // Fields of the two per-node offset scans.
 constexpr int kSelFields = 3 + kXcd;  // scan tasks, histogram words, fill slots, scan tasks per class
-constexpr int kChFields = 5;   // child ids, next-frontier slots, big / small reshuffle tasks, big tiles
+// child ids, next-frontier slots, big / small reshuffle tasks, big tiles,
+// small-subtree queue slots
+constexpr int kChFields = 6;
 // Algorithmic-byte counters: per-wave sums flushed into kByteSlots spread
 // slots (one 128-byte line each) and folded into Ctl at the end of a level.
 constexpr uint32_t kByteSlots = 64, kByteStride = 16;
-enum ByteKind { kBytesScan = 0, kBytesFill = 1, kBytesResh = 2, kBytesBoot = 3, kBytesReshSmall = 4, kBytesRl = 5 };
-constexpr uint32_t kByteKinds = 6;
+enum ByteKind {
+  kBytesScan = 0,
+  kBytesFill = 1,
+  kBytesResh = 2,
+  kBytesBoot = 3,
+  kBytesReshSmall = 4,
+  kBytesRl = 5,
+  kBytesSub = 6
+};
+constexpr uint32_t kByteKinds = 7;
 constexpr uint32_t kErrCapacity = 1u, kErrBadNode = 2u;
 
 // Flattened forest node for prediction: pre-order, so a split node's left
@@ -159,6 +172,11 @@ struct FitArgs {
   uint64_t seed;
   uint32_t Cs, Cf, Cr;  // chunk sizes: scan, fill, reshuffle
   uint32_t hist_lds;    // 1: split-scan class counts in per-wave LDS counters even when K <= 64
+  // Live children of at most sub_max samples (0: none) leave the levels for
+  // the small-subtree queue; a subtree wave grows each one to its leaves.
+  uint32_t sub_max;
+  FNode *sub_queue;
+  uint32_t cap_sub;
   // level state
   const FNode *front;
   const uint32_t *cnt;  // q x K class counts of the frontier

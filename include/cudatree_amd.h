@@ -125,7 +125,11 @@ typedef struct {
   int32_t hist_layout;       /* class-count layout of the split scan: CTA_HIST_AUTO (0), CTA_HIST_REGS (1: lane k
                                 holds class k, K <= 64 only) or CTA_HIST_LDS (2: per-wave LDS counters).  The
                                 trees are identical */
-  int32_t reserved[2];
+  int32_t subtree_max;       /* batched mode: a live node of at most this many samples leaves the levels and
+                                its whole subtree is grown by one wave, every such node of every tree in one
+                                launch after the levels (<= 0, the default: every node stays in the levels).
+                                The trees are identical */
+  int32_t reserved[1];
 } cta_fit_opts;
 
 #define CTA_HIST_AUTO 0

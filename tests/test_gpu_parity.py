@@ -25,7 +25,7 @@ def orc():
 
 
 def gpu_forest(ctx, case, mode="batched", scan_chunk=0, resh_chunk=0, idx_bits=0, tpb=0, resh_global=0, all_rows=0,
-               fill_global=0, hist_layout=0):
+               fill_global=0, hist_layout=0, subtree_max=0):
     X, y, K = case["X"], case["y"], case["K"]
     ds = _lib.Dataset(ctx, X, y, K, idx_bits=idx_bits)
     o = _lib.default_opts()
@@ -43,6 +43,7 @@ def gpu_forest(ctx, case, mode="batched", scan_chunk=0, resh_chunk=0, idx_bits=0
     o.all_rows = all_rows
     o.fill_global = fill_global
     o.hist_layout = hist_layout
+    o.subtree_max = subtree_max
     f = _lib.Forest.fit(ctx, ds, case["n_trees"], o, helpers.case_boot_ids(case))
     return f, ds
 
@@ -396,7 +397,8 @@ def test_two_lanes_match_one(ctx, orc, T, tpb):
 
 def test_level_hook_runs_between_level_groups(ctx, orc):
     """The context's level hook runs on the fitting thread between groups of
-    levels; the trees do not change."""
+    levels; the trees do not change.  Every node stays in the levels here
+    (subtree_max < 0), so there is a group per four depths."""
     X, y, _, _ = datasets.covtype_like(seed=21, n_train=6000, n_test=1)
     case = {"X": X, "y": y, "K": 7, "m": 7, "mss": 1, "max_depth": 0, "identity": False, "seed": 5,
             "boot": True, "n_trees": 3}
@@ -404,7 +406,7 @@ def test_level_hook_runs_between_level_groups(ctx, orc):
     ctx.set_level_hook(lambda: calls.append(1))
     try:
         print("level hook", flush=True)
-        f, _ = gpu_forest(ctx, case)
+        f, _ = gpu_forest(ctx, case, subtree_max=-1)
     finally:
         ctx.set_level_hook(None)
     got = f.export()
