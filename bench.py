@@ -224,6 +224,40 @@ def serving_bench(lib, X, y, Xt, args):
                  e[key]["requests"]))
         e["rows_per_s_ratio"] = round(e["coalesced"]["rows_per_s"] / max(e["per_request"]["rows_per_s"], 1e-9), 2)
         result["sweep"].append(e)
+    def burst(n_req, largest, coalesce, seed, per_call=1000):
+        # every request submitted at once (a client handing over requests it
+        # has gathered, per_call of them per submit call): the completion
+        # rate is the serving capacity of the mode
+        rng = np.random.default_rng(seed)
+        sizes = np.minimum(np.exp(rng.uniform(0, np.log(largest), size=n_req)).astype(int), largest)
+        starts = rng.integers(0, len(Xt) - largest, size=n_req)
+        srv = lib.Server(coalesce=coalesce, fit_workers=1)
+        fid = srv.add_forest(forest)
+        reqs = [Xt[starts[i]:starts[i] + sizes[i]] for i in range(n_req)]
+        tickets = []
+        t0 = time.perf_counter()
+        for c in range(0, n_req, per_call):
+            tickets += srv.submit_many(fid, reqs[c:c + per_call])
+        t_sub = time.perf_counter() - t0
+        for tk in tickets:
+            st, _ = srv.wait(tk)
+            assert st == 0
+        t_all = time.perf_counter() - t0
+        s = srv.stats()
+        srv.close()
+        return {"requests": n_req, "rows": int(sizes.sum()), "submit_s": round(t_sub, 3), "wall_s": round(t_all, 3),
+                "requests_per_s": round(n_req / t_all, 1), "rows_per_s": round(s["rows_per_s"], 1),
+                "p50_ms": round(s["p50_ms"], 3), "p99_ms": round(s["p99_ms"], 3), "launches": s["predict_launches"]}
+
+    if args.serving_burst > 0:
+        e = {"request_rows": "log-uniform 1..64", "submit": "all at once, 1,000 requests per submit call"}
+        for coalesce in (True, False):
+            key = "coalesced" if coalesce else "per_request"
+            e[key] = burst(args.serving_burst, 64, coalesce, 13)
+            log("serving burst %d small requests %-11s: %.0f req/s, %.0f rows/s, %d launches" %
+                (args.serving_burst, key, e[key]["requests_per_s"], e[key]["rows_per_s"], e[key]["launches"]))
+        e["rows_per_s_ratio"] = round(e["coalesced"]["rows_per_s"] / max(e["per_request"]["rows_per_s"], 1e-9), 2)
+        result["saturation_small_requests"] = e
     if args.serving_small_qps > 0:
         e = {"qps": args.serving_small_qps, "request_rows": "log-uniform 1..64, no fit jobs"}
         for coalesce in (True, False):
@@ -334,6 +368,8 @@ def main():
                     help="predict request rates of the serving sweep")
     ap.add_argument("--serving-small-qps", type=float, default=20000.0,
                     help="request rate of the small-request stream (0: skip)")
+    ap.add_argument("--serving-burst", type=int, default=20000,
+                    help="small requests submitted at once to measure each mode's capacity (0: skip)")
     ap.add_argument("--workload-steps", type=int, default=3, help="timed fits per other workload (0: skip them)")
     ap.add_argument("--tune-probe-trees", type=int, default=8,
                     help="trees per autotuner probe on the 200,000 x 256 workload (0: no sweep)")

@@ -150,6 +150,8 @@ SIGNATURES = {
     "cta_server_get_forest": (ctypes.c_int, [_vp, ctypes.c_int32, ctypes.POINTER(_vp)]),
     "cta_server_submit_predict": (ctypes.c_int, [_vp, ctypes.c_int32, _f32p, ctypes.c_int64, ctypes.c_int32, _u32p,
                                                  _u64p]),
+    "cta_server_submit_predict_many": (ctypes.c_int, [_vp, ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(_f32p),
+                                                      _i64p, ctypes.c_int32, ctypes.POINTER(_u32p), _u64p]),
     "cta_server_submit_fit": (ctypes.c_int, [_vp, _f32p, _u32p, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
                                              ctypes.c_int32, ctypes.POINTER(FitOpts), _u64p]),
     "cta_server_wait": (ctypes.c_int, [_vp, ctypes.c_uint64, ctypes.c_int64, _i32p, _i32p]),
@@ -582,6 +584,25 @@ class Server:
         with self._lock:
             self._keep[t.value] = (X, out)
         return t.value
+
+    def submit_many(self, fid, Xs):
+        """Queues every array of Xs (rows, F) in one call; returns the tickets."""
+        Xs = [np.ascontiguousarray(X, dtype=np.float32) for X in Xs]
+        n = len(Xs)
+        if n == 0:
+            return []
+        F = Xs[0].shape[1]
+        outs = [np.zeros(X.shape[0], np.uint32) for X in Xs]
+        xp = (_f32p * n)(*[ptr(X, _f32p) for X in Xs])
+        lp = (_u32p * n)(*[ptr(o, _u32p) for o in outs])
+        R = np.array([X.shape[0] for X in Xs], np.int64)
+        t = np.zeros(n, np.uint64)
+        check_server(self.lib.cta_server_submit_predict_many(self.h, fid, n, xp, ptr(R, _i64p), F, lp,
+                                                             ptr(t, _u64p)))
+        with self._lock:
+            for k in range(n):
+                self._keep[int(t[k])] = (Xs[k], outs[k])
+        return [int(v) for v in t]
 
     def submit_fit(self, X, y, K, n_trees, opts):
         X = np.ascontiguousarray(X, dtype=np.float32)

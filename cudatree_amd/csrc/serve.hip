@@ -145,7 +145,7 @@ struct cta_server {
   std::condition_variable cv_pred, cv_fit, cv_done;
   std::condition_variable cv_idle;  // predict queue empty and no batch on the device
   int busy_lanes = 0;
-  std::deque<PredReq> pq;
+  std::deque<PredReq> pq[2];  // queued predict requests of each lane class, in arrival order
   std::deque<FitJob> fq;
   std::map<int32_t, ForestEntry> forests;
   std::unordered_map<uint64_t, TicketState> tickets;
@@ -221,15 +221,11 @@ struct cta_server {
   // Lane of a request of R rows.
   int lane_of(int64_t R) const { return (lanes.size() > 1 && R > split_rows) ? 1 : 0; }
 
-  // First queued request of lane class `cls`, or pq.end().
-  std::deque<PredReq>::iterator first_of(int cls) {
-    for (auto it = pq.begin(); it != pq.end(); ++it)
-      if (lane_of(it->R) == cls) return it;
-    return pq.end();
-  }
+  bool queues_empty() const { return pq[0].empty() && pq[1].empty(); }
 
-  // One coalesced (or single) launch over the queued requests of a forest,
-  // in arrival order within the lane's class.
+  // One coalesced (or single) launch over the requests at the front of the
+  // lane's queue: the run of requests of the first one's forest, in arrival
+  // order.
   void predict_loop(Lane &L) {
     (void)hipSetDevice(o.device);
     const int64_t max_rows = o.max_batch_rows > 0 ? o.max_batch_rows : (1 << 20);
@@ -240,25 +236,21 @@ struct cta_server {
       int32_t fid = -1;
       {
         std::unique_lock<std::mutex> lk(mu);
-        cv_pred.wait(lk, [&] { return stop || first_of(L.cls) != pq.end(); });
-        auto head = first_of(L.cls);
-        if (head == pq.end()) return;  // stopping, nothing left for this lane
-        fid = head->forest;
+        std::deque<PredReq> &q = pq[L.cls];
+        cv_pred.wait(lk, [&] { return stop || !q.empty(); });
+        if (q.empty()) return;  // stopping, nothing left for this lane
+        fid = q.front().forest;
         auto fit = forests.find(fid);
         const int64_t T = fit != forests.end() ? std::max<int64_t>(1, fit->second.T) : 1;
         int64_t rows = 0;
-        for (auto it = head; it != pq.end();) {
-          // a batch stays under max_rows rows and max_work row-tree visits
-          // (its first request is always taken)
-          if (it->forest == fid && lane_of(it->R) == L.cls &&
-              (batch.empty() || (rows + it->R <= max_rows && (rows + it->R) * T <= max_work))) {
-            rows += it->R;
-            batch.push_back(*it);
-            it = pq.erase(it);
-            if (!o.coalesce) break;
-          } else {
-            ++it;
-          }
+        // a batch stays under max_rows rows and max_work row-tree visits (its
+        // first request is always taken)
+        while (!q.empty() && q.front().forest == fid &&
+               (batch.empty() || (rows + q.front().R <= max_rows && (rows + q.front().R) * T <= max_work))) {
+          rows += q.front().R;
+          batch.push_back(q.front());
+          q.pop_front();
+          if (!o.coalesce) break;
         }
         auto f = forests.find(fid);
         if (f == forests.end()) {
@@ -306,7 +298,7 @@ struct cta_server {
       release_forest_locked(fid);
       L.busy = false;
       --busy_lanes;
-      if (pq.empty() && busy_lanes == 0) cv_idle.notify_all();
+      if (queues_empty() && busy_lanes == 0) cv_idle.notify_all();
     }
   }
 
@@ -318,7 +310,7 @@ struct cta_server {
     if (us < 0) return;
     std::unique_lock<std::mutex> lk(s->mu);
     s->cv_idle.wait_for(lk, std::chrono::microseconds(us ? us : 4000),
-                        [&] { return s->stop || (s->pq.empty() && s->busy_lanes == 0); });
+                        [&] { return s->stop || (s->queues_empty() && s->busy_lanes == 0); });
   }
 
   void fit_loop(size_t w) {
@@ -524,18 +516,29 @@ int cta_server_drop_forest(cta_server *s, int32_t forest_id) {
 
 int cta_server_submit_predict(cta_server *s, int32_t forest_id, const float *X, int64_t R, int32_t F,
                               uint32_t *labels, uint64_t *ticket) {
+  return cta_server_submit_predict_many(s, forest_id, 1, &X, &R, F, &labels, ticket);
+}
+
+int cta_server_submit_predict_many(cta_server *s, int32_t forest_id, int32_t count, const float *const *X,
+                                   const int64_t *R, int32_t F, uint32_t *const *labels, uint64_t *tickets) {
   return sguard([&] {
-    if (!s || !X || !labels || !ticket) sfail(CTA_EINVAL, "NULL argument");
-    if (R <= 0) sfail(CTA_EINVAL, "R must be positive");
+    if (!s || !X || !R || !labels || !tickets) sfail(CTA_EINVAL, "NULL argument");
+    if (count < 0) sfail(CTA_EINVAL, "count must be >= 0");
+    const int64_t max_rows = s->o.max_batch_rows > 0 ? s->o.max_batch_rows : (1 << 20);
+    for (int32_t i = 0; i < count; ++i) {
+      if (!X[i] || !labels[i]) sfail(CTA_EINVAL, "NULL request %d", i);
+      if (R[i] <= 0) sfail(CTA_EINVAL, "request %d: R must be positive", i);
+      if (R[i] > max_rows) sfail(CTA_EINVAL, "request %d of %lld rows exceeds max_batch_rows", i, (long long)R[i]);
+    }
     std::lock_guard<std::mutex> lk(s->mu);
     auto it = s->forests.find(forest_id);
     if (it == s->forests.end() || it->second.dropped) sfail(CTA_EINVAL, "no forest %d", forest_id);
     if (F != it->second.F) sfail(CTA_EINVAL, "rows have %d features, the forest %d", F, it->second.F);
-    const int64_t max_rows = s->o.max_batch_rows > 0 ? s->o.max_batch_rows : (1 << 20);
-    if (R > max_rows) sfail(CTA_EINVAL, "request of %lld rows exceeds max_batch_rows", (long long)R);
-    const uint64_t t = s->new_ticket_locked();
-    s->pq.push_back(PredReq{t, forest_id, X, R, labels});
-    *ticket = t;
+    for (int32_t i = 0; i < count; ++i) {
+      const uint64_t t = s->new_ticket_locked();
+      s->pq[s->lane_of(R[i])].push_back(PredReq{t, forest_id, X[i], R[i], labels[i]});
+      tickets[i] = t;
+    }
     s->cv_pred.notify_all();
   });
 }
@@ -594,7 +597,7 @@ int cta_server_get_stats(cta_server *s, cta_server_stats *out) {
     out->hbm_in_use = s->in_use;
     out->hbm_peak = s->peak;
     out->resident_forests = (int32_t)s->forests.size();
-    out->queued_predict = (int32_t)s->pq.size();
+    out->queued_predict = (int32_t)(s->pq[0].size() + s->pq[1].size());
     out->queued_fit = (int32_t)s->fq.size();
     if (!s->lat_ms.empty()) {
       std::vector<double> v = s->lat_ms;

@@ -297,6 +297,11 @@ int cta_server_drop_forest(cta_server *server, int32_t forest_id);
  * (R uint32) are written before the ticket completes. */
 int cta_server_submit_predict(cta_server *server, int32_t forest_id, const float *X, int64_t R, int32_t F,
                               uint32_t *labels, uint64_t *ticket);
+/* Queues `count` requests at once (request i: R[i] rows at X[i], labels to
+ * labels[i], ticket to tickets[i]), as count calls of
+ * cta_server_submit_predict under one lock; all are refused if any is bad. */
+int cta_server_submit_predict_many(cta_server *server, int32_t forest_id, int32_t count, const float *const *X,
+                                   const int64_t *R, int32_t F, uint32_t *const *labels, uint64_t *tickets);
 /* Queues a fit job (X, y copied at submission; boot_ids come from
  * opts->bootstrap).  On success the new forest becomes resident and its id is
  * reported by cta_server_wait. */

@@ -100,6 +100,31 @@ def test_predict_alone_vs_in_mix(data, base_forest, coalesce, split):
         srv.close()
 
 
+@pytest.mark.parametrize("coalesce", [True, False])
+def test_submit_many_matches_single(data, base_forest, coalesce):
+    """Requests queued in one call give the labels of single predicts (and
+    the oracle's), across both lanes."""
+    _, _, Xt = data
+    reqs = _requests(Xt, 300, 9, largest=9000)
+    trees = base_forest.export()
+    orc = oracle.get()
+    srv = _lib.Server(coalesce=coalesce, split_rows=256)
+    try:
+        fid = srv.add_forest(base_forest)
+        print("submit_many coalesce=%d" % coalesce, flush=True)
+        tickets = srv.submit_many(fid, reqs[:150]) + srv.submit_many(fid, reqs[150:])
+        for r, t in zip(reqs, tickets):
+            st, labels = srv.wait(t)
+            assert st == 0 and np.array_equal(labels, orc.predict(trees, r, 7))
+        s = srv.stats()
+        assert s["predict_requests"] == len(reqs)
+        assert s["predict_launches"] < len(reqs) if coalesce else s["predict_launches"] == len(reqs)
+        with pytest.raises(_lib.CtaError):
+            srv.submit_many(fid, [reqs[0], np.zeros((0, 54), np.float32)])
+    finally:
+        srv.close()
+
+
 def test_fit_admission_by_hbm_budget(data):
     X, y, _ = data
     srv = _lib.Server(hbm_budget=256 << 20)
