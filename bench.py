@@ -358,7 +358,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--cpu-trees", type=int, default=10, help="trees in the bounded CPU-oracle sample")
+    ap.add_argument("--cpu-trees", type=int, default=16, help="trees in the bounded CPU-oracle sample")
+    ap.add_argument("--cpu-threads", type=int, default=0, help="host threads growing them (0: every core)")
+    ap.add_argument("--cpu-trees-1core", type=int, default=4, help="trees of the one-core oracle figure (0: skip)")
     ap.add_argument("--per-node-trees", type=int, default=1, help="trees in the per-node GPU sample (0: skip)")
     ap.add_argument("--sklearn-trees", type=int, default=8, help="trees in the scikit-learn sample (0: skip)")
     ap.add_argument("--no-profile", action="store_true", help="skip per-kernel HIP-event timing")
@@ -527,22 +529,36 @@ def main():
     if args.cpu_trees > 0:
         import oracle
 
-        o = oracle.get()
-        nt = args.cpu_trees
-        ids = o.bootstrap_forest_ids(SEED, nt, N)
-        t = time.perf_counter()
-        fo = o.fit(X, y, K, nt, MAX_FEATURES, boot_ids=ids, seed=SEED)
-        dt = time.perf_counter() - t
         sys.path.insert(0, os.path.join(ROOT, "tests"))
         import helpers  # tests/helpers.py: exact forest comparison
 
+        o = oracle.get()
+        cores = len(os.sched_getaffinity(0))
+        threads = args.cpu_threads if args.cpu_threads > 0 else cores
+        nt = args.cpu_trees
+        ids = o.bootstrap_forest_ids(SEED, nt, N)
+        # the oracle's trees in parallel on the host cores (one tree per thread
+        # at a time), then a few trees on one core
+        t = time.perf_counter()
+        fo = o.fit(X, y, K, nt, MAX_FEATURES, boot_ids=ids, seed=SEED, threads=threads)
+        dt = time.perf_counter() - t
         # every field of the first nt trees: thresholds bit for bit, sample
         # counts, depths and the tree boundaries
         same = helpers.forest_diff(helpers.forest_slice(arrays, 0, nt), fo) is None
-        cpu = {"value": round(dt * N_TREES / nt, 2), "unit": "s/50 trees", "cores": 1, "kind": "port",
-               "sample": "%d of %d trees of the same workload, scaled x%.3g" % (nt, N_TREES, N_TREES / nt),
+        cpu = {"value": round(dt * N_TREES / nt, 2), "unit": "s/50 trees", "cores": threads, "kind": "port",
+               "sample": "%d of %d trees of the same workload grown by %d host threads, scaled x%.3g" %
+                         (nt, N_TREES, threads, N_TREES / nt),
                "identical_trees": bool(same)}
-        log("cpu oracle: %.2fs for %d trees -> %.1fs/50 (identical: %s)" % (dt, nt, dt * N_TREES / nt, same))
+        log("cpu oracle: %.2fs for %d trees on %d threads -> %.1fs/50 (identical: %s)" %
+            (dt, nt, threads, dt * N_TREES / nt, same))
+        if args.cpu_trees_1core > 0:
+            n1 = args.cpu_trees_1core
+            t = time.perf_counter()
+            o.fit(X, y, K, n1, MAX_FEATURES, boot_ids=ids[:n1], seed=SEED)
+            d1 = time.perf_counter() - t
+            cpu["one_core"] = {"value": round(d1 * N_TREES / n1, 2), "unit": "s/50 trees",
+                               "sample": "%d trees on one core, scaled x%.3g" % (n1, N_TREES / n1)}
+            log("cpu oracle one core: %.2fs for %d trees -> %.1fs/50" % (d1, n1, d1 * N_TREES / n1))
     sk = None
     if args.sklearn_trees > 0:
         from sklearn.ensemble import RandomForestClassifier

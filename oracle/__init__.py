@@ -83,6 +83,10 @@ class Oracle:
                                 ctypes.c_int32, _u32p, _i64p, ctypes.POINTER(_Params),
                                 ctypes.POINTER(_Forest)]
         lib.orc_fit.restype = ctypes.c_int
+        lib.orc_fit_threads.argtypes = [_f32p, _u32p, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, _u32p,
+                                        ctypes.c_int32, _u32p, _i64p, ctypes.POINTER(_Params), ctypes.c_int32,
+                                        ctypes.POINTER(_Forest)]
+        lib.orc_fit_threads.restype = ctypes.c_int
         lib.orc_forest_free.argtypes = [ctypes.POINTER(_Forest)]
         lib.orc_forest_free.restype = None
         lib.orc_predict.argtypes = [ctypes.POINTER(_Forest), _f32p, ctypes.c_int64, ctypes.c_int32,
@@ -138,7 +142,7 @@ class Oracle:
 
     # -- forest -----------------------------------------------------------
     def fit(self, X, y, K, n_trees, max_features, boot_ids=None, seed=0, min_samples_split=1,
-            max_depth=0, identity_perm=False, sorted_idx=None):
+            max_depth=0, identity_perm=False, sorted_idx=None, threads=1):
         """Fit on row-major X (N x F) and compact labels y; returns a dict of
         flat node arrays plus 'tree_off'.  boot_ids: list of per-tree sorted
         unique id arrays, or None for bootstrap=False."""
@@ -159,8 +163,8 @@ class Oracle:
             sorted_idx = np.ascontiguousarray(sorted_idx, dtype=np.uint32)
             sp = _ptr(sorted_idx, _u32p)
         fo = _Forest()
-        rc = self.lib.orc_fit(_ptr(X_T, _f32p), _ptr(y, _u32p), N, F, K, sp, n_trees, bptr, optr,
-                              ctypes.byref(prm), ctypes.byref(fo))
+        rc = self.lib.orc_fit_threads(_ptr(X_T, _f32p), _ptr(y, _u32p), N, F, K, sp, n_trees, bptr, optr,
+                                      ctypes.byref(prm), int(threads), ctypes.byref(fo))
         if rc:
             raise ValueError("orc_fit failed with status %d" % rc)
         try:

@@ -36,6 +36,8 @@
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
+#include <pthread.h>
+#include <stdint.h>
 
 #ifndef ORC_MUTANT
 #define ORC_MUTANT 0
@@ -639,9 +641,87 @@ done:
   return rc;
 }
 
+/* One tree (tree index t) appended to `out`. */
+static int fit_one_tree(const float *X_T, const uint32_t *y, int32_t N, int32_t F, int32_t K, const orc_params *prm,
+                        const uint32_t *sorted, int32_t t, const uint32_t *boot_ids, const int64_t *boot_off,
+                        orc_forest *out) {
+  tree_ctx tc;
+  memset(&tc, 0, sizeof(tc));
+  tc.X_T = X_T;
+  tc.y = y;
+  tc.N = N;
+  tc.F = F;
+  tc.K = K;
+  tc.prm = prm;
+  tc.tree = (uint32_t)t;
+  const uint32_t *ids = boot_ids ? boot_ids + boot_off[t] : NULL;
+  int64_t n_ids = boot_ids ? boot_off[t + 1] - boot_off[t] : 0;
+  int rc = build_tree(&tc, sorted, ids, n_ids, out);
+  free_tree_ctx(&tc);
+  return rc;
+}
+
+/* Appends every tree of `src` to `dst`. */
+static int append_forest(orc_forest *dst, const orc_forest *src) {
+  const int64_t base = dst->tree_off[dst->n_trees], n = src->tree_off[src->n_trees], total = base + n;
+#define APPEND2(field, T_)                                                     \
+  do {                                                                         \
+    void *q = realloc(dst->field, sizeof(T_) * (size_t)(total ? total : 1));   \
+    if (!q) return ORC_ENOMEM;                                                 \
+    dst->field = (T_ *)q;                                                      \
+    memcpy(dst->field + base, src->field, sizeof(T_) * (size_t)n);             \
+  } while (0)
+  APPEND2(feature, int32_t);
+  APPEND2(threshold, float);
+  APPEND2(left, uint32_t);
+  APPEND2(right, uint32_t);
+  APPEND2(value, uint32_t);
+  APPEND2(n_samples, uint32_t);
+  APPEND2(depth, uint32_t);
+#undef APPEND2
+  for (int32_t t = 0; t < src->n_trees; ++t) dst->tree_off[dst->n_trees + t + 1] = base + src->tree_off[t + 1];
+  dst->n_trees += src->n_trees;
+  return ORC_OK;
+}
+
+typedef struct {
+  const float *X_T;
+  const uint32_t *y;
+  int32_t N, F, K;
+  const orc_params *prm;
+  const uint32_t *sorted;
+  int32_t T;
+  const uint32_t *boot_ids;
+  const int64_t *boot_off;
+  orc_forest *parts; /* one forest per tree */
+  int32_t next;      /* next tree to grow */
+  pthread_mutex_t mu;
+} fit_job;
+
+static void *fit_worker(void *arg) {
+  fit_job *j = (fit_job *)arg;
+  for (;;) {
+    pthread_mutex_lock(&j->mu);
+    const int32_t t = j->next < j->T ? j->next++ : -1;
+    pthread_mutex_unlock(&j->mu);
+    if (t < 0) return NULL;
+    orc_forest *part = &j->parts[t];
+    part->tree_off = (int64_t *)calloc(2, sizeof(int64_t));
+    if (!part->tree_off) return (void *)(intptr_t)ORC_ENOMEM;
+    int rc = fit_one_tree(j->X_T, j->y, j->N, j->F, j->K, j->prm, j->sorted, t, j->boot_ids, j->boot_off, part);
+    if (rc != ORC_OK) return (void *)(intptr_t)rc;
+  }
+}
+
 int orc_fit(const float *X_T, const uint32_t *y, int32_t N, int32_t F, int32_t K, const uint32_t *sorted,
             int32_t T, const uint32_t *boot_ids, const int64_t *boot_off, const orc_params *prm,
             orc_forest *out) {
+  return orc_fit_threads(X_T, y, N, F, K, sorted, T, boot_ids, boot_off, prm, 1, out);
+}
+
+int orc_fit_threads(const float *X_T, const uint32_t *y, int32_t N, int32_t F, int32_t K, const uint32_t *sorted,
+                    int32_t T, const uint32_t *boot_ids, const int64_t *boot_off, const orc_params *prm,
+                    int32_t threads, orc_forest *out) {
   memset(out, 0, sizeof(*out));
   if (N <= 0 || N > ORC_MAX_SAMPLES || F <= 0 || K <= 0 || T <= 0 || !prm) return ORC_EINVAL;
   if (prm->max_features <= 0 || prm->max_features > F) return ORC_EINVAL;
@@ -678,20 +758,34 @@ int orc_fit(const float *X_T, const uint32_t *y, int32_t N, int32_t F, int32_t K
     return ORC_ENOMEM;
   }
   int rc = ORC_OK;
-  for (int32_t t = 0; t < T && rc == ORC_OK; ++t) {
-    tree_ctx tc;
-    memset(&tc, 0, sizeof(tc));
-    tc.X_T = X_T;
-    tc.y = y;
-    tc.N = N;
-    tc.F = F;
-    tc.K = K;
-    tc.prm = prm;
-    tc.tree = (uint32_t)t;
-    const uint32_t *ids = boot_ids ? boot_ids + boot_off[t] : NULL;
-    int64_t n_ids = boot_ids ? boot_off[t + 1] - boot_off[t] : 0;
-    rc = build_tree(&tc, sorted, ids, n_ids, out);
-    free_tree_ctx(&tc);
+  if (threads <= 1 || T == 1) {
+    for (int32_t t = 0; t < T && rc == ORC_OK; ++t) rc = fit_one_tree(X_T, y, N, F, K, prm, sorted, t, boot_ids,
+                                                                   boot_off, out);
+  } else {
+    /* trees are independent: each worker grows its share into forests of
+     * its own, appended in tree order afterwards */
+    fit_job job = {X_T, y, N, F, K, prm, sorted, T, boot_ids, boot_off, NULL, 0, PTHREAD_MUTEX_INITIALIZER};
+    job.parts = (orc_forest *)calloc((size_t)T, sizeof(orc_forest));
+    pthread_t *th = (pthread_t *)calloc((size_t)threads, sizeof(pthread_t));
+    int32_t started = 0;
+    if (!job.parts || !th) rc = ORC_ENOMEM;
+    for (int32_t i = 0; i < threads && rc == ORC_OK; ++i) {
+      if (pthread_create(&th[i], NULL, fit_worker, &job) != 0) break;
+      ++started;
+    }
+    if (rc == ORC_OK && started == 0) fit_worker(&job); /* no thread to spare: this one */
+    for (int32_t i = 0; i < started; ++i) {
+      void *r = NULL;
+      pthread_join(th[i], &r);
+      if (r && rc == ORC_OK) rc = (int)(intptr_t)r;
+    }
+    for (int32_t t = 0; t < T && rc == ORC_OK && job.parts; ++t)
+      if (!job.parts[t].tree_off) rc = ORC_ENOMEM;
+    for (int32_t t = 0; t < T && rc == ORC_OK; ++t) rc = append_forest(out, &job.parts[t]);
+    if (job.parts)
+      for (int32_t t = 0; t < T; ++t) orc_forest_free(&job.parts[t]);
+    free(job.parts);
+    free(th);
   }
   free(own);
   if (rc != ORC_OK) orc_forest_free(out);

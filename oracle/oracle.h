@@ -99,6 +99,12 @@ int orc_scan_feature(const float *v, const uint32_t *lab, uint32_t n, uint32_t K
 int orc_fit(const float *X_T, const uint32_t *y, int32_t N, int32_t F, int32_t K,
             const uint32_t *sorted, int32_t T, const uint32_t *boot_ids,
             const int64_t *boot_off, const orc_params *prm, orc_forest *out);
+/* As orc_fit with the trees grown by `threads` host threads at once (each
+ * tree is independent; the forest is the same, trees in order). */
+int orc_fit_threads(const float *X_T, const uint32_t *y, int32_t N, int32_t F, int32_t K,
+                    const uint32_t *sorted, int32_t T, const uint32_t *boot_ids,
+                    const int64_t *boot_off, const orc_params *prm, int32_t threads,
+                    orc_forest *out);
 
 void orc_forest_free(orc_forest *f);
 

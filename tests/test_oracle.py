@@ -242,3 +242,19 @@ def _caught(mut, orc):
 @pytest.mark.parametrize("k", sorted(MUTANTS))
 def test_planted_mistake_is_caught(orc, mutant_libs, k):
     assert _caught(mutant_libs[k], orc), "planted mistake %d (%s) not caught" % (k, MUTANTS[k])
+
+
+def test_threaded_fit_equals_sequential():
+    """Trees grown by several host threads at once form the same forest, in
+    tree order, as trees grown one after another."""
+    import oracle
+    from cudatree_amd import datasets
+
+    X, y, _, _ = datasets.covtype_like(seed=4, n_train=3000, n_test=1)
+    orc = oracle.get()
+    ids = orc.bootstrap_forest_ids(11, 9, len(y))
+    a = orc.fit(X, y, 7, 9, 7, boot_ids=ids, seed=11)
+    for threads in (2, 4, 16):
+        b = orc.fit(X, y, 7, 9, 7, boot_ids=ids, seed=11, threads=threads)
+        for k in a:
+            assert np.array_equal(a[k], b[k]), (threads, k)
