@@ -233,15 +233,18 @@ def serving_bench(lib, X, y, Xt, args):
         starts = rng.integers(0, len(Xt) - largest, size=n_req)
         srv = lib.Server(coalesce=coalesce, fit_workers=1)
         fid = srv.add_forest(forest)
-        reqs = [Xt[starts[i]:starts[i] + sizes[i]] for i in range(n_req)]
+        # the gathered requests lie back to back in one array
+        Xall = np.concatenate([Xt[starts[i]:starts[i] + sizes[i]] for i in range(n_req)])
+        first = np.concatenate(([0], np.cumsum(sizes)))
         tickets = []
         t0 = time.perf_counter()
         for c in range(0, n_req, per_call):
-            tickets += srv.submit_many(fid, reqs[c:c + per_call])
+            e = min(c + per_call, n_req)
+            t, _ = srv.submit_rows(fid, Xall[first[c]:first[e]], sizes[c:e])
+            tickets.append(t)
         t_sub = time.perf_counter() - t0
-        for tk in tickets:
-            st, _ = srv.wait(tk)
-            assert st == 0
+        st = srv.wait_many(np.concatenate(tickets))
+        assert (st == 0).all()
         t_all = time.perf_counter() - t0
         s = srv.stats()
         srv.close()
@@ -250,7 +253,8 @@ def serving_bench(lib, X, y, Xt, args):
                 "p50_ms": round(s["p50_ms"], 3), "p99_ms": round(s["p99_ms"], 3), "launches": s["predict_launches"]}
 
     if args.serving_burst > 0:
-        e = {"request_rows": "log-uniform 1..64", "submit": "all at once, 1,000 requests per submit call"}
+        e = {"request_rows": "log-uniform 1..64",
+             "submit": "all at once, 1,000 requests per submit call (rows back to back), one wait for all"}
         for coalesce in (True, False):
             key = "coalesced" if coalesce else "per_request"
             e[key] = burst(args.serving_burst, 64, coalesce, 13)

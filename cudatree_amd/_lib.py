@@ -5,6 +5,7 @@ GPU, every call raises.  There is no CPU fallback.
 """
 import atexit
 import ctypes
+import itertools
 import os
 import threading
 import weakref
@@ -155,6 +156,7 @@ SIGNATURES = {
     "cta_server_submit_fit": (ctypes.c_int, [_vp, _f32p, _u32p, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
                                              ctypes.c_int32, ctypes.POINTER(FitOpts), _u64p]),
     "cta_server_wait": (ctypes.c_int, [_vp, ctypes.c_uint64, ctypes.c_int64, _i32p, _i32p]),
+    "cta_server_wait_many": (ctypes.c_int, [_vp, ctypes.c_int32, _u64p, ctypes.c_int64, _i32p, _i32p]),
     "cta_server_get_stats": (ctypes.c_int, [_vp, ctypes.POINTER(ServerStats)]),
     "cta_server_reset_stats": (ctypes.c_int, [_vp]),
     "cta_ctx_set_profiling": (ctypes.c_int, [_vp, ctypes.c_int]),
@@ -604,6 +606,45 @@ class Server:
                 self._keep[int(t[k])] = (Xs[k], outs[k])
         return [int(v) for v in t]
 
+    def submit_rows(self, fid, X, R):
+        """Queues len(R) requests whose rows lie back to back in X
+        (sum(R) x F): request i is the R[i] rows after the first R[0] + ... +
+        R[i-1].  Returns (tickets, labels): labels (sum(R) uint32) receives
+        every request's labels at the same offsets.  Both arrays are held
+        here until wait_many has reported every ticket of the call."""
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        R = np.ascontiguousarray(R, dtype=np.int64)
+        n = len(R)
+        if n == 0:
+            return np.zeros(0, np.uint64), np.zeros(0, np.uint32)
+        if R.min() < 1 or int(R.sum()) != X.shape[0]:
+            raise ValueError("R must hold positive counts summing to X.shape[0]")
+        F = X.shape[1]
+        labels = np.zeros(X.shape[0], np.uint32)
+        first = np.concatenate(([0], np.cumsum(R)[:-1])).astype(np.uint64)
+        xp = np.uint64(X.ctypes.data) + first * np.uint64(F * 4)
+        lp = np.uint64(labels.ctypes.data) + first * np.uint64(4)
+        t = np.zeros(n, np.uint64)
+        check_server(self.lib.cta_server_submit_predict_many(
+            self.h, fid, n, xp.ctypes.data_as(ctypes.POINTER(_f32p)), ptr(R, _i64p), F,
+            lp.ctypes.data_as(ctypes.POINTER(_u32p)), ptr(t, _u64p)))
+        held = (X, labels)  # every ticket of the call holds the arrays
+        with self._lock:
+            self._keep.update(zip(t.tolist(), itertools.repeat(held)))
+        return t, labels
+
+    def wait_many(self, tickets, timeout_us=-1):
+        """Statuses of tickets (uint64 array, in order) as wait reports them,
+        with one deadline (-1: still pending)."""
+        tickets = np.ascontiguousarray(tickets, dtype=np.uint64)
+        st = np.zeros(len(tickets), np.int32)
+        check_server(self.lib.cta_server_wait_many(self.h, len(tickets), ptr(tickets, _u64p), timeout_us,
+                                                   ptr(st, _i32p), None))
+        with self._lock:
+            for k in tickets[st >= 0].tolist():
+                self._keep.pop(k, None)
+        return st
+
     def submit_fit(self, X, y, K, n_trees, opts):
         X = np.ascontiguousarray(X, dtype=np.float32)
         y = np.ascontiguousarray(y, dtype=np.uint32)
@@ -613,8 +654,9 @@ class Server:
         return t.value
 
     def wait(self, ticket, timeout_us=-1):
-        """(status, result): result is the labels of a predict ticket or the
-        forest id of a fit ticket; status -1 means still pending."""
+        """(status, result): result is the labels of a predict ticket (for a
+        submit_rows ticket, the call's whole label array) or the forest id of
+        a fit ticket; status -1 means still pending."""
         st, fid = ctypes.c_int32(), ctypes.c_int32()
         check_server(self.lib.cta_server_wait(self.h, ticket, timeout_us, ctypes.byref(st), ctypes.byref(fid)))
         if st.value < 0:

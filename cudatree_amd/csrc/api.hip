@@ -994,7 +994,6 @@ struct Fitter {
     fill(a, q);
     {
       Timed tm(ctx, "k_children");
-      k_children<IdxT, LabT><<<G, kBlock, sm, st>>>(a);
       offsets<kChFields>(a, q, &a.ctl->c_ids);
       k_children_emit<IdxT, LabT><<<G, kBlock, 0, st>>>(a);
     }
@@ -2210,7 +2209,8 @@ static void hook_split_scan(cta_ctx *ctx, cta_dataset *ds, const uint32_t *rows,
   DevMem d_rows(hrows.size() * sizeof(IdxT)), d_front(sizeof(FNode)), d_cnt(K * 4), d_aux(sizeof(NAux)),
       d_feats(std::max(nf, 1u) * 4), d_tasks(tasks.size() * sizeof(Task)),
       d_res(std::max(ntask, 1u) * sizeof(ScanRes)), d_hist((size_t)std::max(nf, 1u) * nch * K * 4), d_ctl(sizeof(Ctl)),
-      d_cntL(K * 4), d_ar(7 * 4 * 4), d_fill(((size_t)ax.nch_f + 1) * sizeof(Task)), d_bslot(nbslot * 8);
+      d_cntL(K * 4), d_ar(7 * 4 * 4), d_fill(((size_t)ax.nch_f + 1) * sizeof(Task)), d_bslot(nbslot * 8),
+      d_vals(kChFields * 4);
   k_fill_u32<<<grid_for(nbslot * 2), 256, 0, st>>>(d_bslot.as<uint32_t>(), nbslot * 2, 0u);
   HIPC(hipMemcpyAsync(d_rows.p, hrows.data(), hrows.size() * sizeof(IdxT), hipMemcpyHostToDevice, st));
   HIPC(hipMemcpyAsync(d_front.p, &nd, sizeof nd, hipMemcpyHostToDevice, st));
@@ -2247,6 +2247,7 @@ static void hook_split_scan(cta_ctx *ctx, cta_dataset *ds, const uint32_t *rows,
   a.hist = d_hist.as<uint32_t>();
   a.cntL = d_cntL.as<uint32_t>();
   a.fill_tasks = d_fill.as<Task>();
+  a.vals = d_vals.as<uint32_t>();  // the children-scan counts k_decide writes
   a.bslot = d_bslot.as<unsigned long long>();
   uint32_t *ar = d_ar.as<uint32_t>();
   a.arena = Arena{(int32_t *)ar, (float *)(ar + 4), ar + 8, ar + 12, ar + 16, ar + 20, ar + 24};
@@ -2319,7 +2320,7 @@ static void hook_reshuffle(cta_ctx *ctx, cta_dataset *ds, const uint32_t *rows, 
     if (mark[s]) hbits[s >> 5] |= 1u << (s & 31);
   uint32_t nL = 0;
   for (uint32_t i = 0; i < n; ++i) nL += mark[rows[i]] ? 1 : 0;
-  // the counting step of k_children for one node of tree 0, whose frontier
+  // the children-scan counts of k_decide for one node of tree 0, whose frontier
   // range is [0, 1) and whose tasks start at offset 0
   NAux ax = {};
   ax.nL = nL;

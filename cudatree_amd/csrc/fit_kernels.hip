@@ -20,12 +20,12 @@
 //   k_scan        per task: Gini scan of the chunk, best position
 //                                                (dfs_module.cu:310-443, bfs_module.cu:65-203)
 //   k_decide      per node: best split over tasks, threshold, left-child
-//                 class counts, fill tasks      (dfs_module.cu:446-491, 158-208, bfs_module.cu:206-235, 379-401)
+//                 class counts, fill tasks      (dfs_module.cu:446-491, 158-208, bfs_module.cu:206-235, 379-401),
+//                 leaf tests of both children (random_tree.py:71-165), child
+//                 and reshuffle counts
 //   k_fill_lds    per tree: side bitmap built in LDS
 //                                                (dfs_module.cu:210-221, bfs_module.cu:238-264)
-//   k_children    per split node: leaf tests (random_tree.py:71-165), child
-//                 and reshuffle counts; k_children_emit: ids, next frontier,
-//                 reshuffle tasks
+//   k_children_emit per split node: ids, next frontier, reshuffle tasks
 //   k_plan / k_resh
 //                 stable partition of every row (dfs_module.cu:224-307, bfs_module.cu:266-377)
 //
@@ -812,8 +812,70 @@ __global__ __launch_bounds__(kBlock, 6) void k_scan(FitArgs<IdxT, LabT> a) {
 }
 
 // ------------------------------------------------------------------------
+// Reshuffle geometry.  Rows of a node with at least kBigN samples are moved
+// by whole 1,024-thread workgroups (one task per row, kReshTile-element
+// tiles); smaller nodes by single waves (one task per group of rows holding
+// about Cr elements).
+// ------------------------------------------------------------------------
+constexpr uint32_t kBigN = 4096;
+constexpr int kReshThreads = 1024;
+constexpr int kReshWaves = kReshThreads / kWave;  // 16
+constexpr int kReshE = 8;  // elements per lane in flight in a small task
+// Big tasks: elements per thread per tile, and whether the next tile's
+// loads are issued into a second register set before this tile's stores.
+#ifndef CTA_RESH_BIG_E
+#define CTA_RESH_BIG_E 8
+#endif
+#ifndef CTA_RESH_PIPE
+#define CTA_RESH_PIPE 1
+#endif
+constexpr int kBigE = CTA_RESH_BIG_E;
+constexpr bool kBigPipe = CTA_RESH_PIPE != 0;
+constexpr uint32_t kReshTile = kReshThreads * kBigE;
+constexpr uint32_t kSlots = kBigE * kReshWaves;  // (sub-tile, wave) counts per tile
+constexpr uint32_t kSpl = kSlots / kWave;        // of them per lane in the tile prefix
+static_assert(kSlots % kWave == 0 && kReshWaves % kSpl == 0, "the tile prefix takes whole slots per lane");
+constexpr uint64_t kWgElemsMax = 1ull << 19, kWgElemsMin = 1ull << 16;
+
+// Label of the majority class (lowest label on ties).
+__device__ uint32_t wave_majority(const uint32_t *c, uint32_t K) {
+  const uint32_t lane = lane_id();
+  uint32_t bv = 0, bk = 0xFFFFFFFFu;
+  for (uint32_t k = lane; k < K; k += kWave)
+    if (bk == 0xFFFFFFFFu || c[k] > bv) {
+      bv = c[k];
+      bk = k;
+    }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    uint32_t ov = __shfl_xor(bv, o, kWave), ok = __shfl_xor(bk, o, kWave);
+    if (ok != 0xFFFFFFFFu && (bk == 0xFFFFFFFFu || ov > bv || (ov == bv && ok < bk))) {
+      bv = ov;
+      bk = ok;
+    }
+  }
+  return bk;
+}
+
+// Reshuffle task shape of a node of n samples with a live child; fills
+// v[2..4] (big tasks, small tasks, big tiles) of its children-scan counts.
+__device__ __forceinline__ void resh_shape(uint32_t n, uint32_t F, uint32_t Cr, NAux &ax, uint32_t *v) {
+  const bool big = n >= kBigN;
+  const uint32_t rpt = big ? 1u : max(1u, Cr / n);
+  const uint32_t ntask = big ? F : ceil_div(F, rpt);
+  ax.resh_big = big ? 1u : 0u;
+  ax.rpt = rpt;
+  ax.resh_ntask = ntask;
+  v[2] = big ? F : 0u;
+  v[3] = big ? 0u : ntask;
+  v[4] = big ? F * ceil_div(n, kReshTile) : 0u;
+}
+
+// ------------------------------------------------------------------------
 // k_decide: best split of each node over its tasks, threshold, class counts
-// of the left child and the node's bitmap-fill tasks.
+// of the left child and the node's bitmap-fill tasks; for a split node, the
+// leaf tests of both children (random_tree.py:123-163) and its
+// children-scan counts: child ids, next-frontier slots, reshuffle tasks.
 // ------------------------------------------------------------------------
 
 // Class counts of positions [0, p] of row f of node i into cntL.  The chunk
@@ -891,7 +953,7 @@ __global__ __launch_bounds__(kBlock) void k_decide(FitArgs<IdxT, LabT> a) {
   extern __shared__ uint32_t smem[];
   const uint32_t lane = lane_id();
   uint32_t *h = smem + smem_wave_offset(a.K);
-  unsigned long long bytes = 0;
+  unsigned long long bytes = 0, bytes_resh = 0, bytes_resh_small = 0;
   WAVE_LOOP(a.ctl->q) {
     const uint32_t i = item;
     const FNode nd = a.front[i];
@@ -962,26 +1024,81 @@ __global__ __launch_bounds__(kBlock) void k_decide(FitArgs<IdxT, LabT> a) {
       ax.flags = kSplit | (2 * ax.nL > nd.n ? kInvert : 0u);
       ax.dst = (nd.src == 2) ? 1u : 2u;
     }
-    if ((ax.flags & kSplit) && a.Frl) {
-      // a live child needs row-free counts: the fill tasks count the smaller
-      // side's samples, the ones they mark, into rl_small (from zero)
+    uint32_t v[kChFields] = {0u, 0u, 0u, 0u, 0u, 0u};  // children-scan counts
+    if (ax.flags & kSplit) {
+      // both children: class counts (lane k: class k), how many classes
+      // each holds and the single present one (random_tree.py:123-163)
       const uint32_t K = (uint32_t)a.K, nL = ax.nL, nR = nd.n - ax.nL;
-      uint32_t nzl = 0, nzr = 0;
+      uint32_t *cl = h, *cr = h + K;
+      uint32_t nzl = 0, nzr = 0, onlyl = 0, onlyr = 0;
+      wave_sync();
       for (uint32_t k = lane; k < K; k += kWave) {
         const uint32_t l = a.cntL[(size_t)i * K + k];  // written by this lane above
-        nzl += l ? 1u : 0u;
-        nzr += (a.cnt[(size_t)i * K + k] - l) ? 1u : 0u;
+        const uint32_t r = a.cnt[(size_t)i * K + k] - l;
+        cl[k] = l;
+        cr[k] = r;
+        if (l) {
+          ++nzl;
+          onlyl = k;
+        }
+        if (r) {
+          ++nzr;
+          onlyr = k;
+        }
       }
       nzl = uniform(wave_sum32(nzl));
       nzr = uniform(wave_sum32(nzr));
-      // (children queued for the small subtrees need none)
-      if ((child_kind(a, nzl, nL, nd.depth) == 0u && nL > a.sub_max) ||
-          (child_kind(a, nzr, nR, nd.depth) == 0u && nR > a.sub_max)) {
+      // the single present class (max over lanes: absent lanes hold 0)
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) {
+        onlyl = max(onlyl, (uint32_t)__shfl_xor(onlyl, o, kWave));
+        onlyr = max(onlyr, (uint32_t)__shfl_xor(onlyr, o, kWave));
+      }
+      wave_sync();
+      const uint32_t kl = child_kind(a, nzl, nL, nd.depth), kr = child_kind(a, nzr, nR, nd.depth);
+      uint32_t labl = onlyl, labr = onlyr;
+      if (kl == 3) labl = wave_majority(cl, K);
+      if (kr == 3) labr = wave_majority(cr, K);
+      // a leaf labelled by its first sample needs the side bits: k_children_emit
+      // looks it up after the fill
+      if (kl == 2) labl = kFirstSample;
+      if (kr == 2) labr = kFirstSample;
+      ax.lab[0] = kl ? labl : kLeaf;
+      ax.lab[1] = kr ? labr : kLeaf;
+      if (kl == 0) ax.flags |= kLiveL;
+      if (kr == 0) ax.flags |= kLiveR;
+      v[0] = 2;
+      // a live child of at most sub_max samples goes to the small-subtree
+      // queue, a larger one to the next frontier
+      const uint32_t subl = (kl == 0 && nL <= a.sub_max) ? 1u : 0u, subr = (kr == 0 && nR <= a.sub_max) ? 1u : 0u;
+      v[1] = (kl == 0 ? 1u : 0u) + (kr == 0 ? 1u : 0u) - subl - subr;
+      v[5] = subl + subr;
+      ax.resh_ntask = 0;
+      if (kl == 0 || kr == 0) {
+        resh_shape(nd.n, a.Fr, a.Cr, ax, v);
+        // every row read and written, the mark gathered per element: Fr n (2b + 1)
+        if (lane == 0) {
+          const unsigned long long b = (unsigned long long)a.Fr * nd.n * (2 * sizeof(IdxT) + 1);
+          if (ax.resh_big) bytes_resh += b;
+          else bytes_resh_small += b;
+        }
+      }
+      // a live child that stays in the levels needs row-free counts: the
+      // fill tasks count the smaller side's samples, the ones they mark,
+      // into rl_small (from zero)
+      if (a.Frl && ((kl == 0 && nL > a.sub_max) || (kr == 0 && nR > a.sub_max))) {
         ax.flags |= kCountRl;
-        for (uint32_t w = lane; w < a.Frl * K; w += kWave) a.rl_small[(size_t)i * a.Frl * K + w] = 0u;
+        // (a node of one fill task gets its counts stored whole by that task)
+        if (nfill > a.Cf)
+          for (uint32_t w = lane; w < a.Frl * K; w += kWave) a.rl_small[(size_t)i * a.Frl * K + w] = 0u;
         // signature words of the smaller side
         if (lane == 0) bytes += (unsigned long long)min(nL, nR) * 4 * a.Wsig;
       }
+    }
+    if (lane == 0) {
+      uint32_t *o = a.vals + (size_t)i * kChFields;
+#pragma unroll
+      for (int f = 0; f < kChFields; ++f) o[f] = v[f];
     }
     nfill = ceil_div(nfill, a.Cf);
     // the node's fill slots, one per Cf positions of its rows: the side bits
@@ -992,6 +1109,8 @@ __global__ __launch_bounds__(kBlock) void k_decide(FitArgs<IdxT, LabT> a) {
     if (lane == 0) a.aux[i] = ax;
   }
   flush_bytes(a, kBytesFill, bytes);
+  flush_bytes(a, kBytesResh, bytes_resh);
+  flush_bytes(a, kBytesReshSmall, bytes_resh_small);
 }
 
 // ------------------------------------------------------------------------
@@ -1046,17 +1165,26 @@ __device__ __forceinline__ void rl_add(const FitArgs<IdxT, LabT> &a, const uint3
 constexpr int kFillU = 8;
 
 // Adds the wave's row-free counters (node `node`'s) to rl_small and clears
-// them.
+// them.  `whole`: this wave counted the node's only fill task, so its
+// counters are the node's counts and are stored outright (rl_small was not
+// cleared for such a node); otherwise the non-zero ones are added.
 template <class IdxT, class LabT>
-__device__ __forceinline__ void rl_flush(const FitArgs<IdxT, LabT> &a, uint32_t node, uint32_t *cnt) {
+__device__ __forceinline__ void rl_flush(const FitArgs<IdxT, LabT> &a, uint32_t node, uint32_t *cnt, bool whole) {
   const uint32_t lane = lane_id(), FK = a.Frl * (uint32_t)a.K;
   wave_sync();
   uint32_t *dst = a.rl_small + (size_t)node * FK;
-  for (uint32_t w = lane; w < FK; w += kWave)
-    if (cnt[w]) {
-      atomicAdd(&dst[w], cnt[w]);
+  if (whole) {
+    for (uint32_t w = lane; w < FK; w += kWave) {
+      dst[w] = cnt[w];
       cnt[w] = 0;
     }
+  } else {
+    for (uint32_t w = lane; w < FK; w += kWave)
+      if (cnt[w]) {
+        atomicAdd(&dst[w], cnt[w]);
+        cnt[w] = 0;
+      }
+  }
   wave_sync();
 }
 
@@ -1074,7 +1202,7 @@ __device__ __forceinline__ void rl_zero(uint32_t *cnt, uint32_t FK) {
 // end.
 template <class IdxT, class LabT>
 __device__ __forceinline__ void fill_task_bits(const FitArgs<IdxT, LabT> &a, const Task &tk, uint32_t *bits,
-                                               uint32_t *cnt, uint32_t &cur) {
+                                               uint32_t *cnt, uint32_t &cur, bool &cur_whole) {
   const uint32_t lane = lane_id();
   const FNode nd = a.front[tk.node];
   const NAux ax = a.aux[tk.node];
@@ -1082,8 +1210,9 @@ __device__ __forceinline__ void fill_task_bits(const FitArgs<IdxT, LabT> &a, con
   const bool count = (ax.flags & kCountRl) != 0u;
   const bool small_left = ax.nL <= nd.n - ax.nL;
   if (count && tk.node != cur) {
-    if (cur != kLeaf) rl_flush(a, cur, cnt);
+    if (cur != kLeaf) rl_flush(a, cur, cnt, cur_whole);
     cur = tk.node;
+    cur_whole = tk.nch == 1;
   }
   if (jr >= 0) {
     // row-free split: the samples holding the lower value, over feature 0's row
@@ -1167,12 +1296,13 @@ __global__ __launch_bounds__(kFillThreads) void k_fill_lds(FitArgs<IdxT, LabT> a
   uint32_t *cnt = lbits + W + wave * a.Frl * (uint32_t)a.K;
   rl_zero(cnt, a.Frl * (uint32_t)a.K);
   uint32_t cur = kLeaf;
+  bool cur_whole = false;
   for (uint32_t k = fb + j * kFillWaves + wave; k < fe; k += Je * kFillWaves) {
     const Task tk = a.fill_tasks[k];
     if (tk.node == kLeaf || tk.chunk >= tk.nch) continue;
-    fill_task_bits(a, tk, lbits, cnt, cur);
+    fill_task_bits(a, tk, lbits, cnt, cur, cur_whole);
   }
-  if (cur != kLeaf) rl_flush(a, cur, cnt);
+  if (cur != kLeaf) rl_flush(a, cur, cnt, cur_whole);
   __syncthreads();
   uint32_t *out = (Je == 1) ? a.bits + (size_t)t * W : part + ((size_t)t * J + j) * W;
   for (uint32_t w = tid; w < W; w += kFillThreads) out[w] = lbits[w];
@@ -1206,171 +1336,17 @@ __global__ __launch_bounds__(kBlock) void k_fill_global(FitArgs<IdxT, LabT> a) {
   uint32_t *cnt = smem + (threadIdx.x / kWave) * a.Frl * (uint32_t)a.K;
   rl_zero(cnt, a.Frl * (uint32_t)a.K);
   uint32_t cur = kLeaf;
+  bool cur_whole = false;
   WAVE_LOOP(a.ctl->n_fill) {
     const Task tk = a.fill_tasks[item];
     if (tk.node == kLeaf || tk.chunk >= tk.nch) continue;
-    fill_task_bits(a, tk, a.bits + (size_t)a.front[tk.node].tree * a.W, cnt, cur);
+    fill_task_bits(a, tk, a.bits + (size_t)a.front[tk.node].tree * a.W, cnt, cur, cur_whole);
   }
-  if (cur != kLeaf) rl_flush(a, cur, cnt);
+  if (cur != kLeaf) rl_flush(a, cur, cnt, cur_whole);
 }
 
-// ------------------------------------------------------------------------
-// Reshuffle geometry.  Rows of a node with at least kBigN samples are moved
-// by whole 1,024-thread workgroups (one task per row, kReshTile-element
-// tiles); smaller nodes by single waves (one task per group of rows holding
-// about Cr elements).
-// ------------------------------------------------------------------------
-constexpr uint32_t kBigN = 4096;
-constexpr int kReshThreads = 1024;
-constexpr int kReshWaves = kReshThreads / kWave;  // 16
-constexpr int kReshE = 8;                         // elements per thread per tile
-constexpr uint32_t kReshTile = kReshThreads * kReshE;
-constexpr uint32_t kSlots = kReshE * kReshWaves;  // 128 (sub-tile, wave) counts per tile
-static_assert(kSlots == 2 * kWave, "the tile prefix takes two slots per lane");
-constexpr uint64_t kWgElemsMax = 1ull << 19, kWgElemsMin = 1ull << 16;
 
-// Label of the majority class (lowest label on ties).
-__device__ uint32_t wave_majority(const uint32_t *c, uint32_t K) {
-  const uint32_t lane = lane_id();
-  uint32_t bv = 0, bk = 0xFFFFFFFFu;
-  for (uint32_t k = lane; k < K; k += kWave)
-    if (bk == 0xFFFFFFFFu || c[k] > bv) {
-      bv = c[k];
-      bk = k;
-    }
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) {
-    uint32_t ov = __shfl_xor(bv, o, kWave), ok = __shfl_xor(bk, o, kWave);
-    if (ok != 0xFFFFFFFFu && (bk == 0xFFFFFFFFu || ov > bv || (ov == bv && ok < bk))) {
-      bv = ov;
-      bk = ok;
-    }
-  }
-  return bk;
-}
 
-// Reshuffle task shape of a node of n samples with a live child; fills
-// v[2..4] (big tasks, small tasks, big tiles) of its children-scan counts.
-__device__ __forceinline__ void resh_shape(uint32_t n, uint32_t F, uint32_t Cr, NAux &ax, uint32_t *v) {
-  const bool big = n >= kBigN;
-  const uint32_t rpt = big ? 1u : max(1u, Cr / n);
-  const uint32_t ntask = big ? F : ceil_div(F, rpt);
-  ax.resh_big = big ? 1u : 0u;
-  ax.rpt = rpt;
-  ax.resh_ntask = ntask;
-  v[2] = big ? F : 0u;
-  v[3] = big ? 0u : ntask;
-  v[4] = big ? F * ceil_div(n, kReshTile) : 0u;
-}
-
-// ------------------------------------------------------------------------
-// k_children: leaf tests of both children of every split node
-// (random_tree.py:123-163) and the node's children-scan counts: child ids,
-// next-frontier slots, reshuffle tasks.
-// ------------------------------------------------------------------------
-template <class IdxT, class LabT>
-__global__ __launch_bounds__(kBlock) void k_children(FitArgs<IdxT, LabT> a) {
-  if (a.ctl->err) return;  // a list overflowed: the fit fails, nothing more is read
-  extern __shared__ uint32_t smem[];
-  const uint32_t lane = lane_id();
-  const uint32_t K = (uint32_t)a.K;
-  uint32_t *cl = smem + smem_wave_offset(a.K);
-  uint32_t *cr = cl + K;
-  unsigned long long bytes = 0, bytes_small = 0;
-  WAVE_LOOP(a.ctl->q) {
-    const uint32_t i = item;
-    NAux ax = a.aux[i];
-    uint32_t v[kChFields] = {0u, 0u, 0u, 0u, 0u};
-    if (ax.flags & kSplit) {
-      const FNode nd = a.front[i];
-      const uint32_t nL = ax.nL, nR = nd.n - ax.nL;
-      uint32_t nzl = 0, nzr = 0, onlyl = 0, onlyr = 0;
-      wave_sync();
-      for (uint32_t k = lane; k < K; k += kWave) {
-        const uint32_t l = a.cntL[(size_t)i * K + k];
-        const uint32_t r = a.cnt[(size_t)i * K + k] - l;
-        cl[k] = l;
-        cr[k] = r;
-        if (l) {
-          ++nzl;
-          onlyl = k;
-        }
-        if (r) {
-          ++nzr;
-          onlyr = k;
-        }
-      }
-      nzl = uniform(wave_sum32(nzl));
-      nzr = uniform(wave_sum32(nzr));
-      // the single present class (max over lanes: absent lanes hold 0)
-      uint32_t ol = onlyl, orr = onlyr;
-#pragma unroll
-      for (int o = 32; o > 0; o >>= 1) {
-        ol = max(ol, (uint32_t)__shfl_xor(ol, o, kWave));
-        orr = max(orr, (uint32_t)__shfl_xor(orr, o, kWave));
-      }
-      wave_sync();
-      const uint32_t kl = child_kind(a, nzl, nL, nd.depth), kr = child_kind(a, nzr, nR, nd.depth);
-      uint32_t labl = ol, labr = orr;
-      if (kl == 3) labl = wave_majority(cl, K);
-      if (kr == 3) labr = wave_majority(cr, K);
-      if (kl == 2 || kr == 2) {
-        // first sample of each child in feature 0's row: the first marked
-        // (left) and first unmarked (right) sample of the parent's segment
-        const IdxT *row0 = rows_of(a, nd.src, nd.tree, 0) + nd.start;
-        const uint32_t *bits = a.bits + (size_t)nd.tree * a.W;
-        const uint32_t inv = (ax.flags & kInvert) ? 1u : 0u;  // the marks are on the right
-        bool needl = kl == 2, needr = kr == 2;
-        for (uint32_t pb = 0; pb < nd.n && (needl || needr); pb += kWave) {
-          const uint32_t p = pb + lane;
-          uint32_t e = 0, side = 2;
-          if (p < nd.n) {
-            e = (uint32_t)row0[p];
-            const uint32_t s = sample_of((IdxT)e);
-            side = ((bits[s >> 5] >> (s & 31u)) & 1u) ^ inv;
-          }
-          const uint64_t ml = ballot(side == 1), mr = ballot(side == 0);
-          if (needl && ml) {
-            labl = a.label((IdxT)readlane(e, (uint32_t)__ffsll((unsigned long long)ml) - 1u));
-            needl = false;
-          }
-          if (needr && mr) {
-            labr = a.label((IdxT)readlane(e, (uint32_t)__ffsll((unsigned long long)mr) - 1u));
-            needr = false;
-          }
-        }
-      }
-      ax.lab[0] = kl ? labl : kLeaf;
-      ax.lab[1] = kr ? labr : kLeaf;
-      uint32_t flags = kSplit | (ax.flags & kInvert);
-      if (kl == 0) flags |= kLiveL;
-      if (kr == 0) flags |= kLiveR;
-      ax.flags = flags;
-      v[0] = 2;
-      // a live child of at most sub_max samples goes to the small-subtree
-      // queue, a larger one to the next frontier
-      const uint32_t subl = (kl == 0 && nL <= a.sub_max) ? 1u : 0u, subr = (kr == 0 && nR <= a.sub_max) ? 1u : 0u;
-      v[1] = (kl == 0 ? 1u : 0u) + (kr == 0 ? 1u : 0u) - subl - subr;
-      v[5] = subl + subr;
-      ax.resh_ntask = 0;
-      if (kl == 0 || kr == 0) {
-        resh_shape(nd.n, a.Fr, a.Cr, ax, v);
-        // every row read and written, the mark gathered per element: Fr n (2b + 1)
-        const unsigned long long b = (unsigned long long)a.Fr * nd.n * (2 * sizeof(IdxT) + 1);
-        if (ax.resh_big) bytes += b;
-        else bytes_small += b;
-      }
-    }
-    if (lane == 0) {
-      uint32_t *o = a.vals + (size_t)i * kChFields;
-#pragma unroll
-      for (int f = 0; f < kChFields; ++f) o[f] = v[f];
-      a.aux[i] = ax;
-    }
-  }
-  flush_bytes(a, kBytesResh, bytes);
-  flush_bytes(a, kBytesReshSmall, bytes_small);
-}
 
 // Reshuffle task descriptors of node i at its scanned offsets.
 template <class IdxT, class LabT>
@@ -1424,6 +1400,34 @@ __global__ __launch_bounds__(kBlock) void k_children_emit(FitArgs<IdxT, LabT> a)
       a.arena.left[nd.nid] = id0;
       a.arena.right[nd.nid] = id0 + 1;
     }
+    uint32_t lab[2] = {ax.lab[0], ax.lab[1]};
+    if (lab[0] == kFirstSample || lab[1] == kFirstSample) {
+      // leaves of fewer than min_samples_split samples: labelled by the
+      // first sample of each child in feature 0's row, the first marked and
+      // first unmarked sample of the parent's segment (random_tree.py:628-634)
+      const IdxT *row0 = rows_of(a, nd.src, nd.tree, 0) + nd.start;
+      const uint32_t *bits = a.bits + (size_t)nd.tree * a.W;
+      const uint32_t inv = (ax.flags & kInvert) ? 1u : 0u;  // the marks are on the right
+      bool needl = lab[0] == kFirstSample, needr = lab[1] == kFirstSample;
+      for (uint32_t pb = 0; pb < nd.n && (needl || needr); pb += kWave) {
+        const uint32_t p = pb + lane;
+        uint32_t e = 0, side = 2;
+        if (p < nd.n) {
+          e = (uint32_t)row0[p];
+          const uint32_t s = sample_of((IdxT)e);
+          side = ((bits[s >> 5] >> (s & 31u)) & 1u) ^ inv;
+        }
+        const uint64_t ml = ballot(side == 1), mr = ballot(side == 0);
+        if (needl && ml) {
+          lab[0] = a.label((IdxT)readlane(e, (uint32_t)__ffsll((unsigned long long)ml) - 1u));
+          needl = false;
+        }
+        if (needr && mr) {
+          lab[1] = a.label((IdxT)readlane(e, (uint32_t)__ffsll((unsigned long long)mr) - 1u));
+          needr = false;
+        }
+      }
+    }
     const uint32_t cd = nd.depth + 1;
     for (uint32_t c = 0; c < 2; ++c) {
       const uint32_t id = id0 + c;
@@ -1432,13 +1436,13 @@ __global__ __launch_bounds__(kBlock) void k_children_emit(FitArgs<IdxT, LabT> a)
         a.arena.nsamp[id] = cn;
         a.arena.depth[id] = cd;
       }
-      if (ax.lab[c] != kLeaf) {
+      if (lab[c] != kLeaf) {
         if (lane == 0) {
           a.arena.feat[id] = -1;
           a.arena.thr[id] = 0.f;
           a.arena.left[id] = kLeaf;
           a.arena.right[id] = kLeaf;
-          a.arena.value[id] = ax.lab[c];
+          a.arena.value[id] = lab[c];
         }
         continue;
       }
@@ -1735,57 +1739,76 @@ __device__ void resh_big(const FitArgs<IdxT, LabT> &a, const Task &tk, const Sid
   const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
   uint32_t carry = 0;
   // positions past n repeat the last element and are masked
-  auto load = [&](IdxT(&e)[kReshE], uint32_t base) {
+  auto load = [&](IdxT(&e)[kBigE], uint32_t base) {
 #pragma unroll
-    for (int k = 0; k < kReshE; ++k) e[k] = ld32(in, min(base + (uint32_t)k * kReshThreads + tid, n - 1));
+    for (int k = 0; k < kBigE; ++k) e[k] = ld32(in, min(base + (uint32_t)k * kReshThreads + tid, n - 1));
   };
-  // moves the tile at `base` held in e; loads the next tile into nx first
-  auto step = [&](IdxT(&e)[kReshE], IdxT(&nx)[kReshE], uint32_t base) -> bool {
+  // moves the tile at `base` held in e; with two register sets, loads the
+  // next tile into nx before this tile's stores, else after them into e
+  auto step = [&](IdxT(&e)[kBigE], IdxT(&nx)[kBigE], uint32_t base) -> bool {
+    // Every e[k] is a valid entry, so the side lookups are issued together,
+    // unconditionally, and masked afterwards (branch-free: a lookup under a
+    // condition would wait for its own load and stall the next tile's).
+    uint32_t sv[kBigE];
+#pragma unroll
+    for (int k = 0; k < kBigE; ++k) sv[k] = side(e[k]);
     uint32_t sdm = 0;  // bit k: element k goes left
 #pragma unroll
-    for (int k = 0; k < kReshE; ++k) {
+    for (int k = 0; k < kBigE; ++k) {
       const uint32_t p = base + (uint32_t)k * kReshThreads + tid;
-      sdm |= ((p < n) ? side(e[k]) ^ inv : 0u) << k;
+      sdm |= ((sv[k] ^ inv) & (p < n ? 1u : 0u)) << k;
     }
+    uint64_t mk[kBigE];
 #pragma unroll
-    for (int k = 0; k < kReshE; ++k) {
-      const uint64_t m = ballot((sdm >> k) & 1u);
-      if (lane == 0) wcnt[par * kSlots + k * kReshWaves + wave] = (uint32_t)__popcll(m);
+    for (int k = 0; k < kBigE; ++k) {
+      mk[k] = ballot((sdm >> k) & 1u);
+      if (lane == 0) wcnt[par * kSlots + k * kReshWaves + wave] = (uint32_t)__popcll(mk[k]);
     }
     __syncthreads();
-    // exclusive prefix over the 128 (sub-tile, wave) counts in position
-    // order, two slots per lane
-    const uint32_t v0 = wcnt[par * kSlots + 2 * lane], v1 = wcnt[par * kSlots + 2 * lane + 1];
-    const uint32_t inc = wave_incl_scan(v0 + v1);
-    const uint32_t e0 = inc - v0 - v1, e1 = e0 + v0;
-    const uint32_t total = readlane(inc, 63);
-    const uint32_t esel = (wave & 1u) ? e1 : e0;  // slot parity = wave parity
-    const bool more = base + kReshTile < n;
-    if (more) load(nx, base + kReshTile);
+    // exclusive prefix over the (sub-tile, wave) counts in position order,
+    // kSpl consecutive slots per lane; the slots of this wave's column
+    // (k * kReshWaves + wave) sit at index wave % kSpl of their lane
+    uint32_t v[kSpl], sum = 0;
 #pragma unroll
-    for (int k = 0; k < kReshE; ++k) {
-      const uint32_t slot = (uint32_t)k * kReshWaves + wave;
-      const uint32_t before = carry + readlane(esel, slot >> 1);
-      const uint32_t p = base + (uint32_t)k * kReshThreads + tid;
-      const uint64_t m = ballot((sdm >> k) & 1u);
-      if (p < n) {
-        const uint32_t lb = before + mbcnt(m);
-        if ((sdm >> k) & 1u) {
-          if (liveL) st32(out, lb, e[k]);
-        } else if (liveR) {
-          st32(out, nL + (p - lb), e[k]);
-        }
-      }
+    for (uint32_t j = 0; j < kSpl; ++j) {
+      v[j] = wcnt[par * kSlots + kSpl * lane + j];
+      sum += v[j];
     }
+    const uint32_t inc = wave_incl_scan(sum);
+    const uint32_t total = readlane(inc, 63);
+    const uint32_t js = wave % kSpl;
+    uint32_t esel = inc - sum;
+#pragma unroll
+    for (uint32_t j = 0; j + 1 < kSpl; ++j) esel += (j < js) ? v[j] : 0u;
+    const bool more = base + kReshTile < n;
+    if (kBigPipe && more) load(nx, base + kReshTile);
+#pragma unroll
+    for (int k = 0; k < kBigE; ++k) {
+      const uint32_t slot = (uint32_t)k * kReshWaves + wave;
+      const uint32_t before = carry + readlane(esel, slot / kSpl);
+      const uint32_t p = base + (uint32_t)k * kReshThreads + tid;
+      const bool left = (sdm >> k) & 1u;
+      const uint32_t lb = before + mbcnt(mk[k]);
+      // one store per element: left at its rank, right after the nL left ones
+      if (p < n && (left ? liveL : liveR)) st32(out, left ? lb : nL + (p - lb), e[k]);
+    }
+    if (!kBigPipe && more) load(e, base + kReshTile);
     carry += total;
     par ^= 1u;
     return more;
   };
-  IdxT A[kReshE], B[kReshE];
-  load(A, 0);
-  for (uint32_t base = 0;; base += 2 * kReshTile) {
-    if (!step(A, B, base)) break;
-    if (!step(B, A, base + kReshTile)) break;
+  if constexpr (kBigPipe) {
+    IdxT A[kBigE], B[kBigE];
+    load(A, 0);
+    for (uint32_t base = 0;; base += 2 * kReshTile) {
+      if (!step(A, B, base)) break;
+      if (!step(B, A, base + kReshTile)) break;
+    }
+  } else {
+    IdxT A[kBigE];
+    load(A, 0);
+    for (uint32_t base = 0; step(A, A, base); base += kReshTile) {
+    }
   }
 }
 
@@ -1823,13 +1846,17 @@ __device__ void resh_rows(const FitArgs<IdxT, LabT> &a, uint32_t tree, uint32_t 
           ++r;
         }
       }
+      // every e[k] is a valid entry: lookups issued together, masked after
+      uint32_t sv[kReshE];
+#pragma unroll
+      for (int k = 0; k < kReshE; ++k) sv[k] = side(e[k]);
       uint32_t sdm = 0;
       r = 0;
       pc = 0;
 #pragma unroll
       for (int k = 0; k < kReshE; ++k) {
         const uint32_t p = pc * kWave + lane;
-        if (r < rows && p < n && (side(e[k]) ^ inv)) sdm |= 1u << k;
+        sdm |= ((sv[k] ^ inv) & (r < rows && p < n ? 1u : 0u)) << k;
         if (++pc == c) {
           pc = 0;
           ++r;
@@ -1841,15 +1868,10 @@ __device__ void resh_rows(const FitArgs<IdxT, LabT> &a, uint32_t tree, uint32_t 
 #pragma unroll
       for (int k = 0; k < kReshE; ++k) {
         const uint32_t p = pc * kWave + lane;
-        const uint64_t m = ballot((sdm >> k) & 1u);
-        if (r < rows && p < n) {
-          const uint32_t lb = carry + mbcnt(m);
-          if ((sdm >> k) & 1u) {
-            if (liveL) st32(out, r * N + lb, e[k]);
-          } else if (liveR) {
-            st32(out, r * N + nL + (p - lb), e[k]);
-          }
-        }
+        const bool left = (sdm >> k) & 1u;
+        const uint64_t m = ballot(left);
+        const uint32_t lb = carry + mbcnt(m);
+        if (r < rows && p < n && (left ? liveL : liveR)) st32(out, r * N + (left ? lb : nL + (p - lb)), e[k]);
         carry += (uint32_t)__popcll(m);
         if (++pc == c) {
           pc = 0;
@@ -1882,24 +1904,22 @@ __device__ void resh_rows(const FitArgs<IdxT, LabT> &a, uint32_t tree, uint32_t 
     const bool more = first + r2 < end;
     if (more) load(nx, r2, b2);
     const gptr<IdxT> out = out0 + r * rstride;
+    uint32_t sv[kReshE];
+#pragma unroll
+    for (int k = 0; k < kReshE; ++k) sv[k] = side(e[k]);
     uint32_t sdm = 0;
 #pragma unroll
     for (int k = 0; k < kReshE; ++k) {
       const uint32_t p = base + (uint32_t)k * kWave + lane;
-      if (p < n && (side(e[k]) ^ inv)) sdm |= 1u << k;
+      sdm |= ((sv[k] ^ inv) & (p < n ? 1u : 0u)) << k;
     }
 #pragma unroll
     for (int k = 0; k < kReshE; ++k) {
       const uint32_t p = base + (uint32_t)k * kWave + lane;
-      const uint64_t m = ballot((sdm >> k) & 1u);
-      if (p < n) {
-        const uint32_t lb = carry + mbcnt(m);
-        if ((sdm >> k) & 1u) {
-          if (liveL) st32(out, lb, e[k]);
-        } else if (liveR) {
-          st32(out, nL + (p - lb), e[k]);
-        }
-      }
+      const bool left = (sdm >> k) & 1u;
+      const uint64_t m = ballot(left);
+      const uint32_t lb = carry + mbcnt(m);
+      if (p < n && (left ? liveL : liveR)) st32(out, left ? lb : nL + (p - lb), e[k]);
       carry += (uint32_t)__popcll(m);
     }
     if (r2 != r) carry = 0;

@@ -9,6 +9,9 @@ namespace cta {
 constexpr int kBlock = 256;          // threads per block; every kernel works wave by wave
 constexpr int kWavesPerBlock = kBlock / 64;
 constexpr uint32_t kLeaf = 0xFFFFFFFFu;
+// Label of a leaf child still to be looked up: its first sample, once the
+// side bitmaps are built.
+constexpr uint32_t kFirstSample = 0xFFFFFFFEu;
 
 // A node waiting for its split (one entry of a level's frontier).
 struct FNode {
@@ -40,9 +43,8 @@ struct NAux {
   uint32_t pad;
 };
 constexpr uint32_t kSplit = 1u, kLiveL = 2u, kLiveR = 4u;
-// Set by k_decide on a split node with a live child when there are row-free
-// features: its fill tasks count the smaller side (k_children then resets
-// the flags to kSplit | kLiveL | kLiveR | kInvert).
+// Set by k_decide on a split node with a live child that stays in the levels
+// when there are row-free features: its fill tasks count the smaller side.
 constexpr uint32_t kCountRl = 8u;
 // Set by k_decide on a split node whose right child is the smaller: the side
 // bitmaps mark a split node's smaller child, so its marks are the right one's.

@@ -583,6 +583,28 @@ int cta_server_wait(cta_server *s, uint64_t ticket, int64_t timeout_us, int32_t 
   });
 }
 
+int cta_server_wait_many(cta_server *s, int32_t count, const uint64_t *tickets, int64_t timeout_us, int32_t *status,
+                         int32_t *forest_ids) {
+  return sguard([&] {
+    if (!s || count < 0 || (count > 0 && (!tickets || !status))) sfail(CTA_EINVAL, "NULL argument");
+    std::unique_lock<std::mutex> lk(s->mu);
+    for (int32_t i = 0; i < count; ++i)
+      if (s->tickets.find(tickets[i]) == s->tickets.end())
+        sfail(CTA_EINVAL, "unknown ticket %llu", (unsigned long long)tickets[i]);
+    const auto deadline = std::chrono::steady_clock::now() + std::chrono::microseconds(std::max<int64_t>(0, timeout_us));
+    for (int32_t i = 0; i < count; ++i) {
+      const uint64_t t = tickets[i];
+      auto done = [&] { return s->tickets[t].status >= 0; };
+      if (timeout_us < 0) s->cv_done.wait(lk, done);
+      else s->cv_done.wait_until(lk, deadline, done);
+      const TicketState st = s->tickets[t];
+      status[i] = st.status < 0 ? -1 : st.status;
+      if (forest_ids) forest_ids[i] = st.forest;
+      if (st.status >= 0) s->tickets.erase(t);
+    }
+  });
+}
+
 int cta_server_get_stats(cta_server *s, cta_server_stats *out) {
   return sguard([&] {
     if (!s || !out) sfail(CTA_EINVAL, "NULL argument");

@@ -311,6 +311,13 @@ int cta_server_submit_fit(cta_server *server, const float *X, const uint32_t *y,
  * job's CTA_* status; *forest_id: the fitted forest of a fit job.  A
  * completed ticket is forgotten after it has been reported. */
 int cta_server_wait(cta_server *server, uint64_t ticket, int64_t timeout_us, int32_t *status, int32_t *forest_id);
+/* cta_server_wait over `count` tickets in order, with one deadline
+ * timeout_us from the call (< 0: forever): status[i] (and forest_ids[i] when
+ * forest_ids is not NULL) as cta_server_wait reports them; tickets still
+ * pending at the deadline report -1 and are kept.  All are refused if any
+ * ticket is unknown. */
+int cta_server_wait_many(cta_server *server, int32_t count, const uint64_t *tickets, int64_t timeout_us,
+                         int32_t *status, int32_t *forest_ids);
 int cta_server_get_stats(cta_server *server, cta_server_stats *out);
 int cta_server_reset_stats(cta_server *server);
 

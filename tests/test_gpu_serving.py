@@ -125,6 +125,41 @@ def test_submit_many_matches_single(data, base_forest, coalesce):
         srv.close()
 
 
+@pytest.mark.parametrize("coalesce", [True, False])
+def test_submit_rows_and_wait_many(data, base_forest, coalesce):
+    """Requests laid back to back in one array, queued in one call and
+    waited for in one call: every request's labels equal the oracle's, at
+    the offsets of its rows; unknown tickets are refused and a zero timeout
+    reports pending tickets as -1 without forgetting them."""
+    _, _, Xt = data
+    reqs = _requests(Xt, 200, 21, largest=6000)
+    R = np.array([len(r) for r in reqs], np.int64)
+    X = np.concatenate(reqs)
+    trees = base_forest.export()
+    want = oracle.get().predict(trees, X, 7)
+    srv = _lib.Server(coalesce=coalesce, split_rows=512)
+    try:
+        fid = srv.add_forest(base_forest)
+        print("submit_rows coalesce=%d" % coalesce, flush=True)
+        t1, lab1 = srv.submit_rows(fid, X[:R[:120].sum()], R[:120])
+        t2, lab2 = srv.submit_rows(fid, X[R[:120].sum():], R[120:])
+        tickets = np.concatenate([t1, t2])
+        early = srv.wait_many(tickets, timeout_us=0)
+        assert set(early.tolist()) <= {-1, 0}
+        st = srv.wait_many(tickets[early < 0])
+        assert (st == 0).all()
+        assert np.array_equal(np.concatenate([lab1, lab2]), want)
+        s = srv.stats()
+        assert s["predict_requests"] == len(reqs)
+        assert s["predict_launches"] < len(reqs) if coalesce else s["predict_launches"] == len(reqs)
+        with pytest.raises(_lib.CtaError):
+            srv.wait_many(tickets[:1])  # reported already: forgotten
+        with pytest.raises(ValueError):
+            srv.submit_rows(fid, X[:10], np.array([4, 5]))
+    finally:
+        srv.close()
+
+
 def test_fit_admission_by_hbm_budget(data):
     X, y, _ = data
     srv = _lib.Server(hbm_budget=256 << 20)
