@@ -395,8 +395,10 @@ void launch_resh(cta_ctx *ctx, const FitArgs<IdxT, LabT> &a, bool global_bits, u
 // pair the frontier, node arena and level-list entries.
 size_t tree_bytes(int64_t N, int64_t Fr, int64_t K, int64_t m, size_t idx_bytes) {
   const size_t rows = 2 * (size_t)Fr * N * idx_bytes + N / 8 + 64;
+  // (the level-list counts and offsets hold the larger of the two scans' fields)
+  constexpr size_t kFields = kSelFields > kChFields ? kSelFields : kChFields;
   const size_t per_sample = 2 * sizeof(FNode) + 3 * 4 * (size_t)K + sizeof(NAux) + 4 * (size_t)m + 2 * 7 * 4 +
-                            (sizeof(Task) + sizeof(ScanRes)) + 2 * sizeof(Task) + 2 * kChFields * 4;
+                            (sizeof(Task) + sizeof(ScanRes)) + 2 * sizeof(Task) + 2 * kFields * 4;
   return rows + per_sample * (size_t)N / 2 + 16 * ((size_t)N / 8) + 4096;
 }
 

@@ -107,6 +107,17 @@ __device__ __forceinline__ uint32_t code_at(const uint8_t *cb, uint32_t kind, ui
   return reinterpret_cast<const uint16_t *>(cb)[s];
 }
 
+// Row entries at positions pb + u * kWave + lane (u < U) of a row whose valid
+// positions end at p1 > pb, loaded unconditionally (positions past p1 - 1
+// load the last entry; callers mask them), so that the U loads and what
+// depends on them issue together rather than one masked load at a time.
+template <int U, class IdxT>
+__device__ __forceinline__ void load_row_batch(const IdxT *row, uint32_t pb, uint32_t p1, uint32_t (&e)[U]) {
+  const uint32_t lane = lane_id();
+#pragma unroll
+  for (int u = 0; u < U; ++u) e[u] = (uint32_t)row[min(pb + (uint32_t)u * kWave + lane, p1 - 1u)];
+}
+
 // ------------------------------------------------------------------------
 // Offset scans.  Per-node counts (NF fields, node-major) become exclusive
 // offsets and totals, so every list a level builds (split-scan tasks,
@@ -491,14 +502,13 @@ __global__ __launch_bounds__(kBlock) void k_scan_hist(FitArgs<IdxT, LabT> a) {
     const uint32_t p0 = tk.chunk * a.Cs, p1 = min(nd.n, p0 + a.Cs);
     for (uint32_t pb = p0; pb < p1; pb += kHistU * kWave) {
       uint32_t sid[kHistU];
-#pragma unroll
-      for (int u = 0; u < kHistU; ++u) {
-        const uint32_t p = pb + (uint32_t)u * kWave + lane;
-        sid[u] = (p < p1) ? (uint32_t)row[p] : 0xFFFFFFFFu;
-      }
+      load_row_batch(row, pb, p1, sid);
       uint32_t l[kHistU];
 #pragma unroll
-      for (int u = 0; u < kHistU; ++u) l[u] = (sid[u] != 0xFFFFFFFFu) ? a.label((IdxT)sid[u]) : 0xFFFFFFFFu;
+      for (int u = 0; u < kHistU; ++u) l[u] = a.label((IdxT)sid[u]);
+#pragma unroll
+      for (int u = 0; u < kHistU; ++u)
+        if (pb + (uint32_t)u * kWave + lane >= p1) l[u] = 0xFFFFFFFFu;  // past the chunk
       if (regs) {
 #pragma unroll
         for (int u = 0; u < kHistU; ++u) {
@@ -917,14 +927,13 @@ __device__ void count_left(const FitArgs<IdxT, LabT> &a, uint32_t i, const FNode
   const uint32_t delta = fwd ? 1u : 0xFFFFFFFFu;
   for (uint32_t pb = r0; pb < r1; pb += 4 * kWave) {
     uint32_t sid[4];
-#pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      const uint32_t q = pb + (uint32_t)u * kWave + lane;
-      sid[u] = (q < r1) ? (uint32_t)row[q] : 0xFFFFFFFFu;
-    }
+    load_row_batch(row, pb, r1, sid);
     uint32_t l[4];
 #pragma unroll
-    for (int u = 0; u < 4; ++u) l[u] = (sid[u] != 0xFFFFFFFFu) ? a.label((IdxT)sid[u]) : 0xFFFFFFFFu;
+    for (int u = 0; u < 4; ++u) l[u] = a.label((IdxT)sid[u]);
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+      if (pb + (uint32_t)u * kWave + lane >= r1) l[u] = 0xFFFFFFFFu;  // past the range
 #pragma unroll
     for (int u = 0; u < 4; ++u)
       if (l[u] != 0xFFFFFFFFu) atomicAdd(&h[l[u]], delta);
