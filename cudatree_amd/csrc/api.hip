@@ -374,12 +374,22 @@ void launch_resh(cta_ctx *ctx, const FitArgs<IdxT, LabT> &a, bool global_bits, u
       HIPC(hipFuncSetAttribute((const void *)k_resh<IdxT, LabT, true, false>,
                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)ctx->lds_per_block));
     });
+#ifdef CTA_RESH_FUSED
+    static std::once_flag attr2;
+    std::call_once(attr2, [&] {
+      HIPC(hipFuncSetAttribute((const void *)k_resh_all<IdxT, LabT, true>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)ctx->lds_per_block));
+    });
+    Timed tm(ctx, big_name);
+    k_resh_all<IdxT, LabT, true><<<grid, kReshThreads, lds, st>>>(a);
+#else
     {
       Timed tm(ctx, big_name);
       k_resh<IdxT, LabT, true, true><<<grid, kReshThreads, lds, st>>>(a);
     }
     Timed tm(ctx, small_name);
     k_resh<IdxT, LabT, true, false><<<grid, kReshThreads, lds, st>>>(a);
+#endif
   } else {
     {
       Timed tm(ctx, big_name);
@@ -391,7 +401,7 @@ void launch_resh(cta_ctx *ctx, const FitArgs<IdxT, LabT> &a, bool global_bits, u
 }
 
 // Device bytes one resident tree of a fit needs: ping-pong rows of the Fr
-// features that keep rows, the side bitmap and fill partials, and per sample
+// features that keep rows, the side bitmap, and per sample
 // pair the frontier, node arena and level-list entries.
 size_t tree_bytes(int64_t N, int64_t Fr, int64_t K, int64_t m, size_t idx_bytes) {
   const size_t rows = 2 * (size_t)Fr * N * idx_bytes + N / 8 + 64;
@@ -399,7 +409,7 @@ size_t tree_bytes(int64_t N, int64_t Fr, int64_t K, int64_t m, size_t idx_bytes)
   constexpr size_t kFields = kSelFields > kChFields ? kSelFields : kChFields;
   const size_t per_sample = 2 * sizeof(FNode) + 3 * 4 * (size_t)K + sizeof(NAux) + 4 * (size_t)m + 2 * 7 * 4 +
                             (sizeof(Task) + sizeof(ScanRes)) + 2 * sizeof(Task) + 2 * kFields * 4;
-  return rows + per_sample * (size_t)N / 2 + 16 * ((size_t)N / 8) + 4096;
+  return rows + per_sample * (size_t)N / 2 + 4096;
 }
 
 // Batched levels issued between two host reads of the control block.
@@ -428,7 +438,7 @@ struct Fitter {
   const int32_t *rl_of_d = nullptr, *slot_of_d = nullptr, *row_feat_d = nullptr;
   uint64_t rl_cap = 0;  // frontier slots of each row-free count buffer
   uint32_t Jcap = 1;       // most fill workgroups per tree
-  uint32_t *part = nullptr, *bsum = nullptr;
+  uint32_t *bsum = nullptr;
 
   Fitter(cta_ctx *c, cta_dataset *d, const cta_fit_opts &opts) : ctx(c), ds(d), o(opts) {
     N = d->N;
@@ -527,7 +537,6 @@ struct Fitter {
     unsigned long long *bslot = ctx->ws_as<unsigned long long>("byte_slots", nbslot);
     fill_lds = !o.fill_global && fill_lds_bytes() <= (size_t)ctx->lds_per_block;
     Jcap = fill_lds ? (uint32_t)std::max<int64_t>(1, std::min<int64_t>(16, (3 * (int64_t)ctx->num_cu) / Tb)) : 1u;
-    part = Jcap > 1 ? ctx->ws_as<uint32_t>("fill_part", (size_t)Tb * Jcap * W) : nullptr;
     FNode *front[2] = {ctx->ws_as<FNode>("front0", cap_q), ctx->ws_as<FNode>("front1", cap_q)};
     uint32_t *cnt[2] = {ctx->ws_as<uint32_t>("cnt0", cap_q * K), ctx->ws_as<uint32_t>("cnt1", cap_q * K)};
     uint32_t *cntL = ctx->ws_as<uint32_t>("cntL", cap_q * K);
@@ -895,11 +904,8 @@ struct Fitter {
         HIPC(hipFuncSetAttribute((const void *)k_fill_lds<IdxT, LabT>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                  (int)ctx->lds_per_block));
       });
-      k_fill_lds<IdxT, LabT><<<a.T * J, kFillThreads, fill_lds_bytes(), st>>>(a, J, part);
-      if (J > 1)
-        k_merge_bits<IdxT, LabT><<<dim3(std::min<uint32_t>(64, (a.W + 255) / 256), a.T), 256, 0, st>>>(a, J, part);
+      k_fill_lds<IdxT, LabT><<<a.T * J, kFillThreads, fill_lds_bytes(), st>>>(a, J);
     } else {
-      k_fill_u32<<<grid_for((size_t)a.T * a.W), 256, 0, st>>>(a.bits, (size_t)a.T * a.W, 0u);
       k_fill_global<IdxT, LabT><<<G, kBlock, (size_t)kWavesPerBlock * Frl * K * 4, st>>>(a);
     }
   }

@@ -258,6 +258,11 @@ __global__ __launch_bounds__(kBlock) void k_select(FitArgs<IdxT, LabT> a) {
   const uint32_t lane = lane_id();
   const uint32_t m = (uint32_t)a.m, F = (uint32_t)a.F;
   const uint32_t q = a.ctl->q;
+  // the side bitmaps start the level clear (the fill ORs into them; the
+  // previous level's readers are done)
+  for (size_t w = (size_t)blockIdx.x * blockDim.x + threadIdx.x; w < (size_t)a.T * a.W;
+       w += (size_t)gridDim.x * blockDim.x)
+    a.bits[w] = 0u;
   unsigned long long bytes = 0;
   WAVE_LOOP(q) {
     const uint32_t i = item;
@@ -1129,10 +1134,10 @@ __global__ __launch_bounds__(kBlock) void k_decide(FitArgs<IdxT, LabT> a) {
 // child; marking the smaller one halves the marks on average).
 //
 // k_fill_lds: workgroup (t, j) of J per tree builds the bits of its share of
-// tree t's fill tasks in an LDS copy of the bitmap (no global atomics) and
-// writes the whole copy: directly into the tree's bitmap when J == 1, else
-// into partial bitmap j, which k_merge_bits ORs together.  Every word is
-// written, so the bitmaps need no clearing.
+// tree t's fill tasks in an LDS copy of the bitmap and writes it to the
+// tree's bitmap, which k_select cleared at the start of the level: the
+// whole copy when the tree has one fill workgroup, else its non-zero words
+// by atomic OR.
 // ------------------------------------------------------------------------
 constexpr int kFillThreads = 1024;
 constexpr int kFillWaves = kFillThreads / kWave;
@@ -1289,7 +1294,7 @@ __device__ __forceinline__ uint32_t fill_groups(const FitArgs<IdxT, LabT> &a, ui
 }
 
 template <class IdxT, class LabT>
-__global__ __launch_bounds__(kFillThreads) void k_fill_lds(FitArgs<IdxT, LabT> a, uint32_t J, uint32_t *part) {
+__global__ __launch_bounds__(kFillThreads) void k_fill_lds(FitArgs<IdxT, LabT> a, uint32_t J) {
   if (a.ctl->err) return;  // a list overflowed: the fit fails, nothing more is read
   extern __shared__ uint32_t lbits[];
   const uint32_t t = blockIdx.x / J, j = blockIdx.x % J;
@@ -1313,31 +1318,17 @@ __global__ __launch_bounds__(kFillThreads) void k_fill_lds(FitArgs<IdxT, LabT> a
   }
   if (cur != kLeaf) rl_flush(a, cur, cnt, cur_whole);
   __syncthreads();
-  uint32_t *out = (Je == 1) ? a.bits + (size_t)t * W : part + ((size_t)t * J + j) * W;
-  for (uint32_t w = tid; w < W; w += kFillThreads) out[w] = lbits[w];
-}
-
-// bits of tree blockIdx.y = OR of its partial bitmaps, when it has more than
-// one.
-template <class IdxT, class LabT>
-__global__ void k_merge_bits(FitArgs<IdxT, LabT> a, uint32_t J, const uint32_t *part) {
-  if (a.ctl->err) return;
-  const uint32_t t = blockIdx.y, lo = a.tlo[t], hi = a.thi[t];
-  if (hi <= lo) return;
-  uint32_t fb, fe;
-  const uint32_t Je = fill_groups(a, lo, hi, J, fb, fe);
-  if (Je == 1) return;  // written whole by its one workgroup
-  const uint32_t W = a.W;
-  const uint32_t *src = part + (size_t)t * J * W;
-  for (uint32_t w = blockIdx.x * blockDim.x + threadIdx.x; w < W; w += gridDim.x * blockDim.x) {
-    uint32_t v = 0;
-    for (uint32_t j = 0; j < Je; ++j) v |= src[(size_t)j * W + w];
-    a.bits[(size_t)t * W + w] = v;
+  uint32_t *out = a.bits + (size_t)t * W;
+  if (Je == 1) {
+    for (uint32_t w = tid; w < W; w += kFillThreads) out[w] = lbits[w];
+  } else {
+    for (uint32_t w = tid; w < W; w += kFillThreads)
+      if (lbits[w]) atomicOr(&out[w], lbits[w]);
   }
 }
 
-// Bitmaps too large for LDS: global atomics into bitmaps cleared first
-// (LDS: the waves' row-free counters).
+// Bitmaps too large for LDS: global atomics into the bitmaps k_select
+// cleared (LDS: the waves' row-free counters).
 template <class IdxT, class LabT>
 __global__ __launch_bounds__(kBlock) void k_fill_global(FitArgs<IdxT, LabT> a) {
   if (a.ctl->err) return;  // a list overflowed: the fit fails, nothing more is read
@@ -2001,6 +1992,54 @@ __global__ __launch_bounds__(kReshThreads, 8) void k_resh(FitArgs<IdxT, LabT> a)
     }
   }
 }
+
+#ifdef CTA_RESH_FUSED
+// Both plans in one launch: plan entries [0, P_big) move big tasks, the
+// next P_small small tasks, so small tasks fill the CUs the big ones leave.
+template <class IdxT, class LabT, bool kLds>
+__global__ __launch_bounds__(kReshThreads, 8) void k_resh_all(FitArgs<IdxT, LabT> a) {
+  extern __shared__ uint32_t lds[];
+  uint32_t *wcnt = lds + (kLds ? ((a.W + 3u) & ~3u) : 0u);  // 2 x kSlots counts
+  const uint32_t Pb = a.ctl->n_plan[0], P = Pb + a.ctl->n_plan[1];
+  const uint32_t T = a.T;
+  const uint32_t wave = threadIdx.x / kWave;
+  uint32_t loaded = 0xFFFFFFFFu, par = 0;
+  for (uint32_t e = blockIdx.x; e < P; e += gridDim.x) {
+    const bool big = e < Pb;
+    const uint32_t pe = big ? e : e - Pb;
+    const uint32_t *wg_off = a.wg_off + (big ? 0u : T + 1);
+    uint32_t lo = 0, hi = T;
+    while (hi - lo > 1) {
+      const uint32_t mid = (lo + hi) / 2;
+      if (wg_off[mid] <= pe) lo = mid;
+      else hi = mid;
+    }
+    const uint32_t t = lo;
+    const uint32_t j = pe - wg_off[t], wgs = wg_off[t + 1] - wg_off[t];
+    const uint32_t *gbits = a.bits + (size_t)t * a.W;
+    if (kLds && t != loaded) {
+      __syncthreads();
+      for (uint32_t w = threadIdx.x; w < a.W; w += kReshThreads) lds[w] = gbits[w];
+      __syncthreads();
+      loaded = t;
+    }
+    auto side = [&](IdxT s) -> uint32_t {
+      const uint32_t u = sample_of(s);
+      if constexpr (kLds) return (lds[u >> 5] >> (u & 31u)) & 1u;
+      else return (gbits[u >> 5] >> (u & 31u)) & 1u;
+    };
+    const uint32_t *pl = a.plan + (size_t)t * 4;
+    if (big) {
+      const uint32_t b0 = pl[0], nb = pl[1] - pl[0];
+      for (uint32_t bt = j; bt < nb; bt += wgs) resh_big(a, a.big_tasks[b0 + bt], side, wcnt, par);
+    } else {
+      const uint32_t s0 = pl[2], ns = pl[3] - pl[2];
+      for (uint32_t st = j * kReshWaves + wave; st < ns; st += wgs * kReshWaves)
+        resh_small(a, a.small_tasks[s0 + st], side);
+    }
+  }
+}
+#endif
 
 // ------------------------------------------------------------------------
 // Bootstrap and roots (random_forest.py:113-144, bootstrap_fill.cu,
