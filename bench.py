@@ -284,8 +284,6 @@ def short_kernel(name):
     if base.startswith("cta::"):
         base = base[5:]
     ident = base.split("<")[0]
-    if ident == "k_resh":
-        return "k_resh_big" if base.endswith("true>") else "k_resh_small"
     return ident if base.startswith("k_") else base
 
 
@@ -449,7 +447,7 @@ def main():
 
     # roofline of the dominant HBM kernel, HIP events over the timed steps
     roof = None
-    hbm_kernels = [k for k in ("k_resh_big", "k_resh_small", "k_scan", "k_subtree", "k_rl_count")
+    hbm_kernels = [k for k in ("k_resh", "k_scan", "k_subtree", "k_rl_count")
                    if k in stats and stats[k]["ms"] > 0]
     if hbm_kernels:
         dom = max(hbm_kernels, key=lambda k: stats[k]["ms"])
@@ -466,7 +464,7 @@ def main():
         traffic, tsrc = pmc_traffic(dom)
         # split scan + reshuffle kernels together (their algorithmic bytes over
         # their summed time), the figure the >= 50 % of peak target is set on
-        group = [k for k in ("k_scan", "k_scan_hist", "k_rl_count", "k_resh_big", "k_resh_small", "k_fill",
+        group = [k for k in ("k_scan", "k_scan_hist", "k_rl_count", "k_resh", "k_fill",
                              "k_subtree") if k in stats]
         g_bytes = sum(stats[k]["alg_bytes"] for k in group)
         g_s = sum(stats[k]["ms"] for k in group) * 1e-3

@@ -359,44 +359,23 @@ struct HostEntry {
   std::vector<uint32_t> cnt;
 };
 
-// Both reshuffle launches (big tasks, then small tasks) of a planned level;
-// the side bitmaps are staged in LDS unless `global_bits` or too large.
+// The reshuffle launch of a planned level (big and small tasks); the side
+// bitmaps are staged in LDS unless `global_bits` or too large.
 template <class IdxT, class LabT>
 void launch_resh(cta_ctx *ctx, const FitArgs<IdxT, LabT> &a, bool global_bits, uint32_t grid,
-                 const char *big_name = "k_resh_big", const char *small_name = "k_resh_small") {
+                 const char *name = "k_resh") {
   const size_t lds = (((size_t)a.W + 3) & ~(size_t)3) * 4 + 2 * kSlots * 4;
   hipStream_t st = ctx->stream;
+  Timed tm(ctx, name);
   if (!global_bits && lds <= ctx->lds_per_block) {
     static std::once_flag attr;
     std::call_once(attr, [&] {
-      HIPC(hipFuncSetAttribute((const void *)k_resh<IdxT, LabT, true, true>,
-                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)ctx->lds_per_block));
-      HIPC(hipFuncSetAttribute((const void *)k_resh<IdxT, LabT, true, false>,
-                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)ctx->lds_per_block));
-    });
-#ifdef CTA_RESH_FUSED
-    static std::once_flag attr2;
-    std::call_once(attr2, [&] {
-      HIPC(hipFuncSetAttribute((const void *)k_resh_all<IdxT, LabT, true>, hipFuncAttributeMaxDynamicSharedMemorySize,
+      HIPC(hipFuncSetAttribute((const void *)k_resh<IdxT, LabT, true>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                (int)ctx->lds_per_block));
     });
-    Timed tm(ctx, big_name);
-    k_resh_all<IdxT, LabT, true><<<grid, kReshThreads, lds, st>>>(a);
-#else
-    {
-      Timed tm(ctx, big_name);
-      k_resh<IdxT, LabT, true, true><<<grid, kReshThreads, lds, st>>>(a);
-    }
-    Timed tm(ctx, small_name);
-    k_resh<IdxT, LabT, true, false><<<grid, kReshThreads, lds, st>>>(a);
-#endif
+    k_resh<IdxT, LabT, true><<<grid, kReshThreads, lds, st>>>(a);
   } else {
-    {
-      Timed tm(ctx, big_name);
-      k_resh<IdxT, LabT, false, true><<<grid, kReshThreads, 2 * kSlots * 4, st>>>(a);
-    }
-    Timed tm(ctx, small_name);
-    k_resh<IdxT, LabT, false, false><<<grid, kReshThreads, 2 * kSlots * 4, st>>>(a);
+    k_resh<IdxT, LabT, false><<<grid, kReshThreads, 2 * kSlots * 4, st>>>(a);
   }
 }
 
@@ -686,8 +665,8 @@ struct Fitter {
       const char *lvl_path = ctx->profiling ? std::getenv("CTA_LEVEL_LOG") : nullptr;
       FILE *lvl = lvl_path ? std::fopen(lvl_path, "a") : nullptr;
       if (lvl)
-        std::fprintf(lvl, "level,q,scan_bytes,resh_big_bytes,resh_small_bytes,fill_bytes,k_scan_ms,k_resh_big_ms,"
-                          "k_resh_small_ms,k_fill_ms,k_scan_hist_ms,k_select_ms,k_decide_ms,k_children_ms\n");
+        std::fprintf(lvl, "level,q,scan_bytes,resh_big_bytes,resh_small_bytes,fill_bytes,k_scan_ms,k_resh_ms,"
+                          "k_fill_ms,k_scan_hist_ms,k_select_ms,k_decide_ms,k_children_ms\n");
       int depth = 0;
       // Levels are issued in groups between host reads of the control block:
       // a frontier at most doubles per level, so the host sizes launches for
@@ -737,11 +716,10 @@ struct Fitter {
         if (hc.err) fail(CTA_EINTERNAL, "device capacity check failed (err %u)", hc.err);
         if (lvl) {
           auto d = [&](const char *k) { return ctx->stats[k].ms - before[k].ms; };
-          std::fprintf(lvl, "%d,%u,%llu,%llu,%llu,%llu,%.4f,%.4f,%.4f,%.4f,%.4f,%.4f,%.4f,%.4f\n", depth, q,
+          std::fprintf(lvl, "%d,%u,%llu,%llu,%llu,%llu,%.4f,%.4f,%.4f,%.4f,%.4f,%.4f,%.4f\n", depth, q,
                        hc.bytes_scan - prev.bytes_scan, hc.bytes_resh - prev.bytes_resh,
                        hc.bytes_resh_small - prev.bytes_resh_small, hc.bytes_fill - prev.bytes_fill, d("k_scan"),
-                       d("k_resh_big"), d("k_resh_small"), d("k_fill"), d("k_scan_hist"), d("k_select"), d("k_decide"),
-                       d("k_children"));
+                       d("k_resh"), d("k_fill"), d("k_scan_hist"), d("k_select"), d("k_decide"), d("k_children"));
         }
         if (ctx->level_hook) ctx->level_hook(ctx->level_hook_arg);
         // levels of the group that had a frontier
@@ -870,8 +848,7 @@ struct Fitter {
   void account(const Ctl &prev, const Ctl &now) {
     ctx->add_bytes("k_scan", now.bytes_scan - prev.bytes_scan);
     ctx->add_bytes("k_fill", now.bytes_fill - prev.bytes_fill);
-    ctx->add_bytes("k_resh_big", now.bytes_resh - prev.bytes_resh);
-    ctx->add_bytes("k_resh_small", now.bytes_resh_small - prev.bytes_resh_small);
+    ctx->add_bytes("k_resh", (now.bytes_resh - prev.bytes_resh) + (now.bytes_resh_small - prev.bytes_resh_small));
     ctx->add_bytes("k_rl_count", now.bytes_rl - prev.bytes_rl);
     ctx->add_bytes("k_subtree", now.bytes_sub - prev.bytes_sub);
   }
@@ -1017,7 +994,7 @@ struct Fitter {
     const uint32_t grid = (uint32_t)std::max(8, ctx->num_cu * 4);
     k_plan<IdxT, LabT><<<1, kPlanThreads, 0, st>>>(a, grid, boot ? 0u : 1u);
     if (ev) HIPC(hipEventRecord(ev, st));
-    if (boot) launch_resh(ctx, a, o.resh_global != 0, grid, "k_boot_compact", "k_boot_compact");
+    if (boot) launch_resh(ctx, a, o.resh_global != 0, grid, "k_boot_compact");
     else launch_resh(ctx, a, o.resh_global != 0, grid);
   }
 

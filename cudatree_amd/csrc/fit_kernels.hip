@@ -1947,57 +1947,13 @@ __device__ void resh_small(const FitArgs<IdxT, LabT> &a, const Task &tk, const S
             uniform(ax.flags), first, first + count, side);
 }
 
-// kLds: the tree's bitmap is staged in LDS (N/8 bytes); otherwise it is read
-// from global memory.  kBig: the launch moves the big tasks of its plan,
-// else the small tasks (separate launches keep each path's registers).
-template <class IdxT, class LabT, bool kLds, bool kBig>
-__global__ __launch_bounds__(kReshThreads, 8) void k_resh(FitArgs<IdxT, LabT> a) {
-  extern __shared__ uint32_t lds[];
-  uint32_t *wcnt = lds + (kLds ? ((a.W + 3u) & ~3u) : 0u);  // 2 x kSlots counts
-  const uint32_t P = a.ctl->n_plan[kBig ? 0 : 1];
-  const uint32_t T = a.T;
-  const uint32_t *wg_off = a.wg_off + (kBig ? 0u : T + 1);
-  const uint32_t wave = threadIdx.x / kWave;
-  uint32_t loaded = 0xFFFFFFFFu, par = 0;
-  for (uint32_t pe = blockIdx.x; pe < P; pe += gridDim.x) {
-    // tree of this plan entry: last t with wg_off[t] <= pe
-    uint32_t lo = 0, hi = T;
-    while (hi - lo > 1) {
-      const uint32_t mid = (lo + hi) / 2;
-      if (wg_off[mid] <= pe) lo = mid;
-      else hi = mid;
-    }
-    const uint32_t t = lo;
-    const uint32_t j = pe - wg_off[t], wgs = wg_off[t + 1] - wg_off[t];
-    const uint32_t *gbits = a.bits + (size_t)t * a.W;
-    if (kLds && t != loaded) {
-      __syncthreads();
-      for (uint32_t w = threadIdx.x; w < a.W; w += kReshThreads) lds[w] = gbits[w];
-      __syncthreads();
-      loaded = t;
-    }
-    auto side = [&](IdxT s) -> uint32_t {
-      const uint32_t u = sample_of(s);
-      if constexpr (kLds) return (lds[u >> 5] >> (u & 31u)) & 1u;
-      else return (gbits[u >> 5] >> (u & 31u)) & 1u;
-    };
-    const uint32_t *pl = a.plan + (size_t)t * 4;
-    if constexpr (kBig) {
-      const uint32_t b0 = pl[0], nb = pl[1] - pl[0];
-      for (uint32_t bt = j; bt < nb; bt += wgs) resh_big(a, a.big_tasks[b0 + bt], side, wcnt, par);
-    } else {
-      const uint32_t s0 = pl[2], ns = pl[3] - pl[2];
-      for (uint32_t st = j * kReshWaves + wave; st < ns; st += wgs * kReshWaves)
-        resh_small(a, a.small_tasks[s0 + st], side);
-    }
-  }
-}
-
-#ifdef CTA_RESH_FUSED
-// Both plans in one launch: plan entries [0, P_big) move big tasks, the
-// next P_small small tasks, so small tasks fill the CUs the big ones leave.
+// One launch moves both plans: plan entries [0, P_big) are big-task work,
+// the next P_small small-task work, so the small tasks fill the CUs that the
+// big ones leave while a level's last big tasks finish.  kLds: the tree's
+// bitmap is staged in LDS (N/8 bytes); otherwise it is read from global
+// memory.
 template <class IdxT, class LabT, bool kLds>
-__global__ __launch_bounds__(kReshThreads, 8) void k_resh_all(FitArgs<IdxT, LabT> a) {
+__global__ __launch_bounds__(kReshThreads, 8) void k_resh(FitArgs<IdxT, LabT> a) {
   extern __shared__ uint32_t lds[];
   uint32_t *wcnt = lds + (kLds ? ((a.W + 3u) & ~3u) : 0u);  // 2 x kSlots counts
   const uint32_t Pb = a.ctl->n_plan[0], P = Pb + a.ctl->n_plan[1];
@@ -2007,6 +1963,7 @@ __global__ __launch_bounds__(kReshThreads, 8) void k_resh_all(FitArgs<IdxT, LabT
   for (uint32_t e = blockIdx.x; e < P; e += gridDim.x) {
     const bool big = e < Pb;
     const uint32_t pe = big ? e : e - Pb;
+    // tree of this plan entry: last t with wg_off[t] <= pe
     const uint32_t *wg_off = a.wg_off + (big ? 0u : T + 1);
     uint32_t lo = 0, hi = T;
     while (hi - lo > 1) {
@@ -2039,7 +1996,6 @@ __global__ __launch_bounds__(kReshThreads, 8) void k_resh_all(FitArgs<IdxT, LabT
     }
   }
 }
-#endif
 
 // ------------------------------------------------------------------------
 // Bootstrap and roots (random_forest.py:113-144, bootstrap_fill.cu,
