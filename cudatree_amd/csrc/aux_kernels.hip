@@ -566,11 +566,26 @@ __global__ __launch_bounds__(256) void k_predict_xcd(const PNode *__restrict__ n
   const uint32_t nt = (T - cls + kXcd - 1) / kXcd;  // trees of this class
   const uint64_t total = R * nt;
   const uint64_t step = (uint64_t)(gridDim.x / kXcd) * blockDim.x;
-  for (uint64_t g = (uint64_t)(blockIdx.x / kXcd) * blockDim.x + threadIdx.x; g < total; g += step) {
-    const uint64_t r = g / nt;
-    const uint32_t t = cls + kXcd * (uint32_t)(g % nt);
-    const float *x = X + r * F;
-    atomicAdd(&votes[r * K + walk(nodes, x, roots[t])], 1u);
+  const uint32_t lane = threadIdx.x % kWave;
+  // whole waves step together, so that the lanes holding the same row and
+  // label add their votes with one atomic
+  for (uint64_t g0 = (uint64_t)(blockIdx.x / kXcd) * blockDim.x + (threadIdx.x - lane); g0 < total; g0 += step) {
+    const uint64_t g = g0 + lane;
+    const bool active = g < total;
+    uint64_t key = 0;
+    if (active) {
+      const uint64_t r = g / nt;
+      const uint32_t t = cls + kXcd * (uint32_t)(g % nt);
+      key = r * K + walk(nodes, X + r * F, roots[t]);
+    }
+    uint64_t rem = ballot(active);
+    while (rem) {
+      const uint32_t leader = (uint32_t)__ffsll((unsigned long long)rem) - 1u;
+      const uint64_t kl = ((uint64_t)readlane((uint32_t)(key >> 32), leader) << 32) | readlane((uint32_t)key, leader);
+      const uint64_t mm = ballot(active && key == kl);
+      if (lane == leader) atomicAdd(&votes[kl], (uint32_t)__popcll(mm));
+      rem &= ~mm;
+    }
   }
 }
 
