@@ -8,8 +8,10 @@
 //                      distinct keys); ties keep ascending sample id
 //                      (replaces the host np.argsort of
 //                      random_forest.py:247)
-//   k_predict / k_vote batched traversal of a flattened pre-order forest,
-//                      one thread per (row, tree), votes by integer atomics
+//   k_block_*          blocked predict table (subtrees of three levels
+//                      per 128-byte line) from the pre-order table
+//   k_predict / k_vote batched traversal of the blocked forest, one thread
+//                      per (row, tree), votes by integer atomics
 //                      (predict.cu:12-69, random_forest.py:339-377)
 #include "device_util.h"
 #include "internal.h"
@@ -525,23 +527,106 @@ __global__ void k_code_pack(const uint32_t *d, uint32_t N, const uint64_t *fcode
   }
 }
 
+// ---- blocked predict table ---------------------------------------------------------
+// Built from the pre-order table (PNode, right-child indices global) and the
+// node depths: flags of the block roots, their ranks by an exclusive scan,
+// each node's slot and block root (the roots set their children, then the
+// nodes one level below set theirs), then one scatter.
+
+// flag[g] = 1 when node g roots a block (its depth is a multiple of 3).
+__global__ void k_block_flags(const uint32_t *depth, uint32_t n, uint32_t *flag) {
+  for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < n; g += gridDim.x * blockDim.x)
+    flag[g] = depth[g] % 3u == 0u ? 1u : 0u;
+}
+
+// Block roots: slot 0 and themselves as root; their children slots 1 and 2.
+__global__ void k_block_top(const PNode *nodes, const uint32_t *depth, uint32_t n, uint8_t *slot, uint32_t *root) {
+  for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < n; g += gridDim.x * blockDim.x) {
+    if (depth[g] % 3u != 0u) continue;
+    slot[g] = 0;
+    root[g] = g;
+    const PNode p = nodes[g];
+    if (p.feat < 0) continue;
+    slot[g + 1] = 1;
+    root[g + 1] = g;
+    slot[p.right] = 2;
+    root[p.right] = g;
+  }
+}
+
+// Split nodes one level below a block root: their children's slots.
+__global__ void k_block_mid(const PNode *nodes, const uint32_t *depth, uint32_t n, uint8_t *slot, uint32_t *root) {
+  for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < n; g += gridDim.x * blockDim.x) {
+    if (depth[g] % 3u != 1u) continue;
+    const PNode p = nodes[g];
+    if (p.feat < 0) continue;
+    const uint32_t s = slot[g], r = root[g];
+    slot[g + 1] = (uint8_t)(2u * s + 1u);
+    root[g + 1] = r;
+    slot[p.right] = (uint8_t)(2u * s + 2u);
+    root[p.right] = r;
+  }
+}
+
+// Every node into its block's slot; split nodes at the bottom of a block
+// point at their children's blocks.
+__global__ void k_block_fill(const PNode *nodes, const uint32_t *depth, uint32_t n, const uint8_t *slot,
+                             const uint32_t *root, const uint32_t *rank, BNode *blocks) {
+  for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < n; g += gridDim.x * blockDim.x) {
+    const PNode p = nodes[g];
+    BNode b;
+    b.thr = p.thr;
+    b.feat = p.feat;
+    if (p.feat < 0) {
+      b.a = p.value;
+      b.b = 0u;
+    } else if (depth[g] % 3u == 2u) {
+      b.a = rank[g + 1];
+      b.b = rank[p.right];
+    } else {
+      b.a = b.b = 0u;
+    }
+    blocks[(size_t)rank[root[g]] * kBlockSlots + slot[g]] = b;
+  }
+}
+
+// Root block of every tree.
+__global__ void k_block_roots(const uint32_t *first, uint32_t T, const uint32_t *rank, uint32_t *broots) {
+  for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < T; t += gridDim.x * blockDim.x)
+    broots[t] = rank[first[t]];
+}
+
 // ---- predict ----------------------------------------------------------------
 // Leaf label of sample x from node `node`: each level loads both children
 // (the left one is the next node) with the split feature's value, so a
 // level costs one memory latency rather than two.
-__device__ __forceinline__ uint32_t walk(const PNode *__restrict__ nodes, const float *__restrict__ x, uint32_t node) {
-  PNode nd = nodes[node];
+// The blocked table: inside a block both children come from the line the
+// block's first node fetched; at a block's bottom level both child blocks'
+// first nodes are loaded with the split feature's value, so every level
+// costs one memory latency and three levels one line from L2.
+__device__ __forceinline__ uint32_t walk(const BNode *__restrict__ blocks, const float *__restrict__ x, uint32_t b) {
+  const BNode *blk = blocks + (size_t)b * kBlockSlots;
+  uint32_t s = 0;
+  BNode nd = blk[0];
   while (nd.feat >= 0) {
     const float v = x[nd.feat];
-    const PNode l = nodes[node + 1], r = nodes[nd.right];
+    const bool inner = s < 3u;
+    const BNode *lp = inner ? blk + 2u * s + 1u : blocks + (size_t)nd.a * kBlockSlots;
+    const BNode *rp = inner ? blk + 2u * s + 2u : blocks + (size_t)nd.b * kBlockSlots;
+    const BNode l = *lp, r = *rp;
     const bool left = v < nd.thr;
-    node = left ? node + 1 : nd.right;
+    if (inner) {
+      s = left ? 2u * s + 1u : 2u * s + 2u;
+    } else {
+      blk = left ? lp : rp;
+      s = 0;
+    }
     nd = left ? l : r;
   }
-  return nd.value;
+  return nd.a;
 }
 
-__global__ __launch_bounds__(256) void k_predict(const PNode *__restrict__ nodes, const uint32_t *__restrict__ roots,
+__global__ __launch_bounds__(256) void k_predict(const BNode *__restrict__ nodes, const uint32_t *__restrict__ roots,
                                                  uint32_t T, const float *__restrict__ X, uint64_t R, uint32_t F,
                                                  uint32_t K, uint32_t *__restrict__ votes) {
   const uint64_t total = R * T;
@@ -555,10 +640,9 @@ __global__ __launch_bounds__(256) void k_predict(const PNode *__restrict__ nodes
 
 // The same traversal with the trees split over the XCDs: workgroup b walks
 // the trees t with t % kXcd == b % kXcd, so each XCD's L2 holds its eighth of
-// the node table.  Items (row, tree of the class) run tree-fastest, so the
-// lanes of a wave share a row of X.  Pays when an eighth of the node table
-// fits an L2.  Needs gridDim.x a multiple of kXcd.
-__global__ __launch_bounds__(256) void k_predict_xcd(const PNode *__restrict__ nodes,
+// the blocks.  Items (row, tree of the class) run tree-fastest, so the lanes
+// of a wave share a row of X.  Needs gridDim.x a multiple of kXcd.
+__global__ __launch_bounds__(256) void k_predict_xcd(const BNode *__restrict__ nodes,
                                                      const uint32_t *__restrict__ roots, uint32_t T,
                                                      const float *__restrict__ X, uint64_t R, uint32_t F, uint32_t K,
                                                      uint32_t *__restrict__ votes) {

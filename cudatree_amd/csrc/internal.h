@@ -118,6 +118,18 @@ struct PNode {
   uint32_t value;
 };
 
+// Predict table in blocks: a node at depth 3k of its tree roots a block that
+// holds its subtree down to depth 3k + 2 in heap order (slot 0 the root,
+// slots 2s + 1 and 2s + 2 the children of slot s; slot 7 unused).  A block is
+// one 128-byte line, so a walk fetches one line per three levels and takes
+// the children inside a block from that line.
+struct BNode {
+  float thr;
+  int32_t feat;   // -1: leaf
+  uint32_t a, b;  // leaf: a = label; split node in slots 3..6: blocks of its left / right child
+};
+constexpr uint32_t kBlockSlots = 8;
+
 // Node storage of a resident batch of trees, indexed by arena id.
 struct Arena {
   int32_t *feat;   // -1 for a leaf
