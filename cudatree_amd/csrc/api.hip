@@ -561,7 +561,13 @@ struct Fitter {
     const size_t nbslot = kByteKinds * (size_t)kByteSlots * kByteStride;
     unsigned long long *bslot = ctx->ws_as<unsigned long long>("byte_slots", nbslot);
     fill_lds = !o.fill_global && fill_lds_bytes() <= (size_t)ctx->lds_per_block;
-    Jcap = fill_lds ? (uint32_t)std::max<int64_t>(1, std::min<int64_t>(16, (3 * (int64_t)ctx->num_cu) / Tb)) : 1u;
+#ifndef CTA_FILL_JMUL
+#define CTA_FILL_JMUL 3
+#define CTA_FILL_JMAX 16
+#endif
+    Jcap = fill_lds ? (uint32_t)std::max<int64_t>(
+                          1, std::min<int64_t>(CTA_FILL_JMAX, (CTA_FILL_JMUL * (int64_t)ctx->num_cu) / Tb))
+                    : 1u;
     FNode *front[2] = {ctx->ws_as<FNode>("front0", cap_q), ctx->ws_as<FNode>("front1", cap_q)};
     uint32_t *cnt[2] = {ctx->ws_as<uint32_t>("cnt0", cap_q * K), ctx->ws_as<uint32_t>("cnt1", cap_q * K)};
     uint32_t *cntL = ctx->ws_as<uint32_t>("cntL", cap_q * K);
@@ -919,9 +925,10 @@ struct Fitter {
     Timed tm(ctx, "k_fill");
     if (fill_lds) {
       // fill tasks per tree (at most one per node plus one per Cf samples),
-      // about 32 per workgroup, at most Jcap workgroups per tree
+      // about kFillTasksPerGroup per workgroup, at most Jcap workgroups per tree
       const uint64_t est = ((uint64_t)q + n_tot / Cf) / std::max<uint32_t>(1, a.T);
-      const uint32_t J = (uint32_t)std::min<uint64_t>(Jcap, std::max<uint64_t>(1, (est + 31) / 32));
+      const uint32_t J = (uint32_t)std::min<uint64_t>(
+          Jcap, std::max<uint64_t>(1, (est + kFillTasksPerGroup - 1) / kFillTasksPerGroup));
       static std::once_flag attr;
       std::call_once(attr, [&] {
         HIPC(hipFuncSetAttribute((const void *)k_fill_lds<IdxT, LabT>, hipFuncAttributeMaxDynamicSharedMemorySize,

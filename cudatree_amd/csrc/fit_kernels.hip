@@ -1283,7 +1283,10 @@ __device__ __forceinline__ void fill_task_bits(const FitArgs<IdxT, LabT> &a, con
 
 // Fill workgroups of the tree whose frontier is [lo, hi): one per
 // kFillTasksPerGroup of its fill slots [fb, fe), at most J.
-constexpr uint32_t kFillTasksPerGroup = 32;
+#ifndef CTA_FILL_TPG
+#define CTA_FILL_TPG 32
+#endif
+constexpr uint32_t kFillTasksPerGroup = CTA_FILL_TPG;
 template <class IdxT, class LabT>
 __device__ __forceinline__ uint32_t fill_groups(const FitArgs<IdxT, LabT> &a, uint32_t lo, uint32_t hi, uint32_t J,
                                                 uint32_t &fb, uint32_t &fe) {
