@@ -910,7 +910,10 @@ struct Fitter {
   template <int NF>
   void offsets(const FitArgs<IdxT, LabT> &a, uint32_t qb, uint32_t *tot) {
     const uint32_t *q = &a.ctl->q;
-    if (qb <= 8 * kOffTile) {
+#ifndef CTA_OFF_SINGLE_TILES
+#define CTA_OFF_SINGLE_TILES 2
+#endif
+    if (qb <= CTA_OFF_SINGLE_TILES * kOffTile) {
       k_off_single<NF><<<1, kOffSingleThreads, 0, st>>>(a.vals, q, 1u, a.offs, tot);
     } else {
       const uint32_t nb = (qb + kOffTile - 1) / kOffTile;
@@ -1044,7 +1047,10 @@ struct Fitter {
   void reshuffle(const FitArgs<IdxT, LabT> &a, bool boot = false, hipEvent_t ev = nullptr) {
     // the plan splits the work into about 4 units per CU; as many workgroups
     // (two stay resident per CU) let the CUs that finish early take more
-    const uint32_t grid = (uint32_t)std::max(8, ctx->num_cu * 4);
+#ifndef CTA_RESH_PER_CU
+#define CTA_RESH_PER_CU 4
+#endif
+    const uint32_t grid = (uint32_t)std::max(8, ctx->num_cu * CTA_RESH_PER_CU);
     k_plan<IdxT, LabT><<<1, kPlanThreads, 0, st>>>(a, grid, boot ? 0u : 1u);
     if (ev) HIPC(hipEventRecord(ev, st));
     if (boot) launch_resh(ctx, a, o.resh_global != 0, grid, "k_boot_compact");

@@ -850,7 +850,10 @@ constexpr uint32_t kReshTile = kReshThreads * kBigE;
 constexpr uint32_t kSlots = kBigE * kReshWaves;  // (sub-tile, wave) counts per tile
 constexpr uint32_t kSpl = kSlots / kWave;        // of them per lane in the tile prefix
 static_assert(kSlots % kWave == 0 && kReshWaves % kSpl == 0, "the tile prefix takes whole slots per lane");
-constexpr uint64_t kWgElemsMax = 1ull << 19, kWgElemsMin = 1ull << 16;
+#ifndef CTA_WG_ELEMS_MIN_LOG2
+#define CTA_WG_ELEMS_MIN_LOG2 16
+#endif
+constexpr uint64_t kWgElemsMax = 1ull << 19, kWgElemsMin = 1ull << CTA_WG_ELEMS_MIN_LOG2;
 
 // Label of the majority class (lowest label on ties).
 __device__ uint32_t wave_majority(const uint32_t *c, uint32_t K) {
