@@ -101,10 +101,16 @@ __device__ __forceinline__ void flush_bytes(const FitArgs<IdxT, LabT> &a, ByteKi
 
 // Distinct-value index of sample s in a value code column of kind 1 (bits),
 // 2 (bytes) or 3 (16-bit).
+template <uint32_t kKind>
+__device__ __forceinline__ uint32_t code_at_k(const uint8_t *cb, uint32_t s) {
+  if constexpr (kKind == 1) return (reinterpret_cast<const uint32_t *>(cb)[s >> 5] >> (s & 31u)) & 1u;
+  else if constexpr (kKind == 2) return cb[s];
+  else return reinterpret_cast<const uint16_t *>(cb)[s];
+}
 __device__ __forceinline__ uint32_t code_at(const uint8_t *cb, uint32_t kind, uint32_t s) {
-  if (kind == 1) return (reinterpret_cast<const uint32_t *>(cb)[s >> 5] >> (s & 31u)) & 1u;
-  if (kind == 2) return cb[s];
-  return reinterpret_cast<const uint16_t *>(cb)[s];
+  if (kind == 1) return code_at_k<1>(cb, s);
+  if (kind == 2) return code_at_k<2>(cb, s);
+  return code_at_k<3>(cb, s);
 }
 
 // Row entries at positions pb + u * kWave + lane (u < U) of a row whose valid
@@ -603,29 +609,25 @@ __device__ __forceinline__ Cand scan_positions(const FitArgs<IdxT, LabT> &a, con
   const uint32_t lane = lane_id();
   Cand lb = {0, 1, 0, (int32_t)f, 0};
   for (uint32_t pb = p0; pb < p1; pb += kWave * kScanU) {
-    // kScanU tiles in flight: index loads, then label and value gathers
+    // kScanU tiles in flight: index loads, then label and value gathers,
+    // all unmasked (positions past p1 - 1 read the last entry; the scan
+    // below masks them) so that each group issues without waits between
     uint32_t sid[kScanU], lab[kScanU], val[kScanU];
 #pragma unroll
-    for (int u = 0; u < kScanU; ++u) {
-      const uint32_t p = pb + (uint32_t)u * kWave + lane;
-      sid[u] = (p < p1) ? (uint32_t)row[p] : 0xFFFFFFFFu;
-    }
+    for (int u = 0; u < kScanU; ++u) sid[u] = (uint32_t)row[min(pb + (uint32_t)u * kWave + lane, p1 - 1u)];
     const uint32_t pe = pb + kWave * kScanU;  // first position after the batch
-    const uint32_t batch_next = (pe < p1 && lane == kWave - 1) ? vat(sample_of(row[pe])) : 0u;
+    const uint32_t e_next = (uint32_t)row[min(pe, p1 - 1u)];  // one address for the wave
 #pragma unroll
     for (int u = 0; u < kScanU; ++u) {
-      lab[u] = 0;
-      val[u] = 0;
-      if (sid[u] != 0xFFFFFFFFu) {
-        lab[u] = a.label((IdxT)sid[u]);
-        val[u] = vat(sample_of((IdxT)sid[u]));
-      }
+      lab[u] = a.label((IdxT)sid[u]);
+      val[u] = vat(sample_of((IdxT)sid[u]));
     }
+    const uint32_t batch_next = vat(sample_of((IdxT)e_next));  // used only when pe < p1
 #pragma unroll
     for (int u = 0; u < kScanU; ++u) {
       const uint32_t p = pb + (uint32_t)u * kWave + lane;
       const bool valid = p < p1;
-      if (!ballot(valid)) break;
+      if (u > 0 && !ballot(valid)) break;  // the first tile holds position pb < p1
       // rank of this lane among earlier lanes with the same label (one
       // ballot per distinct label of the tile), and its class total
       uint32_t cprev = 0, dD = 0;
@@ -735,6 +737,7 @@ __global__ __launch_bounds__(kBlock, 6) void k_scan(FitArgs<IdxT, LabT> a) {
       auto cand = [&](uint32_t v, uint32_t w) -> bool {
         return kind ? v != w : gap_ok(__uint_as_float(v), __uint_as_float(w));
       };
+      auto differ = [](uint32_t v, uint32_t w) -> bool { return v != w; };
       if (kind == 1u && tk.nch == 1 && regs && n > kWave * kScanU) {
         // Two-valued feature (the node holds both values: the selection
         // dropped constant features): its one candidate is the position
@@ -808,7 +811,21 @@ __global__ __launch_bounds__(kBlock, 6) void k_scan(FitArgs<IdxT, LabT> a) {
       wave_sync();
       // value just past the chunk end (the next chunk's first sample)
       const uint32_t chunk_next = (p1 < n) ? vpe : 0u;
-      Cand lb = scan_positions(a, row, p0, p1, n, f, regs, cv, tv, carry, tot, SL, D, T2, chunk_next, vat, cand);
+      // one instance per column kind, so the position loop's gathers carry
+      // no branches on the kind
+      Cand lb;
+      if (kind == 2u)
+        lb = scan_positions(a, row, p0, p1, n, f, regs, cv, tv, carry, tot, SL, D, T2, chunk_next,
+                            [&](uint32_t s) { return code_at_k<2>(cb, s); }, differ);
+      else if (kind == 3u)
+        lb = scan_positions(a, row, p0, p1, n, f, regs, cv, tv, carry, tot, SL, D, T2, chunk_next,
+                            [&](uint32_t s) { return code_at_k<3>(cb, s); }, differ);
+      else if (kind == 1u)
+        lb = scan_positions(a, row, p0, p1, n, f, regs, cv, tv, carry, tot, SL, D, T2, chunk_next,
+                            [&](uint32_t s) { return code_at_k<1>(cb, s); }, differ);
+      else
+        lb = scan_positions(a, row, p0, p1, n, f, regs, cv, tv, carry, tot, SL, D, T2, chunk_next,
+                            [&](uint32_t s) { return __float_as_uint(x[s]); }, cand);
       if (cand_better(lb, best)) best = lb;
     }
     if (lane == 0) {
