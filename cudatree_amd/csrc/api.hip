@@ -898,7 +898,7 @@ struct Fitter {
   }
 
   void account(const Ctl &prev, const Ctl &now) {
-    ctx->add_bytes("k_scan", now.bytes_scan - prev.bytes_scan);
+    ctx->add_bytes("k_scan", (now.bytes_scan - prev.bytes_scan) - (now.bytes_scan_skip - prev.bytes_scan_skip));
     ctx->add_bytes("k_fill", now.bytes_fill - prev.bytes_fill);
     ctx->add_bytes("k_resh", (now.bytes_resh - prev.bytes_resh) + (now.bytes_resh_small - prev.bytes_resh_small));
     ctx->add_bytes("k_rl_count", now.bytes_rl - prev.bytes_rl);

@@ -548,19 +548,51 @@ __global__ __launch_bounds__(kBlock) void k_scan_hist(FitArgs<IdxT, LabT> a) {
   }
 }
 
-// Exclusive prefix over the chunks of each multi-chunk (node, feature).
+// Result-entry mark of a chunk that k_scan_carry has ruled out.
+constexpr uint32_t kScanSkipped = 1u;
+
+// Exclusive prefix over the chunks of each multi-chunk (node, feature);
+// each chunk's result entry is cleared on the way, and marked skipped when
+// no position of the chunk can hold the node's split.  The node's best
+// row-free candidate (scored exactly by the selection) beats every position
+// that scores lower.  For p in [p0, p1) the left class counts lie between
+// the chunk's carries and the next chunk's (sum of squares at most SLe,
+// with nL >= p0 + 1) and the right counts only shrink (sum of squares at
+// most SRs = sum_k (t_k - c_k)^2 at p0, with nR >= n - p1), so no position
+// of the chunk scores above SLe / (p0 + 1) + SRs / (n - p1).  The test
+// keeps a relative margin of 2^-30 on each side, far above the rounding of
+// these doubles (the integers are below 2^53, the row-free score's
+// numerator below 2^64).  It is made for up to kC classes, whose counts
+// the prefix pass holds in registers.
 template <class IdxT, class LabT>
 __global__ __launch_bounds__(kBlock) void k_scan_carry(FitArgs<IdxT, LabT> a) {
   if (a.ctl->err) return;  // a list overflowed: the fit fails, nothing more is read
   const uint32_t lane = lane_id();
   const uint32_t K = (uint32_t)a.K;
   constexpr uint32_t kC = 8;  // classes in flight
+  unsigned long long skipped = 0;  // algorithmic split-scan bytes of the chunks ruled out
   WAVE_LOOP(a.ctl->n_scan) {
     const Task tk = a.scan_tasks[item];
     if (tk.nch <= 1 || tk.chunk != 0) continue;
     const NAux ax = a.aux[tk.node];
     const uint32_t j = tk.first_count & 0xFFFFu;
     uint32_t *base = a.hist + ax.hist_base + (size_t)j * tk.nch * K;
+    ScanRes *rs = a.res + ax.scan_base + (size_t)j * tk.nch;
+    const int32_t f = a.feats[(size_t)tk.node * a.m + j];
+    const bool prune = ax.rl_feat >= 0 && K <= kC;
+    uint32_t t[kC] = {};
+    uint64_t T2 = 0;
+    uint32_t n = 0;
+    double rl = 0.0;
+    if (prune) {
+      n = a.front[tk.node].n;
+#pragma unroll
+      for (uint32_t c = 0; c < kC; ++c) {
+        t[c] = c < K ? a.cnt[(size_t)tk.node * K + c] : 0u;
+        T2 += (uint64_t)t[c] * t[c];
+      }
+      rl = (double)ax.rl_num / (double)ax.rl_den;
+    }
     // per class, the chunk histograms become exclusive prefix sums over
     // the chunks: lanes take 64 chunks at a time, kC classes at once
     for (uint32_t k0 = 0; k0 < K; k0 += kC) {
@@ -573,15 +605,39 @@ __global__ __launch_bounds__(kBlock) void k_scan_carry(FitArgs<IdxT, LabT> a) {
 #pragma unroll
         for (uint32_t c = 0; c < kC; ++c)
           v[c] = (ch < tk.nch && k0 + c < K) ? base[(size_t)ch * K + k0 + c] : 0u;
+        uint64_t sl = 0, d = 0, sle = 0;  // at the chunk's start: sum c^2, sum t c; at its end: sum c^2
 #pragma unroll
         for (uint32_t c = 0; c < kC; ++c) {
           const uint32_t inc = wave_incl_scan(v[c]);
-          if (ch < tk.nch && k0 + c < K) base[(size_t)ch * K + k0 + c] = carry[c] + inc - v[c];
+          const uint32_t before = carry[c] + inc - v[c], after = carry[c] + inc;
+          if (ch < tk.nch && k0 + c < K) base[(size_t)ch * K + k0 + c] = before;
+          sl += (uint64_t)before * before;
+          d += (uint64_t)t[c] * before;
+          sle += (uint64_t)after * after;
           carry[c] += readlane(inc, kWave - 1);
+        }
+        if (k0 == 0 && ch < tk.nch) {
+          bool skip = false;
+          if (prune && ch + 1 < tk.nch) {
+            const uint32_t p0 = ch * a.Cs, p1 = p0 + a.Cs;  // p1 < n: a later chunk exists
+            const uint64_t SRs = T2 + sl - 2 * d;
+            const double ub = (double)sle / (double)(p0 + 1u) + (double)SRs / (double)(n - p1);
+            skip = ub * (1.0 + 0x1p-30) < rl * (1.0 - 0x1p-30);
+            if (skip) skipped += (unsigned long long)a.Cs * (sizeof(IdxT) + sizeof(LabT) + 4);
+          }
+          ScanRes r;
+          r.num = 0;
+          r.den = 1;
+          r.pos = 0;
+          r.feat = f;
+          r.ok = 0;
+          r.pad = skip ? kScanSkipped : 0u;
+          rs[ch] = r;
         }
       }
     }
   }
+  flush_bytes(a, kBytesScanSkip, wave_sum64(skipped));
 }
 
 // ------------------------------------------------------------------------
@@ -707,6 +763,9 @@ __global__ __launch_bounds__(kBlock, 6) void k_scan(FitArgs<IdxT, LabT> a) {
     const uint32_t i = tk.node;
     const FNode nd = a.front[i];
     const NAux ax = a.aux[i];
+    // a chunk k_scan_carry ruled out keeps the result entry it wrote
+    if (tk.nch > 1 && a.res[ax.scan_base + (size_t)(tk.first_count & 0xFFFFu) * tk.nch + tk.chunk].pad == kScanSkipped)
+      continue;
     const uint32_t n = nd.n;
     // class totals: lane k holds class k when K <= 64 (else in LDS)
     const bool regs = K <= (uint32_t)kWave && !a.hist_lds;
@@ -849,7 +908,10 @@ __global__ __launch_bounds__(kBlock, 6) void k_scan(FitArgs<IdxT, LabT> a) {
 // tiles); smaller nodes by single waves (one task per group of rows holding
 // about Cr elements).
 // ------------------------------------------------------------------------
-constexpr uint32_t kBigN = 4096;
+#ifndef CTA_BIG_N
+#define CTA_BIG_N 4096
+#endif
+constexpr uint32_t kBigN = CTA_BIG_N;
 constexpr int kReshThreads = 1024;
 constexpr int kReshWaves = kReshThreads / kWave;  // 16
 constexpr int kReshE = 8;  // elements per lane in flight in a small task

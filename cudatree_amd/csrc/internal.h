@@ -82,7 +82,8 @@ struct Ctl {
   uint32_t pad1[2];
   uint32_t n_scan, n_hist, n_fill, n_cls[kXcd], pad2;    // selection-scan totals
   uint32_t c_ids, c_live, c_big, c_small, c_tiles, c_sub, pad3[2];  // children-scan totals
-  unsigned long long bytes_scan, bytes_fill, bytes_resh, bytes_boot, bytes_resh_small, bytes_rl, bytes_sub;
+  unsigned long long bytes_scan, bytes_fill, bytes_resh, bytes_boot, bytes_resh_small, bytes_rl, bytes_sub,
+      bytes_scan_skip;
   // small subtrees: nodes queued so far (whole levels), the next one a
   // subtree wave takes, and the deepest node depth the subtree waves made
   uint32_t n_sub, sub_next, sub_depth, pad4;
@@ -104,9 +105,10 @@ enum ByteKind {
   kBytesBoot = 3,
   kBytesReshSmall = 4,
   kBytesRl = 5,
-  kBytesSub = 6
+  kBytesSub = 6,
+  kBytesScanSkip = 7  // split-scan bytes of chunks skipped by the score bound
 };
-constexpr uint32_t kByteKinds = 7;
+constexpr uint32_t kByteKinds = 8;
 constexpr uint32_t kErrCapacity = 1u, kErrBadNode = 2u;
 
 // Flattened forest node for prediction: pre-order, so a split node's left
