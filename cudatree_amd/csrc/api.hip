@@ -456,6 +456,7 @@ struct Fitter {
   hipStream_t st;
   uint32_t G;      // blocks of the wave-loop kernels
   uint32_t Gscan;  // blocks of k_scan: all resident at once, a multiple of kXcd
+  bool scan_regs;  // k_scan keeps the class counts in lane registers
   uint64_t n_tot = 0;      // samples over the batch's trees
   bool fill_lds = false;   // side bitmaps built in LDS (N / 8 bytes fit a workgroup)
   // rows kept: every feature, or all but the row-free ones
@@ -486,7 +487,9 @@ struct Fitter {
     // a wave-loop grid larger than what stays resident leaves a tail of
     // late workgroups; k_scan's registers allow fewer than 8 per CU
     int nb = 0;
-    HIPC(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_scan<IdxT, LabT>, kBlock, smem()));
+    scan_regs = (uint32_t)K <= kScanRegsK && opts.hist_layout != CTA_HIST_LDS;
+    HIPC(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+        &nb, scan_regs ? k_scan<IdxT, LabT, true> : k_scan<IdxT, LabT, false>, kBlock, smem()));
     nb = std::min(std::max(nb, 1), 8);
     Gscan = std::max<uint32_t>(kXcd, (uint32_t)(ctx->num_cu * nb) / kXcd * kXcd);
   }
@@ -1026,7 +1029,8 @@ struct Fitter {
     k_scan_carry<IdxT, LabT><<<G, kBlock, 0, st>>>(a);
     {
       Timed tm(ctx, "k_scan");
-      k_scan<IdxT, LabT><<<Gscan, kBlock, sm, st>>>(a);
+      if (scan_regs) k_scan<IdxT, LabT, true><<<Gscan, kBlock, sm, st>>>(a);
+      else k_scan<IdxT, LabT, false><<<Gscan, kBlock, sm, st>>>(a);
     }
     {
       Timed tm(ctx, "k_decide");
@@ -2314,7 +2318,8 @@ static void hook_split_scan(cta_ctx *ctx, cta_dataset *ds, const uint32_t *rows,
   const uint32_t G = 64;
   k_scan_hist<IdxT, LabT><<<G, kBlock, sm, st>>>(a);
   k_scan_carry<IdxT, LabT><<<G, kBlock, 0, st>>>(a);
-  k_scan<IdxT, LabT><<<G, kBlock, sm, st>>>(a);
+  if ((uint32_t)K <= kScanRegsK && !a.hist_lds) k_scan<IdxT, LabT, true><<<G, kBlock, sm, st>>>(a);
+  else k_scan<IdxT, LabT, false><<<G, kBlock, sm, st>>>(a);
   k_decide<IdxT, LabT><<<G, kBlock, sm, st>>>(a);
   HIPC(hipGetLastError());
   NAux out = {};
@@ -2381,7 +2386,7 @@ static void hook_reshuffle(cta_ctx *ctx, cta_dataset *ds, const uint32_t *rows, 
   ax.dst = 2;
   uint32_t v[kChFields] = {0u, 0u, 0u, 0u, 0u};
   {
-    const bool big = n >= kBigN;
+    const bool big = n >= kBigN && (uint32_t)F < kBigMaxRows;
     const uint32_t rpt = big ? 1u : std::max(1u, Cr / n);
     const uint32_t ntask = big ? (uint32_t)F : ((uint32_t)F + rpt - 1) / rpt;
     ax.resh_big = big ? 1u : 0u;

@@ -486,6 +486,15 @@ __global__ __launch_bounds__(kBlock) void k_select_emit(FitArgs<IdxT, LabT> a) {
 // ------------------------------------------------------------------------
 // positions per lane in flight in a chunk-histogram task
 constexpr int kHistU = 8;
+// Chunks are ruled out (k_scan_carry) for up to kPruneMaxK classes; the
+// chunk boundaries of nodes of at least kBoundMinN samples join the known
+// candidates (below that, a node's chunks are too few for their value
+// gathers to pay).
+constexpr uint32_t kPruneMaxK = 8;
+#ifndef CTA_BOUND_MIN_N
+#define CTA_BOUND_MIN_N 32768
+#endif
+constexpr uint32_t kBoundMinN = CTA_BOUND_MIN_N;
 
 template <class IdxT, class LabT>
 __global__ __launch_bounds__(kBlock) void k_scan_hist(FitArgs<IdxT, LabT> a) {
@@ -553,23 +562,30 @@ constexpr uint32_t kScanSkipped = 1u;
 
 // Exclusive prefix over the chunks of each multi-chunk (node, feature);
 // each chunk's result entry is cleared on the way, and marked skipped when
-// no position of the chunk can hold the node's split.  The node's best
-// row-free candidate (scored exactly by the selection) beats every position
-// that scores lower.  For p in [p0, p1) the left class counts lie between
-// the chunk's carries and the next chunk's (sum of squares at most SLe,
-// with nL >= p0 + 1) and the right counts only shrink (sum of squares at
-// most SRs = sum_k (t_k - c_k)^2 at p0, with nR >= n - p1), so no position
-// of the chunk scores above SLe / (p0 + 1) + SRs / (n - p1).  The test
-// keeps a relative margin of 2^-30 on each side, far above the rounding of
-// these doubles (the integers are below 2^53, the row-free score's
-// numerator below 2^64).  It is made for up to kC classes, whose counts
-// the prefix pass holds in registers.
+// no position of the chunk can hold the node's split.  A split needs a
+// strictly higher score than any other candidate of the node, and two are
+// known here: the node's best row-free candidate (scored exactly by the
+// selection) and the feature's chunk boundaries whose two values pass the
+// candidate test (their left class counts are the carries).  For p in
+// [p0, p1) the left class counts lie between the chunk's carries and the
+// next chunk's (sum of squares at most SLe, with nL >= p0 + 1) and the
+// right counts only shrink (sum of squares at most SRs = sum_k (t_k -
+// c_k)^2 at p0, with nR >= n - p1), so no position of the chunk scores above
+// SLe / (p0 + 1) + SRs / (n - p1).  A chunk whose bound is below a known
+// candidate's score is skipped; the chunk holding that candidate is not
+// (its bound is at least the candidate's score).  The test keeps a relative
+// margin of 2^-30 on each side, far above the rounding of these doubles
+// (counts and sums of squares below 2^53, scores' numerators below 2^64).
+// It is made for up to kC classes, whose counts the prefix pass holds in
+// registers; while the prefix pass runs, a chunk's result entry holds its
+// bound.
 template <class IdxT, class LabT>
 __global__ __launch_bounds__(kBlock) void k_scan_carry(FitArgs<IdxT, LabT> a) {
   if (a.ctl->err) return;  // a list overflowed: the fit fails, nothing more is read
   const uint32_t lane = lane_id();
   const uint32_t K = (uint32_t)a.K;
   constexpr uint32_t kC = 8;  // classes in flight
+  static_assert(kPruneMaxK <= kC, "the bounds take every class from one prefix pass");
   unsigned long long skipped = 0;  // algorithmic split-scan bytes of the chunks ruled out
   WAVE_LOOP(a.ctl->n_scan) {
     const Task tk = a.scan_tasks[item];
@@ -579,19 +595,33 @@ __global__ __launch_bounds__(kBlock) void k_scan_carry(FitArgs<IdxT, LabT> a) {
     uint32_t *base = a.hist + ax.hist_base + (size_t)j * tk.nch * K;
     ScanRes *rs = a.res + ax.scan_base + (size_t)j * tk.nch;
     const int32_t f = a.feats[(size_t)tk.node * a.m + j];
-    const bool prune = ax.rl_feat >= 0 && K <= kC;
+    // prune: chunks are tested against the known candidates; bounds: the
+    // chunk boundaries join them, and the marks wait for a second pass
+    const FNode nd = a.front[tk.node];
+    const uint32_t n = nd.n;
+    const bool bounds = K <= kPruneMaxK && n >= kBoundMinN;
+    const bool prune = K <= kPruneMaxK && (bounds || ax.rl_feat >= 0);
     uint32_t t[kC] = {};
     uint64_t T2 = 0;
-    uint32_t n = 0;
-    double rl = 0.0;
+    uint32_t kind = 0;
+    double known = 0.0;  // the best score of the candidates known so far
+    const IdxT *row = nullptr;
+    const uint8_t *cb = nullptr;
+    const float *x = nullptr;
     if (prune) {
-      n = a.front[tk.node].n;
 #pragma unroll
       for (uint32_t c = 0; c < kC; ++c) {
         t[c] = c < K ? a.cnt[(size_t)tk.node * K + c] : 0u;
         T2 += (uint64_t)t[c] * t[c];
       }
-      rl = (double)ax.rl_num / (double)ax.rl_den;
+      if (ax.rl_feat >= 0) known = (double)ax.rl_num / (double)ax.rl_den;
+    }
+    if (bounds) {
+      row = rows_of(a, nd.src, nd.tree, (uint32_t)f) + nd.start;
+      const uint64_t fc = a.fcode[f];
+      kind = uniform(fc_kind(fc));
+      cb = a.codes + fc_off(fc);
+      x = a.X_T + (size_t)f * a.N;
     }
     // per class, the chunk histograms become exclusive prefix sums over
     // the chunks: lanes take 64 chunks at a time, kC classes at once
@@ -617,16 +647,30 @@ __global__ __launch_bounds__(kBlock) void k_scan_carry(FitArgs<IdxT, LabT> a) {
           carry[c] += readlane(inc, kWave - 1);
         }
         if (k0 == 0 && ch < tk.nch) {
-          bool skip = false;
-          if (prune && ch + 1 < tk.nch) {
-            const uint32_t p0 = ch * a.Cs, p1 = p0 + a.Cs;  // p1 < n: a later chunk exists
+          double ub = -1.0;  // no bound: the chunk is kept
+          if (prune) {
             const uint64_t SRs = T2 + sl - 2 * d;
-            const double ub = (double)sle / (double)(p0 + 1u) + (double)SRs / (double)(n - p1);
-            skip = ub * (1.0 + 0x1p-30) < rl * (1.0 - 0x1p-30);
-            if (skip) skipped += (unsigned long long)a.Cs * (sizeof(IdxT) + sizeof(LabT) + 4);
+            if (ch + 1 < tk.nch) {
+              const uint32_t p0 = ch * a.Cs, p1 = p0 + a.Cs;  // p1 < n: a later chunk exists
+              ub = (double)sle / (double)(p0 + 1u) + (double)SRs / (double)(n - p1);
+            }
+            if (bounds && ch >= 1) {
+              // the boundary before the chunk: positions p, p + 1 < n
+              const uint32_t p = ch * a.Cs - 1u;
+              const uint32_t s0 = sample_of(row[p]), s1 = sample_of(row[p + 1u]);
+              const uint32_t w0 = kind ? code_at(cb, kind, s0) : __float_as_uint(x[s0]);
+              const uint32_t w1 = kind ? code_at(cb, kind, s1) : __float_as_uint(x[s1]);
+              if (kind ? w0 != w1 : gap_ok(__uint_as_float(w0), __uint_as_float(w1))) {
+                const uint64_t nL = (uint64_t)ch * a.Cs, nR = (uint64_t)n - nL;
+                known = fmax(known, (double)(sl * nR + SRs * nL) / (double)(nL * nR));
+              }
+            }
           }
+          // without the boundaries the known score is final: mark now
+          const bool skip = !bounds && ub >= 0.0 && ub * (1.0 + 0x1p-30) < known * (1.0 - 0x1p-30);
+          if (skip) skipped += (unsigned long long)a.Cs * (sizeof(IdxT) + sizeof(LabT) + 4);
           ScanRes r;
-          r.num = 0;
+          r.num = bounds ? (unsigned long long)__double_as_longlong(ub) : 0ull;
           r.den = 1;
           r.pos = 0;
           r.feat = f;
@@ -634,6 +678,28 @@ __global__ __launch_bounds__(kBlock) void k_scan_carry(FitArgs<IdxT, LabT> a) {
           r.pad = skip ? kScanSkipped : 0u;
           rs[ch] = r;
         }
+      }
+    }
+    if (bounds) {
+      // the best known score over the lanes, rounded down
+#pragma unroll
+      for (uint32_t s = kWave / 2; s > 0; s >>= 1) known = fmax(known, __shfl_xor(known, (int)s, kWave));
+      const double thr = known * (1.0 - 0x1p-30);
+      for (uint32_t c0 = 0; c0 < tk.nch; c0 += kWave) {
+        const uint32_t ch = c0 + lane;
+        if (ch >= tk.nch) break;
+        // this lane stored the entry above
+        const double ub = __longlong_as_double((long long)rs[ch].num);
+        const bool skip = ub >= 0.0 && ub * (1.0 + 0x1p-30) < thr;
+        if (skip) skipped += (unsigned long long)a.Cs * (sizeof(IdxT) + sizeof(LabT) + 4);
+        ScanRes r;
+        r.num = 0;
+        r.den = 1;
+        r.pos = 0;
+        r.feat = f;
+        r.ok = 0;
+        r.pad = skip ? kScanSkipped : 0u;
+        rs[ch] = r;
       }
     }
   }
@@ -647,6 +713,17 @@ __global__ __launch_bounds__(kBlock) void k_scan_carry(FitArgs<IdxT, LabT> a) {
 // in order.
 // ------------------------------------------------------------------------
 constexpr int kScanU = 4;
+// Label ranks in the split scan: with the class counts in registers (lane k
+// holds class k), one ballot per distinct label of a tile; with them in LDS
+// (more classes), one ballot per label bit.
+#ifndef CTA_SCAN_REGS_K
+#define CTA_SCAN_REGS_K 8
+#endif
+#ifndef CTA_RANK_BITS
+#define CTA_RANK_BITS 1
+#endif
+constexpr uint32_t kScanRegsK = CTA_SCAN_REGS_K;
+constexpr bool kRankByBits = CTA_RANK_BITS != 0;
 
 // Gini scan of positions [p0, p1) of a node's sorted row (n samples, feature
 // f): the best candidate position of the range, in every lane.  SL and D are
@@ -657,11 +734,11 @@ constexpr int kScanU = 4;
 // cand(v, w) the candidate test between adjacent values.  kScanU tiles of 64
 // positions are in flight: their index loads, then their label and value
 // gathers, are issued together before the tiles are scanned in order.
-template <class IdxT, class LabT, class VAt, class CandF>
+template <bool regs, class IdxT, class LabT, class VAt, class CandF>
 __device__ __forceinline__ Cand scan_positions(const FitArgs<IdxT, LabT> &a, const IdxT *row, uint32_t p0,
-                                               uint32_t p1, uint32_t n, uint32_t f, bool regs, uint32_t cv,
-                                               uint32_t tv, uint32_t *carry, const uint32_t *tot, uint64_t SL,
-                                               uint64_t D, uint64_t T2, uint32_t chunk_next, VAt vat, CandF cand) {
+                                               uint32_t p1, uint32_t n, uint32_t f, uint32_t cv, uint32_t tv,
+                                               uint32_t *carry, const uint32_t *tot, uint64_t SL, uint64_t D,
+                                               uint64_t T2, uint32_t chunk_next, VAt vat, CandF cand) {
   const uint32_t lane = lane_id();
   Cand lb = {0, 1, 0, (int32_t)f, 0};
   for (uint32_t pb = p0; pb < p1; pb += kWave * kScanU) {
@@ -688,7 +765,7 @@ __device__ __forceinline__ Cand scan_positions(const FitArgs<IdxT, LabT> &a, con
       // ballot per distinct label of the tile), and its class total
       uint32_t cprev = 0, dD = 0;
       uint64_t rem = ballot(valid);
-      if (regs) {
+      if constexpr (regs) {
         while (rem) {
           const uint32_t leader = (uint32_t)__ffsll((unsigned long long)rem) - 1u;
           const uint32_t L = readlane(lab[u], leader);
@@ -701,6 +778,24 @@ __device__ __forceinline__ Cand scan_positions(const FitArgs<IdxT, LabT> &a, con
           if (lane == L) cv += (uint32_t)__popcll(mm);
           rem &= ~mm;
         }
+      } else if (kRankByBits) {
+        // the lanes holding this lane's label are those that agree with it
+        // on every label bit (one ballot per bit, whatever the number of
+        // labels in the tile); the last of them stores the label's new count
+        const uint32_t nbits = a.K > 1 ? 32u - (uint32_t)__builtin_clz((uint32_t)a.K - 1u) : 1u;
+        uint64_t same = rem;
+        for (uint32_t b = 0; b < nbits; ++b) {
+          const uint32_t bit = (lab[u] >> b) & 1u;
+          const uint64_t mb = ballot(bit != 0u);
+          same &= bit ? mb : ~mb;
+        }
+        if (valid) {
+          cprev = carry[lab[u]] + mbcnt(same);
+          dD = tot[lab[u]];
+        }
+        wave_sync();
+        if (valid && (same >> lane) == 1ull) carry[lab[u]] = cprev + 1u;
+        wave_sync();
       } else {
         while (rem) {
           const uint32_t leader = (uint32_t)__ffsll((unsigned long long)rem) - 1u;
@@ -743,7 +838,10 @@ __device__ __forceinline__ Cand scan_positions(const FitArgs<IdxT, LabT> &a, con
   return wave_best(lb);
 }
 
-template <class IdxT, class LabT>
+// kRegs: the class counts of a wave in lane registers (lane k holds class
+// k; at most kScanRegsK classes), else in LDS.  The host launches the one
+// instance that fits K.
+template <class IdxT, class LabT, bool kRegs>
 // Six waves per SIMD (a few registers spilled) beat five without spills.
 __global__ __launch_bounds__(kBlock, 6) void k_scan(FitArgs<IdxT, LabT> a) {
   if (a.ctl->err) return;  // a list overflowed: the fit fails, nothing more is read
@@ -767,8 +865,8 @@ __global__ __launch_bounds__(kBlock, 6) void k_scan(FitArgs<IdxT, LabT> a) {
     if (tk.nch > 1 && a.res[ax.scan_base + (size_t)(tk.first_count & 0xFFFFu) * tk.nch + tk.chunk].pad == kScanSkipped)
       continue;
     const uint32_t n = nd.n;
-    // class totals: lane k holds class k when K <= 64 (else in LDS)
-    const bool regs = K <= (uint32_t)kWave && !a.hist_lds;
+    // class totals: lane k holds class k (kRegs), else in LDS
+    constexpr bool regs = kRegs;
     uint64_t t2 = 0;
     uint32_t tv = 0;
     for (uint32_t k = lane; k < K; k += kWave) {
@@ -872,19 +970,18 @@ __global__ __launch_bounds__(kBlock, 6) void k_scan(FitArgs<IdxT, LabT> a) {
       const uint32_t chunk_next = (p1 < n) ? vpe : 0u;
       // one instance per column kind, so the position loop's gathers carry
       // no branches on the kind
+      auto scan = [&](auto at, auto test) -> Cand {
+        return scan_positions<kRegs>(a, row, p0, p1, n, f, cv, tv, carry, tot, SL, D, T2, chunk_next, at, test);
+      };
       Cand lb;
       if (kind == 2u)
-        lb = scan_positions(a, row, p0, p1, n, f, regs, cv, tv, carry, tot, SL, D, T2, chunk_next,
-                            [&](uint32_t s) { return code_at_k<2>(cb, s); }, differ);
+        lb = scan([&](uint32_t s) { return code_at_k<2>(cb, s); }, differ);
       else if (kind == 3u)
-        lb = scan_positions(a, row, p0, p1, n, f, regs, cv, tv, carry, tot, SL, D, T2, chunk_next,
-                            [&](uint32_t s) { return code_at_k<3>(cb, s); }, differ);
+        lb = scan([&](uint32_t s) { return code_at_k<3>(cb, s); }, differ);
       else if (kind == 1u)
-        lb = scan_positions(a, row, p0, p1, n, f, regs, cv, tv, carry, tot, SL, D, T2, chunk_next,
-                            [&](uint32_t s) { return code_at_k<1>(cb, s); }, differ);
+        lb = scan([&](uint32_t s) { return code_at_k<1>(cb, s); }, differ);
       else
-        lb = scan_positions(a, row, p0, p1, n, f, regs, cv, tv, carry, tot, SL, D, T2, chunk_next,
-                            [&](uint32_t s) { return __float_as_uint(x[s]); }, cand);
+        lb = scan([&](uint32_t s) { return __float_as_uint(x[s]); }, cand);
       if (cand_better(lb, best)) best = lb;
     }
     if (lane == 0) {
@@ -912,6 +1009,13 @@ __global__ __launch_bounds__(kBlock, 6) void k_scan(FitArgs<IdxT, LabT> a) {
 #define CTA_BIG_N 4096
 #endif
 constexpr uint32_t kBigN = CTA_BIG_N;
+// A node with at least kBigMaxRows rows gives the level enough waves with
+// one wave per group of rows, which needs no workgroup barriers: its rows
+// never take big tasks.
+#ifndef CTA_BIG_MAX_ROWS
+#define CTA_BIG_MAX_ROWS 1024
+#endif
+constexpr uint32_t kBigMaxRows = CTA_BIG_MAX_ROWS;
 constexpr int kReshThreads = 1024;
 constexpr int kReshWaves = kReshThreads / kWave;  // 16
 constexpr int kReshE = 8;  // elements per lane in flight in a small task
@@ -957,7 +1061,7 @@ __device__ uint32_t wave_majority(const uint32_t *c, uint32_t K) {
 // Reshuffle task shape of a node of n samples with a live child; fills
 // v[2..4] (big tasks, small tasks, big tiles) of its children-scan counts.
 __device__ __forceinline__ void resh_shape(uint32_t n, uint32_t F, uint32_t Cr, NAux &ax, uint32_t *v) {
-  const bool big = n >= kBigN;
+  const bool big = n >= kBigN && F < kBigMaxRows;
   const uint32_t rpt = big ? 1u : max(1u, Cr / n);
   const uint32_t ntask = big ? F : ceil_div(F, rpt);
   ax.resh_big = big ? 1u : 0u;

@@ -190,7 +190,10 @@ __global__ __launch_bounds__(kSubThreads) void k_subtree(FitArgs<IdxT, LabT> a) 
           };
           for (uint32_t c = lane; c < K; c += kWave) carry[c] = 0;
           wave_sync();
-          const Cand c = scan_positions(a, row, 0u, n, n, fr, regs, 0u, tv, carry, tot, 0ull, 0ull, T2, 0u, vat, cand);
+          const Cand c = regs ? scan_positions<true>(a, row, 0u, n, n, fr, 0u, tv, carry, tot, 0ull, 0ull, T2, 0u,
+                                                     vat, cand)
+                              : scan_positions<false>(a, row, 0u, n, n, fr, 0u, tv, carry, tot, 0ull, 0ull, T2, 0u,
+                                                      vat, cand);
           if (cand_better(c, best)) best = c;
         }
       }
