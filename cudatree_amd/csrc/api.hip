@@ -438,7 +438,10 @@ size_t tree_bytes(int64_t N, int64_t Fr, int64_t K, int64_t m, size_t idx_bytes)
 }
 
 // Batched levels issued between two host reads of the control block.
-constexpr uint32_t kLevelGroup = 4;
+#ifndef CTA_LEVEL_GROUP
+#define CTA_LEVEL_GROUP 4
+#endif
+constexpr uint32_t kLevelGroup = CTA_LEVEL_GROUP;
 // Default positions per split-scan task and elements per small-node
 // reshuffle task, and positions per fill task (measured best at
 // 581,012 x 54, 50 trees).
@@ -565,7 +568,7 @@ struct Fitter {
     unsigned long long *bslot = ctx->ws_as<unsigned long long>("byte_slots", nbslot);
     fill_lds = !o.fill_global && fill_lds_bytes() <= (size_t)ctx->lds_per_block;
 #ifndef CTA_FILL_JMUL
-#define CTA_FILL_JMUL 3
+#define CTA_FILL_JMUL 1
 #define CTA_FILL_JMAX 16
 #endif
     Jcap = fill_lds ? (uint32_t)std::max<int64_t>(
