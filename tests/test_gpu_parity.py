@@ -289,6 +289,31 @@ def test_forest_tabular_u32_indices(ctx, orc, idx_bits, all_rows):
     assert helpers.forest_diff(got, helpers.oracle_forest(case, orc)) is None
 
 
+@pytest.mark.parametrize("rowfree_first", [False, True])
+def test_score_bound_keeps_ties(ctx, orc, rowfree_first):
+    """Chunks of a large node are ruled out by a score bound against the
+    candidates the node already has (its best row-free split, the feature's
+    chunk boundaries).  A continuous feature and a two-valued one give the
+    same perfect split of 65,536 samples, tied in score: the lower feature
+    index wins, so the chunk holding the continuous feature's tied position
+    is scanned, while its chunks far from the cut are ruled out."""
+    n, cut = 65536, 20000
+    rng = np.random.default_rng(41)
+    v = rng.permutation(n).astype(np.float32)  # distinct values
+    y = (v >= cut).astype(np.int64)
+    two = (v >= cut).astype(np.float32)
+    noise = rng.normal(size=n).astype(np.float32)
+    X = np.stack([noise, two, v] if rowfree_first else [noise, v, two], axis=1)
+    case = {"X": X, "y": y, "K": 2, "m": 3, "mss": 2, "max_depth": 0, "identity": True, "seed": 5, "boot": False,
+            "n_trees": 2}
+    print("score-bound tie rowfree_first=%d" % rowfree_first, flush=True)
+    f, _ = gpu_forest(ctx, case)
+    got = f.export()
+    assert helpers.forest_diff(got, helpers.oracle_forest(case, orc)) is None
+    assert got["feature"][0] == 1
+    assert got["threshold"][0] == (np.float32(0.5) if rowfree_first else np.float32(19999.5))
+
+
 @pytest.mark.parametrize("idx_bits,all_rows", [(0, 0), (32, 0), (0, 1)])
 def test_value_code_columns(ctx, orc, idx_bits, all_rows):
     """Features of every value-code kind: binary (bit column), < 256 integer
