@@ -486,15 +486,6 @@ __global__ __launch_bounds__(kBlock) void k_select_emit(FitArgs<IdxT, LabT> a) {
 // ------------------------------------------------------------------------
 // positions per lane in flight in a chunk-histogram task
 constexpr int kHistU = 8;
-// Chunks are ruled out (k_scan_carry) for up to kPruneMaxK classes; the
-// chunk boundaries of nodes of at least kBoundMinN samples join the known
-// candidates (below that, a node's chunks are too few for their value
-// gathers to pay).
-constexpr uint32_t kPruneMaxK = 8;
-#ifndef CTA_BOUND_MIN_N
-#define CTA_BOUND_MIN_N 32768
-#endif
-constexpr uint32_t kBoundMinN = CTA_BOUND_MIN_N;
 
 template <class IdxT, class LabT>
 __global__ __launch_bounds__(kBlock) void k_scan_hist(FitArgs<IdxT, LabT> a) {
@@ -559,14 +550,23 @@ __global__ __launch_bounds__(kBlock) void k_scan_hist(FitArgs<IdxT, LabT> a) {
 
 // Result-entry mark of a chunk that k_scan_carry has ruled out.
 constexpr uint32_t kScanSkipped = 1u;
+// Chunks are ruled out for up to kPruneMaxK classes; the chunk boundaries
+// of nodes of at least kBoundMinN samples join the known candidates (below
+// that, a node's chunks are too few for their value gathers to pay).
+constexpr uint32_t kPruneMaxK = 8;
+#ifndef CTA_BOUND_MIN_N
+#define CTA_BOUND_MIN_N 32768
+#endif
+constexpr uint32_t kBoundMinN = CTA_BOUND_MIN_N;
 
 // Exclusive prefix over the chunks of each multi-chunk (node, feature);
 // each chunk's result entry is cleared on the way, and marked skipped when
 // no position of the chunk can hold the node's split.  A split needs a
 // strictly higher score than any other candidate of the node, and two are
 // known here: the node's best row-free candidate (scored exactly by the
-// selection) and the feature's chunk boundaries whose two values pass the
-// candidate test (their left class counts are the carries).  For p in
+// selection) and, in nodes of at least kBoundMinN samples, the feature's
+// chunk boundaries whose two values pass the candidate test (their left
+// class counts are the carries).  For p in
 // [p0, p1) the left class counts lie between the chunk's carries and the
 // next chunk's (sum of squares at most SLe, with nL >= p0 + 1) and the
 // right counts only shrink (sum of squares at most SRs = sum_k (t_k -
