@@ -712,7 +712,13 @@ __global__ __launch_bounds__(kBlock) void k_scan_carry(FitArgs<IdxT, LabT> a) {
 // label and value gathers, are issued together before the tiles are scanned
 // in order.
 // ------------------------------------------------------------------------
-constexpr int kScanU = 4;
+#ifndef CTA_SCAN_U
+#define CTA_SCAN_U 3
+#endif
+#ifndef CTA_SCAN_WAVES
+#define CTA_SCAN_WAVES 6
+#endif
+constexpr int kScanU = CTA_SCAN_U;
 // Label ranks in the split scan: with the class counts in registers (lane k
 // holds class k), one ballot per distinct label of a tile; with them in LDS
 // (more classes), one ballot per label bit.
@@ -843,7 +849,7 @@ __device__ __forceinline__ Cand scan_positions(const FitArgs<IdxT, LabT> &a, con
 // instance that fits K.
 template <class IdxT, class LabT, bool kRegs>
 // Six waves per SIMD (a few registers spilled) beat five without spills.
-__global__ __launch_bounds__(kBlock, 6) void k_scan(FitArgs<IdxT, LabT> a) {
+__global__ __launch_bounds__(kBlock, CTA_SCAN_WAVES) void k_scan(FitArgs<IdxT, LabT> a) {
   if (a.ctl->err) return;  // a list overflowed: the fit fails, nothing more is read
   extern __shared__ uint32_t smem[];
   const uint32_t lane = lane_id();
