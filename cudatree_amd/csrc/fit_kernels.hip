@@ -1024,11 +1024,14 @@ constexpr uint32_t kBigN = CTA_BIG_N;
 constexpr uint32_t kBigMaxRows = CTA_BIG_MAX_ROWS;
 constexpr int kReshThreads = 1024;
 constexpr int kReshWaves = kReshThreads / kWave;  // 16
-constexpr int kReshE = 8;  // elements per lane in flight in a small task
+#ifndef CTA_RESH_SMALL_E
+#define CTA_RESH_SMALL_E 4
+#endif
+constexpr int kReshE = CTA_RESH_SMALL_E;  // elements per lane in flight in a small task
 // Big tasks: elements per thread per tile, and whether the next tile's
 // loads are issued into a second register set before this tile's stores.
 #ifndef CTA_RESH_BIG_E
-#define CTA_RESH_BIG_E 8
+#define CTA_RESH_BIG_E 4
 #endif
 #ifndef CTA_RESH_PIPE
 #define CTA_RESH_PIPE 1
