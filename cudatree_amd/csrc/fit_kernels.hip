@@ -485,7 +485,10 @@ __global__ __launch_bounds__(kBlock) void k_select_emit(FitArgs<IdxT, LabT> a) {
 // the decomposition of scan_gini_large + scan_reduce (dfs_module.cu:24-106).
 // ------------------------------------------------------------------------
 // positions per lane in flight in a chunk-histogram task
-constexpr int kHistU = 8;
+#ifndef CTA_HIST_U
+#define CTA_HIST_U 8
+#endif
+constexpr int kHistU = CTA_HIST_U;
 
 template <class IdxT, class LabT>
 __global__ __launch_bounds__(kBlock) void k_scan_hist(FitArgs<IdxT, LabT> a) {
@@ -1371,7 +1374,10 @@ __device__ __forceinline__ void rl_add(const FitArgs<IdxT, LabT> &a, const uint3
 }
 
 // positions per lane in flight in a fill task
-constexpr int kFillU = 8;
+#ifndef CTA_FILL_U
+#define CTA_FILL_U 8
+#endif
+constexpr int kFillU = CTA_FILL_U;
 
 // Adds the wave's row-free counters (node `node`'s) to rl_small and clears
 // them.  `whole`: this wave counted the node's only fill task, so its
