@@ -445,7 +445,10 @@ constexpr uint32_t kLevelGroup = CTA_LEVEL_GROUP;
 // Default positions per split-scan task and elements per small-node
 // reshuffle task, and positions per fill task (measured best at
 // 581,012 x 54, 50 trees).
-constexpr uint32_t kScanChunk = 1024, kReshChunk = 4096, kFillChunk = 512;
+#ifndef CTA_FILL_CHUNK
+#define CTA_FILL_CHUNK 512
+#endif
+constexpr uint32_t kScanChunk = 1024, kReshChunk = 4096, kFillChunk = CTA_FILL_CHUNK;
 
 
 template <class IdxT, class LabT>

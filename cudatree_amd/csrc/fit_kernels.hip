@@ -66,7 +66,10 @@ __device__ __forceinline__ const IdxT *rows_of(const FitArgs<IdxT, LabT> &a, uin
 }
 
 // Split-scan tasks aimed for at a level whose frontier is small.
-constexpr uint32_t kScanTaskTarget = 4096;
+#ifndef CTA_SCAN_TASK_TARGET
+#define CTA_SCAN_TASK_TARGET 4096
+#endif
+constexpr uint32_t kScanTaskTarget = CTA_SCAN_TASK_TARGET;
 
 // Split-scan class (< kXcd) of the tasks of feature f in frontier node i.
 // Workgroups are dispatched to the XCDs round-robin (workgroup b on XCD
