@@ -46,7 +46,10 @@ __global__ __launch_bounds__(256) void k_transpose(const float *__restrict__ X, 
 
 // ---- stable radix presort -------------------------------------------------
 constexpr int kSortThreads = 256;
-constexpr int kSortItems = 8;
+#ifndef CTA_SORT_ITEMS
+#define CTA_SORT_ITEMS 8
+#endif
+constexpr int kSortItems = CTA_SORT_ITEMS;
 constexpr int kSortTile = kSortThreads * kSortItems;  // elements per block and pass
 constexpr int kDigits = 16;
 
