@@ -482,7 +482,10 @@ struct Fitter {
     Cf = kFillChunk;
     sub_max = (opts.mode == CTA_MODE_BATCHED && opts.subtree_max > 0) ? (uint32_t)opts.subtree_max : 0u;
     st = c->stream;
-    G = (uint32_t)std::max(64, ctx->num_cu * 8);
+#ifndef CTA_G_PER_CU
+#define CTA_G_PER_CU 8
+#endif
+    G = (uint32_t)std::max(64, ctx->num_cu * CTA_G_PER_CU);
     const bool rowfree = !opts.all_rows && d->Frl > 0;
     Frl = rowfree ? (uint32_t)d->Frl : 0u;
     Wsig = rowfree ? (uint32_t)d->Wsig : 0u;

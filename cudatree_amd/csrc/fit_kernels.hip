@@ -729,7 +729,7 @@ constexpr int kScanU = CTA_SCAN_U;
 // holds class k), one ballot per distinct label of a tile; with them in LDS
 // (more classes), one ballot per label bit.
 #ifndef CTA_SCAN_REGS_K
-#define CTA_SCAN_REGS_K 8
+#define CTA_SCAN_REGS_K 4
 #endif
 #ifndef CTA_RANK_BITS
 #define CTA_RANK_BITS 1
